@@ -1,0 +1,228 @@
+"""Benchmark of the H1 Laplace hot path on one MI355X.
+
+Workload (the single-GPU configuration the metric is quoted on): 3D Poisson
+on the unit cube, 32^3 cubes x 6 Kuhn tetrahedra (196,608 tets), H1 order 4
+(2,146,689 dofs, 35 dofs per tet), integration order 8 (125 points),
+x ~ U[-1,1] from numpy default_rng(1).
+
+One step = one FP64 CSR operator application y = A x (the reference's
+SparseMatrix<double>::MultAdd path).  `value` is its throughput in GDoF/s
+(ndof / time per apply); the FP64/FP32 matrix-free applies, the FP64 CSR
+Jacobi-CG and the mixed-precision (FP32 matrix-free inner, FP64 refinement)
+time-to-1e-10 on the lognormal-kappa variant are reported beside it.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-cg]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+FP32_PEAK_TFLOPS = 157.3
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def timed(N, fn, steps, warmup):
+    """Average milliseconds per call of fn over `steps` calls, bracketed by
+    HIP events on the library stream (synchronised on both sides)."""
+    for _ in range(warmup):
+        fn()
+    N.synchronize()
+    e0, e1 = N.event_create(), N.event_create()
+    N.event_record(e0)
+    for _ in range(steps):
+        fn()
+    N.event_record(e1)
+    ms = N.event_elapsed_ms(e0, e1)
+    N.synchronize()
+    N.event_destroy(e0)
+    N.event_destroy(e1)
+    return ms / steps
+
+
+def build_mf(N, fes, mesh, ir, kappa, precision):
+    classes = N.BuildClasses(fes, ir)
+    detj, ginv = N.ComputeGeometry(mesh)
+    el = np.concatenate([c["elements"] for c in classes]).astype(np.int32)
+    off = np.cumsum([0] + [len(c["elements"]) for c in classes]).astype(np.int64)
+    G = detj[el, None, None] * ginv[el]
+    metric = np.ascontiguousarray(np.stack([G[:, 0, 0], G[:, 1, 1], G[:, 2, 2], G[:, 0, 1], G[:, 0, 2], G[:, 1, 2]], 1))
+    kap = None if kappa is None else np.ascontiguousarray(kappa[el])
+    return N.CreateMatrixFree(3, fes.nd_el, len(ir), fes.ndof, off,
+                              np.ascontiguousarray(np.concatenate([c["bmat"].ravel() for c in classes])),
+                              np.ascontiguousarray(ir.weights), np.ascontiguousarray(fes.el_dofs[el]), metric, kap,
+                              el, precision, 0)
+
+
+def cpu_baseline(rowptr, cols, vals, x, ndof, budget_s=10.0):
+    """The oracle's row-parallel CSR SpMV (OpenMP, all host threads) on the
+    same matrix; a bounded sample of about budget_s seconds."""
+    import oracle
+    y = np.empty(ndof)
+    oracle.spmv(rowptr, cols, vals, x, y)  # warm-up
+    t0 = time.perf_counter()
+    reps = 0
+    while True:
+        oracle.spmv(rowptr, cols, vals, x, y)
+        reps += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s or reps >= 200:
+            break
+    return {"value": round(ndof * reps / el / 1e9, 4), "unit": "GDoF/s", "cores": oracle.num_threads(),
+            "kind": "port",
+            "sample": f"{reps} FP64 CSR applies y=Ax of the 32^3 p=4 matrix ({int(rowptr[-1])} nonzeros), "
+                      f"oracle C SpMV with OpenMP, {el:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=32)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-cg", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path (see DESIGN.md for how the "
+              "operator would shard)", file=sys.stderr)
+        sys.exit(2)
+
+    from ngsolve_amd import _ngsamd as N
+    from ngsolve_amd._native import ensure_device
+    ensure_device(0)
+
+    n, p, order = args.n, 4, 8
+    t = time.perf_counter()
+    mesh = N.MakeStructured3DMesh(n, n, n)
+    fes = N.H1Space(mesh, p, 0xFFFFFFFFFFFFFFFF)
+    ir = N.TetRule(order)
+    rowptr, cols = N.BuildPattern(fes)
+    vals = N.AssembleCSR(fes, rowptr, cols, ir, True, None, False, None)
+    ndof, nnz, ne = fes.ndof, int(rowptr[-1]), mesh.ne
+    log(f"[bench] setup {time.perf_counter() - t:.1f}s ndof={ndof} nnz={nnz} ne={ne}")
+
+    A = N.CreateCSR(ndof, ndof, rowptr, cols, vals)
+    x = np.random.default_rng(1).uniform(-1, 1, ndof)
+    dx, dy = N.DeviceVector(ndof, 8), N.DeviceVector(ndof, 8)
+    dx.upload(x)
+
+    # ---------------------------------------------------- CSR SpMV (value)
+    ms_csr = timed(N, lambda: A.mult(dx, dy), args.steps, args.warmup)
+    y_csr = dy.download()
+    csr_bytes = nnz * 12 + (ndof + 1) * 4 + 2 * ndof * 8
+    gbs = csr_bytes / (ms_csr * 1e-3) / 1e9
+    gdofs = ndof / (ms_csr * 1e-3) / 1e9
+    log(f"[bench] csr spmv {ms_csr * 1e3:.1f} us  {gbs:.0f} GB/s  {gdofs:.2f} GDoF/s")
+
+    # ------------------------------------------------- matrix-free applies
+    extra = {}
+    nq = len(ir)
+    flops_mf = 435.0 * nq * ne
+    for name, prec, peak in [("mf_f64", 0, FP64_PEAK_TFLOPS), ("mf_f32", 1, FP32_PEAK_TFLOPS)]:
+        M = build_mf(N, fes, mesh, ir, None, prec)
+        ms = timed(N, lambda: M.mult(dx, dy), args.steps, args.warmup)
+        err = float(np.abs(dy.download() - y_csr).max() / np.abs(y_csr).max())
+        tf = flops_mf / (ms * 1e-3) / 1e12
+        extra[name] = {"us_per_apply": round(ms * 1e3, 1), "gdofs": round(ndof / (ms * 1e-3) / 1e9, 3),
+                       "tflops": round(tf, 2), "frac_of_peak": round(tf / peak, 4),
+                       "rel_err_vs_fp64_csr": err}
+        log(f"[bench] {name} {ms * 1e3:.1f} us  {tf:.1f} TFLOP/s  err {err:.1e}")
+        del M
+
+    # ----------------------------------------------------------------- CG
+    if not args.no_cg:
+        diag = np.empty(ndof)
+        dd = N.DeviceVector(ndof, 8)
+        A.diagonal(dd)
+        diag[:] = dd.download()
+        C = N.CreateJacobi(diag, fes.freedofs)
+        f = N.AssembleLinearForm(fes, ir, np.ones((ne, nq)))
+        df, du = N.DeviceVector(ndof, 8), N.DeviceVector(ndof, 8)
+        df.upload(f)
+        t0 = time.perf_counter()
+        its, res = N.CGSolve(A, C, df, du, True, 1e-10, -1.0, 100000, 16)
+        t_cg = time.perf_counter() - t0
+        extra["cg_fp64_csr_kappa1"] = {"iterations": its, "seconds": round(t_cg, 4),
+                                       "ms_per_iteration": round(1e3 * t_cg / max(its - 1, 1), 4)}
+        log(f"[bench] FP64 CSR CG kappa=1: {its} its {t_cg:.3f}s")
+
+        # lognormal kappa variant: FP64 CSR CG vs mixed-precision IR
+        kappa = np.exp(np.random.default_rng(2).standard_normal(ne))
+        vals_k = N.AssembleCSR(fes, rowptr, cols, ir, True, kappa, False, None)
+        Ak = N.CreateCSR(ndof, ndof, rowptr, cols, vals_k)
+        Ak.diagonal(dd)
+        Ck = N.CreateJacobi(dd.download(), fes.freedofs)
+        free = fes.freedofs.astype(bool)
+
+        def true_res(u):
+            r = f - _spmv_host(rowptr, cols, vals_k, u)
+            return float(np.linalg.norm(r[free]) / np.linalg.norm(f[free]))
+
+        t0 = time.perf_counter()
+        its_k, _ = N.CGSolve(Ak, Ck, df, du, True, 1e-10, -1.0, 200000, 16)
+        t_k = time.perf_counter() - t0
+        u64 = du.download()
+        log(f"[bench] FP64 CSR CG lognormal: {its_k} its {t_k:.3f}s")
+        M32 = build_mf(N, fes, mesh, ir, kappa, 1)
+        du2 = N.DeviceVector(ndof, 8)
+        t0 = time.perf_counter()
+        outer, inner, rel = N.IRCGSolve(Ak, M32, Ck, df, du2, 1e-10, 1e-4, 60, 200000)
+        t_ir = time.perf_counter() - t0
+        u_ir = du2.download()
+        log(f"[bench] mixed IR: outer {outer} inner {inner} rel {rel:.1e} {t_ir:.3f}s")
+        extra["cg_lognormal_kappa"] = {
+            "fp64_csr_cg": {"iterations": its_k, "seconds": round(t_k, 4), "true_rel_residual": true_res(u64)},
+            "mixed_ir_mf_f32": {"outer": outer, "inner_total": inner, "seconds": round(t_ir, 4),
+                                "true_rel_residual": true_res(u_ir),
+                                "rel_diff_vs_fp64_solution": float(np.linalg.norm(u_ir - u64) / np.linalg.norm(u64))},
+            "speedup_mixed_vs_fp64_csr": round(t_k / t_ir, 3)}
+
+    cpu = None
+    if not args.no_cpu:
+        log("[bench] cpu baseline ...")
+        cpu = cpu_baseline(rowptr, cols, vals, x, ndof)
+        log(f"[bench] cpu {cpu['value']} GDoF/s on {cpu['cores']} threads")
+
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tfile):
+        with open(tfile) as fh:
+            traffic = json.load(fh).get("csr_spmv_32cube_p4")
+
+    out = {
+        "metric": "GDoF/s apply + time-to-1e-10 CG, mixed vs FP64 on 1 GPU, % HBM/MFMA roofline",
+        "value": round(gdofs, 4), "unit": "GDoF/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms_csr, 5), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"3D Poisson unit cube {n}^3 cubes x 6 Kuhn tets, H1 p=4, intorder 8 "
+                               f"({ndof} dofs, {nnz} nnz), FP64 CSR y=Ax, x~U[-1,1] seed 1",
+                   "ndof": ndof, "nnz": nnz, "elements": ne},
+        "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": "CsrSpmvKernel", "algorithmic_bytes": csr_bytes},
+        "cpu_baseline": cpu,
+        "extra": extra,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def _spmv_host(rowptr, cols, vals, x):
+    import scipy.sparse as sp
+    n = len(rowptr) - 1
+    return sp.csr_matrix((vals, cols, rowptr), shape=(n, n)) @ x
+
+
+if __name__ == "__main__":
+    main()

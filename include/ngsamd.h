@@ -1,0 +1,160 @@
+/* ngsamd.h -- C ABI of the MI355X (gfx950) H1 operator / Krylov library.
+ *
+ * Every entry point returns 0 on success and a nonzero NGSAMD_E_* code on
+ * failure; ngsamd_last_error() then returns a message (thread-local).
+ * Pointers named d_* are device pointers (allocated with ngsamd_malloc or by
+ * any HIP allocator on the current device); pointers named h_* are host.
+ * All work is enqueued on the library stream (ngsamd_set_stream) in order;
+ * calls that return host values synchronise that stream.
+ *
+ * Reference interfaces replaced (file:line in NGSolve):
+ *   operators   ngla::BaseMatrix::Mult / MultAdd      linalg/basematrix.hpp:117-121
+ *   CSR SpMV    SparseMatrix<double>::MultAdd          linalg/sparsematrix_impl.hpp:264-306
+ *               DevSparseMatrix::MultAdd (cuSPARSE)    ngscuda/cuda_linalg.cpp:243-275
+ *   matrix-free MatrixFreeBTDTB::MultAdd               comp/matrixfree.cpp:681-1087
+ *               DevMatrixFreeBTDTB                     ngscuda/cuda_matrix_free_btdtb.cpp:16-427
+ *   Jacobi      JacobiPrecond::MultAdd                 linalg/jacobi.cpp:41-90
+ *   vectors     S_BaseVector<double>::InnerProduct     linalg/basevector.cpp:1127-1160
+ *               UnifiedVector Add/Scale/SetScalar      ngscuda/unifiedvector.cpp:136-212,437-515
+ *   CG          CGSolver._SolveImpl                    python/krylovspace.py:263-291
+ *               DevCGSolver (graph-captured loop)      ngscuda/cuda_krylov.cpp:19-203
+ */
+#ifndef NGSAMD_H
+#define NGSAMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NGSAMD_OK 0
+#define NGSAMD_E_HIP 1        /* HIP runtime error */
+#define NGSAMD_E_ARG 2        /* invalid argument / shape mismatch */
+#define NGSAMD_E_STATE 3      /* library not initialised, wrong handle kind */
+#define NGSAMD_E_NOMEM 4      /* device allocation failed */
+
+/* contraction precision of a matrix-free operator */
+#define NGSAMD_PREC_F64 0
+#define NGSAMD_PREC_F32 1
+#define NGSAMD_PREC_BF16 2
+
+/* scatter mode of a matrix-free operator */
+#define NGSAMD_SCATTER_GATHER 0   /* deterministic two-pass owner-computes gather */
+#define NGSAMD_SCATTER_ATOMIC 1   /* hardware FP64/FP32 atomics, as the reference kernel */
+
+typedef struct ngsamd_op_s* ngsamd_op_t;
+
+/* ---------------------------------------------------------------- runtime */
+int ngsamd_init(int device);
+const char* ngsamd_last_error(void);
+int ngsamd_set_stream(void* hip_stream);   /* NULL selects the library's own stream */
+void* ngsamd_get_stream(void);
+int ngsamd_synchronize(void);
+int ngsamd_malloc(void** d_ptr, size_t bytes);
+int ngsamd_free(void* d_ptr);
+int ngsamd_memcpy_htod(void* d_dst, const void* h_src, size_t bytes);
+int ngsamd_memcpy_dtoh(void* h_dst, const void* d_src, size_t bytes);
+int ngsamd_memcpy_dtod(void* d_dst, const void* d_src, size_t bytes);
+
+/* Timing utilities (no reference counterpart): HIP events recorded on the
+ * library stream, so they bracket exactly the kernels this library enqueues. */
+int ngsamd_event_create(void** ev);
+int ngsamd_event_record(void* ev);
+int ngsamd_event_elapsed_ms(void* start, void* stop, float* ms); /* synchronises `stop` */
+int ngsamd_event_destroy(void* ev);
+
+/* ------------------------------------------------- vectors (BaseVector ops) */
+/* *h_result = sum_i x_i y_i, FP64 accumulation, fixed (deterministic) order */
+int ngsamd_vec_dot(int64_t n, const double* d_x, const double* d_y, double* h_result);
+int ngsamd_vec_dot_f32(int64_t n, const float* d_x, const float* d_y, double* h_result);
+int ngsamd_vec_set(int64_t n, double a, double* d_x);                       /* x = a */
+int ngsamd_vec_scale(int64_t n, double a, double* d_x);                     /* x *= a */
+int ngsamd_vec_axpy(int64_t n, double a, const double* d_x, double* d_y);   /* y += a x */
+int ngsamd_vec_axpby(int64_t n, double a, const double* d_x, double b, double* d_y); /* y = a x + b y */
+int ngsamd_vec_f64_to_f32(int64_t n, const double* d_x, float* d_y);
+int ngsamd_vec_f32_to_f64(int64_t n, const float* d_x, double* d_y);
+
+/* ---------------------------------------------------- generic operators */
+int ngsamd_op_height(ngsamd_op_t op, int64_t* h);
+int ngsamd_op_width(ngsamd_op_t op, int64_t* w);
+/* y += s * A x   (BaseMatrix::MultAdd, linalg/basematrix.hpp:121) */
+int ngsamd_op_mult_add(ngsamd_op_t op, double s, const double* d_x, double* d_y);
+/* y = A x        (BaseMatrix::Mult, linalg/basematrix.hpp:117) */
+int ngsamd_op_mult(ngsamd_op_t op, const double* d_x, double* d_y);
+int ngsamd_op_destroy(ngsamd_op_t op);
+
+/* ------------------------------------------------------------- CSR matrix */
+/* Copies a host CSR (int64 row pointers as NGSolve's size_t firsti, int32
+ * column numbers sorted per row, FP64 values) to the device.
+ * SparseMatrix<double> storage, linalg/sparsematrix.hpp. */
+int ngsamd_csr_create(ngsamd_op_t* op, int64_t height, int64_t width,
+                      const int64_t* h_rowptr, const int* h_cols, const double* h_vals);
+/* diagonal of a CSR operator (0 where the row has no diagonal entry) */
+int ngsamd_csr_diagonal(ngsamd_op_t op, double* d_diag);
+
+/* ---------------------------------------------------------------- Jacobi */
+/* y += s * diag^{-1} x on free dofs, 0 on the others (linalg/jacobi.cpp:41-90).
+ * h_freedofs may be NULL (all free). */
+int ngsamd_jacobi_create(ngsamd_op_t* op, int64_t n, const double* h_diag, const uint8_t* h_freedofs);
+int ngsamd_jacobi_create_device(ngsamd_op_t* op, int64_t n, const double* d_diag, const uint8_t* h_freedofs);
+
+/* ------------------------------------------------- matrix-free H1 Laplace */
+/* Fused element-batched operator y += s * sum_e P_e^T B^T D_e B P_e x with
+ * D_e = kappa_e w_q |det F_e| F_e^{-1} F_e^{-T} (comp/matrixfree.cpp:681-1087).
+ *   dim        2 or 3
+ *   nd         local dofs per element
+ *   nq         quadrature points
+ *   nclass     number of vertex-permutation classes
+ *   h_class_offsets[nclass+1]  element ranges of the classes in the tables below
+ *   h_bmat     [nclass][nq][nd][dim] reference gradients d phi_i / d xi at ip q
+ *   h_weights  [nq]
+ *   h_dofs     [nel][nd] global dof numbers (int32)
+ *   h_metric   [nel][dim*(dim+1)/2] |det F| F^{-1}F^{-T}: (xx,yy,xy) in 2D,
+ *              (xx,yy,zz,xy,xz,yz) in 3D
+ *   h_kappa    [nel] element coefficient or NULL (=1)
+ *   h_elnr     [nel] global element number of each table row or NULL (=row)
+ *   precision  NGSAMD_PREC_*  (B and the contraction in that type, FP32
+ *              accumulation for BF16; input/output vectors stay FP64)
+ *   scatter    NGSAMD_SCATTER_* */
+int ngsamd_mf_create(ngsamd_op_t* op, int dim, int nd, int nq, int64_t ndof, int nclass,
+                     const int64_t* h_class_offsets, const double* h_bmat, const double* h_weights,
+                     const int* h_dofs, const double* h_metric, const double* h_kappa,
+                     const int* h_elnr, int precision, int scatter);
+/* replace the element coefficient (device array indexed by global element number) */
+int ngsamd_mf_set_kappa(ngsamd_op_t op, const double* d_kappa);
+/* y = A x on FP32 vectors (inner solves of iterative refinement) */
+int ngsamd_mf_mult_f32(ngsamd_op_t op, const float* d_x, float* d_y);
+/* diagonal of the operator, FP64 */
+int ngsamd_mf_diagonal(ngsamd_op_t op, double* d_diag);
+/* g_e = -int_e grad(lam) . grad(u) = -(lam_e^T K_e u_e)/kappa_e, indexed by
+ * global element number (adjoint gradient of an element coefficient). */
+int ngsamd_mf_element_grad(ngsamd_op_t op, const double* d_lam, const double* d_u, double* d_grad);
+
+/* ----------------------------------------------------------------- CG */
+/* Preconditioned CG with the recurrence and stopping rule of the reference's
+ * CGSolver (python/krylovspace.py:227-291): stop when
+ * sqrt|<w,d>| <= max(tol*sqrt|<w0,d0>|, atol) or after maxiter checks; the
+ * iteration count includes the initial check.  pre must be a Jacobi
+ * operator (fused) or any operator (generic path).  Scalars, dots and
+ * convergence tests stay on the device; the iteration is replayed from a
+ * captured HIP graph in chunks of `chunk` iterations (0 = no graph).
+ * h_residuals (may be NULL) receives up to maxiter residuals. */
+int ngsamd_cg_solve(ngsamd_op_t mat, ngsamd_op_t pre, const double* d_rhs, double* d_sol,
+                    int initialize, double tol, double atol, int maxiter, int chunk,
+                    int* h_iterations, double* h_residuals);
+
+/* Mixed-precision iterative refinement: outer FP64 residual r = f - A u with
+ * `mat`, inner Jacobi-PCG on `inner` (FP32 vectors, FP64 dots and scalars)
+ * to relative tolerance inner_tol, u += correction, until
+ * ||r||_2 <= tol ||f||_2.  h_stats[0]=outer its, [1]=total inner its,
+ * [2]=final relative residual. */
+int ngsamd_ir_cg_solve(ngsamd_op_t mat, ngsamd_op_t inner, ngsamd_op_t pre, const double* d_rhs,
+                       double* d_sol, double tol, double inner_tol, int max_outer, int max_inner,
+                       double* h_stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NGSAMD_H */

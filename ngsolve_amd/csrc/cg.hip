@@ -1,0 +1,520 @@
+// Jacobi preconditioner, the device-resident preconditioned CG loop and the
+// mixed-precision iterative refinement built on it.
+//
+// Jacobi: JacobiPrecond (linalg/jacobi.cpp:41-90): invdiag = 1/a_ii on free
+// dofs and 0 elsewhere, y += s * invdiag .* x.
+//
+// CG: the recurrence and stopping rule of CGSolver._SolveImpl
+// (python/krylovspace.py:263-291) and LinearSolver.CheckResidual (:129-150):
+//   d = f - A u;  w = C d;  s = w;  wdn = <w,d>;  check(sqrt|wdn|)
+//   loop: w = A s; wd = wdn; as = <s,w>; stop if as == 0 or wd == 0
+//         alpha = wd/as; u += alpha s; d -= alpha w; w = C d; wdn = <w,d>
+//         check(sqrt|wdn|); beta = wdn/wd; s = beta s + w
+// check: iterations += 1; converged when iterations >= maxiter or
+// residual <= final, final = tol*res0 (max'ed with atol when given).
+//
+// One iteration is three kernels: (1) w = A s fused with <s,w> (CSR) or the
+// operator followed by a dot kernel, (2) the two updates, the Jacobi apply
+// and <w,d> fused, (3) s = beta s + w.  Reductions finish in the last
+// workgroup, which also evaluates alpha, beta and the stopping test on the
+// device, so the loop needs no host round trip.  `chunk` iterations are
+// captured once into a HIP graph that is replayed until the device flag says
+// done; every kernel returns at once when the flag is set, so the overshoot
+// of the last chunk costs only launch latency.  (Reference: DevCGSolver,
+// ngscuda/cuda_krylov.cpp:19-203, which uses a CUDA conditional WHILE node.)
+//
+// The vector type V is double for the FP64 solver and float for the inner
+// solver of iterative refinement; inner products, alpha, beta and the
+// stopping test are FP64 in both.
+#include <cmath>
+#include <vector>
+
+#include "common.hpp"
+#include "csr.hpp"
+#include "matfree.hpp"
+#include "reduce.hpp"
+#include "runtime_state.hpp"
+
+namespace ngsamd {
+
+// ------------------------------------------------------------------ Jacobi
+
+template <class V>
+__global__ void InvDiagKernel(const double* __restrict__ diag, const uint8_t* __restrict__ free, V* __restrict__ inv,
+                              int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    inv[i] = (free == nullptr || free[i]) ? (V)(1.0 / diag[i]) : (V)0;
+}
+
+JacobiOp::JacobiOp(int64_t n, const double* diag, bool on_device, const uint8_t* freedofs)
+    : ngsamd_op_s(OpKind::Jacobi) {
+  height = width = n;
+  invdiag.alloc(n);
+  if (n == 0) return;
+  hipStream_t st = Stream();
+  DevBuf<uint8_t> dfree;
+  if (freedofs) {
+    dfree.alloc(n);
+    dfree.upload(freedofs, n);
+  }
+  DevBuf<double> ddiag;
+  const double* src = diag;
+  if (!on_device) {
+    ddiag.alloc(n);
+    ddiag.upload(diag, n);
+    src = ddiag.p;
+  }
+  InvDiagKernel<double><<<GridFor(n, 256, 8192), 256, 0, st>>>(src, dfree.p, invdiag.p, n);
+  NGSAMD_CHECK_LAUNCH();
+  NGSAMD_HIP(hipStreamSynchronize(st));
+}
+
+__global__ void JacobiMultAddKernel(const double* __restrict__ inv, const double* __restrict__ x,
+                                    double* __restrict__ y, int64_t n, double s, bool accum) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    double v = s * (inv[i] * x[i]);
+    y[i] = accum ? y[i] + v : v;
+  }
+}
+
+void JacobiOp::MultAdd(double s, const double* x, double* y, hipStream_t st) {
+  if (height == 0) return;
+  JacobiMultAddKernel<<<GridFor(height, 256, 8192), 256, 0, st>>>(invdiag.p, x, y, height, s, true);
+  NGSAMD_CHECK_LAUNCH();
+}
+
+void JacobiOp::Mult(const double* x, double* y, hipStream_t st) {
+  if (height == 0) return;
+  JacobiMultAddKernel<<<GridFor(height, 256, 8192), 256, 0, st>>>(invdiag.p, x, y, height, 1.0, false);
+  NGSAMD_CHECK_LAUNCH();
+}
+
+// ---------------------------------------------------------- CG device state
+
+struct CGScalars {
+  double wdn;        // current <w,d>
+  double as_s;       // <s, A s>
+  double beta;
+  double final_res;  // stopping threshold
+  double tol, atol;  // tol < 0: not given; atol < 0: not given
+  int it;            // CheckResidual calls so far
+  int done;          // 0 running, 1 converged / maxiter, 2 breakdown
+  int maxiter;
+  int pad;
+};
+
+constexpr int kCgBlock = 256;
+
+__device__ __forceinline__ void CheckResidual(CGScalars* S, double* residuals, double res) {
+  S->it += 1;
+  if (S->it <= S->maxiter) residuals[S->it - 1] = res;
+  if (S->it == 1) {
+    if (S->tol < 0) S->final_res = S->atol;
+    else {
+      S->final_res = res * S->tol;
+      if (S->atol >= 0 && S->atol > S->final_res) S->final_res = S->atol;
+    }
+  }
+  if (S->it >= S->maxiter || res <= S->final_res) S->done = 1;
+}
+
+// w = C d; s = w; wdn = <w,d>; first check
+template <class V>
+__global__ __launch_bounds__(kCgBlock) void CgInitKernel(const V* __restrict__ inv, const V* __restrict__ d,
+                                                         V* __restrict__ w, V* __restrict__ s, int64_t n,
+                                                         CGScalars* S, double* residuals, double* partials,
+                                                         unsigned* counter) {
+  __shared__ double lds[kCgBlock / 64];
+  __shared__ bool s_last;
+  double acc = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)kCgBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kCgBlock) {
+    const V di = d[i];
+    const V wi = inv[i] * di;
+    w[i] = wi;
+    s[i] = wi;
+    acc += (double)wi * (double)di;
+  }
+  double v = BlockSum<kCgBlock>(acc, lds);
+  double vals[1] = {v};
+  if (PublishPartials<1>(vals, partials, counter, &s_last)) {
+    double tot[1];
+    ReducePartials<kCgBlock, 1>(partials, tot, lds);
+    if (threadIdx.x == 0) {
+      S->wdn = tot[0];
+      CheckResidual(S, residuals, sqrt(fabs(tot[0])));
+      *counter = 0u;
+    }
+  }
+}
+
+// u += alpha s; d -= alpha w; w = C d; wdn = <w,d>; check; beta
+template <class V>
+__global__ __launch_bounds__(kCgBlock) void CgUpdateKernel(const V* __restrict__ inv, V* __restrict__ u,
+                                                           V* __restrict__ d, V* __restrict__ w,
+                                                           const V* __restrict__ s, int64_t n, CGScalars* S,
+                                                           double* residuals, double* partials, unsigned* counter) {
+  if (S->done) return;
+  const double wd = S->wdn, as_s = S->as_s;
+  if (as_s == 0.0 || wd == 0.0) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) S->done = 2;
+    return;
+  }
+  const double alpha = wd / as_s;
+  const V alv = (V)alpha;
+  __shared__ double lds[kCgBlock / 64];
+  __shared__ bool s_last;
+  double acc = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)kCgBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kCgBlock) {
+    const V si = s[i], wi = w[i];
+    u[i] += alv * si;
+    const V di = d[i] - alv * wi;
+    d[i] = di;
+    const V zi = inv[i] * di;
+    w[i] = zi;
+    acc += (double)zi * (double)di;
+  }
+  double v = BlockSum<kCgBlock>(acc, lds);
+  double vals[1] = {v};
+  if (PublishPartials<1>(vals, partials, counter, &s_last)) {
+    double tot[1];
+    ReducePartials<kCgBlock, 1>(partials, tot, lds);
+    if (threadIdx.x == 0) {
+      const double wdn = tot[0];
+      S->wdn = wdn;
+      CheckResidual(S, residuals, sqrt(fabs(wdn)));
+      S->beta = wdn / wd;
+      *counter = 0u;
+    }
+  }
+}
+
+// s = beta s + w
+template <class V>
+__global__ __launch_bounds__(kCgBlock) void CgDirectionKernel(V* __restrict__ s, const V* __restrict__ w, int64_t n,
+                                                              const CGScalars* S) {
+  if (S->done) return;
+  const V beta = (V)S->beta;
+  for (int64_t i = blockIdx.x * (int64_t)kCgBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kCgBlock)
+    s[i] = beta * s[i] + w[i];
+}
+
+// <x,y> (FP64 accumulation) into *out, skipped when *done
+template <class V>
+__global__ __launch_bounds__(kCgBlock) void CgDotKernel(const V* __restrict__ x, const V* __restrict__ y, int64_t n,
+                                                        double* out, const int* done, double* partials,
+                                                        unsigned* counter) {
+  if (*done) return;
+  __shared__ double lds[kCgBlock / 64];
+  __shared__ bool s_last;
+  double acc = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)kCgBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kCgBlock)
+    acc += (double)x[i] * (double)y[i];
+  double v = BlockSum<kCgBlock>(acc, lds);
+  double vals[1] = {v};
+  if (PublishPartials<1>(vals, partials, counter, &s_last)) {
+    double tot[1];
+    ReducePartials<kCgBlock, 1>(partials, tot, lds);
+    if (threadIdx.x == 0) {
+      *out = tot[0];
+      *counter = 0u;
+    }
+  }
+}
+
+// *out = sum over free dofs (mask != 0) of x_i^2
+__global__ __launch_bounds__(kCgBlock) void MaskedSqKernel(const double* __restrict__ x, const double* __restrict__ mask,
+                                                           int64_t n, double* out, double* partials,
+                                                           unsigned* counter) {
+  __shared__ double lds[kCgBlock / 64];
+  __shared__ bool s_last;
+  double acc = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)kCgBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kCgBlock)
+    if (mask[i] != 0.0) acc += x[i] * x[i];
+  double v = BlockSum<kCgBlock>(acc, lds);
+  double vals[1] = {v};
+  if (PublishPartials<1>(vals, partials, counter, &s_last)) {
+    double tot[1];
+    ReducePartials<kCgBlock, 1>(partials, tot, lds);
+    if (threadIdx.x == 0) {
+      *out = tot[0];
+      *counter = 0u;
+    }
+  }
+}
+
+__global__ void ScaledToF32Kernel(const double* __restrict__ x, float* __restrict__ y, int64_t n,
+                                  const double* __restrict__ scale) {
+  const double s = *scale;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    y[i] = (float)(x[i] * s);
+}
+
+__global__ void AddScaledF32Kernel(double* __restrict__ u, const float* __restrict__ c, int64_t n,
+                                   const double* __restrict__ scale) {
+  const double s = *scale;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    u[i] += s * (double)c[i];
+}
+
+static int CgGrid(int64_t n) { return GridFor(n, kCgBlock, 2048); }
+
+// w = A s and as_s = <s, w>, skipped when done
+template <class V>
+static void EnqueueMatDot(ngsamd_op_t A, const V* s, V* w, int64_t n, CGScalars* dS, hipStream_t st) {
+  auto& R = State();
+  int* done = &dS->done;
+  if constexpr (sizeof(V) == 8) {
+    if (A->kind == OpKind::CSR) {
+      static_cast<CsrOp*>(A)->MultDot(s, w, &dS->as_s, done, st);
+      return;
+    }
+    if (A->kind == OpKind::MatrixFree) static_cast<MatFreeOp*>(A)->MultStop(s, w, done, st);
+    else A->Mult(s, w, st);
+  } else {
+    NGSAMD_REQUIRE(A->kind == OpKind::MatrixFree, "cg: FP32 vectors need a matrix-free operator");
+    static_cast<MatFreeOp*>(A)->MultF32(s, w, done, st);
+  }
+  CgDotKernel<V><<<CgGrid(n), kCgBlock, 0, st>>>(s, w, n, &dS->as_s, done, R.partials, R.counters + kCounterCG1);
+  NGSAMD_CHECK_LAUNCH();
+}
+
+// One CG iteration on the stream (graph-capturable: kernels only).
+template <class V>
+static void EnqueueIteration(ngsamd_op_t A, const V* inv, V* u, V* d, V* w, V* s, int64_t n, CGScalars* dS,
+                             double* residuals, hipStream_t st) {
+  auto& R = State();
+  EnqueueMatDot<V>(A, s, w, n, dS, st);
+  CgUpdateKernel<V><<<CgGrid(n), kCgBlock, 0, st>>>(inv, u, d, w, s, n, dS, residuals, R.partials,
+                                                     R.counters + kCounterCG2);
+  NGSAMD_CHECK_LAUNCH();
+  CgDirectionKernel<V><<<CgGrid(n), kCgBlock, 0, st>>>(s, w, n, dS);
+  NGSAMD_CHECK_LAUNCH();
+}
+
+struct CgResult {
+  int iterations = 0;
+  int done = 0;
+};
+
+// Work vectors and device scalars of one CG instance.
+template <class V>
+struct CgWork {
+  DevBuf<V> d, w, s;
+  DevBuf<double> residuals;
+  DevBuf<CGScalars> S;
+  CgWork(int64_t n, int maxiter) : d(n), w(n), s(n), residuals(maxiter), S(1) {}
+};
+
+// Iterates from an initialised state (d = f - A u already set, u set) until
+// the device flag is raised.  Uses a graph of `chunk` iterations if possible.
+template <class V>
+static CgResult IterateCg(ngsamd_op_t A, const V* inv, V* u, CgWork<V>& W, int64_t n, double tol, double atol,
+                          int maxiter, int chunk, hipStream_t st) {
+  auto& R = State();
+  CGScalars hS{};
+  hS.tol = tol;
+  hS.atol = atol;
+  hS.maxiter = maxiter;
+  NGSAMD_HIP(hipMemcpyAsync(W.S.p, &hS, sizeof(CGScalars), hipMemcpyHostToDevice, st));
+  CgInitKernel<V><<<CgGrid(n), kCgBlock, 0, st>>>(inv, W.d.p, W.w.p, W.s.p, n, W.S.p, W.residuals.p, R.partials,
+                                                  R.counters + kCounterCG2);
+  NGSAMD_CHECK_LAUNCH();
+
+  CGScalars* pin = nullptr;
+  NGSAMD_HIP(hipHostMalloc((void**)&pin, 2 * sizeof(CGScalars), hipHostMallocDefault));
+  struct PinGuard {
+    CGScalars* p;
+    ~PinGuard() { (void)hipHostFree(p); }
+  } pin_guard{pin};
+
+  hipGraphExec_t exec = nullptr;
+  hipGraph_t graph = nullptr;
+  bool use_graph = chunk > 0;
+  if (use_graph) {
+    hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+      bool ok = true;
+      try {
+        for (int k = 0; k < chunk; k++) EnqueueIteration<V>(A, inv, u, W.d.p, W.w.p, W.s.p, n, W.S.p, W.residuals.p, st);
+      } catch (...) {
+        ok = false;
+      }
+      e = hipStreamEndCapture(st, &graph);
+      if (!ok && e == hipSuccess) e = hipErrorUnknown;
+      if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    }
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      if (graph) (void)hipGraphDestroy(graph);
+      graph = nullptr;
+      exec = nullptr;
+      use_graph = false;
+    }
+  }
+  struct GraphGuard {
+    hipGraph_t g;
+    hipGraphExec_t x;
+    ~GraphGuard() {
+      if (x) (void)hipGraphExecDestroy(x);
+      if (g) (void)hipGraphDestroy(g);
+    }
+  } graph_guard{graph, exec};
+
+  const int per_launch = use_graph ? chunk : 1;
+  auto enqueue = [&](int slot) {
+    if (use_graph) NGSAMD_HIP(hipGraphLaunch(exec, st));
+    else EnqueueIteration<V>(A, inv, u, W.d.p, W.w.p, W.s.p, n, W.S.p, W.residuals.p, st);
+    NGSAMD_HIP(hipMemcpyAsync(&pin[slot], W.S.p, sizeof(CGScalars), hipMemcpyDeviceToHost, st));
+  };
+  hipEvent_t ev[2];
+  NGSAMD_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+  NGSAMD_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+  struct EvGuard {
+    hipEvent_t* e;
+    ~EvGuard() { (void)hipEventDestroy(e[0]); (void)hipEventDestroy(e[1]); }
+  } ev_guard{ev};
+
+  NGSAMD_HIP(hipMemcpyAsync(&pin[0], W.S.p, sizeof(CGScalars), hipMemcpyDeviceToHost, st));
+  NGSAMD_HIP(hipStreamSynchronize(st));
+  if (!pin[0].done) {
+    // keep two launches in flight; poll the older one
+    enqueue(0);
+    NGSAMD_HIP(hipEventRecord(ev[0], st));
+    enqueue(1);
+    NGSAMD_HIP(hipEventRecord(ev[1], st));
+    int slot = 0;
+    const long long max_launches = (long long)maxiter / per_launch + 4;
+    for (long long k = 0; k < max_launches; k++) {
+      NGSAMD_HIP(hipEventSynchronize(ev[slot]));
+      if (pin[slot].done) break;
+      enqueue(slot);
+      NGSAMD_HIP(hipEventRecord(ev[slot], st));
+      slot ^= 1;
+    }
+  }
+  NGSAMD_HIP(hipMemcpyAsync(&hS, W.S.p, sizeof(CGScalars), hipMemcpyDeviceToHost, st));
+  NGSAMD_HIP(hipStreamSynchronize(st));
+  NGSAMD_REQUIRE(hS.done != 0, "cg: iteration did not terminate within maxiter (internal error)");
+  CgResult res;
+  res.iterations = hS.it;
+  res.done = hS.done;
+  return res;
+}
+
+static void CheckCgOperators(ngsamd_op_t A, ngsamd_op_t pre) {
+  NGSAMD_REQUIRE(A && pre, "cg: null operator");
+  NGSAMD_REQUIRE(pre->kind == OpKind::Jacobi, "cg: the device CG needs a Jacobi preconditioner");
+  NGSAMD_REQUIRE(A->height == A->width && pre->height == A->height, "cg: operator shapes do not match");
+}
+
+}  // namespace ngsamd
+
+using namespace ngsamd;
+
+extern "C" int ngsamd_jacobi_create(ngsamd_op_t* op, int64_t n, const double* diag, const uint8_t* freedofs) {
+  return Guard([&] {
+    RequireInit();
+    NGSAMD_REQUIRE(op && (n == 0 || diag), "ngsamd_jacobi_create: null argument");
+    *op = nullptr;
+    *op = new JacobiOp(n, diag, false, freedofs);
+  });
+}
+
+extern "C" int ngsamd_jacobi_create_device(ngsamd_op_t* op, int64_t n, const double* d_diag,
+                                           const uint8_t* freedofs) {
+  return Guard([&] {
+    RequireInit();
+    NGSAMD_REQUIRE(op && (n == 0 || d_diag), "ngsamd_jacobi_create_device: null argument");
+    *op = nullptr;
+    *op = new JacobiOp(n, d_diag, true, freedofs);
+  });
+}
+
+extern "C" int ngsamd_cg_solve(ngsamd_op_t A, ngsamd_op_t pre, const double* f, double* u, int initialize,
+                               double tol, double atol, int maxiter, int chunk, int* h_iterations,
+                               double* h_residuals) {
+  return Guard([&] {
+    RequireInit();
+    CheckCgOperators(A, pre);
+    NGSAMD_REQUIRE(maxiter >= 1, "cg: maxiter must be >= 1");
+    const int64_t n = A->height;
+    hipStream_t st = Stream();
+    CgWork<double> W(n, maxiter);
+    // d = f - A u
+    if (initialize) LaunchFill(u, n, 0.0, st);
+    if (n) NGSAMD_HIP(hipMemcpyAsync(W.d.p, f, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+    if (!initialize) A->MultAdd(-1.0, u, W.d.p, st);
+    const double* inv = static_cast<JacobiOp*>(pre)->invdiag.p;
+    CgResult r = IterateCg<double>(A, inv, u, W, n, tol, atol, maxiter, chunk, st);
+    if (h_iterations) *h_iterations = r.iterations;
+    if (h_residuals && r.iterations > 0) {
+      const int cnt = r.iterations < maxiter ? r.iterations : maxiter;
+      NGSAMD_HIP(hipMemcpyAsync(h_residuals, W.residuals.p, cnt * sizeof(double), hipMemcpyDeviceToHost, st));
+      NGSAMD_HIP(hipStreamSynchronize(st));
+    }
+  });
+}
+
+extern "C" int ngsamd_ir_cg_solve(ngsamd_op_t A, ngsamd_op_t inner, ngsamd_op_t pre, const double* f, double* u,
+                                  double tol, double inner_tol, int max_outer, int max_inner, double* h_stats) {
+  return Guard([&] {
+    RequireInit();
+    CheckCgOperators(A, pre);
+    NGSAMD_REQUIRE(inner && inner->kind == OpKind::MatrixFree && inner->height == A->height,
+                   "ir_cg: inner operator must be matrix-free of the same size");
+    NGSAMD_REQUIRE(max_outer >= 1 && max_inner >= 1, "ir_cg: iteration limits must be >= 1");
+    const int64_t n = A->height;
+    hipStream_t st = Stream();
+    auto& R = State();
+    MatFreeOp* Ai = static_cast<MatFreeOp*>(inner);
+    DevBuf<double> r(n);
+    DevBuf<float> inv32(n), c32(n);
+    DevBuf<double> scal(2);  // [0] = 1/||r||, [1] = ||r||
+    // FP32 copy of the Jacobi inverse diagonal
+    const double* inv64 = static_cast<JacobiOp*>(pre)->invdiag.p;
+    ngsamd_vec_f64_to_f32(n, inv64, inv32.p);
+    CgWork<float> W(n, max_inner);
+
+    // residual norms over the free dofs (where the Jacobi inverse is nonzero)
+    DevBuf<double> sq(1);
+    auto free_norm = [&](const double* x) {
+      MaskedSqKernel<<<CgGrid(n), kCgBlock, 0, st>>>(x, inv64, n, sq.p, R.partials, R.counters + kCounterMisc);
+      NGSAMD_CHECK_LAUNCH();
+      double h = 0.0;
+      NGSAMD_HIP(hipMemcpyAsync(&h, sq.p, sizeof(double), hipMemcpyDeviceToHost, st));
+      NGSAMD_HIP(hipStreamSynchronize(st));
+      return std::sqrt(h);
+    };
+    LaunchFill(u, n, 0.0, st);
+    const double fn = n ? free_norm(f) : 0.0;
+    int outer = 0;
+    long long inner_total = 0;
+    double relres = 1.0;
+    if (n) NGSAMD_HIP(hipMemcpyAsync(r.p, f, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+    while (true) {
+      const double rn = n ? free_norm(r.p) : 0.0;
+      relres = fn > 0 ? rn / fn : 0.0;
+      if (rn <= tol * fn || outer >= max_outer || rn == 0.0) break;
+      // inner solve A c = r / ||r|| in FP32 vectors
+      double hsc[2] = {1.0 / rn, rn};
+      NGSAMD_HIP(hipMemcpyAsync(scal.p, hsc, sizeof(hsc), hipMemcpyHostToDevice, st));
+      ScaledToF32Kernel<<<GridFor(n, 256, 8192), 256, 0, st>>>(r.p, W.d.p, n, scal.p);
+      NGSAMD_CHECK_LAUNCH();
+      LaunchFillF32(c32.p, n, 0.0f, st);
+      CgResult ir = IterateCg<float>(Ai, inv32.p, c32.p, W, n, inner_tol, -1.0, max_inner, 16, st);
+      inner_total += ir.iterations;
+      // u += ||r|| c; r = f - A u (FP64)
+      AddScaledF32Kernel<<<GridFor(n, 256, 8192), 256, 0, st>>>(u, c32.p, n, scal.p + 1);
+      NGSAMD_CHECK_LAUNCH();
+      NGSAMD_HIP(hipMemcpyAsync(r.p, f, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+      A->MultAdd(-1.0, u, r.p, st);
+      outer++;
+    }
+    (void)R;
+    if (h_stats) {
+      h_stats[0] = outer;
+      h_stats[1] = (double)inner_total;
+      h_stats[2] = relres;
+    }
+  });
+}

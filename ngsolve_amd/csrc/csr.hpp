@@ -1,0 +1,30 @@
+// Device CSR operator and Jacobi preconditioner types shared by the SpMV and
+// CG translation units.
+#pragma once
+#include <vector>
+
+#include "common.hpp"
+
+namespace ngsamd {
+
+struct CsrOp : ngsamd_op_s {
+  DevBuf<int> rowptr, cols;
+  DevBuf<double> vals;
+  int64_t nnz = 0;
+  int group = 32;  // lanes per row
+  CsrOp(int64_t h, int64_t w, const int64_t* rp, const int* c, const double* v);
+  void MultAdd(double s, const double* x, double* y, hipStream_t st) override;
+  void Mult(const double* x, double* y, hipStream_t st) override;
+  // y = A x and *d_result = <x, A x>; skipped entirely when *d_stop != 0
+  void MultDot(const double* x, double* y, double* d_result, const int* d_stop, hipStream_t st);
+  void Diagonal(double* d, hipStream_t st);
+};
+
+struct JacobiOp : ngsamd_op_s {
+  DevBuf<double> invdiag;  // 1/a_ii on free dofs, 0 elsewhere
+  JacobiOp(int64_t n, const double* diag, bool diag_on_device, const uint8_t* freedofs);
+  void MultAdd(double s, const double* x, double* y, hipStream_t st) override;
+  void Mult(const double* x, double* y, hipStream_t st) override;
+};
+
+}  // namespace ngsamd

@@ -1,0 +1,597 @@
+// Fused element-batched matrix-free H1 Laplace operator on gfx950.
+//
+// Reference: MatrixFreeBTDTB::MultAdd (comp/matrixfree.cpp:681-1087) and its
+// CUDA counterpart DevMatrixFreeBTDTB (ngscuda/cuda_matrix_free_btdtb.cpp:16-427).
+// For every element e of a vertex-permutation class (shared reference
+// gradients B, [q][i][d] = d phi_i / d xi_d at quadrature point q):
+//     u_q = B_q^T x_e            (dim values per point)
+//     v_q = w_q kappa_e G_e u_q  (G_e = |det F| F^-1 F^-T, symmetric)
+//     y_e = sum_q B_q v_q
+// followed by the scatter y[dofs_e] += y_e.
+//
+// MFMA path (FP64 v_mfma_f64_16x16x4f64, FP32 v_mfma_f32_16x16x4f32): one
+// wavefront owns a tile of 16 elements.  Per block of 16 quadrature points it
+// forms U^T (16 q x 16 el, one tile per direction d) = B^T X with the B block
+// as the A operand (read from LDS) and the gathered element vectors as the B
+// operand (held in registers for the whole tile), applies the 2x2/3x3 metric
+// to the accumulators in registers, and feeds them straight back as the B
+// operand of the second product Y^T (dof x el) += B V^T.  The output layout of
+// a 16x16 MFMA (lane l holds rows 4*(l/16)..+3 of column l%16) is used as the
+// K index of the second product with the matching permutation of the A
+// operand, so U/V never leave registers.  B of one class (nq padded to 16,
+// nd padded to 4, one pad word per point against LDS bank conflicts) is loaded
+// into LDS once per workgroup; every wavefront then walks element tiles.
+//
+// Scatter: GATHER (default) writes each element's result row to a
+// contribution buffer; a second kernel sums, per dof, the contributions in a
+// fixed order (deterministic, no atomics).  ATOMIC adds the element results
+// with hardware floating-point atomics, as the reference kernel does.
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "common.hpp"
+#include "matfree.hpp"
+
+namespace ngsamd {
+
+typedef double double4v __attribute__((ext_vector_type(4)));
+typedef float float4v __attribute__((ext_vector_type(4)));
+
+constexpr int kMfWaves = 8;
+constexpr int kMfBlock = 64 * kMfWaves;
+
+__device__ __forceinline__ double4v Mfma(double a, double b, double4v c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ float4v Mfma(float a, float b, float4v c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+template <class T> struct Acc4;
+template <> struct Acc4<double> { typedef double4v type; };
+template <> struct Acc4<float> { typedef float4v type; };
+
+template <class V>
+__device__ __forceinline__ void AtomicAddV(V* p, V v) {
+  atomicAdd(p, v);
+}
+
+// Metric G (symmetric, packed) times vector: 2D (xx,yy,xy), 3D (xx,yy,zz,xy,xz,yz).
+template <int DIM, class T>
+__device__ __forceinline__ void ApplyMetric(const T* g, const T* u, T* v) {
+  if constexpr (DIM == 2) {
+    v[0] = g[0] * u[0] + g[2] * u[1];
+    v[1] = g[2] * u[0] + g[1] * u[1];
+  } else {
+    v[0] = g[0] * u[0] + g[3] * u[1] + g[4] * u[2];
+    v[1] = g[3] * u[0] + g[1] * u[1] + g[5] * u[2];
+    v[2] = g[4] * u[0] + g[5] * u[1] + g[2] * u[2];
+  }
+}
+
+// MODE 0: contributions (row-major [nel][nd], table row numbering)
+// MODE 1: atomic add s*y_e into y
+template <class T, int DIM, int NDP, int MODE, class V>
+__global__ __launch_bounds__(kMfBlock) void MfMfmaKernel(
+    const T* __restrict__ bclass, const double* __restrict__ weights, int nqb, const int* __restrict__ dofs,
+    const double* __restrict__ metric, const double* __restrict__ kappa, const int* __restrict__ elnr,
+    int64_t begin, int64_t end, int nd, const V* __restrict__ x, V* __restrict__ out, double s,
+    const int* __restrict__ stop) {
+  if (stop && *stop) return;
+  typedef typename Acc4<T>::type acc_t;
+  constexpr int QS = DIM * NDP + 1;  // LDS words per quadrature point
+  constexpr int KS = NDP / 4;        // k-steps of the first product
+  constexpr int MT = (NDP + 15) / 16;
+  constexpr int NC = DIM * (DIM + 1) / 2;
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  T* lb = reinterpret_cast<T*>(lds_raw);
+  const int nlds = nqb * 16 * QS;
+  for (int i = threadIdx.x; i < nlds; i += kMfBlock) lb[i] = bclass[i];
+  __syncthreads();
+
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int col = lane & 15, grp = lane >> 4;
+  const int64_t ntiles = (end - begin + 15) / 16;
+  for (int64_t tile = (int64_t)blockIdx.x * kMfWaves + wave; tile < ntiles; tile += (int64_t)gridDim.x * kMfWaves) {
+    const int64_t e = begin + tile * 16 + col;
+    const bool valid = e < end;
+    // element vector, K layout of the first product: lane holds dof 4s+grp
+    T xk[KS];
+#pragma unroll
+    for (int st = 0; st < KS; st++) {
+      const int dof = 4 * st + grp;
+      xk[st] = (valid && dof < nd) ? (T)x[dofs[e * nd + dof]] : (T)0;
+    }
+    T g[NC];
+    {
+      const double kap = valid ? kappa[elnr[e]] : 0.0;
+#pragma unroll
+      for (int c = 0; c < NC; c++) g[c] = valid ? (T)(kap * metric[e * NC + c]) : (T)0;
+    }
+    acc_t yacc[MT];
+#pragma unroll
+    for (int m = 0; m < MT; m++) yacc[m] = acc_t{0, 0, 0, 0};
+
+    for (int t = 0; t < nqb; t++) {
+      acc_t u[DIM];
+#pragma unroll
+      for (int d = 0; d < DIM; d++) {
+        u[d] = acc_t{0, 0, 0, 0};
+        const T* arow = lb + (16 * t + col) * QS + d * NDP + grp;
+#pragma unroll
+        for (int st = 0; st < KS; st++) u[d] = Mfma(arow[4 * st], xk[st], u[d]);
+      }
+      // metric at the 4 points this lane holds: q = 16t + 4grp + r
+      T v[DIM][4];
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const T w = (T)weights[16 * t + 4 * grp + r];
+        T uu[DIM], vv[DIM];
+#pragma unroll
+        for (int d = 0; d < DIM; d++) uu[d] = u[d][r];
+        ApplyMetric<DIM, T>(g, uu, vv);
+#pragma unroll
+        for (int d = 0; d < DIM; d++) v[d][r] = w * vv[d];
+      }
+#pragma unroll
+      for (int m = 0; m < MT; m++) {
+        const int dof = 16 * m + col;
+        const bool inrange = dof < NDP;
+#pragma unroll
+        for (int d = 0; d < DIM; d++)
+#pragma unroll
+          for (int r = 0; r < 4; r++) {
+            const T a = inrange ? lb[(16 * t + 4 * grp + r) * QS + d * NDP + dof] : (T)0;
+            yacc[m] = Mfma(a, v[d][r], yacc[m]);
+          }
+      }
+    }
+    if (valid) {
+#pragma unroll
+      for (int m = 0; m < MT; m++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const int dof = 16 * m + 4 * grp + r;
+          if (dof < nd) {
+            const V val = (V)yacc[m][r];
+            if (MODE == 0) out[e * nd + dof] = val;
+            else AtomicAddV(&out[dofs[e * nd + dof]], (V)(s * (double)val));
+          }
+        }
+    }
+  }
+}
+
+// Scalar fallback for shapes the MFMA path does not cover: one thread per
+// element, FP64, element vectors staged in LDS columns.
+constexpr int kScalarBlock = 64;
+constexpr int kScalarMaxNd = 64;
+
+template <int DIM, int MODE, class V>
+__global__ __launch_bounds__(kScalarBlock) void MfScalarKernel(
+    const double* __restrict__ bref /*[nq][nd][dim]*/, const double* __restrict__ weights, int nq,
+    const int* __restrict__ dofs, const double* __restrict__ metric, const double* __restrict__ kappa,
+    const int* __restrict__ elnr, int64_t begin, int64_t end, int nd, const V* __restrict__ x,
+    V* __restrict__ out, double s, const int* __restrict__ stop) {
+  if (stop && *stop) return;
+  constexpr int NC = DIM * (DIM + 1) / 2;
+  __shared__ double xe[kScalarMaxNd][kScalarBlock];
+  __shared__ double ye[kScalarMaxNd][kScalarBlock];
+  const int tid = threadIdx.x;
+  for (int64_t base = begin + (int64_t)blockIdx.x * kScalarBlock; base < end; base += (int64_t)gridDim.x * kScalarBlock) {
+    const int64_t e = base + tid;
+    if (e < end) {
+      for (int i = 0; i < nd; i++) { xe[i][tid] = (double)x[dofs[e * nd + i]]; ye[i][tid] = 0.0; }
+      double g[NC];
+      const double kap = kappa[elnr[e]];
+      for (int c = 0; c < NC; c++) g[c] = kap * metric[e * NC + c];
+      for (int q = 0; q < nq; q++) {
+        double u[DIM] = {}, v[DIM];
+        const double* Bq = bref + (size_t)q * nd * DIM;
+        for (int i = 0; i < nd; i++)
+          for (int d = 0; d < DIM; d++) u[d] += Bq[i * DIM + d] * xe[i][tid];
+        ApplyMetric<DIM, double>(g, u, v);
+        for (int d = 0; d < DIM; d++) v[d] *= weights[q];
+        for (int i = 0; i < nd; i++) {
+          double acc = 0.0;
+          for (int d = 0; d < DIM; d++) acc += Bq[i * DIM + d] * v[d];
+          ye[i][tid] += acc;
+        }
+      }
+      for (int i = 0; i < nd; i++) {
+        if (MODE == 0) out[e * nd + i] = (V)ye[i][tid];
+        else AtomicAddV(&out[dofs[e * nd + i]], (V)(s * ye[i][tid]));
+      }
+    }
+  }
+}
+
+// y[r] = (ACC ? y[r] : 0) + s * sum of the contributions of dof r, in slot order.
+template <class C, class V>
+__global__ void MfGatherKernel(const int* __restrict__ rowptr, const int* __restrict__ slots,
+                               const C* __restrict__ contrib, V* __restrict__ y, int64_t n, double s, int accumulate,
+                               const int* __restrict__ stop) {
+  if (stop && *stop) return;
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+    double acc = 0.0;
+    for (int k = rowptr[r]; k < rowptr[r + 1]; k++) acc += (double)contrib[slots[k]];
+    const double v = s * acc;
+    y[r] = accumulate ? (V)((double)y[r] + v) : (V)v;
+  }
+}
+
+template <class V>
+__global__ void ScaleFillKernel(V* __restrict__ y, int64_t n, const int* __restrict__ stop) {
+  if (stop && *stop) return;
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x)
+    y[r] = (V)0;
+}
+
+// Diagonal: per element, diag_i = kappa sum_q w_q B_qi^T G B_qi, into contributions.
+template <int DIM>
+__global__ void MfDiagKernel(const double* __restrict__ bref, const double* __restrict__ weights, int nq,
+                             const double* __restrict__ metric, const double* __restrict__ kappa,
+                             const int* __restrict__ elnr, int64_t begin, int64_t end, int nd,
+                             double* __restrict__ contrib) {
+  constexpr int NC = DIM * (DIM + 1) / 2;
+  for (int64_t e = begin + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < end;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    double g[NC];
+    const double kap = kappa[elnr[e]];
+    for (int c = 0; c < NC; c++) g[c] = kap * metric[e * NC + c];
+    for (int i = 0; i < nd; i++) {
+      double acc = 0.0;
+      for (int q = 0; q < nq; q++) {
+        const double* b = bref + ((size_t)q * nd + i) * DIM;
+        double v[DIM];
+        ApplyMetric<DIM, double>(g, b, v);
+        double dd = 0.0;
+        for (int d = 0; d < DIM; d++) dd += b[d] * v[d];
+        acc += weights[q] * dd;
+      }
+      contrib[e * nd + i] = acc;
+    }
+  }
+}
+
+// g_e = -int_e grad(lam).grad(u) = -sum_q w_q (B_q^T lam_e)^T G_e (B_q^T u_e)
+template <int DIM>
+__global__ void MfElementGradKernel(const double* __restrict__ bref, const double* __restrict__ weights, int nq,
+                                    const int* __restrict__ dofs, const double* __restrict__ metric,
+                                    const int* __restrict__ elnr, int64_t begin, int64_t end, int nd,
+                                    const double* __restrict__ lam, const double* __restrict__ u,
+                                    double* __restrict__ grad) {
+  constexpr int NC = DIM * (DIM + 1) / 2;
+  for (int64_t e = begin + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < end;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const double* g = metric + e * NC;
+    double acc = 0.0;
+    for (int q = 0; q < nq; q++) {
+      double a[DIM] = {}, b[DIM] = {}, gb[DIM];
+      for (int i = 0; i < nd; i++) {
+        const int dof = dofs[e * nd + i];
+        const double li = lam[dof], ui = u[dof];
+        const double* B = bref + ((size_t)q * nd + i) * DIM;
+        for (int d = 0; d < DIM; d++) { a[d] += B[d] * li; b[d] += B[d] * ui; }
+      }
+      ApplyMetric<DIM, double>(g, b, gb);
+      double dd = 0.0;
+      for (int d = 0; d < DIM; d++) dd += a[d] * gb[d];
+      acc += weights[q] * dd;
+    }
+    grad[elnr[e]] = -acc;
+  }
+}
+
+// ----------------------------------------------------------------- host side
+
+static int DeviceMaxLds() {
+  static int v = -1;
+  if (v < 0) {
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    int val = 0;
+    if (hipDeviceGetAttribute(&val, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) val = 65536;
+    v = val;
+  }
+  return v;
+}
+
+static int DeviceCUs() {
+  static int v = -1;
+  if (v < 0) {
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    int val = 0;
+    if (hipDeviceGetAttribute(&val, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) val = 256;
+    v = val;
+  }
+  return v;
+}
+
+static bool SupportedNdp(int ndp) {
+  return ndp == 4 || ndp == 8 || ndp == 12 || ndp == 16 || ndp == 20 || ndp == 24 || ndp == 36;
+}
+
+static uint16_t ToBf16(double v) {
+  float f = (float)v;
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  uint32_t r = u + 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even
+  return (uint16_t)(r >> 16);
+}
+
+MatFreeOp::MatFreeOp(int dim_, int nd_, int nq_, int64_t ndof, int nclass, const int64_t* off, const double* bmat,
+                     const double* w, const int* hdofs, const double* hmetric, const double* hkappa,
+                     const int* helnr, int prec, int scat)
+    : ngsamd_op_s(OpKind::MatrixFree) {
+  NGSAMD_REQUIRE(dim_ == 2 || dim_ == 3, "ngsamd_mf_create: dim must be 2 or 3");
+  NGSAMD_REQUIRE(nd_ >= 1 && nd_ <= kScalarMaxNd, "ngsamd_mf_create: nd out of range (1..64)");
+  NGSAMD_REQUIRE(nq_ >= 1 && nclass >= 1, "ngsamd_mf_create: empty rule or no classes");
+  NGSAMD_REQUIRE(prec == NGSAMD_PREC_F64 || prec == NGSAMD_PREC_F32 || prec == NGSAMD_PREC_BF16,
+                 "ngsamd_mf_create: unknown precision");
+  NGSAMD_REQUIRE(scat == NGSAMD_SCATTER_GATHER || scat == NGSAMD_SCATTER_ATOMIC, "ngsamd_mf_create: unknown scatter");
+  NGSAMD_REQUIRE(ndof >= 0 && ndof < (int64_t(1) << 31), "ngsamd_mf_create: ndof out of range");
+  dim = dim_;
+  nd = nd_;
+  nq = nq_;
+  ncomp = dim * (dim + 1) / 2;
+  precision = prec;
+  scatter = scat;
+  height = width = ndof;
+  nel = off[nclass];
+  NGSAMD_REQUIRE(off[0] == 0 && nel >= 0 && nel * nd < (int64_t(1) << 31), "ngsamd_mf_create: bad class offsets");
+  for (int c = 0; c < nclass; c++) {
+    NGSAMD_REQUIRE(off[c + 1] >= off[c], "ngsamd_mf_create: class offsets not monotone");
+    classes.push_back({off[c], off[c + 1]});
+  }
+  for (int64_t i = 0; i < nel * nd; i++)
+    NGSAMD_REQUIRE(hdofs[i] >= 0 && hdofs[i] < ndof, "ngsamd_mf_create: dof number out of range");
+  std::vector<int> el(nel);
+  ne_global = 0;
+  for (int64_t e = 0; e < nel; e++) {
+    el[e] = helnr ? helnr[e] : (int)e;
+    NGSAMD_REQUIRE(el[e] >= 0, "ngsamd_mf_create: negative element number");
+    ne_global = std::max<int64_t>(ne_global, el[e] + 1);
+  }
+
+  ndp = (nd + 3) / 4 * 4;
+  nqp = (nq + 15) / 16 * 16;
+  const int qs = dim * ndp + 1;
+  bstride = (int64_t)nqp * qs;
+  // MFMA path if the padded shape is instantiated and B fits in LDS
+  const size_t esize = prec == NGSAMD_PREC_F64 ? 8 : 4;
+  use_mfma = SupportedNdp(ndp) && (size_t)bstride * esize <= (size_t)DeviceMaxLds() && prec != NGSAMD_PREC_BF16;
+
+  // FP64 reference gradients [class][nq][nd][dim] for the scalar kernels
+  const size_t bsz = (size_t)nq * nd * dim;
+  bref.alloc(bsz * nclass);
+  bref.upload(bmat, bsz * nclass);
+  std::vector<double> wp(nqp, 0.0);
+  for (int q = 0; q < nq; q++) wp[q] = w[q];
+  weights.alloc(nqp);
+  weights.upload(wp.data(), nqp);
+  if (use_mfma) {
+    std::vector<double> lay((size_t)bstride * nclass, 0.0);
+    for (int c = 0; c < nclass; c++)
+      for (int q = 0; q < nq; q++)
+        for (int i = 0; i < nd; i++)
+          for (int d = 0; d < dim; d++)
+            lay[(size_t)c * bstride + (size_t)q * qs + d * ndp + i] = bmat[c * bsz + ((size_t)q * nd + i) * dim + d];
+    if (prec == NGSAMD_PREC_F64) {
+      bmat64.alloc(lay.size());
+      bmat64.upload(lay.data(), lay.size());
+    } else {
+      std::vector<float> l32(lay.begin(), lay.end());
+      bmat32.alloc(l32.size());
+      bmat32.upload(l32.data(), l32.size());
+    }
+  }
+  (void)ToBf16;
+  dofs.alloc((size_t)nel * nd);
+  dofs.upload(hdofs, (size_t)nel * nd);
+  metric.alloc((size_t)nel * ncomp);
+  metric.upload(hmetric, (size_t)nel * ncomp);
+  elnr.alloc(nel);
+  elnr.upload(el.data(), nel);
+  std::vector<double> kap(ne_global, 1.0);
+  if (hkappa)
+    for (int64_t e = 0; e < nel; e++) kap[el[e]] = hkappa[e];
+  kappa.alloc(ne_global);
+  kappa.upload(kap.data(), ne_global);
+
+  // dof -> contribution slots, ascending slot order
+  std::vector<int> cnt(ndof + 1, 0);
+  for (int64_t i = 0; i < nel * nd; i++) cnt[hdofs[i] + 1]++;
+  for (int64_t r = 0; r < ndof; r++) cnt[r + 1] += cnt[r];
+  std::vector<int> sl(nel * nd), pos(cnt.begin(), cnt.end() - 1);
+  for (int64_t i = 0; i < nel * nd; i++) sl[pos[hdofs[i]]++] = (int)i;
+  g_rowptr.alloc(ndof + 1);
+  g_rowptr.upload(cnt.data(), ndof + 1);
+  g_slots.alloc(sl.size());
+  g_slots.upload(sl.data(), sl.size());
+  NGSAMD_HIP(hipStreamSynchronize(Stream()));
+}
+
+template <class T, int DIM, int NDP, int MODE, class V>
+static void LaunchMfma(const MatFreeOp& A, const T* b, const MfClass& c, const V* x, V* out, double s,
+                       const int* stop, hipStream_t st) {
+  const int nqb = A.nqp / 16;
+  const size_t lds = (size_t)nqb * 16 * (DIM * NDP + 1) * sizeof(T);
+  auto kern = MfMfmaKernel<T, DIM, NDP, MODE, V>;
+  static bool attr_set = false;
+  if (!attr_set) {
+    (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    attr_set = true;
+  } else {
+    (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  }
+  const int64_t ntiles = (c.end - c.begin + 15) / 16;
+  int64_t grid = (ntiles + kMfWaves - 1) / kMfWaves;
+  const int64_t maxg = (int64_t)DeviceCUs() * (lds > 81920 ? 1 : 2);
+  if (grid > maxg) grid = maxg;
+  if (grid < 1) grid = 1;
+  kern<<<(int)grid, kMfBlock, lds, st>>>(b, A.weights.p, nqb, A.dofs.p, A.metric.p, A.Kappa(), A.elnr.p, c.begin,
+                                         c.end, A.nd, x, out, s, stop);
+  NGSAMD_CHECK_LAUNCH();
+}
+
+template <class T, int DIM, int MODE, class V>
+static void DispatchNdp(const MatFreeOp& A, const T* b, const MfClass& c, const V* x, V* out, double s,
+                        const int* stop, hipStream_t st) {
+  switch (A.ndp) {
+    case 4: LaunchMfma<T, DIM, 4, MODE, V>(A, b, c, x, out, s, stop, st); break;
+    case 8: LaunchMfma<T, DIM, 8, MODE, V>(A, b, c, x, out, s, stop, st); break;
+    case 12: LaunchMfma<T, DIM, 12, MODE, V>(A, b, c, x, out, s, stop, st); break;
+    case 16: LaunchMfma<T, DIM, 16, MODE, V>(A, b, c, x, out, s, stop, st); break;
+    case 20: LaunchMfma<T, DIM, 20, MODE, V>(A, b, c, x, out, s, stop, st); break;
+    case 24: LaunchMfma<T, DIM, 24, MODE, V>(A, b, c, x, out, s, stop, st); break;
+    case 36: LaunchMfma<T, DIM, 36, MODE, V>(A, b, c, x, out, s, stop, st); break;
+    default: throw Error(NGSAMD_E_STATE, "matrix-free: unsupported padded dof count");
+  }
+}
+
+template <int MODE, class V>
+static void LaunchClass(const MatFreeOp& A, int ci, const V* x, V* out, double s, const int* stop, hipStream_t st) {
+  const MfClass& c = A.classes[ci];
+  if (c.end <= c.begin) return;
+  if (A.use_mfma) {
+    if (A.precision == NGSAMD_PREC_F64) {
+      const double* b = A.bmat64.p + (size_t)ci * A.bstride;
+      if (A.dim == 2) DispatchNdp<double, 2, MODE, V>(A, b, c, x, out, s, stop, st);
+      else DispatchNdp<double, 3, MODE, V>(A, b, c, x, out, s, stop, st);
+    } else {
+      const float* b = A.bmat32.p + (size_t)ci * A.bstride;
+      if (A.dim == 2) DispatchNdp<float, 2, MODE, V>(A, b, c, x, out, s, stop, st);
+      else DispatchNdp<float, 3, MODE, V>(A, b, c, x, out, s, stop, st);
+    }
+  } else {
+    const double* b = A.bref.p + (size_t)ci * A.nq * A.nd * A.dim;
+    const int64_t n = c.end - c.begin;
+    const int grid = GridFor(n, kScalarBlock, 4096);
+    if (A.dim == 2)
+      MfScalarKernel<2, MODE, V><<<grid, kScalarBlock, 0, st>>>(b, A.weights.p, A.nq, A.dofs.p, A.metric.p, A.Kappa(),
+                                                               A.elnr.p, c.begin, c.end, A.nd, x, out, s, stop);
+    else
+      MfScalarKernel<3, MODE, V><<<grid, kScalarBlock, 0, st>>>(b, A.weights.p, A.nq, A.dofs.p, A.metric.p, A.Kappa(),
+                                                               A.elnr.p, c.begin, c.end, A.nd, x, out, s, stop);
+    NGSAMD_CHECK_LAUNCH();
+  }
+}
+
+template <class V>
+void MatFreeOp::Apply(double s, const V* x, V* y, bool accumulate, const int* stop, hipStream_t st) {
+  if (height == 0) return;
+  if (scatter == NGSAMD_SCATTER_ATOMIC) {
+    if (!accumulate) {
+      ScaleFillKernel<V><<<GridFor(height, 256, 8192), 256, 0, st>>>(y, height, stop);
+      NGSAMD_CHECK_LAUNCH();
+    }
+    for (size_t c = 0; c < classes.size(); c++) LaunchClass<1, V>(*this, (int)c, x, y, s, stop, st);
+    return;
+  }
+  V* contrib;
+  if constexpr (sizeof(V) == 8) {
+    if (contrib64.n < (size_t)(nel * nd)) contrib64.alloc(nel * nd);
+    contrib = reinterpret_cast<V*>(contrib64.p);
+  } else {
+    if (contrib32.n < (size_t)(nel * nd)) contrib32.alloc(nel * nd);
+    contrib = reinterpret_cast<V*>(contrib32.p);
+  }
+  for (size_t c = 0; c < classes.size(); c++) LaunchClass<0, V>(*this, (int)c, x, contrib, 1.0, stop, st);
+  MfGatherKernel<V, V><<<GridFor(height, 256, 16384), 256, 0, st>>>(g_rowptr.p, g_slots.p, contrib, y, height, s,
+                                                                    accumulate ? 1 : 0, stop);
+  NGSAMD_CHECK_LAUNCH();
+}
+
+void MatFreeOp::MultAdd(double s, const double* x, double* y, hipStream_t st) { Apply<double>(s, x, y, true, nullptr, st); }
+void MatFreeOp::Mult(const double* x, double* y, hipStream_t st) { Apply<double>(1.0, x, y, false, nullptr, st); }
+void MatFreeOp::MultStop(const double* x, double* y, const int* stop, hipStream_t st) {
+  Apply<double>(1.0, x, y, false, stop, st);
+}
+void MatFreeOp::MultF32(const float* x, float* y, const int* stop, hipStream_t st) {
+  Apply<float>(1.0, x, y, false, stop, st);
+}
+
+void MatFreeOp::Diagonal(double* d, hipStream_t st) {
+  if (height == 0) return;
+  if (contrib64.n < (size_t)(nel * nd)) contrib64.alloc(nel * nd);
+  for (size_t ci = 0; ci < classes.size(); ci++) {
+    const MfClass& c = classes[ci];
+    if (c.end <= c.begin) continue;
+    const double* b = bref.p + ci * (size_t)nq * nd * dim;
+    const int grid = GridFor(c.end - c.begin, 128, 8192);
+    if (dim == 2)
+      MfDiagKernel<2><<<grid, 128, 0, st>>>(b, weights.p, nq, metric.p, Kappa(), elnr.p, c.begin, c.end, nd, contrib64.p);
+    else
+      MfDiagKernel<3><<<grid, 128, 0, st>>>(b, weights.p, nq, metric.p, Kappa(), elnr.p, c.begin, c.end, nd, contrib64.p);
+    NGSAMD_CHECK_LAUNCH();
+  }
+  MfGatherKernel<double, double><<<GridFor(height, 256, 16384), 256, 0, st>>>(g_rowptr.p, g_slots.p, contrib64.p, d,
+                                                                              height, 1.0, 0, nullptr);
+  NGSAMD_CHECK_LAUNCH();
+}
+
+void MatFreeOp::ElementGrad(const double* lam, const double* u, double* grad, hipStream_t st) {
+  for (size_t ci = 0; ci < classes.size(); ci++) {
+    const MfClass& c = classes[ci];
+    if (c.end <= c.begin) continue;
+    const double* b = bref.p + ci * (size_t)nq * nd * dim;
+    const int grid = GridFor(c.end - c.begin, 128, 8192);
+    if (dim == 2)
+      MfElementGradKernel<2><<<grid, 128, 0, st>>>(b, weights.p, nq, dofs.p, metric.p, elnr.p, c.begin, c.end, nd,
+                                                   lam, u, grad);
+    else
+      MfElementGradKernel<3><<<grid, 128, 0, st>>>(b, weights.p, nq, dofs.p, metric.p, elnr.p, c.begin, c.end, nd,
+                                                   lam, u, grad);
+    NGSAMD_CHECK_LAUNCH();
+  }
+}
+
+}  // namespace ngsamd
+
+using namespace ngsamd;
+
+extern "C" int ngsamd_mf_create(ngsamd_op_t* op, int dim, int nd, int nq, int64_t ndof, int nclass,
+                                const int64_t* off, const double* bmat, const double* w, const int* dofs,
+                                const double* metric, const double* kappa, const int* elnr, int precision,
+                                int scatter) {
+  return Guard([&] {
+    RequireInit();
+    NGSAMD_REQUIRE(op && off && bmat && w && (off[nclass] == 0 || (dofs && metric)), "ngsamd_mf_create: null argument");
+    *op = nullptr;
+    *op = new MatFreeOp(dim, nd, nq, ndof, nclass, off, bmat, w, dofs, metric, kappa, elnr, precision, scatter);
+  });
+}
+
+extern "C" int ngsamd_mf_set_kappa(ngsamd_op_t op, const double* d_kappa) {
+  return Guard([&] {
+    NGSAMD_REQUIRE(op && op->kind == OpKind::MatrixFree, "ngsamd_mf_set_kappa: not a matrix-free operator");
+    static_cast<MatFreeOp*>(op)->kappa_ext = d_kappa;
+  });
+}
+
+extern "C" int ngsamd_mf_mult_f32(ngsamd_op_t op, const float* x, float* y) {
+  return Guard([&] {
+    RequireInit();
+    NGSAMD_REQUIRE(op && op->kind == OpKind::MatrixFree, "ngsamd_mf_mult_f32: not a matrix-free operator");
+    static_cast<MatFreeOp*>(op)->MultF32(x, y, nullptr, Stream());
+  });
+}
+
+extern "C" int ngsamd_mf_diagonal(ngsamd_op_t op, double* d) {
+  return Guard([&] {
+    RequireInit();
+    NGSAMD_REQUIRE(op && op->kind == OpKind::MatrixFree, "ngsamd_mf_diagonal: not a matrix-free operator");
+    static_cast<MatFreeOp*>(op)->Diagonal(d, Stream());
+  });
+}
+
+extern "C" int ngsamd_mf_element_grad(ngsamd_op_t op, const double* lam, const double* u, double* grad) {
+  return Guard([&] {
+    RequireInit();
+    NGSAMD_REQUIRE(op && op->kind == OpKind::MatrixFree, "ngsamd_mf_element_grad: not a matrix-free operator");
+    static_cast<MatFreeOp*>(op)->ElementGrad(lam, u, grad, Stream());
+  });
+}

@@ -1,0 +1,58 @@
+// Fused element-batched matrix-free H1 Laplace operator (device side).
+#pragma once
+#include <vector>
+
+#include "common.hpp"
+
+namespace ngsamd {
+
+struct MfClass {
+  int64_t begin = 0, end = 0;  // element range in the (class-sorted) tables
+};
+
+struct MatFreeOp : ngsamd_op_s {
+  int dim = 3, nd = 0, nq = 0, ncomp = 6;
+  int ndp = 0;                // nd padded to a multiple of 4 (MFMA K)
+  int nqp = 0;                // nq padded to a multiple of 16
+  int precision = NGSAMD_PREC_F64;
+  int scatter = NGSAMD_SCATTER_GATHER;
+  bool use_mfma = false;
+  int64_t nel = 0;
+  std::vector<MfClass> classes;
+  // per class reference gradients, MFMA layout [nqp][dim][ndp] (+1 pad per q),
+  // in the contraction type (double / float / bf16 bits)
+  DevBuf<double> bmat64;
+  DevBuf<float> bmat32;
+  DevBuf<uint16_t> bmat16;
+  int64_t bstride = 0;        // elements per class block in bmat*
+  DevBuf<double> weights;     // [nqp], 0 beyond nq
+  DevBuf<double> bref;        // [nclass][nq][nd][dim] FP64 (scalar path, diagonal, gradient)
+  DevBuf<int> dofs;           // [nel][nd]
+  DevBuf<double> metric;      // [nel][ncomp]
+  DevBuf<double> kappa;       // [ne_global] (owned copy)
+  const double* kappa_ext = nullptr;  // set by ngsamd_mf_set_kappa
+  DevBuf<int> elnr;           // [nel] global element numbers
+  int64_t ne_global = 0;
+  // gather scatter: dof -> contribution slots (element row * nd + local)
+  DevBuf<int> g_rowptr, g_slots;
+  DevBuf<double> contrib64;
+  DevBuf<float> contrib32;
+
+  MatFreeOp(int dim, int nd, int nq, int64_t ndof, int nclass, const int64_t* class_offsets, const double* bmat,
+            const double* w, const int* dofs, const double* metric, const double* kappa, const int* elnr,
+            int precision, int scatter);
+  const double* Kappa() const { return kappa_ext ? kappa_ext : kappa.p; }
+  void MultAdd(double s, const double* x, double* y, hipStream_t st) override;
+  void Mult(const double* x, double* y, hipStream_t st) override;
+  // y = A x, all kernels return immediately when *d_stop != 0
+  void MultStop(const double* x, double* y, const int* d_stop, hipStream_t st);
+  void MultF32(const float* x, float* y, const int* d_stop, hipStream_t st);
+  void Diagonal(double* d, hipStream_t st);
+  void ElementGrad(const double* lam, const double* u, double* grad, hipStream_t st);
+
+ private:
+  template <class V>
+  void Apply(double s, const V* x, V* y, bool accumulate, const int* d_stop, hipStream_t st);
+};
+
+}  // namespace ngsamd

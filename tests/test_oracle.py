@@ -1,0 +1,259 @@
+"""Pins the CPU oracle against the reference's known answers and against what
+the mathematics fixes (closed forms, exactness, orthogonality, invariants).
+CPU only."""
+import gzip
+import math
+import os
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spl
+from numpy.polynomial import legendre as L
+from scipy.special import eval_jacobi
+
+from oracle import fe_oracle as O
+import oracle
+from conftest import GOLDEN
+
+
+def _vol(name):
+    with gzip.open(os.path.join(GOLDEN, name), "rt") as f:
+        return O.read_vol(f.read())
+
+
+# ---------------------------------------------------------------- known answers
+@pytest.mark.parametrize("name,ref", [("square.vol.gz", 0.004820065534967336),
+                                      ("cube.vol.gz", 0.0002480226944690391)])
+def test_mass_known_answer(name, ref):
+    """tests/pytest/test_matrix.py:58-93: order-1 mass entry a.mat[1,1]."""
+    m = _vol(name)
+    pr = O.Problem(m, 1, intorder=2)
+    rp, c, v = pr.assemble(stiffness=False, mass=True)
+    k = np.searchsorted(c[rp[1]:rp[2]], 1)
+    assert abs(v[rp[1] + k] - ref) < 1e-8
+    # closed form: sum over elements at vertex 1 of |T|/6 (2D) or |T|/10 (3D)
+    els = m["els"]
+    vol = np.abs(pr.detF) / (2 if m["dim"] == 2 else 6)
+    cf = vol[(els == 1).any(axis=1)].sum() / (6 if m["dim"] == 2 else 10)
+    assert abs(v[rp[1] + k] - cf) < 1e-15
+
+
+def test_element_volumes():
+    """tests/catch/meshaccess.cpp:16-21 style: element volumes sum to the domain."""
+    for m, dom in [(O.structured_2d(5), 1.0), (O.structured_3d(3), 1.0), (_vol("square.vol.gz"), 1.0),
+                   (_vol("cube.vol.gz"), 1.0)]:
+        pr = O.Problem(m, 1)
+        fac = 2.0 if m["dim"] == 2 else 6.0
+        assert abs(np.abs(pr.detF).sum() / fac - dom) < 1e-12
+
+
+# ------------------------------------------------------------------ quadrature
+@pytest.mark.parametrize("order", range(0, 11))
+def test_trig_rule_exact(order):
+    pts, w = O.trig_rule(order)
+    for a in range(order + 1):
+        for b in range(order + 1 - a):
+            exact = math.factorial(a) * math.factorial(b) / math.factorial(a + b + 2)
+            tol = 1e-14 if order > 6 else 5e-14  # tabulated rules carry 15 digits
+            assert abs(np.sum(w * pts[:, 0] ** a * pts[:, 1] ** b) - exact) < tol
+
+
+@pytest.mark.parametrize("order", range(0, 11))
+def test_tet_rule_exact(order):
+    pts, w = O.tet_rule(order)
+    assert len(w) == (order // 2 + 1) ** 3
+    for a in range(order + 1):
+        for b in range(order + 1 - a):
+            for c in range(order + 1 - a - b):
+                exact = math.factorial(a) * math.factorial(b) * math.factorial(c) / math.factorial(a + b + c + 3)
+                got = np.sum(w * pts[:, 0] ** a * pts[:, 1] ** b * pts[:, 2] ** c)
+                assert abs(got - exact) < 1e-15
+
+
+def test_tet_order8_is_125_points():
+    pts, w = O.tet_rule(8)
+    assert len(w) == 125 and abs(w.sum() - 1 / 6) < 1e-15
+
+
+# ------------------------------------------------------------- 1D polynomials
+def test_intleg_nobubble_closed_form():
+    """IntLegNoBubble_i(x) = L_{i+2}(x)/(1-x^2) = -P'_{i+1}(x)/((i+2)(i+1))."""
+    x = np.linspace(-0.95, 0.95, 11)
+    vals = O.intleg_nobubble_scaled(8, x, np.ones_like(x), 1.0)
+    for i, v in enumerate(vals):
+        c = np.zeros(i + 3)
+        c[i + 1] = 1
+        ref = -L.legval(x, L.legder(c)) / ((i + 2) * (i + 1))
+        assert np.abs(v - ref).max() < 1e-14
+
+
+def test_jacobi11_closed_form():
+    x = np.linspace(-0.95, 0.95, 11)
+    for i, v in enumerate(O.jacobi11_scaled(8, x, np.ones_like(x), 1.0)):
+        assert np.abs(v - eval_jacobi(i, 1, 1, x)).max() < 1e-12
+
+
+@pytest.mark.parametrize("alpha", [4, 6, 8, 10])
+def test_integrated_jacobi_closed_form(alpha):
+    """IntegratedJacobiAlpha_n(x) = (1/(1+x)) int_{-1}^x P_n^(alpha,0)(s) ds."""
+    x = np.linspace(-0.9, 0.9, 7)
+    gx, gw = np.polynomial.legendre.leggauss(30)
+    for n, v in enumerate(O.intjac_alpha(6, alpha, x, 1.0)):
+        ref = np.array([(xx + 1) / 2 * np.sum(gw * eval_jacobi(n, alpha, 0, (xx + 1) / 2 * gx + (xx - 1) / 2))
+                        / (1 + xx) for xx in x])
+        assert np.abs(v - ref).max() < 1e-11 * max(1.0, np.abs(ref).max())
+
+
+def test_scaled_polynomials_are_homogeneous():
+    rng = np.random.default_rng(1)
+    x, t = rng.uniform(-0.5, 0.5, 9), rng.uniform(0.5, 1.5, 9)
+    for k, (a, b) in enumerate(zip(O.intleg_nobubble_scaled(6, x, t, 1.0),
+                                   O.intleg_nobubble_scaled(6, x / t, np.ones_like(t), 1.0))):
+        assert np.abs(a - t ** k * b).max() < 1e-14
+    for k, (a, b) in enumerate(zip(O.intjac_alpha(5, 6, x, 1.0, y=t), O.intjac_alpha(5, 6, x / t, 1.0))):
+        assert np.abs(a - t ** k * b).max() < 1e-13
+
+
+# ------------------------------------------------------- 2D / 3D bubble bases
+def test_trig_interior_bubble_orthogonal():
+    """DubinerBasisOrthoBub is orthogonal w.r.t. the cubic bubble."""
+    pts, w = O.trig_rule(16)
+    lam = [pts[:, 0], pts[:, 1], 1 - pts[:, 0] - pts[:, 1]]
+    D = np.array(O.dubiner_orthobub(5, lam[0], lam[1], np.ones(len(w))))
+    G = (D * (w * lam[0] * lam[1] * lam[2])) @ D.T
+    assert np.abs(G - np.diag(np.diag(G))).max() < 1e-13 * np.abs(np.diag(G)).max()
+
+
+def test_tet_interior_bubble_orthogonal():
+    pts, w = O.tet_rule(14)
+    lam = [pts[:, 0], pts[:, 1], pts[:, 2], 1 - pts.sum(1)]
+    D = np.array(O.dubiner3d_orthobub(3, lam[0], lam[1], lam[2], np.ones(len(w))))
+    G = (D * (w * lam[0] * lam[1] * lam[2] * lam[3])) @ D.T
+    assert np.abs(G - np.diag(np.diag(G))).max() < 1e-13 * np.abs(np.diag(G)).max()
+
+
+def test_face_functions_scaling_identity():
+    rng = np.random.default_rng(2)
+    xx, yy, t = rng.uniform(0.1, 0.3, 5), rng.uniform(0.1, 0.3, 5), rng.uniform(0.7, 1.3, 5)
+    n = 4
+    sc = O.dubiner_orthobub_scaled(n, xx, yy, t, np.ones(5))
+    un = O.dubiner_orthobub(n, xx / t, yy / t, np.ones(5))
+    deg = [i + j for i in range(n + 1) for j in range(n - i + 1)]
+    for s, u, d in zip(sc, un, deg):
+        assert np.abs(s - t ** d * u).max() < 1e-14
+
+
+@pytest.mark.parametrize("dim,p", [(2, 1), (2, 3), (2, 5), (3, 1), (3, 2), (3, 4)])
+def test_vertex_partition_of_unity_and_edge_traces(dim, p):
+    pts, _ = O.rule(dim, 6)
+    vn = np.array([[3, 1, 2] if dim == 2 else [3, 0, 2, 1]])
+    val, grad = O.element_shapes(dim, p, vn, pts)
+    nv = dim + 1
+    assert np.abs(val[0, :, :nv].sum(axis=1) - 1).max() < 1e-15
+    assert np.abs(grad[0, :, :nv, :].sum(axis=1)).max() < 1e-15
+    # non-vertex functions vanish at the vertices
+    verts = np.vstack([np.eye(dim), np.zeros(dim)])
+    v2, _ = O.element_shapes(dim, p, vn, verts)
+    if p > 1:
+        assert np.abs(v2[0, :, nv:]).max() < 1e-15
+
+
+# ------------------------------------------------------------ whole chain
+def _poisson_l2_error(m, p, f_fn, exact_fn):
+    pr = O.Problem(m, p, intorder=2 * p)
+    rp, c, v = pr.assemble()
+    A = sp.csr_matrix((v, c, rp), shape=(pr.ndof, pr.ndof))
+    f = pr.linear_form(f_fn(pr.points()))
+    fr = np.where(pr.free)[0]
+    u = np.zeros(pr.ndof)
+    u[fr] = spl.spsolve(A[fr][:, fr].tocsc(), f[fr])
+    pr2 = O.Problem(m, p, intorder=2 * p + 4)
+    e = pr2.evaluate(u) - exact_fn(pr2.points())
+    return math.sqrt(np.sum(pr2.wts[None, :] * np.abs(pr2.detF)[:, None] * e * e))
+
+
+@pytest.mark.parametrize("mesh", ["structured", "square.vol.gz"])
+def test_p4_poisson_exact_2d(mesh):
+    """tests/pytest/test_solvers.py:58-77: 'p4 should be exact'."""
+    m = O.structured_2d(5) if mesh == "structured" else _vol(mesh)
+    err = _poisson_l2_error(m, 4, lambda X: 32 * (X[..., 1] * (1 - X[..., 1]) + X[..., 0] * (1 - X[..., 0])),
+                            lambda X: 16 * X[..., 0] * (1 - X[..., 0]) * X[..., 1] * (1 - X[..., 1]))
+    assert err < 1e-12
+
+
+def test_p6_poisson_exact_3d():
+    def f(X):
+        bx, by, bz = (X[..., k] * (1 - X[..., k]) for k in range(3))
+        return 2 * (by * bz + bx * bz + bx * by)
+
+    err = _poisson_l2_error(O.structured_3d(2), 6, f,
+                            lambda X: np.prod([X[..., k] * (1 - X[..., k]) for k in range(3)], axis=0))
+    assert err < 1e-12
+
+
+@pytest.mark.parametrize("dim,p", [(2, 3), (3, 4)])
+def test_stiffness_invariants(dim, p):
+    m = O.structured_2d(4) if dim == 2 else O.structured_3d(2)
+    pr = O.Problem(m, p)
+    rp, c, v = pr.assemble()
+    A = sp.csr_matrix((v, c, rp), shape=(pr.ndof, pr.ndof))
+    assert abs(A - A.T).max() < 1e-14 * abs(A).max()
+    one = np.zeros(pr.ndof)
+    one[:len(m["coords"])] = 1.0            # interpolant of the constant 1
+    assert np.abs(A @ one).max() < 1e-12
+    xv = np.zeros(pr.ndof)
+    xv[:len(m["coords"])] = m["coords"][:, 0]  # interpolant of u = x
+    assert abs(xv @ (A @ xv) - 1.0) < 1e-12   # int |grad x|^2 over the unit domain
+
+
+def test_class_numbers_of_structured_meshes():
+    c2 = O.class_number(2, O.structured_2d(4)["els"])
+    assert sorted(set(c2.tolist())) == [0, 2]
+    c3 = O.class_number(3, O.structured_3d(3)["els"])
+    vals, cnt = np.unique(c3, return_counts=True)
+    assert vals.tolist() == [0, 2, 26] and cnt.tolist() == [81, 54, 27]
+
+
+# -------------------------------------------------------------- C kernels
+def test_oracle_spmv_and_mf_match_assembly():
+    pr = O.Problem(O.structured_3d(2), 4, intorder=8)
+    kappa = np.exp(np.random.default_rng(0).standard_normal(pr.ne))
+    rp, c, v = pr.assemble(kappa=kappa)
+    x = np.random.default_rng(1).uniform(-1, 1, pr.ndof)
+    y_sp = sp.csr_matrix((v, c, rp), shape=(pr.ndof, pr.ndof)) @ x
+    y_c = oracle.spmv(rp, c, v, x)
+    assert np.abs(y_c - y_sp).max() <= 1e-13 * np.abs(y_sp).max()
+    cls = O.class_number(3, pr.mesh["els"])
+    uniq = np.unique(cls)
+    _, g = pr.shapes()
+    bref = np.stack([g[np.where(cls == k)[0][0]] for k in uniq])
+    y_mf = oracle.mf_apply(3, bref, np.searchsorted(uniq, cls), pr.wts, pr.dofs, pr.Finv, pr.detF, kappa, x)
+    assert np.abs(y_mf - y_sp).max() <= 1e-13 * np.abs(y_sp).max()
+
+
+def test_cg_converges_in_k_steps_for_k_eigenvalues():
+    """CG on an operator with k distinct eigenvalues stops after k steps: the
+    reference's `iterations` then equals k+1 (the initial check counts)."""
+    eig = np.repeat([1.0, 2.0, 5.0, 9.0], 25)
+    n = len(eig)
+    rp = np.arange(n + 1, dtype=np.int64)
+    c = np.arange(n, dtype=np.int32)
+    f = np.random.default_rng(3).uniform(-1, 1, n)
+    u, its, res = oracle.pcg_csr(rp, c, eig, np.ones(n), f, tol=1e-12, maxiter=100)
+    assert its == 5 and np.abs(u - f / eig).max() < 1e-12
+    u2, its2, res2 = O.pcg(lambda v: eig * v, np.ones(n), f, tol=1e-12, maxiter=100)
+    assert its2 == its and np.allclose(res, res2, rtol=1e-12)
+
+
+def test_cg_maxiter_and_zero_rhs():
+    eig = np.linspace(1, 100, 50)
+    rp = np.arange(51, dtype=np.int64)
+    c = np.arange(50, dtype=np.int32)
+    # a zero preconditioner gives residual 0 at the first check: converged
+    u, its, res = oracle.pcg_csr(rp, c, eig, np.zeros(50), np.ones(50), tol=1e-12, maxiter=3)
+    assert its == 1 and res[0] == 0.0
+    u, its, _ = oracle.pcg_csr(rp, c, eig, np.ones(50), np.zeros(50), tol=1e-12, maxiter=10)
+    assert its == 1 and not u.any()
+    u, its, res = oracle.pcg_csr(rp, c, eig, np.ones(50), np.ones(50), tol=1e-30, maxiter=7)
+    assert its == 7 and len(res) == 7
