@@ -16,9 +16,10 @@
 // operand (held in registers for the whole tile), applies the 2x2/3x3 metric
 // to the accumulators in registers, and feeds them straight back as the B
 // operand of the second product Y^T (dof x el) += B V^T.  The output layout of
-// a 16x16 MFMA (lane l holds rows 4*(l/16)..+3 of column l%16) is used as the
-// K index of the second product with the matching permutation of the A
-// operand, so U/V never leave registers.  B of one class (nq padded to 16,
+// a 16x16 MFMA (lane l holds 4 rows of column l%16: rows 4*(l/16)+r for FP32,
+// (l/16)+4r for FP64, see AccRow) is used as the K index of the second
+// product with the matching permutation of the A operand, so U/V never leave
+// registers.  B of one class (nq padded to 16,
 // nd padded to 4, one pad word per point against LDS bank conflicts) is loaded
 // into LDS once per workgroup; every wavefront then walks element tiles.
 //
@@ -51,6 +52,15 @@ __device__ __forceinline__ float4v Mfma(float a, float b, float4v c) {
 template <class T> struct Acc4;
 template <> struct Acc4<double> { typedef double4v type; };
 template <> struct Acc4<float> { typedef float4v type; };
+
+// Row of the 16x16 result held in accumulator element r of a lane in lane
+// group grp = lane/16 (the column is lane%16).  Measured on gfx950:
+// v_mfma_f32_16x16x4f32 holds rows 4*grp+r, v_mfma_f64_16x16x4f64 holds
+// rows grp+4*r.
+template <class T>
+__device__ __forceinline__ constexpr int AccRow(int grp, int r) {
+  return sizeof(T) == 8 ? grp + 4 * r : 4 * grp + r;
+}
 
 template <class V>
 __device__ __forceinline__ void AtomicAddV(V* p, V v) {
@@ -122,11 +132,11 @@ __global__ __launch_bounds__(kMfBlock) void MfMfmaKernel(
 #pragma unroll
         for (int st = 0; st < KS; st++) u[d] = Mfma(arow[4 * st], xk[st], u[d]);
       }
-      // metric at the 4 points this lane holds: q = 16t + 4grp + r
+      // metric at the 4 points this lane holds: q = 16t + AccRow(grp, r)
       T v[DIM][4];
 #pragma unroll
       for (int r = 0; r < 4; r++) {
-        const T w = (T)weights[16 * t + 4 * grp + r];
+        const T w = (T)weights[16 * t + AccRow<T>(grp, r)];
         T uu[DIM], vv[DIM];
 #pragma unroll
         for (int d = 0; d < DIM; d++) uu[d] = u[d][r];
@@ -142,7 +152,7 @@ __global__ __launch_bounds__(kMfBlock) void MfMfmaKernel(
         for (int d = 0; d < DIM; d++)
 #pragma unroll
           for (int r = 0; r < 4; r++) {
-            const T a = inrange ? lb[(16 * t + 4 * grp + r) * QS + d * NDP + dof] : (T)0;
+            const T a = inrange ? lb[(16 * t + AccRow<T>(grp, r)) * QS + d * NDP + dof] : (T)0;
             yacc[m] = Mfma(a, v[d][r], yacc[m]);
           }
       }
@@ -152,7 +162,7 @@ __global__ __launch_bounds__(kMfBlock) void MfMfmaKernel(
       for (int m = 0; m < MT; m++)
 #pragma unroll
         for (int r = 0; r < 4; r++) {
-          const int dof = 16 * m + 4 * grp + r;
+          const int dof = 16 * m + AccRow<T>(grp, r);
           if (dof < nd) {
             const V val = (V)yacc[m][r];
             if (MODE == 0) out[e * nd + dof] = val;
@@ -322,6 +332,10 @@ static uint16_t ToBf16(double v) {
   return (uint16_t)(r >> 16);
 }
 
+template <int MODE, class V>
+static void LaunchClass(const MatFreeOp& A, int ci, const V* x, V* out, double s, const int* stop, hipStream_t st,
+                        bool prepare);
+
 MatFreeOp::MatFreeOp(int dim_, int nd_, int nq_, int64_t ndof, int nclass, const int64_t* off, const double* bmat,
                      const double* w, const int* hdofs, const double* hmetric, const double* hkappa,
                      const int* helnr, int prec, int scat)
@@ -411,21 +425,29 @@ MatFreeOp::MatFreeOp(int dim_, int nd_, int nq_, int64_t ndof, int nclass, const
   g_rowptr.upload(cnt.data(), ndof + 1);
   g_slots.alloc(sl.size());
   g_slots.upload(sl.data(), sl.size());
+  contrib64.alloc((size_t)nel * nd);
+  contrib32.alloc((size_t)nel * nd);
+  // enable the dynamic LDS size of every kernel variant this operator uses
+  for (size_t c = 0; c < classes.size(); c++) {
+    LaunchClass<0, double>(*this, (int)c, nullptr, nullptr, 1.0, nullptr, nullptr, true);
+    LaunchClass<1, double>(*this, (int)c, nullptr, nullptr, 1.0, nullptr, nullptr, true);
+    LaunchClass<0, float>(*this, (int)c, nullptr, nullptr, 1.0, nullptr, nullptr, true);
+    LaunchClass<1, float>(*this, (int)c, nullptr, nullptr, 1.0, nullptr, nullptr, true);
+  }
   NGSAMD_HIP(hipStreamSynchronize(Stream()));
 }
 
 template <class T, int DIM, int NDP, int MODE, class V>
 static void LaunchMfma(const MatFreeOp& A, const T* b, const MfClass& c, const V* x, V* out, double s,
-                       const int* stop, hipStream_t st) {
+                       const int* stop, hipStream_t st, bool prepare) {
   const int nqb = A.nqp / 16;
   const size_t lds = (size_t)nqb * 16 * (DIM * NDP + 1) * sizeof(T);
   auto kern = MfMfmaKernel<T, DIM, NDP, MODE, V>;
-  static bool attr_set = false;
-  if (!attr_set) {
-    (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    attr_set = true;
-  } else {
-    (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (prepare) {
+    // dynamic LDS above the default limit must be enabled per kernel, once,
+    // outside any stream capture
+    NGSAMD_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    return;
   }
   const int64_t ntiles = (c.end - c.begin + 15) / 16;
   int64_t grid = (ntiles + kMfWaves - 1) / kMfWaves;
@@ -439,34 +461,36 @@ static void LaunchMfma(const MatFreeOp& A, const T* b, const MfClass& c, const V
 
 template <class T, int DIM, int MODE, class V>
 static void DispatchNdp(const MatFreeOp& A, const T* b, const MfClass& c, const V* x, V* out, double s,
-                        const int* stop, hipStream_t st) {
+                        const int* stop, hipStream_t st, bool prepare) {
   switch (A.ndp) {
-    case 4: LaunchMfma<T, DIM, 4, MODE, V>(A, b, c, x, out, s, stop, st); break;
-    case 8: LaunchMfma<T, DIM, 8, MODE, V>(A, b, c, x, out, s, stop, st); break;
-    case 12: LaunchMfma<T, DIM, 12, MODE, V>(A, b, c, x, out, s, stop, st); break;
-    case 16: LaunchMfma<T, DIM, 16, MODE, V>(A, b, c, x, out, s, stop, st); break;
-    case 20: LaunchMfma<T, DIM, 20, MODE, V>(A, b, c, x, out, s, stop, st); break;
-    case 24: LaunchMfma<T, DIM, 24, MODE, V>(A, b, c, x, out, s, stop, st); break;
-    case 36: LaunchMfma<T, DIM, 36, MODE, V>(A, b, c, x, out, s, stop, st); break;
+    case 4: LaunchMfma<T, DIM, 4, MODE, V>(A, b, c, x, out, s, stop, st, prepare); break;
+    case 8: LaunchMfma<T, DIM, 8, MODE, V>(A, b, c, x, out, s, stop, st, prepare); break;
+    case 12: LaunchMfma<T, DIM, 12, MODE, V>(A, b, c, x, out, s, stop, st, prepare); break;
+    case 16: LaunchMfma<T, DIM, 16, MODE, V>(A, b, c, x, out, s, stop, st, prepare); break;
+    case 20: LaunchMfma<T, DIM, 20, MODE, V>(A, b, c, x, out, s, stop, st, prepare); break;
+    case 24: LaunchMfma<T, DIM, 24, MODE, V>(A, b, c, x, out, s, stop, st, prepare); break;
+    case 36: LaunchMfma<T, DIM, 36, MODE, V>(A, b, c, x, out, s, stop, st, prepare); break;
     default: throw Error(NGSAMD_E_STATE, "matrix-free: unsupported padded dof count");
   }
 }
 
 template <int MODE, class V>
-static void LaunchClass(const MatFreeOp& A, int ci, const V* x, V* out, double s, const int* stop, hipStream_t st) {
+static void LaunchClass(const MatFreeOp& A, int ci, const V* x, V* out, double s, const int* stop, hipStream_t st,
+                        bool prepare) {
   const MfClass& c = A.classes[ci];
   if (c.end <= c.begin) return;
   if (A.use_mfma) {
     if (A.precision == NGSAMD_PREC_F64) {
       const double* b = A.bmat64.p + (size_t)ci * A.bstride;
-      if (A.dim == 2) DispatchNdp<double, 2, MODE, V>(A, b, c, x, out, s, stop, st);
-      else DispatchNdp<double, 3, MODE, V>(A, b, c, x, out, s, stop, st);
+      if (A.dim == 2) DispatchNdp<double, 2, MODE, V>(A, b, c, x, out, s, stop, st, prepare);
+      else DispatchNdp<double, 3, MODE, V>(A, b, c, x, out, s, stop, st, prepare);
     } else {
       const float* b = A.bmat32.p + (size_t)ci * A.bstride;
-      if (A.dim == 2) DispatchNdp<float, 2, MODE, V>(A, b, c, x, out, s, stop, st);
-      else DispatchNdp<float, 3, MODE, V>(A, b, c, x, out, s, stop, st);
+      if (A.dim == 2) DispatchNdp<float, 2, MODE, V>(A, b, c, x, out, s, stop, st, prepare);
+      else DispatchNdp<float, 3, MODE, V>(A, b, c, x, out, s, stop, st, prepare);
     }
   } else {
+    if (prepare) return;
     const double* b = A.bref.p + (size_t)ci * A.nq * A.nd * A.dim;
     const int64_t n = c.end - c.begin;
     const int grid = GridFor(n, kScalarBlock, 4096);
@@ -488,18 +512,15 @@ void MatFreeOp::Apply(double s, const V* x, V* y, bool accumulate, const int* st
       ScaleFillKernel<V><<<GridFor(height, 256, 8192), 256, 0, st>>>(y, height, stop);
       NGSAMD_CHECK_LAUNCH();
     }
-    for (size_t c = 0; c < classes.size(); c++) LaunchClass<1, V>(*this, (int)c, x, y, s, stop, st);
+    for (size_t c = 0; c < classes.size(); c++) LaunchClass<1, V>(*this, (int)c, x, y, s, stop, st, false);
     return;
   }
+  // contribution buffers are allocated by the constructor: nothing here
+  // allocates, so the apply can be captured into a graph
   V* contrib;
-  if constexpr (sizeof(V) == 8) {
-    if (contrib64.n < (size_t)(nel * nd)) contrib64.alloc(nel * nd);
-    contrib = reinterpret_cast<V*>(contrib64.p);
-  } else {
-    if (contrib32.n < (size_t)(nel * nd)) contrib32.alloc(nel * nd);
-    contrib = reinterpret_cast<V*>(contrib32.p);
-  }
-  for (size_t c = 0; c < classes.size(); c++) LaunchClass<0, V>(*this, (int)c, x, contrib, 1.0, stop, st);
+  if constexpr (sizeof(V) == 8) contrib = reinterpret_cast<V*>(contrib64.p);
+  else contrib = reinterpret_cast<V*>(contrib32.p);
+  for (size_t c = 0; c < classes.size(); c++) LaunchClass<0, V>(*this, (int)c, x, contrib, 1.0, stop, st, false);
   MfGatherKernel<V, V><<<GridFor(height, 256, 16384), 256, 0, st>>>(g_rowptr.p, g_slots.p, contrib, y, height, s,
                                                                     accumulate ? 1 : 0, stop);
   NGSAMD_CHECK_LAUNCH();
@@ -516,7 +537,6 @@ void MatFreeOp::MultF32(const float* x, float* y, const int* stop, hipStream_t s
 
 void MatFreeOp::Diagonal(double* d, hipStream_t st) {
   if (height == 0) return;
-  if (contrib64.n < (size_t)(nel * nd)) contrib64.alloc(nel * nd);
   for (size_t ci = 0; ci < classes.size(); ci++) {
     const MfClass& c = classes[ci];
     if (c.end <= c.begin) continue;
