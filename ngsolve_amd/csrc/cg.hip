@@ -13,11 +13,14 @@
 // check: iterations += 1; converged when iterations >= maxiter or
 // residual <= final, final = tol*res0 (max'ed with atol when given).
 //
-// One iteration is three kernels: (1) w = A s fused with <s,w> (CSR) or the
-// operator followed by a dot kernel, (2) the two updates, the Jacobi apply
-// and <w,d> fused, (3) s = beta s + w.  Reductions finish in the last
-// workgroup, which also evaluates alpha, beta and the stopping test on the
-// device, so the loop needs no host round trip.  `chunk` iterations are
+// One iteration is five kernels: (1) w = A s fused with the partial sums of
+// <s,w> (CSR) or the operator followed by a partial-dot kernel, (2) a
+// one-workgroup kernel forming alpha, (3) the two updates, the Jacobi apply
+// and the partials of <w,d> fused, (4) a one-workgroup kernel doing the
+// stopping test and beta, (5) s = beta s + w.  Scalars, the iteration count
+// and the residual history stay on the device, so the loop needs no host
+// round trip; the one-workgroup kernels sum the partials in index order, so
+// every run gives the same iterates bit for bit.  `chunk` iterations are
 // captured once into a HIP graph that is replayed until the device flag says
 // done; every kernel returns at once when the flag is set, so the overshoot
 // of the last chunk costs only launch latency.  (Reference: DevCGSolver,
@@ -26,6 +29,7 @@
 // The vector type V is double for the FP64 solver and float for the inner
 // solver of iterative refinement; inner products, alpha, beta and the
 // stopping test are FP64 in both.
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -93,8 +97,7 @@ void JacobiOp::Mult(const double* x, double* y, hipStream_t st) {
 
 struct CGScalars {
   double wdn;        // current <w,d>
-  double as_s;       // <s, A s>
-  double beta;
+  double alpha, beta;
   double final_res;  // stopping threshold
   double tol, atol;  // tol < 0: not given; atol < 0: not given
   int it;            // CheckResidual calls so far
@@ -118,14 +121,12 @@ __device__ __forceinline__ void CheckResidual(CGScalars* S, double* residuals, d
   if (S->it >= S->maxiter || res <= S->final_res) S->done = 1;
 }
 
-// w = C d; s = w; wdn = <w,d>; first check
+// w = C d; s = w; partials of <w,d>
 template <class V>
 __global__ __launch_bounds__(kCgBlock) void CgInitKernel(const V* __restrict__ inv, const V* __restrict__ d,
                                                          V* __restrict__ w, V* __restrict__ s, int64_t n,
-                                                         CGScalars* S, double* residuals, double* partials,
-                                                         unsigned* counter) {
+                                                         double* __restrict__ partials) {
   __shared__ double lds[kCgBlock / 64];
-  __shared__ bool s_last;
   double acc = 0.0;
   for (int64_t i = blockIdx.x * (int64_t)kCgBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kCgBlock) {
     const V di = d[i];
@@ -134,35 +135,42 @@ __global__ __launch_bounds__(kCgBlock) void CgInitKernel(const V* __restrict__ i
     s[i] = wi;
     acc += (double)wi * (double)di;
   }
-  double v = BlockSum<kCgBlock>(acc, lds);
-  double vals[1] = {v};
-  if (PublishPartials<1>(vals, partials, counter, &s_last)) {
-    double tot[1];
-    ReducePartials<kCgBlock, 1>(partials, tot, lds);
-    if (threadIdx.x == 0) {
-      S->wdn = tot[0];
-      CheckResidual(S, residuals, sqrt(fabs(tot[0])));
-      *counter = 0u;
-    }
+  StorePartial<kCgBlock>(acc, partials, lds);
+}
+
+// one workgroup: wdn = sum of partials; first check
+__global__ __launch_bounds__(kCgBlock) void CgInitCheckKernel(const double* __restrict__ partials, int count,
+                                                              CGScalars* S, double* residuals) {
+  __shared__ double lds[kCgBlock / 64];
+  const double wdn = SumPartials<kCgBlock>(partials, count, lds);
+  if (threadIdx.x == 0) {
+    S->wdn = wdn;
+    CheckResidual(S, residuals, sqrt(fabs(wdn)));
   }
 }
 
-// u += alpha s; d -= alpha w; w = C d; wdn = <w,d>; check; beta
+// one workgroup: as = <s, A s> from the partials; alpha or breakdown
+__global__ __launch_bounds__(kCgBlock) void CgAlphaKernel(const double* __restrict__ partials, int count,
+                                                          CGScalars* S) {
+  if (S->done) return;
+  __shared__ double lds[kCgBlock / 64];
+  const double as_s = SumPartials<kCgBlock>(partials, count, lds);
+  if (threadIdx.x == 0) {
+    const double wd = S->wdn;
+    if (as_s == 0.0 || wd == 0.0) S->done = 2;
+    else S->alpha = wd / as_s;
+  }
+}
+
+// u += alpha s; d -= alpha w; w = C d; partials of <w,d>
 template <class V>
 __global__ __launch_bounds__(kCgBlock) void CgUpdateKernel(const V* __restrict__ inv, V* __restrict__ u,
                                                            V* __restrict__ d, V* __restrict__ w,
-                                                           const V* __restrict__ s, int64_t n, CGScalars* S,
-                                                           double* residuals, double* partials, unsigned* counter) {
+                                                           const V* __restrict__ s, int64_t n, const CGScalars* S,
+                                                           double* __restrict__ partials) {
   if (S->done) return;
-  const double wd = S->wdn, as_s = S->as_s;
-  if (as_s == 0.0 || wd == 0.0) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) S->done = 2;
-    return;
-  }
-  const double alpha = wd / as_s;
-  const V alv = (V)alpha;
+  const V alv = (V)S->alpha;
   __shared__ double lds[kCgBlock / 64];
-  __shared__ bool s_last;
   double acc = 0.0;
   for (int64_t i = blockIdx.x * (int64_t)kCgBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kCgBlock) {
     const V si = s[i], wi = w[i];
@@ -173,18 +181,20 @@ __global__ __launch_bounds__(kCgBlock) void CgUpdateKernel(const V* __restrict__
     w[i] = zi;
     acc += (double)zi * (double)di;
   }
-  double v = BlockSum<kCgBlock>(acc, lds);
-  double vals[1] = {v};
-  if (PublishPartials<1>(vals, partials, counter, &s_last)) {
-    double tot[1];
-    ReducePartials<kCgBlock, 1>(partials, tot, lds);
-    if (threadIdx.x == 0) {
-      const double wdn = tot[0];
-      S->wdn = wdn;
-      CheckResidual(S, residuals, sqrt(fabs(wdn)));
-      S->beta = wdn / wd;
-      *counter = 0u;
-    }
+  StorePartial<kCgBlock>(acc, partials, lds);
+}
+
+// one workgroup: wdn from the partials; check; beta
+__global__ __launch_bounds__(kCgBlock) void CgCheckKernel(const double* __restrict__ partials, int count,
+                                                          CGScalars* S, double* residuals) {
+  if (S->done) return;
+  __shared__ double lds[kCgBlock / 64];
+  const double wdn = SumPartials<kCgBlock>(partials, count, lds);
+  if (threadIdx.x == 0) {
+    const double wd = S->wdn;
+    S->wdn = wdn;
+    CheckResidual(S, residuals, sqrt(fabs(wdn)));
+    S->beta = wdn / wd;
   }
 }
 
@@ -198,48 +208,33 @@ __global__ __launch_bounds__(kCgBlock) void CgDirectionKernel(V* __restrict__ s,
     s[i] = beta * s[i] + w[i];
 }
 
-// <x,y> (FP64 accumulation) into *out, skipped when *done
+// partials of <x,y> (FP64 accumulation), skipped when *done
 template <class V>
 __global__ __launch_bounds__(kCgBlock) void CgDotKernel(const V* __restrict__ x, const V* __restrict__ y, int64_t n,
-                                                        double* out, const int* done, double* partials,
-                                                        unsigned* counter) {
+                                                        const int* done, double* __restrict__ partials) {
   if (*done) return;
   __shared__ double lds[kCgBlock / 64];
-  __shared__ bool s_last;
   double acc = 0.0;
   for (int64_t i = blockIdx.x * (int64_t)kCgBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kCgBlock)
     acc += (double)x[i] * (double)y[i];
-  double v = BlockSum<kCgBlock>(acc, lds);
-  double vals[1] = {v};
-  if (PublishPartials<1>(vals, partials, counter, &s_last)) {
-    double tot[1];
-    ReducePartials<kCgBlock, 1>(partials, tot, lds);
-    if (threadIdx.x == 0) {
-      *out = tot[0];
-      *counter = 0u;
-    }
-  }
+  StorePartial<kCgBlock>(acc, partials, lds);
 }
 
-// *out = sum over free dofs (mask != 0) of x_i^2
+// partials of sum over free dofs (mask != 0) of x_i^2
 __global__ __launch_bounds__(kCgBlock) void MaskedSqKernel(const double* __restrict__ x, const double* __restrict__ mask,
-                                                           int64_t n, double* out, double* partials,
-                                                           unsigned* counter) {
+                                                           int64_t n, double* __restrict__ partials) {
   __shared__ double lds[kCgBlock / 64];
-  __shared__ bool s_last;
   double acc = 0.0;
   for (int64_t i = blockIdx.x * (int64_t)kCgBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kCgBlock)
     if (mask[i] != 0.0) acc += x[i] * x[i];
-  double v = BlockSum<kCgBlock>(acc, lds);
-  double vals[1] = {v};
-  if (PublishPartials<1>(vals, partials, counter, &s_last)) {
-    double tot[1];
-    ReducePartials<kCgBlock, 1>(partials, tot, lds);
-    if (threadIdx.x == 0) {
-      *out = tot[0];
-      *counter = 0u;
-    }
-  }
+  StorePartial<kCgBlock>(acc, partials, lds);
+}
+
+__global__ __launch_bounds__(kCgBlock) void SumToKernel(const double* __restrict__ partials, int count,
+                                                        double* __restrict__ out) {
+  __shared__ double lds[kCgBlock / 64];
+  const double v = SumPartials<kCgBlock>(partials, count, lds);
+  if (threadIdx.x == 0) *out = v;
 }
 
 __global__ void ScaledToF32Kernel(const double* __restrict__ x, float* __restrict__ y, int64_t n,
@@ -258,24 +253,23 @@ __global__ void AddScaledF32Kernel(double* __restrict__ u, const float* __restri
 
 static int CgGrid(int64_t n) { return GridFor(n, kCgBlock, 2048); }
 
-// w = A s and as_s = <s, w>, skipped when done
+// w = A s and the partials of <s, w>; returns the number of partials
 template <class V>
-static void EnqueueMatDot(ngsamd_op_t A, const V* s, V* w, int64_t n, CGScalars* dS, hipStream_t st) {
+static int EnqueueMatDot(ngsamd_op_t A, const V* s, V* w, int64_t n, CGScalars* dS, hipStream_t st) {
   auto& R = State();
-  int* done = &dS->done;
+  const int* done = &dS->done;
   if constexpr (sizeof(V) == 8) {
-    if (A->kind == OpKind::CSR) {
-      static_cast<CsrOp*>(A)->MultDot(s, w, &dS->as_s, done, st);
-      return;
-    }
+    if (A->kind == OpKind::CSR) return static_cast<CsrOp*>(A)->MultDot(s, w, R.partials, done, st);
     if (A->kind == OpKind::MatrixFree) static_cast<MatFreeOp*>(A)->MultStop(s, w, done, st);
     else A->Mult(s, w, st);
   } else {
     NGSAMD_REQUIRE(A->kind == OpKind::MatrixFree, "cg: FP32 vectors need a matrix-free operator");
     static_cast<MatFreeOp*>(A)->MultF32(s, w, done, st);
   }
-  CgDotKernel<V><<<CgGrid(n), kCgBlock, 0, st>>>(s, w, n, &dS->as_s, done, R.partials, R.counters + kCounterCG1);
+  const int g = CgGrid(n);
+  CgDotKernel<V><<<g, kCgBlock, 0, st>>>(s, w, n, done, R.partials);
   NGSAMD_CHECK_LAUNCH();
+  return g;
 }
 
 // One CG iteration on the stream (graph-capturable: kernels only).
@@ -283,11 +277,15 @@ template <class V>
 static void EnqueueIteration(ngsamd_op_t A, const V* inv, V* u, V* d, V* w, V* s, int64_t n, CGScalars* dS,
                              double* residuals, hipStream_t st) {
   auto& R = State();
-  EnqueueMatDot<V>(A, s, w, n, dS, st);
-  CgUpdateKernel<V><<<CgGrid(n), kCgBlock, 0, st>>>(inv, u, d, w, s, n, dS, residuals, R.partials,
-                                                     R.counters + kCounterCG2);
+  const int p1 = EnqueueMatDot<V>(A, s, w, n, dS, st);
+  CgAlphaKernel<<<1, kCgBlock, 0, st>>>(R.partials, p1, dS);
   NGSAMD_CHECK_LAUNCH();
-  CgDirectionKernel<V><<<CgGrid(n), kCgBlock, 0, st>>>(s, w, n, dS);
+  const int g = CgGrid(n);
+  CgUpdateKernel<V><<<g, kCgBlock, 0, st>>>(inv, u, d, w, s, n, dS, R.partials);
+  NGSAMD_CHECK_LAUNCH();
+  CgCheckKernel<<<1, kCgBlock, 0, st>>>(R.partials, g, dS, residuals);
+  NGSAMD_CHECK_LAUNCH();
+  CgDirectionKernel<V><<<g, kCgBlock, 0, st>>>(s, w, n, dS);
   NGSAMD_CHECK_LAUNCH();
 }
 
@@ -316,8 +314,10 @@ static CgResult IterateCg(ngsamd_op_t A, const V* inv, V* u, CgWork<V>& W, int64
   hS.atol = atol;
   hS.maxiter = maxiter;
   NGSAMD_HIP(hipMemcpyAsync(W.S.p, &hS, sizeof(CGScalars), hipMemcpyHostToDevice, st));
-  CgInitKernel<V><<<CgGrid(n), kCgBlock, 0, st>>>(inv, W.d.p, W.w.p, W.s.p, n, W.S.p, W.residuals.p, R.partials,
-                                                  R.counters + kCounterCG2);
+  const int g = CgGrid(n);
+  CgInitKernel<V><<<g, kCgBlock, 0, st>>>(inv, W.d.p, W.w.p, W.s.p, n, R.partials);
+  NGSAMD_CHECK_LAUNCH();
+  CgInitCheckKernel<<<1, kCgBlock, 0, st>>>(R.partials, g, W.S.p, W.residuals.p);
   NGSAMD_CHECK_LAUNCH();
 
   CGScalars* pin = nullptr;
@@ -478,7 +478,10 @@ extern "C" int ngsamd_ir_cg_solve(ngsamd_op_t A, ngsamd_op_t inner, ngsamd_op_t 
     // residual norms over the free dofs (where the Jacobi inverse is nonzero)
     DevBuf<double> sq(1);
     auto free_norm = [&](const double* x) {
-      MaskedSqKernel<<<CgGrid(n), kCgBlock, 0, st>>>(x, inv64, n, sq.p, R.partials, R.counters + kCounterMisc);
+      const int g = CgGrid(n);
+      MaskedSqKernel<<<g, kCgBlock, 0, st>>>(x, inv64, n, R.partials);
+      NGSAMD_CHECK_LAUNCH();
+      SumToKernel<<<1, kCgBlock, 0, st>>>(R.partials, g, sq.p);
       NGSAMD_CHECK_LAUNCH();
       double h = 0.0;
       NGSAMD_HIP(hipMemcpyAsync(&h, sq.p, sizeof(double), hipMemcpyDeviceToHost, st));
@@ -501,7 +504,10 @@ extern "C" int ngsamd_ir_cg_solve(ngsamd_op_t A, ngsamd_op_t inner, ngsamd_op_t 
       ScaledToF32Kernel<<<GridFor(n, 256, 8192), 256, 0, st>>>(r.p, W.d.p, n, scal.p);
       NGSAMD_CHECK_LAUNCH();
       LaunchFillF32(c32.p, n, 0.0f, st);
-      CgResult ir = IterateCg<float>(Ai, inv32.p, c32.p, W, n, inner_tol, -1.0, max_inner, 16, st);
+      // the last refinement step only needs to gain the factor still missing
+      double itol = inner_tol;
+      if (fn > 0) itol = std::max(itol, std::min(0.5, 0.3 * tol * fn / rn));
+      CgResult ir = IterateCg<float>(Ai, inv32.p, c32.p, W, n, itol, -1.0, max_inner, 16, st);
       inner_total += ir.iterations;
       // u += ||r|| c; r = f - A u (FP64)
       AddScaledF32Kernel<<<GridFor(n, 256, 8192), 256, 0, st>>>(u, c32.p, n, scal.p + 1);

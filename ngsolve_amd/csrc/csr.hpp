@@ -11,12 +11,17 @@ struct CsrOp : ngsamd_op_s {
   DevBuf<int> rowptr, cols;
   DevBuf<double> vals;
   int64_t nnz = 0;
-  int group = 32;  // lanes per row
+  int group = 32;  // lanes per row (rowgroup kernel)
+  int algo = 1;    // 0: rowgroup, 1: stream
+  DevBuf<int> blk_row;  // stream kernel row blocks
+  int nblocks = 0;
+  int block_nnz = 2048;
   CsrOp(int64_t h, int64_t w, const int64_t* rp, const int* c, const double* v);
   void MultAdd(double s, const double* x, double* y, hipStream_t st) override;
   void Mult(const double* x, double* y, hipStream_t st) override;
-  // y = A x and *d_result = <x, A x>; skipped entirely when *d_stop != 0
-  void MultDot(const double* x, double* y, double* d_result, const int* d_stop, hipStream_t st);
+  // y = A x plus per-workgroup partial sums of <x, A x> in partials[];
+  // returns the number of partials; skipped entirely when *d_stop != 0
+  int MultDot(const double* x, double* y, double* partials, const int* d_stop, hipStream_t st);
   void Diagonal(double* d, hipStream_t st);
 };
 

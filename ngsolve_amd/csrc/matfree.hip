@@ -173,6 +173,123 @@ __global__ __launch_bounds__(kMfBlock) void MfMfmaKernel(
   }
 }
 
+
+// BF16 path (v_mfma_f32_16x16x32_bf16, FP32 accumulation).  Operand layout
+// measured on gfx950: lane l supplies A[l%16][8*(l/16)+j] and
+// B[8*(l/16)+j][l%16], j = 0..7, and holds result rows 4*(l/16)+r.
+// Quadrature points are taken in chunks of 32 so that the K dimension of the
+// second product (one direction x 32 points) is one MFMA.  B is held in LDS
+// twice, each copy laid out so that a lane's 8 A-operand values are one
+// contiguous 16-byte read:
+//   B1[q][d][K1P]            first product, dofs contiguous (K1P = nd -> 8k)
+//   B2[chunk][d][MT*16][32]  second product, the 32 points of a chunk in the
+//                            order the first product's accumulators hold them
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+template <int DIM, int KS1, int MT, int MODE, class V>
+__global__ __launch_bounds__(kMfBlock) void MfBf16Kernel(
+    const uint16_t* __restrict__ bclass, int k1p, int nqc, const double* __restrict__ weights,
+    const int* __restrict__ dofs, const double* __restrict__ metric, const double* __restrict__ kappa,
+    const int* __restrict__ elnr, int64_t begin, int64_t end, int nd, const V* __restrict__ x, V* __restrict__ out,
+    double s, const int* __restrict__ stop) {
+  if (stop && *stop) return;
+  constexpr int NC = DIM * (DIM + 1) / 2;
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  const int size1 = nqc * 32 * DIM * k1p;
+  const int size2 = nqc * DIM * MT * 16 * 32;
+  {
+    const uint4* src = reinterpret_cast<const uint4*>(bclass);
+    uint4* dst = reinterpret_cast<uint4*>(lds_raw);
+    for (int i = threadIdx.x; i < (size1 + size2) / 8; i += kMfBlock) dst[i] = src[i];
+  }
+  __syncthreads();
+  const __bf16* B1 = reinterpret_cast<const __bf16*>(lds_raw);
+  const __bf16* B2 = B1 + size1;
+
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int col = lane & 15, grp = lane >> 4;
+  const int64_t ntiles = (end - begin + 15) / 16;
+  for (int64_t tile = (int64_t)blockIdx.x * kMfWaves + wave; tile < ntiles; tile += (int64_t)gridDim.x * kMfWaves) {
+    const int64_t e = begin + tile * 16 + col;
+    const bool valid = e < end;
+    bf16x8 xk[KS1];
+#pragma unroll
+    for (int ks = 0; ks < KS1; ks++)
+#pragma unroll
+      for (int j = 0; j < 8; j++) {
+        const int dof = 32 * ks + 8 * grp + j;
+        xk[ks][j] = (__bf16)((valid && dof < nd) ? (float)x[dofs[e * nd + dof]] : 0.0f);
+      }
+    float g[NC];
+    {
+      const double kap = valid ? kappa[elnr[e]] : 0.0;
+#pragma unroll
+      for (int c = 0; c < NC; c++) g[c] = valid ? (float)(kap * metric[e * NC + c]) : 0.0f;
+    }
+    float4v yacc[MT];
+#pragma unroll
+    for (int m = 0; m < MT; m++) yacc[m] = float4v{0, 0, 0, 0};
+    const bf16x8 zero8 = {};
+
+    for (int c = 0; c < nqc; c++) {
+      float4v u[2][DIM];
+#pragma unroll
+      for (int h = 0; h < 2; h++)
+#pragma unroll
+        for (int d = 0; d < DIM; d++) {
+          u[h][d] = float4v{0, 0, 0, 0};
+#pragma unroll
+          for (int ks = 0; ks < KS1; ks++) {
+            const int k0 = 32 * ks + 8 * grp;
+            const bf16x8 a = k0 < k1p
+                                 ? *reinterpret_cast<const bf16x8*>(B1 + ((32 * c + 16 * h + col) * DIM + d) * k1p + k0)
+                                 : zero8;
+            u[h][d] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, xk[ks], u[h][d], 0, 0, 0);
+          }
+        }
+      // metric at the 8 points this lane holds: q = 32c + 16h + 4grp + r
+      float v[DIM][8];
+#pragma unroll
+      for (int h = 0; h < 2; h++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const float w = (float)weights[32 * c + 16 * h + 4 * grp + r];
+          float uu[DIM], vv[DIM];
+#pragma unroll
+          for (int d = 0; d < DIM; d++) uu[d] = u[h][d][r];
+          ApplyMetric<DIM, float>(g, uu, vv);
+#pragma unroll
+          for (int d = 0; d < DIM; d++) v[d][4 * h + r] = w * vv[d];
+        }
+#pragma unroll
+      for (int d = 0; d < DIM; d++) {
+        bf16x8 vb;
+#pragma unroll
+        for (int j = 0; j < 8; j++) vb[j] = (__bf16)v[d][j];
+#pragma unroll
+        for (int m = 0; m < MT; m++) {
+          const bf16x8 a =
+              *reinterpret_cast<const bf16x8*>(B2 + (((c * DIM + d) * MT * 16 + 16 * m + col) * 32 + 8 * grp));
+          yacc[m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, vb, yacc[m], 0, 0, 0);
+        }
+      }
+    }
+    if (valid) {
+#pragma unroll
+      for (int m = 0; m < MT; m++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const int dof = 16 * m + 4 * grp + r;
+          if (dof < nd) {
+            const V val = (V)yacc[m][r];
+            if (MODE == 0) out[e * nd + dof] = val;
+            else AtomicAddV(&out[dofs[e * nd + dof]], (V)(s * (double)val));
+          }
+        }
+    }
+  }
+}
+
 // Scalar fallback for shapes the MFMA path does not cover: one thread per
 // element, FP64, element vectors staged in LDS columns.
 constexpr int kScalarBlock = 64;
@@ -324,6 +441,10 @@ static bool SupportedNdp(int ndp) {
   return ndp == 4 || ndp == 8 || ndp == 12 || ndp == 16 || ndp == 20 || ndp == 24 || ndp == 36;
 }
 
+static bool SupportedBf16(int ks1, int mt) {
+  return (ks1 == 1 && (mt == 1 || mt == 2)) || (ks1 == 2 && mt == 3);
+}
+
 static uint16_t ToBf16(double v) {
   float f = (float)v;
   uint32_t u;
@@ -375,18 +496,47 @@ MatFreeOp::MatFreeOp(int dim_, int nd_, int nq_, int64_t ndof, int nclass, const
   const int qs = dim * ndp + 1;
   bstride = (int64_t)nqp * qs;
   // MFMA path if the padded shape is instantiated and B fits in LDS
-  const size_t esize = prec == NGSAMD_PREC_F64 ? 8 : 4;
-  use_mfma = SupportedNdp(ndp) && (size_t)bstride * esize <= (size_t)DeviceMaxLds() && prec != NGSAMD_PREC_BF16;
+  if (prec == NGSAMD_PREC_BF16) {
+    nqc = (nq + 31) / 32;
+    k1p = (nd + 7) / 8 * 8;
+    ks1 = (nd + 31) / 32;
+    mt16 = (nd + 15) / 16;
+    bstride = (int64_t)nqc * 32 * dim * k1p + (int64_t)nqc * dim * mt16 * 16 * 32;
+    use_mfma = SupportedBf16(ks1, mt16) && (size_t)bstride * 2 <= (size_t)DeviceMaxLds();
+  } else {
+    const size_t esize = prec == NGSAMD_PREC_F64 ? 8 : 4;
+    use_mfma = SupportedNdp(ndp) && (size_t)bstride * esize <= (size_t)DeviceMaxLds();
+  }
 
   // FP64 reference gradients [class][nq][nd][dim] for the scalar kernels
   const size_t bsz = (size_t)nq * nd * dim;
   bref.alloc(bsz * nclass);
   bref.upload(bmat, bsz * nclass);
-  std::vector<double> wp(nqp, 0.0);
+  const int nw = (nq + 31) / 32 * 32;  // covers the FP64/FP32 (16) and BF16 (32) point padding
+  std::vector<double> wp(nw, 0.0);
   for (int q = 0; q < nq; q++) wp[q] = w[q];
-  weights.alloc(nqp);
-  weights.upload(wp.data(), nqp);
-  if (use_mfma) {
+  weights.alloc(nw);
+  weights.upload(wp.data(), nw);
+  if (use_mfma && prec == NGSAMD_PREC_BF16) {
+    std::vector<uint16_t> lay((size_t)bstride * nclass, 0);
+    const int64_t size1 = (int64_t)nqc * 32 * dim * k1p;
+    for (int c = 0; c < nclass; c++) {
+      uint16_t* b1 = lay.data() + (size_t)c * bstride;
+      uint16_t* b2 = b1 + size1;
+      for (int q = 0; q < nq; q++)
+        for (int i = 0; i < nd; i++)
+          for (int d = 0; d < dim; d++) {
+            const uint16_t v = ToBf16(bmat[c * bsz + ((size_t)q * nd + i) * dim + d]);
+            b1[((size_t)q * dim + d) * k1p + i] = v;
+            // position of point q inside its chunk in the accumulator order
+            const int ch = q / 32, ql = q % 32, h = ql / 16, grp = (ql % 16) / 4, r = ql % 4;
+            const int kk = 8 * grp + 4 * h + r;
+            b2[(((size_t)ch * dim + d) * mt16 * 16 + i) * 32 + kk] = v;
+          }
+    }
+    bmat16.alloc(lay.size());
+    bmat16.upload(lay.data(), lay.size());
+  } else if (use_mfma) {
     std::vector<double> lay((size_t)bstride * nclass, 0.0);
     for (int c = 0; c < nclass; c++)
       for (int q = 0; q < nq; q++)
@@ -402,7 +552,6 @@ MatFreeOp::MatFreeOp(int dim_, int nd_, int nq_, int64_t ndof, int nclass, const
       bmat32.upload(l32.data(), l32.size());
     }
   }
-  (void)ToBf16;
   dofs.alloc((size_t)nel * nd);
   dofs.upload(hdofs, (size_t)nel * nd);
   metric.alloc((size_t)nel * ncomp);
@@ -474,13 +623,45 @@ static void DispatchNdp(const MatFreeOp& A, const T* b, const MfClass& c, const 
   }
 }
 
+template <int DIM, int KS1, int MT, int MODE, class V>
+static void LaunchBf16(const MatFreeOp& A, const uint16_t* b, const MfClass& c, const V* x, V* out, double s,
+                       const int* stop, hipStream_t st, bool prepare) {
+  const size_t lds = (size_t)A.bstride * 2;
+  auto kern = MfBf16Kernel<DIM, KS1, MT, MODE, V>;
+  if (prepare) {
+    NGSAMD_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    return;
+  }
+  const int64_t ntiles = (c.end - c.begin + 15) / 16;
+  int64_t grid = (ntiles + kMfWaves - 1) / kMfWaves;
+  const int64_t maxg = (int64_t)DeviceCUs() * (lds > 81920 ? 1 : 2);
+  if (grid > maxg) grid = maxg;
+  if (grid < 1) grid = 1;
+  kern<<<(int)grid, kMfBlock, lds, st>>>(b, A.k1p, A.nqc, A.weights.p, A.dofs.p, A.metric.p, A.Kappa(), A.elnr.p,
+                                         c.begin, c.end, A.nd, x, out, s, stop);
+  NGSAMD_CHECK_LAUNCH();
+}
+
+template <int DIM, int MODE, class V>
+static void DispatchBf16(const MatFreeOp& A, const uint16_t* b, const MfClass& c, const V* x, V* out, double s,
+                         const int* stop, hipStream_t st, bool prepare) {
+  if (A.ks1 == 1 && A.mt16 == 1) LaunchBf16<DIM, 1, 1, MODE, V>(A, b, c, x, out, s, stop, st, prepare);
+  else if (A.ks1 == 1 && A.mt16 == 2) LaunchBf16<DIM, 1, 2, MODE, V>(A, b, c, x, out, s, stop, st, prepare);
+  else if (A.ks1 == 2 && A.mt16 == 3) LaunchBf16<DIM, 2, 3, MODE, V>(A, b, c, x, out, s, stop, st, prepare);
+  else throw Error(NGSAMD_E_STATE, "matrix-free: unsupported BF16 shape");
+}
+
 template <int MODE, class V>
 static void LaunchClass(const MatFreeOp& A, int ci, const V* x, V* out, double s, const int* stop, hipStream_t st,
                         bool prepare) {
   const MfClass& c = A.classes[ci];
   if (c.end <= c.begin) return;
   if (A.use_mfma) {
-    if (A.precision == NGSAMD_PREC_F64) {
+    if (A.precision == NGSAMD_PREC_BF16) {
+      const uint16_t* b = A.bmat16.p + (size_t)ci * A.bstride;
+      if (A.dim == 2) DispatchBf16<2, MODE, V>(A, b, c, x, out, s, stop, st, prepare);
+      else DispatchBf16<3, MODE, V>(A, b, c, x, out, s, stop, st, prepare);
+    } else if (A.precision == NGSAMD_PREC_F64) {
       const double* b = A.bmat64.p + (size_t)ci * A.bstride;
       if (A.dim == 2) DispatchNdp<double, 2, MODE, V>(A, b, c, x, out, s, stop, st, prepare);
       else DispatchNdp<double, 3, MODE, V>(A, b, c, x, out, s, stop, st, prepare);

@@ -25,6 +25,7 @@ struct MatFreeOp : ngsamd_op_s {
   DevBuf<float> bmat32;
   DevBuf<uint16_t> bmat16;
   int64_t bstride = 0;        // elements per class block in bmat*
+  int nqc = 0, k1p = 0, ks1 = 0, mt16 = 0;  // BF16 layout: point chunks of 32, padded dofs, k-steps, dof tiles
   DevBuf<double> weights;     // [nqp], 0 beyond nq
   DevBuf<double> bref;        // [nclass][nq][nd][dim] FP64 (scalar path, diagonal, gradient)
   DevBuf<int> dofs;           // [nel][nd]

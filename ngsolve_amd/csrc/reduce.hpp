@@ -1,18 +1,16 @@
-// Wavefront / workgroup reductions and the last-workgroup finalisation used by
-// the dot products and the fused CG kernels.  Sums are formed in a fixed
-// order for a fixed launch shape, so results are bit-reproducible run to run.
+// Wavefront / workgroup reductions.  Grid-wide sums are two-stage: every
+// workgroup stores its partial sum with a plain store, and a one-workgroup
+// kernel launched next on the same stream sums the partials in index order.
+// The kernel boundary makes the partials visible; no grid-wide fence is
+// needed (on a multi-XCD device an agent-scope release writes back L2, which
+// is far too expensive to do once per workgroup).  For a fixed launch shape
+// the result is bit-reproducible.
 #pragma once
 #include <hip/hip_runtime.h>
 
 namespace ngsamd {
 
 __device__ __forceinline__ double WaveSum(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
-}
-
-__device__ __forceinline__ float WaveSumF(float v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
   return v;
@@ -35,37 +33,19 @@ __device__ __forceinline__ double BlockSum(double v, double* lds) {
   return r;
 }
 
-// Publish this workgroup's partial sums (NV values) and return true in the
-// workgroup that finishes last.  That workgroup then reads all partials
-// (written by every workgroup before it) and resets the counter to zero, so
-// the next launch (or graph replay) starts from a clean state.
-template <int NV>
-__device__ __forceinline__ bool PublishPartials(const double* vals, double* partials, unsigned* counter,
-                                                bool* s_last) {
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int k = 0; k < NV; k++) partials[(size_t)k * gridDim.x + blockIdx.x] = vals[k];
-    __threadfence();
-    unsigned prev = atomicAdd(counter, 1u);
-    *s_last = (prev == gridDim.x - 1);
-  }
-  __syncthreads();
-  const bool last = *s_last;
-  if (last) __threadfence();  // acquire: see every other workgroup's partials
-  return last;
+// Workgroup sum of v stored to partials[blockIdx.x] (thread 0).
+template <int BLOCK>
+__device__ __forceinline__ void StorePartial(double v, double* partials, double* lds) {
+  v = BlockSum<BLOCK>(v, lds);
+  if (threadIdx.x == 0) partials[blockIdx.x] = v;
 }
 
-// In the last workgroup: deterministic sum of the NV partial arrays.
-template <int BLOCK, int NV>
-__device__ __forceinline__ void ReducePartials(const double* partials, double* out, double* lds) {
-#pragma unroll
-  for (int k = 0; k < NV; k++) {
-    double v = 0.0;
-    for (unsigned i = threadIdx.x; i < gridDim.x; i += BLOCK)
-      v += partials[(size_t)k * gridDim.x + i];
-    v = BlockSum<BLOCK>(v, lds);
-    if (threadIdx.x == 0) out[k] = v;
-  }
+// Inside a one-workgroup kernel: deterministic sum of partials[0..count).
+template <int BLOCK>
+__device__ __forceinline__ double SumPartials(const double* partials, int count, double* lds) {
+  double v = 0.0;
+  for (int i = threadIdx.x; i < count; i += BLOCK) v += partials[i];
+  return BlockSum<BLOCK>(v, lds);
 }
 
 }  // namespace ngsamd

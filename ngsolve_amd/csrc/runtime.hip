@@ -70,23 +70,23 @@ __global__ void ToF64Kernel(const float* __restrict__ x, double* __restrict__ y,
     y[i] = (double)x[i];
 }
 
-// Dot product: fixed grid for a given n, per-workgroup partials, the last
-// workgroup sums them in index order -> deterministic result.
+// Dot product: fixed grid for a given n, per-workgroup partials, then a
+// one-workgroup kernel sums them in index order -> deterministic result.
 template <class T>
-__global__ __launch_bounds__(256) void DotKernel(const T* __restrict__ x, const T* __restrict__ y, int64_t n,
-                                                 double* __restrict__ partials, unsigned* __restrict__ counter,
-                                                 double* __restrict__ result) {
+__global__ __launch_bounds__(256) void DotPartialKernel(const T* __restrict__ x, const T* __restrict__ y, int64_t n,
+                                                        double* __restrict__ partials) {
   __shared__ double lds[4];
-  __shared__ bool s_last;
   double v = 0.0;
   for (int64_t i = blockIdx.x * (int64_t)256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
     v += (double)x[i] * (double)y[i];
-  v = BlockSum<256>(v, lds);
-  double vals[1] = {v};
-  if (PublishPartials<1>(vals, partials, counter, &s_last)) {
-    ReducePartials<256, 1>(partials, result, lds);
-    if (threadIdx.x == 0) *counter = 0u;
-  }
+  StorePartial<256>(v, partials, lds);
+}
+
+__global__ __launch_bounds__(256) void SumPartialsKernel(const double* __restrict__ partials, int count,
+                                                         double* __restrict__ result) {
+  __shared__ double lds[4];
+  double v = SumPartials<256>(partials, count, lds);
+  if (threadIdx.x == 0) *result = v;
 }
 
 template <class T>
@@ -94,8 +94,10 @@ static double Dot(int64_t n, const T* x, const T* y) {
   RequireInit();
   auto& S = State();
   hipStream_t st = Stream();
-  int grid = GridFor(n, 256, kMaxReduceBlocks);
-  DotKernel<T><<<grid, 256, 0, st>>>(x, y, n, S.partials, S.counters, S.dev_scalars);
+  int grid = GridFor(n, 256, 2048);
+  DotPartialKernel<T><<<grid, 256, 0, st>>>(x, y, n, S.partials);
+  NGSAMD_CHECK_LAUNCH();
+  SumPartialsKernel<<<1, 256, 0, st>>>(S.partials, grid, S.dev_scalars);
   NGSAMD_CHECK_LAUNCH();
   NGSAMD_HIP(hipMemcpyAsync(S.host_scalars, S.dev_scalars, sizeof(double), hipMemcpyDeviceToHost, st));
   NGSAMD_HIP(hipStreamSynchronize(st));
@@ -124,9 +126,7 @@ int ngsamd_init(int device) {
     NGSAMD_HIP(hipSetDevice(device));
     S.device = device;
     NGSAMD_HIP(hipStreamCreateWithFlags(&S.stream, hipStreamNonBlocking));
-    NGSAMD_HIP(hipMalloc((void**)&S.partials, sizeof(double) * kMaxReduceBlocks * kMaxPartialVals));
-    NGSAMD_HIP(hipMalloc((void**)&S.counters, sizeof(unsigned) * kNumCounters));
-    NGSAMD_HIP(hipMemset(S.counters, 0, sizeof(unsigned) * kNumCounters));
+    NGSAMD_HIP(hipMalloc((void**)&S.partials, sizeof(double) * kMaxReduceBlocks));
     NGSAMD_HIP(hipMalloc((void**)&S.dev_scalars, sizeof(double) * kNumScalars));
     NGSAMD_HIP(hipHostMalloc((void**)&S.host_scalars, sizeof(double) * kNumScalars, hipHostMallocDefault));
     NGSAMD_HIP(hipDeviceSynchronize());

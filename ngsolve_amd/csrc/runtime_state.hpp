@@ -1,5 +1,5 @@
 // Process-wide state of libngsamd: device, stream and the small reduction
-// workspace (per-workgroup partials, completion counters, device scalars).
+// workspace (per-workgroup partial sums, device scalars).
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -8,11 +8,7 @@
 namespace ngsamd {
 
 constexpr int kMaxReduceBlocks = 4096;  // upper bound on workgroups of a reducing kernel
-constexpr int kMaxPartialVals = 4;      // partial sums per workgroup
-constexpr int kNumCounters = 16;        // completion counters (one per concurrent reducing kernel kind)
 constexpr int kNumScalars = 64;
-
-enum CounterSlot { kCounterDot = 0, kCounterCG1 = 1, kCounterCG2 = 2, kCounterMisc = 3 };
 
 struct RuntimeState {
   std::mutex mutex;
@@ -20,8 +16,7 @@ struct RuntimeState {
   int device = -1;
   hipStream_t stream = nullptr;       // library stream
   hipStream_t user_stream = nullptr;  // stream set by ngsamd_set_stream
-  double* partials = nullptr;         // kMaxReduceBlocks * kMaxPartialVals
-  unsigned* counters = nullptr;       // kNumCounters, zero between launches
+  double* partials = nullptr;         // kMaxReduceBlocks partial sums
   double* dev_scalars = nullptr;      // kNumScalars
   double* host_scalars = nullptr;     // pinned mirror
 };

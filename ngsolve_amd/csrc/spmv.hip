@@ -5,13 +5,25 @@
 // reference's cuSPARSE call (ngscuda/cuda_linalg.cpp:243-275).
 //
 // Layout: int32 row pointers, int32 column numbers, FP64 values (12 B per
-// nonzero streamed once).  A group of G lanes of a 64-wide wavefront works on
-// one row: lane l of the group reads nonzeros l, l+G, ... so every load
-// instruction of a wavefront covers 64/G contiguous segments of the column
-// and value streams; the G partial sums are combined with cross-lane
-// shuffles.  G is picked from the mean row length.  The workgroups walk
-// contiguous row ranges that are grouped per XCD (workgroup i runs on XCD
-// i mod 8), so the x entries a row range gathers stay in one XCD's L2.
+// nonzero streamed once).  Two kernels:
+//
+// stream (default): the rows are cut on the host into blocks of consecutive
+// rows holding at most 2048 nonzeros (NGSAMD_SPMV_NNZ=1024/4096 selects other
+// sizes; a single longer row is a block of its own).  A 256-thread workgroup streams a block's column and value
+// ranges with fully coalesced non-temporal loads (they are read once and
+// must not evict x from L2), multiplies by the gathered x entries into LDS,
+// and then reduces each row from LDS with a power-of-two group of lanes sized
+// to the block's row count.  Every lane is busy whatever the row lengths.
+//
+// rowgroup: a group of G lanes works on one row, lane l of the group reading
+// nonzeros l, l+G, ...; the partial sums are combined with cross-lane
+// shuffles.  Kept for comparison (NGSAMD_SPMV=rowgroup).
+//
+// Both walk contiguous row ranges that are grouped per XCD (workgroup i runs
+// on XCD i mod 8), so the x entries a row range gathers stay in one XCD's L2.
+#include <cstdlib>
+#include <string>
+
 #include "common.hpp"
 #include "csr.hpp"
 #include "reduce.hpp"
@@ -28,14 +40,13 @@ __device__ __forceinline__ int XcdContiguousBlock(int b, int grid) {
   return (b & 7) * (grid >> 3) + (b >> 3);
 }
 
-// y[r] = (ACCUM ? y[r] : 0) + s * (A x)[r]; optionally dot = sum_r x[r]*(A x)[r]
-// (the <s, A s> of CG) published as per-workgroup partials.
+// y[r] = (ACCUM ? y[r] : 0) + s * (A x)[r]; optionally the per-workgroup
+// partial sums of x[r]*(A x)[r] (the <s, A s> of CG) into partials[].
 template <int G, bool ACCUM, bool DOT>
 __global__ __launch_bounds__(kSpmvBlock) void CsrSpmvKernel(
     const int* __restrict__ rowptr, const int* __restrict__ cols, const double* __restrict__ vals,
-    const double* __restrict__ x, double* __restrict__ y, int n, double s, int rows_per_wg_iter,
-    int iters_per_wg, double* __restrict__ partials, unsigned* __restrict__ counter,
-    double* __restrict__ result, const int* __restrict__ stop_flag) {
+    const double* __restrict__ x, double* __restrict__ y, int n, double s, int iters_per_wg,
+    double* __restrict__ partials, const int* __restrict__ stop_flag) {
   if (stop_flag && *stop_flag) return;
   constexpr int RPB = kSpmvBlock / G;  // rows per workgroup pass
   const int lb = XcdContiguousBlock(blockIdx.x, gridDim.x);
@@ -66,17 +77,82 @@ __global__ __launch_bounds__(kSpmvBlock) void CsrSpmvKernel(
       }
     }
   }
-  (void)rows_per_wg_iter;
   if (DOT) {
     __shared__ double lds[kSpmvBlock / 64];
-    __shared__ bool s_last;
-    double v = BlockSum<kSpmvBlock>(dotacc, lds);
-    double vals1[1] = {v};
-    if (PublishPartials<1>(vals1, partials, counter, &s_last)) {
-      ReducePartials<kSpmvBlock, 1>(partials, result, lds);
-      if (threadIdx.x == 0) *counter = 0u;
-    }
+    StorePartial<kSpmvBlock>(dotacc, partials, lds);
   }
+}
+
+
+constexpr int kStreamBlock = 256;
+
+// y[r] = (ACCUM ? y[r] : 0) + s * (A x)[r] over the row blocks
+// [blk_row[b], blk_row[b+1]) of at most NNZ nonzeros; DOT as in CsrSpmvKernel.
+template <int NNZ, bool ACCUM, bool DOT>
+__global__ __launch_bounds__(kStreamBlock) void CsrStreamKernel(
+    const int* __restrict__ rowptr, const int* __restrict__ cols, const double* __restrict__ vals,
+    const double* __restrict__ x, double* __restrict__ y, const int* __restrict__ blk_row, int nblocks,
+    int blocks_per_wg, double s, double* __restrict__ partials, const int* __restrict__ stop_flag) {
+  if (stop_flag && *stop_flag) return;
+  constexpr int PER = NNZ / kStreamBlock;  // nonzeros per thread and block
+  __shared__ double prod[NNZ];
+  __shared__ double lds[kStreamBlock / 64];
+  const int tid = threadIdx.x;
+  const int lb = XcdContiguousBlock(blockIdx.x, gridDim.x);
+  const int b0 = lb * blocks_per_wg;
+  const int b1 = min(b0 + blocks_per_wg, nblocks);
+  double dotacc = 0.0;
+  for (int b = b0; b < b1; b++) {
+    const int r0 = blk_row[b], r1 = blk_row[b + 1];
+    const int k0 = rowptr[r0], k1 = rowptr[r1];
+    const int nz = k1 - k0;
+    if (nz > NNZ) {
+      // one long row: the whole workgroup strides over it
+      double a = 0.0;
+      for (int k = k0 + tid; k < k1; k += kStreamBlock)
+        a += __builtin_nontemporal_load(&vals[k]) * x[__builtin_nontemporal_load(&cols[k])];
+      a = BlockSum<kStreamBlock>(a, lds);
+      if (tid == 0) {
+        if (ACCUM) y[r0] += s * a; else y[r0] = s * a;
+        if (DOT) dotacc += x[r0] * a;
+      }
+      continue;
+    }
+    // all loads of this thread first (memory-level parallelism), then the products
+    int c[PER];
+    double v[PER];
+#pragma unroll
+    for (int i = 0; i < PER; i++) {
+      const int k = tid + i * kStreamBlock;
+      c[i] = k < nz ? __builtin_nontemporal_load(&cols[k0 + k]) : -1;
+      v[i] = k < nz ? __builtin_nontemporal_load(&vals[k0 + k]) : 0.0;
+    }
+#pragma unroll
+    for (int i = 0; i < PER; i++) {
+      const int k = tid + i * kStreamBlock;
+      if (c[i] >= 0) prod[k] = v[i] * x[c[i]];
+    }
+    __syncthreads();
+    const int nr = r1 - r0;
+    int tpr = 64;
+    while (tpr > 1 && tpr * nr > kStreamBlock) tpr >>= 1;
+    const int l = tid & (tpr - 1);
+    for (int j0 = 0; j0 < nr; j0 += kStreamBlock / tpr) {
+      const int j = j0 + tid / tpr;
+      double sum = 0.0;
+      if (j < nr) {
+        const int s0 = rowptr[r0 + j] - k0, s1 = rowptr[r0 + j + 1] - k0;
+        for (int k = s0 + l; k < s1; k += tpr) sum += prod[k];
+      }
+      for (int off = tpr / 2; off > 0; off >>= 1) sum += __shfl_xor(sum, off, 64);
+      if (j < nr && l == 0) {
+        if (ACCUM) y[r0 + j] += s * sum; else y[r0 + j] = s * sum;
+        if (DOT) dotacc += x[r0 + j] * sum;
+      }
+    }
+    __syncthreads();
+  }
+  if (DOT) StorePartial<kStreamBlock>(dotacc, partials, lds);
 }
 
 CsrOp::CsrOp(int64_t h, int64_t w, const int64_t* rp, const int* c, const double* v)
@@ -104,11 +180,34 @@ CsrOp::CsrOp(int64_t h, int64_t w, const int64_t* rp, const int* c, const double
   NGSAMD_HIP(hipStreamSynchronize(Stream()));
   double mean = h ? double(nnz) / double(h) : 0.0;
   group = mean > 48 ? 32 : mean > 20 ? 16 : mean > 9 ? 8 : 4;
+  // row blocks of at most block_nnz nonzeros (or one longer row)
+  const char* nnz_env = getenv("NGSAMD_SPMV_NNZ");
+  block_nnz = nnz_env ? atoi(nnz_env) : 2048;
+  NGSAMD_REQUIRE(block_nnz == 1024 || block_nnz == 2048 || block_nnz == 4096,
+                 "NGSAMD_SPMV_NNZ must be 1024, 2048 or 4096");
+  std::vector<int> br;
+  br.push_back(0);
+  int64_t acc = 0;
+  for (int64_t r = 0; r < h; r++) {
+    const int64_t len = rp[r + 1] - rp[r];
+    if (r > br.back() && acc + len > block_nnz) {
+      br.push_back((int)r);
+      acc = 0;
+    }
+    acc += len;
+  }
+  if (br.back() != h || h == 0) br.push_back((int)h);
+  nblocks = (int)br.size() - 1;
+  blk_row.alloc(br.size());
+  blk_row.upload(br.data(), br.size());
+  NGSAMD_HIP(hipStreamSynchronize(Stream()));
+  const char* algo_env = getenv("NGSAMD_SPMV");
+  algo = (algo_env && std::string(algo_env) == "rowgroup") ? 0 : 1;
 }
 
 template <int G, bool ACCUM, bool DOT>
-static void LaunchSpmvG(const CsrOp& A, double s, const double* x, double* y, hipStream_t st, double* result,
-                        const int* stop) {
+static int LaunchSpmvG(const CsrOp& A, double s, const double* x, double* y, hipStream_t st, double* partials,
+                       const int* stop) {
   constexpr int RPB = kSpmvBlock / G;
   const int64_t n = A.height;
   const int64_t passes = (n + RPB - 1) / RPB;
@@ -119,21 +218,37 @@ static void LaunchSpmvG(const CsrOp& A, double s, const double* x, double* y, hi
   if (wg > 8) wg &= ~int64_t(7);
   if (wg < 1) wg = 1;
   const int iters = (int)((passes + wg - 1) / wg);
-  auto& S = State();
-  CsrSpmvKernel<G, ACCUM, DOT><<<(int)wg, kSpmvBlock, 0, st>>>(
-      A.rowptr.p, A.cols.p, A.vals.p, x, y, (int)n, s, RPB, iters, S.partials,
-      S.counters + kCounterCG1, result, stop);
+  CsrSpmvKernel<G, ACCUM, DOT><<<(int)wg, kSpmvBlock, 0, st>>>(A.rowptr.p, A.cols.p, A.vals.p, x, y, (int)n, s,
+                                                               iters, partials, stop);
   NGSAMD_CHECK_LAUNCH();
+  return (int)wg;
 }
 
 template <bool ACCUM, bool DOT>
-static void LaunchSpmv(const CsrOp& A, double s, const double* x, double* y, hipStream_t st, double* result,
-                       const int* stop) {
+static int LaunchStream(const CsrOp& A, double s, const double* x, double* y, hipStream_t st, double* partials,
+                        const int* stop) {
+  int64_t wg = A.nblocks < kMaxReduceBlocks ? A.nblocks : kMaxReduceBlocks;
+  if (wg > 8) wg &= ~int64_t(7);
+  if (wg < 1) wg = 1;
+  const int per = (int)((A.nblocks + wg - 1) / wg);
+  auto kern = A.block_nnz == 1024   ? CsrStreamKernel<1024, ACCUM, DOT>
+              : A.block_nnz == 4096 ? CsrStreamKernel<4096, ACCUM, DOT>
+                                    : CsrStreamKernel<2048, ACCUM, DOT>;
+  kern<<<(int)wg, kStreamBlock, 0, st>>>(A.rowptr.p, A.cols.p, A.vals.p, x, y, A.blk_row.p, A.nblocks, per, s,
+                                         partials, stop);
+  NGSAMD_CHECK_LAUNCH();
+  return (int)wg;
+}
+
+template <bool ACCUM, bool DOT>
+static int LaunchSpmv(const CsrOp& A, double s, const double* x, double* y, hipStream_t st, double* partials,
+                      const int* stop) {
+  if (A.algo == 1) return LaunchStream<ACCUM, DOT>(A, s, x, y, st, partials, stop);
   switch (A.group) {
-    case 4: LaunchSpmvG<4, ACCUM, DOT>(A, s, x, y, st, result, stop); break;
-    case 8: LaunchSpmvG<8, ACCUM, DOT>(A, s, x, y, st, result, stop); break;
-    case 16: LaunchSpmvG<16, ACCUM, DOT>(A, s, x, y, st, result, stop); break;
-    default: LaunchSpmvG<32, ACCUM, DOT>(A, s, x, y, st, result, stop); break;
+    case 4: return LaunchSpmvG<4, ACCUM, DOT>(A, s, x, y, st, partials, stop);
+    case 8: return LaunchSpmvG<8, ACCUM, DOT>(A, s, x, y, st, partials, stop);
+    case 16: return LaunchSpmvG<16, ACCUM, DOT>(A, s, x, y, st, partials, stop);
+    default: return LaunchSpmvG<32, ACCUM, DOT>(A, s, x, y, st, partials, stop);
   }
 }
 
@@ -147,8 +262,8 @@ void CsrOp::Mult(const double* x, double* y, hipStream_t st) {
   LaunchSpmv<false, false>(*this, 1.0, x, y, st, nullptr, nullptr);
 }
 
-void CsrOp::MultDot(const double* x, double* y, double* d_result, const int* d_stop, hipStream_t st) {
-  LaunchSpmv<false, true>(*this, 1.0, x, y, st, d_result, d_stop);
+int CsrOp::MultDot(const double* x, double* y, double* partials, const int* d_stop, hipStream_t st) {
+  return LaunchSpmv<false, true>(*this, 1.0, x, y, st, partials, d_stop);
 }
 
 __global__ void CsrDiagKernel(const int* __restrict__ rowptr, const int* __restrict__ cols,

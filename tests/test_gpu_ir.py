@@ -1,6 +1,6 @@
-"""Mixed-precision iterative refinement: FP64 residual around an FP32
-matrix-free inner CG reaches 1e-10 relative residual and agrees with the
-FP64 solution."""
+"""Mixed-precision iterative refinement: FP64 residual around an FP32 or
+BF16 matrix-free inner CG reaches 1e-10 relative residual and agrees with
+the FP64 solution."""
 import numpy as np
 import pytest
 
@@ -11,12 +11,13 @@ from helpers import N, dvec, mf_operator, problem, say
 pytestmark = pytest.mark.gpu
 
 
+@pytest.mark.parametrize("precision", [1, 2])
 @pytest.mark.parametrize("dim,n,p,order", [(3, 4, 4, 8), (2, 32, 3, 6)])
-def test_ir_fp32_reaches_1e10(device, dim, n, p, order):
-    say(f"[ir] dim={dim} n={n} p={p}")
+def test_ir_reaches_1e10(device, precision, dim, n, p, order):
+    say(f"[ir prec={precision}] dim={dim} n={n} p={p}")
     P = problem(dim, n, p, order=order, seed_kappa=2)
     A64 = N.CreateCSR(P["ndof"], P["ndof"], P["rowptr"], P["cols"], P["vals"])
-    A32 = mf_operator(P, precision=1)
+    A32 = mf_operator(P, precision=precision)
     _, inv = O.jacobi_invdiag(P["rowptr"], P["cols"], P["vals"], P["fes"].freedofs)
     diag = 1.0 / np.where(inv == 0, 1, inv)
     C = N.CreateJacobi(diag, P["fes"].freedofs)
