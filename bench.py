@@ -86,6 +86,53 @@ def cpu_baseline(rowptr, cols, vals, x, ndof, budget_s=10.0):
                       f"oracle C SpMV with OpenMP, {el:.1f} s"}
 
 
+def bench_batch(N, args):
+    """Batching configuration: 512 independent 2D Poisson problems (seed 5:
+    nx ~ U{8..96}, p ~ U{1..5}, kappa_e = exp(N(0,1)), rhs ~ U[-1,1]) solved
+    together to 1e-10, FP64 block CG and mixed (FP32 inner, FP64 refinement),
+    against solving them one after the other with the FP64 device CG."""
+    from ngsolve_amd.batch import BatchSolver, diagonal, make_problems
+    t = time.perf_counter()
+    probs = make_problems(args.batch_problems, seed=5)
+    bs = BatchSolver(probs)
+    log(f"[bench] batch setup {time.perf_counter() - t:.1f}s ndof={bs.n} nnz={bs.nnz} block_nnz={bs.block_nnz}")
+    out = {"problems": len(probs), "total_dofs": bs.n, "total_nnz": bs.nnz, "autotuned_block_nnz": bs.block_nnz}
+    for name, fn in [("fp64_block_cg", lambda: bs.solve_fp64(tol=1e-10)),
+                     ("mixed_block_ir", lambda: bs.solve_mixed(tol=1e-10))]:
+        N.synchronize()
+        t0 = time.perf_counter()
+        r = fn()
+        el = time.perf_counter() - t0
+        ms = r["finish_ms"]
+        out[name] = {"seconds": round(el, 4), "problems_per_s": round(len(probs) / el, 1),
+                     "p50_ms": round(float(np.percentile(ms, 50)), 3),
+                     "p99_ms": round(float(np.percentile(ms, 99)), 3)}
+        if "iterations" in r:
+            out[name]["max_iterations"] = int(r["iterations"].max())
+        else:
+            out[name]["max_outer"] = int(r["outer"].max())
+            out[name]["max_relres"] = float(r["relres"].max())
+        log(f"[bench] batch {name}: {el:.3f}s p50 {out[name]['p50_ms']} ms p99 {out[name]['p99_ms']} ms")
+    # the same problems one after the other (FP64 CSR + Jacobi device CG)
+    seq = []
+    for pr in probs:
+        A = N.CreateCSR(pr.ndof, pr.ndof, pr.rowptr, pr.cols, pr.vals)
+        C = N.CreateJacobi(diagonal(pr.rowptr, pr.cols, pr.vals), pr.free)
+        f, u = N.DeviceVector(pr.ndof, 8), N.DeviceVector(pr.ndof, 8)
+        f.upload(pr.rhs)
+        N.synchronize()
+        t0 = time.perf_counter()
+        N.CGSolve(A, C, f, u, True, 1e-10, -1.0, 100000, 16)
+        seq.append(time.perf_counter() - t0)
+    seq = np.array(seq)
+    out["sequential_fp64_cg"] = {"seconds": round(float(seq.sum()), 4),
+                                 "problems_per_s": round(len(probs) / seq.sum(), 1),
+                                 "p50_ms": round(float(np.percentile(seq, 50) * 1e3), 3),
+                                 "p99_ms": round(float(np.percentile(seq, 99) * 1e3), 3)}
+    log(f"[bench] sequential: {seq.sum():.3f}s")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -94,6 +141,11 @@ def main():
     ap.add_argument("--n", type=int, default=32)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-cg", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-problems", type=int, default=512)
+    ap.add_argument("--profile-mode", action="store_true",
+                    help="short run for rocprofv3 counter passes: a few CSR and matrix-free applies and one "
+                         "1 GiB device-to-device copy (the calibration kernel of known size); no JSON line")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (see DESIGN.md for how the "
@@ -118,6 +170,20 @@ def main():
     x = np.random.default_rng(1).uniform(-1, 1, ndof)
     dx, dy = N.DeviceVector(ndof, 8), N.DeviceVector(ndof, 8)
     dx.upload(x)
+
+    if args.profile_mode:
+        for _ in range(3):
+            A.mult(dx, dy)
+        for prec in (0, 1, 2):
+            M = build_mf(N, fes, mesh, ir, None, prec)
+            for _ in range(3):
+                M.mult(dx, dy)
+            del M
+        big_a, big_b = N.DeviceVector(1 << 27, 8), N.DeviceVector(1 << 27, 8)  # 1 GiB each
+        big_b.copy_from(big_a)
+        N.synchronize()
+        log("[bench] profile mode done")
+        return
 
     # ---------------------------------------------------- CSR SpMV (value)
     ms_csr = timed(N, lambda: A.mult(dx, dy), args.steps, args.warmup)
@@ -217,6 +283,9 @@ def main():
             res["best_mixed"] = best[0]
             res["speedup_best_mixed_vs_fp64_csr"] = round(t_k / best[1], 3)
         extra["cg_lognormal_kappa"] = res
+
+    if not args.no_batch:
+        extra["batch_512_problems"] = bench_batch(N, args)
 
     cpu = None
     if not args.no_cpu:

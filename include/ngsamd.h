@@ -154,6 +154,29 @@ int ngsamd_ir_cg_solve(ngsamd_op_t mat, ngsamd_op_t inner, ngsamd_op_t pre, cons
                        double* d_sol, double tol, double inner_tol, int max_outer, int max_inner,
                        double* h_stats);
 
+/* ------------------------------------------------ batched independent solves */
+/* Many independent problems as one block-diagonal CSR system: problem p owns
+ * rows/columns [h_seg[p], h_seg[p+1]).  with_f32 keeps an FP32 copy of the
+ * values for the inner solves of ngsamd_batch_ir_cg_solve.  The nonzeros per
+ * SpMV row block are autotuned when the operator is created.
+ * Reference: CGSolver per problem (python/krylovspace.py:227-291) and the
+ * lockstep multi-RHS CGSolver<double>::MultiMult (linalg/cg.cpp:132-250). */
+int ngsamd_batch_csr_create(ngsamd_op_t* op, int64_t n, const int64_t* h_rowptr, const int* h_cols,
+                            const double* h_vals, int nseg, const int64_t* h_seg, int with_f32);
+int ngsamd_batch_info(ngsamd_op_t op, int* nseg, int* block_nnz, int* nblocks);
+/* Jacobi-PCG on every problem with the CGSolver recurrence and stopping rule
+ * per problem (tol relative to each problem's initial residual).  Per problem:
+ * h_iterations (checks incl. the initial one), h_final_res (last residual),
+ * h_finish_ms (device time from the start of the call to its convergence). */
+int ngsamd_batch_cg_solve(ngsamd_op_t op, ngsamd_op_t pre, const double* d_rhs, double* d_sol, double tol,
+                          int maxiter, int chunk, int* h_iterations, double* h_final_res, double* h_finish_ms);
+/* FP64 iterative refinement per problem around FP32 batched inner solves
+ * (FP32 matrix values and vectors, FP64 inner products and scalars) until
+ * ||f_p - A_p u_p||_free <= tol ||f_p||_free for every problem. */
+int ngsamd_batch_ir_cg_solve(ngsamd_op_t op, ngsamd_op_t pre, const double* d_rhs, double* d_sol, double tol,
+                             double inner_tol, int max_outer, int max_inner, int* h_outer, int* h_inner,
+                             double* h_relres, double* h_finish_ms);
+
 #ifdef __cplusplus
 }
 #endif

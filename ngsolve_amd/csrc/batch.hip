@@ -1,0 +1,804 @@
+// Batched CG for many independent problems of uneven size (one block-diagonal
+// system, one set of launches per iteration for all problems).
+//
+// Reference: CGSolver (python/krylovspace.py:227-291) applied to each problem
+// separately, and the lockstep multi-right-hand-side CG
+// CGSolver<double>::MultiMult (linalg/cg.cpp:132-250), generalised to
+// independent operators: every problem keeps its own alpha, beta, residual
+// history, stopping test and iteration count; a converged problem is frozen
+// (its row blocks and vector chunks return at once) while the others go on.
+//
+// Layout: the problems' CSR matrices are stacked into one block-diagonal CSR
+// (int32 row pointers / columns, FP64 values and optionally an FP32 copy of
+// the values for the inner solves of iterative refinement).  Row blocks of
+// the stream SpMV never straddle two problems, so each block's <s, A s>
+// partial belongs to one problem; vectors are cut into chunks of at most
+// kChunk dofs inside one problem for the update kernels.  Per-problem sums
+// are formed by one workgroup per problem over its contiguous range of
+// partials (deterministic).
+//
+// Tiles: the nonzeros per row block (1024/2048/4096) are picked when the
+// operator is created by timing each candidate on the device.
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+#include "csr.hpp"
+#include "reduce.hpp"
+#include "runtime_state.hpp"
+
+namespace ngsamd {
+
+constexpr int kBatchBlock = 256;
+constexpr int kChunk = 4096;  // dofs per vector chunk
+
+struct RowBlocks {
+  std::vector<int> row;  // nblocks+1
+  std::vector<int> seg;  // nblocks
+  std::vector<int> seg_begin;  // nseg+1: first block of each problem
+};
+
+static RowBlocks MakeRowBlocks(const int64_t* rp, const std::vector<int64_t>& seg_rows, int block_nnz) {
+  RowBlocks b;
+  const int nseg = (int)seg_rows.size() - 1;
+  for (int p = 0; p < nseg; p++) {
+    b.seg_begin.push_back((int)b.seg.size());
+    int64_t start = seg_rows[p];
+    int64_t acc = 0;
+    if (seg_rows[p + 1] == seg_rows[p]) continue;
+    b.row.push_back((int)start);
+    b.seg.push_back(p);
+    for (int64_t r = seg_rows[p]; r < seg_rows[p + 1]; r++) {
+      const int64_t len = rp[r + 1] - rp[r];
+      if (r > start && acc + len > block_nnz) {
+        b.row.push_back((int)r);
+        b.seg.push_back(p);
+        start = r;
+        acc = 0;
+      }
+      acc += len;
+    }
+  }
+  b.seg_begin.push_back((int)b.seg.size());
+  b.row.push_back((int)seg_rows[nseg]);
+  return b;
+}
+
+struct BatchOp : ngsamd_op_s {
+  DevBuf<int> rowptr, cols;
+  DevBuf<double> vals;
+  DevBuf<float> vals32;
+  int nseg = 0;
+  std::vector<int64_t> seg_rows;
+  int block_nnz = 2048;
+  int nblocks = 0;
+  DevBuf<int> blk_row, blk_seg, seg_blk;
+  int nchunks = 0;
+  DevBuf<int64_t> chunk_start;
+  DevBuf<int> chunk_seg, seg_chunk;
+  DevBuf<double> partials;
+  BatchOp(int64_t n, const int64_t* rp, const int* c, const double* v, int nseg, const int64_t* seg, bool with_f32);
+  void SetBlocks(const RowBlocks& b);
+  void MultAdd(double s, const double* x, double* y, hipStream_t st) override;
+  void Mult(const double* x, double* y, hipStream_t st) override;
+};
+
+// ------------------------------------------------------------------ SpMV
+
+__device__ __forceinline__ int XcdBlock(int b, int grid) {
+  if ((grid & 7) != 0) return b;
+  return (b & 7) * (grid >> 3) + (b >> 3);
+}
+
+// y = A x (ACCUM: y += s A x) over row blocks; with DOT the partial
+// sum_r x[r] (A x)[r] of every row block goes to partials[block]; blocks of
+// problems with done[seg] != 0 are skipped.
+template <int NNZ, class V, class M, bool ACCUM, bool DOT>
+__global__ __launch_bounds__(kBatchBlock) void BatchSpmvKernel(
+    const int* __restrict__ rowptr, const int* __restrict__ cols, const M* __restrict__ vals, const V* __restrict__ x,
+    V* __restrict__ y, double s, const int* __restrict__ blk_row, const int* __restrict__ blk_seg, int nblocks,
+    int blocks_per_wg, const int* __restrict__ done, double* __restrict__ partials) {
+  constexpr int PER = NNZ / kBatchBlock;
+  __shared__ double prod[NNZ];
+  __shared__ double lds[kBatchBlock / 64];
+  const int tid = threadIdx.x;
+  const int lb = XcdBlock(blockIdx.x, gridDim.x);
+  const int b0 = lb * blocks_per_wg;
+  const int b1 = min(b0 + blocks_per_wg, nblocks);
+  for (int b = b0; b < b1; b++) {
+    if (done && done[blk_seg[b]]) {
+      if (DOT && tid == 0) partials[b] = 0.0;
+      continue;
+    }
+    const int r0 = blk_row[b], r1 = blk_row[b + 1];
+    const int k0 = rowptr[r0], k1 = rowptr[r1];
+    const int nz = k1 - k0;
+    double dotacc = 0.0;
+    if (nz > NNZ) {
+      double a = 0.0;
+      for (int k = k0 + tid; k < k1; k += kBatchBlock)
+        a += (double)__builtin_nontemporal_load(&vals[k]) * (double)x[__builtin_nontemporal_load(&cols[k])];
+      a = BlockSum<kBatchBlock>(a, lds);
+      if (tid == 0) {
+        if (ACCUM) y[r0] = (V)((double)y[r0] + s * a); else y[r0] = (V)(s * a);
+        dotacc = (double)x[r0] * a;
+        if (DOT) partials[b] = dotacc;
+      }
+      continue;
+    }
+    int c[PER];
+    double v[PER];
+#pragma unroll
+    for (int i = 0; i < PER; i++) {
+      const int k = tid + i * kBatchBlock;
+      c[i] = k < nz ? __builtin_nontemporal_load(&cols[k0 + k]) : -1;
+      v[i] = k < nz ? (double)__builtin_nontemporal_load(&vals[k0 + k]) : 0.0;
+    }
+#pragma unroll
+    for (int i = 0; i < PER; i++) {
+      const int k = tid + i * kBatchBlock;
+      if (c[i] >= 0) prod[k] = v[i] * (double)x[c[i]];
+    }
+    __syncthreads();
+    const int nr = r1 - r0;
+    int tpr = 64;
+    while (tpr > 1 && tpr * nr > kBatchBlock) tpr >>= 1;
+    const int l = tid & (tpr - 1);
+    for (int j0 = 0; j0 < nr; j0 += kBatchBlock / tpr) {
+      const int j = j0 + tid / tpr;
+      double sum = 0.0;
+      if (j < nr) {
+        const int s0 = rowptr[r0 + j] - k0, s1 = rowptr[r0 + j + 1] - k0;
+        for (int k = s0 + l; k < s1; k += tpr) sum += prod[k];
+      }
+      for (int off = tpr / 2; off > 0; off >>= 1) sum += __shfl_xor(sum, off, 64);
+      if (j < nr && l == 0) {
+        if (ACCUM) y[r0 + j] = (V)((double)y[r0 + j] + s * sum); else y[r0 + j] = (V)(s * sum);
+        if (DOT) dotacc += (double)x[r0 + j] * sum;
+      }
+    }
+    if (DOT) {
+      dotacc = BlockSum<kBatchBlock>(dotacc, lds);
+      if (tid == 0) partials[b] = dotacc;
+    }
+    __syncthreads();
+  }
+}
+
+template <class V, class M, bool ACCUM, bool DOT>
+static void LaunchBatchSpmv(const BatchOp& A, const M* vals, const V* x, V* y, double s, const int* done,
+                            hipStream_t st, int block_nnz) {
+  int64_t wg = A.nblocks < 8192 ? A.nblocks : 8192;
+  if (wg > 8) wg &= ~int64_t(7);
+  if (wg < 1) wg = 1;
+  const int per = (int)((A.nblocks + wg - 1) / wg);
+  auto kern = block_nnz == 1024   ? BatchSpmvKernel<1024, V, M, ACCUM, DOT>
+              : block_nnz == 4096 ? BatchSpmvKernel<4096, V, M, ACCUM, DOT>
+                                  : BatchSpmvKernel<2048, V, M, ACCUM, DOT>;
+  kern<<<(int)wg, kBatchBlock, 0, st>>>(A.rowptr.p, A.cols.p, vals, x, y, s, A.blk_row.p, A.blk_seg.p, A.nblocks,
+                                        per, done, A.partials.p);
+  NGSAMD_CHECK_LAUNCH();
+}
+
+void BatchOp::SetBlocks(const RowBlocks& b) {
+  nblocks = (int)b.seg.size();
+  blk_row.alloc(b.row.size());
+  blk_row.upload(b.row.data(), b.row.size());
+  blk_seg.alloc(std::max<size_t>(b.seg.size(), 1));
+  blk_seg.upload(b.seg.data(), b.seg.size());
+  seg_blk.alloc(b.seg_begin.size());
+  seg_blk.upload(b.seg_begin.data(), b.seg_begin.size());
+}
+
+BatchOp::BatchOp(int64_t n, const int64_t* rp, const int* c, const double* v, int ns, const int64_t* seg,
+                 bool with_f32)
+    : ngsamd_op_s(OpKind::Batch) {
+  height = width = n;
+  nseg = ns;
+  NGSAMD_REQUIRE(n >= 0 && n < (int64_t(1) << 31), "ngsamd_batch_csr_create: size out of range");
+  NGSAMD_REQUIRE(ns >= 1 && seg[0] == 0 && seg[ns] == n, "ngsamd_batch_csr_create: bad problem offsets");
+  const int64_t nnz = rp[n];
+  NGSAMD_REQUIRE(rp[0] == 0 && nnz < (int64_t(1) << 31), "ngsamd_batch_csr_create: bad row pointer");
+  seg_rows.assign(seg, seg + ns + 1);
+  for (int p = 0; p < ns; p++) {
+    NGSAMD_REQUIRE(seg[p + 1] >= seg[p], "ngsamd_batch_csr_create: problem offsets not monotone");
+    // block-diagonal: every column of a problem's rows lies inside the problem
+    for (int64_t r = seg[p]; r < seg[p + 1]; r++)
+      for (int64_t k = rp[r]; k < rp[r + 1]; k++)
+        NGSAMD_REQUIRE(c[k] >= seg[p] && c[k] < seg[p + 1], "ngsamd_batch_csr_create: matrix is not block-diagonal");
+  }
+  std::vector<int> rp32(n + 1);
+  for (int64_t i = 0; i <= n; i++) rp32[i] = (int)rp[i];
+  rowptr.alloc(n + 1);
+  rowptr.upload(rp32.data(), n + 1);
+  cols.alloc(nnz);
+  cols.upload(c, nnz);
+  vals.alloc(nnz);
+  vals.upload(v, nnz);
+  if (with_f32) {
+    std::vector<float> v32(v, v + nnz);
+    vals32.alloc(nnz);
+    vals32.upload(v32.data(), nnz);
+  }
+  // vector chunks
+  std::vector<int64_t> cs;
+  std::vector<int> csg, sgc;
+  for (int p = 0; p < ns; p++) {
+    sgc.push_back((int)csg.size());
+    for (int64_t a = seg[p]; a < seg[p + 1]; a += kChunk) {
+      cs.push_back(a);
+      csg.push_back(p);
+    }
+  }
+  sgc.push_back((int)csg.size());
+  cs.push_back(n);
+  nchunks = (int)csg.size();
+  chunk_start.alloc(cs.size());
+  chunk_start.upload(cs.data(), cs.size());
+  chunk_seg.alloc(std::max<size_t>(csg.size(), 1));
+  chunk_seg.upload(csg.data(), csg.size());
+  seg_chunk.alloc(sgc.size());
+  seg_chunk.upload(sgc.data(), sgc.size());
+
+  // autotune the row-block size: time each candidate on the device
+  hipStream_t st = Stream();
+  const char* env = getenv("NGSAMD_SPMV_NNZ");
+  std::vector<int> cand = env ? std::vector<int>{atoi(env)} : std::vector<int>{1024, 2048, 4096};
+  int best = cand[0];
+  float best_ms = 1e30f;
+  RowBlocks best_blocks;
+  size_t max_blocks = (size_t)nchunks;
+  DevBuf<double> tx(std::max<int64_t>(n, 1)), ty(std::max<int64_t>(n, 1));
+  LaunchFill(tx.p, n, 1.0, st);
+  hipEvent_t e0, e1;
+  NGSAMD_HIP(hipEventCreate(&e0));
+  NGSAMD_HIP(hipEventCreate(&e1));
+  for (int bn : cand) {
+    NGSAMD_REQUIRE(bn == 1024 || bn == 2048 || bn == 4096, "NGSAMD_SPMV_NNZ must be 1024, 2048 or 4096");
+    RowBlocks rb = MakeRowBlocks(rp, seg_rows, bn);
+    SetBlocks(rb);
+    partials.alloc(std::max<size_t>({rb.seg.size(), (size_t)nchunks, (size_t)1}));
+    float ms = 0.f;
+    if (cand.size() > 1 && n > 0) {
+      LaunchBatchSpmv<double, double, false, true>(*this, vals.p, tx.p, ty.p, 1.0, nullptr, st, bn);
+      NGSAMD_HIP(hipEventRecord(e0, st));
+      for (int k = 0; k < 3; k++)
+        LaunchBatchSpmv<double, double, false, true>(*this, vals.p, tx.p, ty.p, 1.0, nullptr, st, bn);
+      NGSAMD_HIP(hipEventRecord(e1, st));
+      NGSAMD_HIP(hipEventSynchronize(e1));
+      NGSAMD_HIP(hipEventElapsedTime(&ms, e0, e1));
+    }
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = bn;
+      best_blocks = rb;
+    }
+    max_blocks = std::max(max_blocks, rb.seg.size());
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  block_nnz = best;
+  SetBlocks(best_blocks);
+  partials.alloc(std::max<size_t>(max_blocks, 1));
+  NGSAMD_HIP(hipStreamSynchronize(st));
+}
+
+void BatchOp::MultAdd(double s, const double* x, double* y, hipStream_t st) {
+  if (height == 0) return;
+  LaunchBatchSpmv<double, double, true, false>(*this, vals.p, x, y, s, nullptr, st, block_nnz);
+}
+
+void BatchOp::Mult(const double* x, double* y, hipStream_t st) {
+  if (height == 0) return;
+  LaunchBatchSpmv<double, double, false, false>(*this, vals.p, x, y, 1.0, nullptr, st, block_nnz);
+}
+
+// ------------------------------------------------------- per-problem state
+
+template <class T>
+__global__ void FillKernelT(T* __restrict__ x, int64_t n, T a) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    x[i] = a;
+}
+
+template <class T>
+static void Fill(T* x, int64_t n, T a, hipStream_t st) {
+  if (n <= 0) return;
+  FillKernelT<T><<<GridFor(n, 256, 8192), 256, 0, st>>>(x, n, a);
+  NGSAMD_CHECK_LAUNCH();
+}
+
+struct BatchState {
+  double* wdn;
+  double* alpha;
+  double* beta;
+  double* final_res;
+  double* tol;        // per problem relative tolerance
+  double* last_res;
+  long long* finish;  // device wall clock when the problem finished
+  int* it;
+  int* done;          // 0 running, 1 converged / maxiter, 2 breakdown
+  int maxiter;
+};
+
+struct BatchWork {
+  int nseg;
+  DevBuf<double> wdn, alpha, beta, final_res, tol, last_res;
+  DevBuf<long long> finish;
+  DevBuf<int> it, done;
+  explicit BatchWork(int n)
+      : nseg(n), wdn(n), alpha(n), beta(n), final_res(n), tol(n), last_res(n), finish(n), it(n), done(n) {}
+  BatchState state(int maxiter) {
+    return BatchState{wdn.p, alpha.p, beta.p, final_res.p, tol.p, last_res.p, finish.p, it.p, done.p, maxiter};
+  }
+};
+
+__device__ __forceinline__ long long DeviceClock() { return (long long)wall_clock64(); }
+
+// reset the state of the problems with active[p] != 0 (all if active == null)
+__global__ void BatchResetKernel(BatchState S, int nseg, const double* tol, double tol_all, const int* active) {
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < nseg; p += gridDim.x * blockDim.x) {
+    const bool act = active == nullptr || active[p] == 0;
+    S.it[p] = 0;
+    S.done[p] = act ? 0 : 1;
+    S.tol[p] = tol ? tol[p] : tol_all;
+    S.finish[p] = 0;
+  }
+}
+
+template <class V>
+__global__ __launch_bounds__(kBatchBlock) void BatchInitKernel(const V* __restrict__ inv, const V* __restrict__ d,
+                                                               V* __restrict__ w, V* __restrict__ s,
+                                                               const int64_t* __restrict__ cstart,
+                                                               const int* __restrict__ cseg, BatchState S,
+                                                               double* __restrict__ partials) {
+  __shared__ double lds[kBatchBlock / 64];
+  const int c = blockIdx.x;
+  if (S.done[cseg[c]]) {
+    if (threadIdx.x == 0) partials[c] = 0.0;
+    return;
+  }
+  double acc = 0.0;
+  for (int64_t i = cstart[c] + threadIdx.x; i < cstart[c + 1]; i += kBatchBlock) {
+    const V di = d[i];
+    const V wi = inv[i] * di;
+    w[i] = wi;
+    s[i] = wi;
+    acc += (double)wi * (double)di;
+  }
+  StorePartial<kBatchBlock>(acc, partials, lds);
+}
+
+__device__ __forceinline__ void BatchCheck(BatchState& S, int p, double res) {
+  const int it = ++S.it[p];
+  S.last_res[p] = res;
+  if (it == 1) S.final_res[p] = res * S.tol[p];
+  if (it >= S.maxiter || res <= S.final_res[p]) {
+    S.done[p] = 1;
+    S.finish[p] = DeviceClock();
+  }
+}
+
+// one workgroup per problem: sum of partials[first[p] .. first[p+1])
+__device__ __forceinline__ double SegmentSum(const double* partials, const int* first, int p, double* lds) {
+  double v = 0.0;
+  for (int i = first[p] + threadIdx.x; i < first[p + 1]; i += kBatchBlock) v += partials[i];
+  return BlockSum<kBatchBlock>(v, lds);
+}
+
+__global__ __launch_bounds__(kBatchBlock) void BatchInitCheckKernel(const double* __restrict__ partials,
+                                                                    const int* __restrict__ seg_chunk, BatchState S) {
+  __shared__ double lds[kBatchBlock / 64];
+  const int p = blockIdx.x;
+  if (S.done[p]) return;
+  const double wdn = SegmentSum(partials, seg_chunk, p, lds);
+  if (threadIdx.x == 0) {
+    S.wdn[p] = wdn;
+    BatchCheck(S, p, sqrt(fabs(wdn)));
+  }
+}
+
+__global__ __launch_bounds__(kBatchBlock) void BatchAlphaKernel(const double* __restrict__ partials,
+                                                                const int* __restrict__ seg_blk, BatchState S) {
+  __shared__ double lds[kBatchBlock / 64];
+  const int p = blockIdx.x;
+  if (S.done[p]) return;
+  const double as_s = SegmentSum(partials, seg_blk, p, lds);
+  if (threadIdx.x == 0) {
+    const double wd = S.wdn[p];
+    if (as_s == 0.0 || wd == 0.0) {
+      S.done[p] = 2;
+      S.finish[p] = DeviceClock();
+    } else {
+      S.alpha[p] = wd / as_s;
+    }
+  }
+}
+
+template <class V>
+__global__ __launch_bounds__(kBatchBlock) void BatchUpdateKernel(const V* __restrict__ inv, V* __restrict__ u,
+                                                                 V* __restrict__ d, V* __restrict__ w,
+                                                                 const V* __restrict__ s,
+                                                                 const int64_t* __restrict__ cstart,
+                                                                 const int* __restrict__ cseg, BatchState S,
+                                                                 double* __restrict__ partials) {
+  __shared__ double lds[kBatchBlock / 64];
+  const int c = blockIdx.x;
+  const int p = cseg[c];
+  if (S.done[p]) {
+    if (threadIdx.x == 0) partials[c] = 0.0;
+    return;
+  }
+  const V al = (V)S.alpha[p];
+  double acc = 0.0;
+  for (int64_t i = cstart[c] + threadIdx.x; i < cstart[c + 1]; i += kBatchBlock) {
+    const V si = s[i], wi = w[i];
+    u[i] += al * si;
+    const V di = d[i] - al * wi;
+    d[i] = di;
+    const V zi = inv[i] * di;
+    w[i] = zi;
+    acc += (double)zi * (double)di;
+  }
+  StorePartial<kBatchBlock>(acc, partials, lds);
+}
+
+__global__ __launch_bounds__(kBatchBlock) void BatchCheckKernel(const double* __restrict__ partials,
+                                                                const int* __restrict__ seg_chunk, BatchState S) {
+  __shared__ double lds[kBatchBlock / 64];
+  const int p = blockIdx.x;
+  if (S.done[p]) return;
+  const double wdn = SegmentSum(partials, seg_chunk, p, lds);
+  if (threadIdx.x == 0) {
+    const double wd = S.wdn[p];
+    S.wdn[p] = wdn;
+    BatchCheck(S, p, sqrt(fabs(wdn)));
+    S.beta[p] = wdn / wd;
+  }
+}
+
+template <class V>
+__global__ __launch_bounds__(kBatchBlock) void BatchDirectionKernel(V* __restrict__ s, const V* __restrict__ w,
+                                                                    const int64_t* __restrict__ cstart,
+                                                                    const int* __restrict__ cseg, BatchState S) {
+  const int c = blockIdx.x;
+  const int p = cseg[c];
+  if (S.done[p]) return;
+  const V beta = (V)S.beta[p];
+  for (int64_t i = cstart[c] + threadIdx.x; i < cstart[c + 1]; i += kBatchBlock) s[i] = beta * s[i] + w[i];
+}
+
+// all problems done?  one workgroup
+__global__ __launch_bounds__(kBatchBlock) void BatchAllDoneKernel(const int* __restrict__ done, int nseg,
+                                                                  int* __restrict__ all) {
+  __shared__ double lds[kBatchBlock / 64];
+  double open = 0.0;
+  for (int p = threadIdx.x; p < nseg; p += kBatchBlock) open += done[p] ? 0.0 : 1.0;
+  open = BlockSum<kBatchBlock>(open, lds);
+  if (threadIdx.x == 0) *all = open == 0.0 ? 1 : 0;
+}
+
+template <class V, class M>
+static void EnqueueBatchIteration(BatchOp& A, const M* vals, const V* inv, V* u, V* d, V* w, V* s, BatchState S,
+                                  int* all_done, hipStream_t st) {
+  LaunchBatchSpmv<V, M, false, true>(A, vals, s, w, 1.0, S.done, st, A.block_nnz);
+  BatchAlphaKernel<<<A.nseg, kBatchBlock, 0, st>>>(A.partials.p, A.seg_blk.p, S);
+  NGSAMD_CHECK_LAUNCH();
+  BatchUpdateKernel<V><<<A.nchunks, kBatchBlock, 0, st>>>(inv, u, d, w, s, A.chunk_start.p, A.chunk_seg.p, S,
+                                                           A.partials.p);
+  NGSAMD_CHECK_LAUNCH();
+  BatchCheckKernel<<<A.nseg, kBatchBlock, 0, st>>>(A.partials.p, A.seg_chunk.p, S);
+  NGSAMD_CHECK_LAUNCH();
+  BatchDirectionKernel<V><<<A.nchunks, kBatchBlock, 0, st>>>(s, w, A.chunk_start.p, A.chunk_seg.p, S);
+  NGSAMD_CHECK_LAUNCH();
+  BatchAllDoneKernel<<<1, kBatchBlock, 0, st>>>(S.done, A.nseg, all_done);
+  NGSAMD_CHECK_LAUNCH();
+}
+
+// Runs the batched CG from d = rhs (u = 0) until every active problem is
+// done; returns the number of launched iterations.
+template <class V, class M>
+static long long RunBatchCg(BatchOp& A, const M* vals, const V* inv, const V* rhs, V* u, BatchWork& W,
+                            const double* d_tol, double tol, const int* d_frozen, int maxiter, int chunk,
+                            hipStream_t st) {
+  const int64_t n = A.height;
+  DevBuf<V> d(std::max<int64_t>(n, 1)), w(std::max<int64_t>(n, 1)), s(std::max<int64_t>(n, 1));
+  DevBuf<int> all(1);
+  BatchState S = W.state(maxiter);
+  BatchResetKernel<<<GridFor(A.nseg, 256, 64), 256, 0, st>>>(S, A.nseg, d_tol, tol, d_frozen);
+  NGSAMD_CHECK_LAUNCH();
+  if (n) {
+    NGSAMD_HIP(hipMemcpyAsync(d.p, rhs, n * sizeof(V), hipMemcpyDeviceToDevice, st));
+    Fill<V>(u, n, (V)0, st);
+  }
+  BatchInitKernel<V><<<A.nchunks, kBatchBlock, 0, st>>>(inv, d.p, w.p, s.p, A.chunk_start.p, A.chunk_seg.p, S,
+                                                         A.partials.p);
+  NGSAMD_CHECK_LAUNCH();
+  BatchInitCheckKernel<<<A.nseg, kBatchBlock, 0, st>>>(A.partials.p, A.seg_chunk.p, S);
+  NGSAMD_CHECK_LAUNCH();
+  BatchAllDoneKernel<<<1, kBatchBlock, 0, st>>>(S.done, A.nseg, all.p);
+  NGSAMD_CHECK_LAUNCH();
+
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  bool use_graph = chunk > 0;
+  if (use_graph) {
+    hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+      bool ok = true;
+      try {
+        for (int k = 0; k < chunk; k++) EnqueueBatchIteration<V, M>(A, vals, inv, u, d.p, w.p, s.p, S, all.p, st);
+      } catch (...) {
+        ok = false;
+      }
+      e = hipStreamEndCapture(st, &graph);
+      if (!ok && e == hipSuccess) e = hipErrorUnknown;
+      if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    }
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      if (graph) (void)hipGraphDestroy(graph);
+      graph = nullptr;
+      exec = nullptr;
+      use_graph = false;
+    }
+  }
+  struct GraphGuard {
+    hipGraph_t g;
+    hipGraphExec_t x;
+    ~GraphGuard() {
+      if (x) (void)hipGraphExecDestroy(x);
+      if (g) (void)hipGraphDestroy(g);
+    }
+  } guard{graph, exec};
+  int* pin = nullptr;
+  NGSAMD_HIP(hipHostMalloc((void**)&pin, 2 * sizeof(int), hipHostMallocDefault));
+  struct PinGuard {
+    int* p;
+    ~PinGuard() { (void)hipHostFree(p); }
+  } pguard{pin};
+  hipEvent_t ev[2];
+  NGSAMD_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+  NGSAMD_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+  struct EvGuard {
+    hipEvent_t* e;
+    ~EvGuard() { (void)hipEventDestroy(e[0]); (void)hipEventDestroy(e[1]); }
+  } eguard{ev};
+  const int per = use_graph ? chunk : 1;
+  long long launched = 0;
+  auto enqueue = [&](int slot) {
+    if (use_graph) NGSAMD_HIP(hipGraphLaunch(exec, st));
+    else EnqueueBatchIteration<V, M>(A, vals, inv, u, d.p, w.p, s.p, S, all.p, st);
+    launched += per;
+    NGSAMD_HIP(hipMemcpyAsync(&pin[slot], all.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    NGSAMD_HIP(hipEventRecord(ev[slot], st));
+  };
+  NGSAMD_HIP(hipMemcpyAsync(&pin[0], all.p, sizeof(int), hipMemcpyDeviceToHost, st));
+  NGSAMD_HIP(hipStreamSynchronize(st));
+  if (!pin[0]) {
+    enqueue(0);
+    enqueue(1);
+    int slot = 0;
+    const long long max_launches = (long long)maxiter / per + 4;
+    for (long long k = 0; k < max_launches; k++) {
+      NGSAMD_HIP(hipEventSynchronize(ev[slot]));
+      if (pin[slot]) break;
+      enqueue(slot);
+      slot ^= 1;
+    }
+  }
+  NGSAMD_HIP(hipStreamSynchronize(st));
+  return launched;
+}
+
+__global__ void ToF32ScaledSegKernel(const double* __restrict__ r, float* __restrict__ out,
+                                     const int64_t* __restrict__ cstart, const int* __restrict__ cseg,
+                                     const double* __restrict__ inv_norm, const int* __restrict__ frozen) {
+  const int c = blockIdx.x;
+  const int p = cseg[c];
+  const double sc = frozen[p] ? 0.0 : inv_norm[p];
+  for (int64_t i = cstart[c] + threadIdx.x; i < cstart[c + 1]; i += blockDim.x) out[i] = (float)(r[i] * sc);
+}
+
+__global__ void AddCorrectionSegKernel(double* __restrict__ u, const float* __restrict__ c32,
+                                       const int64_t* __restrict__ cstart, const int* __restrict__ cseg,
+                                       const double* __restrict__ norm, const int* __restrict__ frozen) {
+  const int c = blockIdx.x;
+  const int p = cseg[c];
+  if (frozen[p]) return;
+  const double sc = norm[p];
+  for (int64_t i = cstart[c] + threadIdx.x; i < cstart[c + 1]; i += blockDim.x) u[i] += sc * (double)c32[i];
+}
+
+// per chunk: partial sum over free dofs of r^2
+__global__ __launch_bounds__(kBatchBlock) void MaskedSqChunkKernel(const double* __restrict__ r,
+                                                                   const double* __restrict__ inv,
+                                                                   const int64_t* __restrict__ cstart,
+                                                                   double* __restrict__ partials) {
+  __shared__ double lds[kBatchBlock / 64];
+  const int c = blockIdx.x;
+  double acc = 0.0;
+  for (int64_t i = cstart[c] + threadIdx.x; i < cstart[c + 1]; i += kBatchBlock)
+    if (inv[i] != 0.0) acc += r[i] * r[i];
+  StorePartial<kBatchBlock>(acc, partials, lds);
+}
+
+// per problem: out[p] = sqrt(sum of its chunk partials)
+__global__ __launch_bounds__(kBatchBlock) void SegNormKernel(const double* __restrict__ partials,
+                                                             const int* __restrict__ seg_chunk,
+                                                             double* __restrict__ out) {
+  __shared__ double lds[kBatchBlock / 64];
+  const double v = SegmentSum(partials, seg_chunk, blockIdx.x, lds);
+  if (threadIdx.x == 0) out[blockIdx.x] = sqrt(v);
+}
+
+// per problem: norm, convergence against tol*fnorm, inner tolerance of the next step
+__global__ __launch_bounds__(kBatchBlock) void RefineCheckKernel(
+    const double* __restrict__ partials, const int* __restrict__ seg_chunk, const double* __restrict__ fnorm,
+    double tol, double inner_tol, int* __restrict__ frozen, double* __restrict__ rnorm, double* __restrict__ inv_rnorm,
+    double* __restrict__ itol, double* __restrict__ relres, long long* __restrict__ finish, int* __restrict__ outer,
+    int final_pass) {
+  __shared__ double lds[kBatchBlock / 64];
+  const int p = blockIdx.x;
+  const double rn = sqrt(SegmentSum(partials, seg_chunk, p, lds));
+  if (threadIdx.x == 0) {
+    if (frozen[p]) return;
+    const double fn = fnorm[p];
+    relres[p] = fn > 0 ? rn / fn : 0.0;
+    if (rn <= tol * fn || rn == 0.0 || final_pass) {
+      frozen[p] = 1;
+      finish[p] = DeviceClock();
+      return;
+    }
+    outer[p] += 1;
+    rnorm[p] = rn;
+    inv_rnorm[p] = 1.0 / rn;
+    double t = inner_tol;
+    const double need = 0.3 * tol * fn / rn;
+    if (need > t) t = need < 0.5 ? need : 0.5;
+    itol[p] = t;
+  }
+}
+
+}  // namespace ngsamd
+
+using namespace ngsamd;
+
+extern "C" int ngsamd_batch_csr_create(ngsamd_op_t* op, int64_t n, const int64_t* rp, const int* cols,
+                                       const double* vals, int nseg, const int64_t* seg, int with_f32) {
+  return Guard([&] {
+    RequireInit();
+    NGSAMD_REQUIRE(op && rp && seg && (rp[n] == 0 || (cols && vals)), "ngsamd_batch_csr_create: null argument");
+    *op = nullptr;
+    *op = new BatchOp(n, rp, cols, vals, nseg, seg, with_f32 != 0);
+  });
+}
+
+extern "C" int ngsamd_batch_info(ngsamd_op_t op, int* nseg, int* block_nnz, int* nblocks) {
+  return Guard([&] {
+    NGSAMD_REQUIRE(op && op->kind == OpKind::Batch, "ngsamd_batch_info: not a batched operator");
+    auto* A = static_cast<BatchOp*>(op);
+    if (nseg) *nseg = A->nseg;
+    if (block_nnz) *block_nnz = A->block_nnz;
+    if (nblocks) *nblocks = A->nblocks;
+  });
+}
+
+static void ReadClock(const long long* d_finish, long long start, int nseg, double* h_ms, hipStream_t st) {
+  std::vector<long long> fin(nseg);
+  NGSAMD_HIP(hipMemcpyAsync(fin.data(), d_finish, nseg * sizeof(long long), hipMemcpyDeviceToHost, st));
+  NGSAMD_HIP(hipStreamSynchronize(st));
+  int dev = 0, khz = 100000;
+  (void)hipGetDevice(&dev);
+  if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess || khz <= 0) khz = 100000;
+  for (int p = 0; p < nseg; p++) h_ms[p] = fin[p] > 0 ? double(fin[p] - start) / double(khz) : -1.0;
+}
+
+__global__ void ClockKernel(long long* out) { *out = DeviceClock(); }
+
+static long long StartClock(hipStream_t st) {
+  DevBuf<long long> c(1);
+  ClockKernel<<<1, 1, 0, st>>>(c.p);
+  NGSAMD_CHECK_LAUNCH();
+  long long h = 0;
+  NGSAMD_HIP(hipMemcpyAsync(&h, c.p, sizeof(long long), hipMemcpyDeviceToHost, st));
+  NGSAMD_HIP(hipStreamSynchronize(st));
+  return h;
+}
+
+extern "C" int ngsamd_batch_cg_solve(ngsamd_op_t op, ngsamd_op_t pre, const double* f, double* u, double tol,
+                                     int maxiter, int chunk, int* h_iterations, double* h_final_res,
+                                     double* h_finish_ms) {
+  return Guard([&] {
+    RequireInit();
+    NGSAMD_REQUIRE(op && op->kind == OpKind::Batch, "ngsamd_batch_cg_solve: not a batched operator");
+    NGSAMD_REQUIRE(pre && pre->kind == OpKind::Jacobi && pre->height == op->height,
+                   "ngsamd_batch_cg_solve: needs a Jacobi preconditioner of the same size");
+    NGSAMD_REQUIRE(maxiter >= 1, "ngsamd_batch_cg_solve: maxiter must be >= 1");
+    auto& A = *static_cast<BatchOp*>(op);
+    hipStream_t st = Stream();
+    const double* inv = static_cast<JacobiOp*>(pre)->invdiag.p;
+    BatchWork W(A.nseg);
+    const long long t0 = StartClock(st);
+    RunBatchCg<double, double>(A, A.vals.p, inv, f, u, W, nullptr, tol, nullptr, maxiter, chunk, st);
+    if (h_iterations) {
+      NGSAMD_HIP(hipMemcpyAsync(h_iterations, W.it.p, A.nseg * sizeof(int), hipMemcpyDeviceToHost, st));
+    }
+    if (h_final_res) {
+      NGSAMD_HIP(hipMemcpyAsync(h_final_res, W.last_res.p, A.nseg * sizeof(double), hipMemcpyDeviceToHost, st));
+    }
+    NGSAMD_HIP(hipStreamSynchronize(st));
+    if (h_finish_ms) ReadClock(W.finish.p, t0, A.nseg, h_finish_ms, st);
+  });
+}
+
+extern "C" int ngsamd_batch_ir_cg_solve(ngsamd_op_t op, ngsamd_op_t pre, const double* f, double* u, double tol,
+                                        double inner_tol, int max_outer, int max_inner, int* h_outer,
+                                        int* h_inner, double* h_relres, double* h_finish_ms) {
+  return Guard([&] {
+    RequireInit();
+    NGSAMD_REQUIRE(op && op->kind == OpKind::Batch, "ngsamd_batch_ir_cg_solve: not a batched operator");
+    auto& A = *static_cast<BatchOp*>(op);
+    NGSAMD_REQUIRE(A.vals32.p != nullptr, "ngsamd_batch_ir_cg_solve: operator created without FP32 values");
+    NGSAMD_REQUIRE(pre && pre->kind == OpKind::Jacobi && pre->height == op->height,
+                   "ngsamd_batch_ir_cg_solve: needs a Jacobi preconditioner of the same size");
+    hipStream_t st = Stream();
+    const int64_t n = A.height;
+    const int ns = A.nseg;
+    const double* inv = static_cast<JacobiOp*>(pre)->invdiag.p;
+    DevBuf<float> inv32(std::max<int64_t>(n, 1)), r32(std::max<int64_t>(n, 1)), c32(std::max<int64_t>(n, 1));
+    DevBuf<double> r(std::max<int64_t>(n, 1));
+    NGSAMD_REQUIRE(ngsamd_vec_f64_to_f32(n, inv, inv32.p) == 0, "ir: conversion failed");
+    DevBuf<int> frozen(ns), outer(ns), inner_sum(ns);
+    DevBuf<double> fnorm(ns), rnorm(ns), inv_rnorm(ns), itol(ns), relres(ns);
+    DevBuf<long long> finish(ns);
+    Fill<int>(frozen.p, ns, 0, st);
+    Fill<int>(outer.p, ns, 0, st);
+    Fill<long long>(finish.p, ns, 0LL, st);
+    std::vector<int> hin(ns, 0), hit(ns);
+    BatchWork W(ns);
+    const long long t0 = StartClock(st);
+    // ||f||_free per problem
+    MaskedSqChunkKernel<<<A.nchunks, kBatchBlock, 0, st>>>(f, inv, A.chunk_start.p, A.partials.p);
+    NGSAMD_CHECK_LAUNCH();
+    SegNormKernel<<<ns, kBatchBlock, 0, st>>>(A.partials.p, A.seg_chunk.p, fnorm.p);
+    NGSAMD_CHECK_LAUNCH();
+    Fill<double>(u, n, 0.0, st);
+    if (n) NGSAMD_HIP(hipMemcpyAsync(r.p, f, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+    for (int k = 0; k <= max_outer; k++) {
+      MaskedSqChunkKernel<<<A.nchunks, kBatchBlock, 0, st>>>(r.p, inv, A.chunk_start.p, A.partials.p);
+      NGSAMD_CHECK_LAUNCH();
+      RefineCheckKernel<<<ns, kBatchBlock, 0, st>>>(A.partials.p, A.seg_chunk.p, fnorm.p, tol, inner_tol, frozen.p,
+                                                    rnorm.p, inv_rnorm.p, itol.p, relres.p, finish.p, outer.p,
+                                                    k == max_outer ? 1 : 0);
+      NGSAMD_CHECK_LAUNCH();
+      std::vector<int> hf(ns);
+      NGSAMD_HIP(hipMemcpyAsync(hf.data(), frozen.p, ns * sizeof(int), hipMemcpyDeviceToHost, st));
+      NGSAMD_HIP(hipStreamSynchronize(st));
+      if (std::all_of(hf.begin(), hf.end(), [](int v) { return v != 0; })) break;
+      // inner FP32 solve of A c = r/||r|| for the problems still open
+      ToF32ScaledSegKernel<<<A.nchunks, 256, 0, st>>>(r.p, r32.p, A.chunk_start.p, A.chunk_seg.p, inv_rnorm.p,
+                                                      frozen.p);
+      NGSAMD_CHECK_LAUNCH();
+      RunBatchCg<float, float>(A, A.vals32.p, inv32.p, r32.p, c32.p, W, itol.p, 0.0, frozen.p, max_inner, 16, st);
+      NGSAMD_HIP(hipMemcpyAsync(hit.data(), W.it.p, ns * sizeof(int), hipMemcpyDeviceToHost, st));
+      NGSAMD_HIP(hipStreamSynchronize(st));
+      for (int p = 0; p < ns; p++)
+        if (!hf[p]) hin[p] += hit[p];
+      AddCorrectionSegKernel<<<A.nchunks, 256, 0, st>>>(u, c32.p, A.chunk_start.p, A.chunk_seg.p, rnorm.p, frozen.p);
+      NGSAMD_CHECK_LAUNCH();
+      // r = f - A u  (FP64)
+      if (n) NGSAMD_HIP(hipMemcpyAsync(r.p, f, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+      A.MultAdd(-1.0, u, r.p, st);
+    }
+    if (h_outer) NGSAMD_HIP(hipMemcpyAsync(h_outer, outer.p, ns * sizeof(int), hipMemcpyDeviceToHost, st));
+    if (h_relres) NGSAMD_HIP(hipMemcpyAsync(h_relres, relres.p, ns * sizeof(double), hipMemcpyDeviceToHost, st));
+    NGSAMD_HIP(hipStreamSynchronize(st));
+    if (h_inner)
+      for (int p = 0; p < ns; p++) h_inner[p] = hin[p];
+    if (h_finish_ms) ReadClock(finish.p, t0, ns, h_finish_ms, st);
+  });
+}

@@ -89,7 +89,7 @@ struct DevBuf {
   }
 };
 
-enum class OpKind { CSR, Jacobi, MatrixFree };
+enum class OpKind { CSR, Jacobi, MatrixFree, Batch };
 
 }  // namespace ngsamd
 

@@ -493,6 +493,57 @@ PYBIND11_MODULE(_ngsamd, m) {
         py::arg("mat"), py::arg("pre"), py::arg("rhs"), py::arg("sol"), py::arg("initialize") = true,
         py::arg("tol") = 1e-12, py::arg("atol") = -1.0, py::arg("maxiter") = 100, py::arg("chunk") = 16);
 
+  m.def("CreateBatchCSR",
+        [](Arr<int64_t> rowptr, Arr<int> cols, Arr<double> vals, Arr<int64_t> seg, bool with_f32) {
+          const int64_t n = rowptr.size() - 1;
+          if (n < 0 || cols.size() != vals.size() || rowptr.data()[n] != (int64_t)cols.size())
+            throw std::runtime_error("CreateBatchCSR: inconsistent CSR arrays");
+          ngsamd_op_t op = nullptr;
+          Check(ngsamd_batch_csr_create(&op, n, rowptr.data(), cols.data(), vals.data(), (int)seg.size() - 1,
+                                        seg.data(), with_f32 ? 1 : 0));
+          return std::make_shared<Operator>(op, "batch");
+        });
+  m.def("BatchInfo", [](Operator& A) {
+    int ns = 0, bn = 0, nb = 0;
+    Check(ngsamd_batch_info(A.op, &ns, &bn, &nb));
+    return py::make_tuple(ns, bn, nb);
+  });
+  m.def("BatchCGSolve",
+        [](Operator& A, Operator& pre, DeviceVector& f, DeviceVector& u, double tol, int maxiter, int chunk) {
+          RequireF64(f, A.height, "BatchCGSolve");
+          RequireF64(u, A.height, "BatchCGSolve");
+          int ns = 0;
+          Check(ngsamd_batch_info(A.op, &ns, nullptr, nullptr));
+          std::vector<int> its(ns);
+          std::vector<double> res(ns), ms(ns);
+          {
+            py::gil_scoped_release rel;
+            Check(ngsamd_batch_cg_solve(A.op, pre.op, f.d(), u.d(), tol, maxiter, chunk, its.data(), res.data(),
+                                        ms.data()));
+          }
+          return py::make_tuple(ToNumpy(its), ToNumpy(res), ToNumpy(ms));
+        },
+        py::arg("mat"), py::arg("pre"), py::arg("rhs"), py::arg("sol"), py::arg("tol") = 1e-10,
+        py::arg("maxiter") = 100000, py::arg("chunk") = 16);
+  m.def("BatchIRCGSolve",
+        [](Operator& A, Operator& pre, DeviceVector& f, DeviceVector& u, double tol, double inner_tol, int max_outer,
+           int max_inner) {
+          RequireF64(f, A.height, "BatchIRCGSolve");
+          RequireF64(u, A.height, "BatchIRCGSolve");
+          int ns = 0;
+          Check(ngsamd_batch_info(A.op, &ns, nullptr, nullptr));
+          std::vector<int> outer(ns), inner(ns);
+          std::vector<double> rel(ns), ms(ns);
+          {
+            py::gil_scoped_release g;
+            Check(ngsamd_batch_ir_cg_solve(A.op, pre.op, f.d(), u.d(), tol, inner_tol, max_outer, max_inner,
+                                           outer.data(), inner.data(), rel.data(), ms.data()));
+          }
+          return py::make_tuple(ToNumpy(outer), ToNumpy(inner), ToNumpy(rel), ToNumpy(ms));
+        },
+        py::arg("mat"), py::arg("pre"), py::arg("rhs"), py::arg("sol"), py::arg("tol") = 1e-10,
+        py::arg("inner_tol") = 1e-4, py::arg("max_outer") = 50, py::arg("max_inner") = 100000);
+
   m.def("IRCGSolve",
         [](Operator& A, Operator& inner, Operator& pre, DeviceVector& f, DeviceVector& u, double tol, double inner_tol,
            int max_outer, int max_inner) {

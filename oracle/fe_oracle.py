@@ -610,3 +610,26 @@ def pcg(matvec, invdiag, f, tol=1e-12, maxiter=100, atol=None):
         beta = wdn / wd
         s = beta * s + w
     return u, len(res), res
+
+
+def adjoint_gradient(pr, kappa, f, u_obs, M):
+    """J(kappa) = 0.5 (u-u_obs)^T M (u-u_obs) with A(kappa) u = f on the free
+    dofs (u = 0 on Dirichlet dofs), and dJ/dkappa_e = -lam_e^T K_e u_e / kappa_e
+    with A lam = M (u - u_obs): the CPU FP64 adjoint gradient (direct solves).
+    Returns (J, grad, u)."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spl
+    rp, c, v = pr.assemble(kappa=kappa)
+    A = sp.csr_matrix((v, c, rp), shape=(pr.ndof, pr.ndof))
+    fr = np.where(pr.free)[0]
+    lu = spl.splu(A[fr][:, fr].tocsc())
+    u = np.zeros(pr.ndof)
+    u[fr] = lu.solve(f[fr])
+    r = u - u_obs
+    Mr = M @ r
+    J = 0.5 * float(r @ Mr)
+    lam = np.zeros(pr.ndof)
+    lam[fr] = lu.solve(Mr[fr])
+    K = pr.element_matrices()
+    g = -np.einsum("ei,eij,ej->e", lam[pr.dofs], K, u[pr.dofs])
+    return J, g, u

@@ -1,0 +1,55 @@
+"""Batched independent solves (the batching configuration at reduced size):
+every problem's iteration count and solution against the oracle's CG on
+that problem alone, FP64 and mixed precision (FP32 inner, FP64 refinement)."""
+import numpy as np
+import pytest
+
+import oracle
+from oracle import fe_oracle as O
+from helpers import say
+
+pytestmark = pytest.mark.gpu
+
+
+def _problems():
+    from ngsolve_amd.batch import make_problems
+    return make_problems(24, seed=5, nx_range=(2, 20), p_range=(1, 5))
+
+
+def test_batch_fp64_matches_per_problem_cg(device):
+    from ngsolve_amd.batch import BatchSolver
+    say("[batch] fp64 24 problems")
+    probs = _problems()
+    bs = BatchSolver(probs)
+    r = bs.solve_fp64(tol=1e-10)
+    say(f"[batch] block_nnz={bs.block_nnz} its={r['iterations'].tolist()}")
+    for k, pr in enumerate(probs):
+        _, inv = O.jacobi_invdiag(pr.rowptr, pr.cols, pr.vals, pr.free)
+        u_ref, its_ref, _ = oracle.pcg_csr(pr.rowptr, pr.cols, pr.vals, inv, pr.rhs, tol=1e-10, maxiter=100000)
+        u = bs.solution(k)
+        assert abs(int(r["iterations"][k]) - its_ref) <= 1, (k, r["iterations"][k], its_ref)
+        assert np.linalg.norm(u - u_ref) <= 1e-9 * max(np.linalg.norm(u_ref), 1e-300)
+        assert r["finish_ms"][k] >= 0
+
+
+def test_batch_mixed_reaches_1e10(device):
+    from ngsolve_amd.batch import BatchSolver
+    say("[batch] mixed 24 problems")
+    probs = _problems()
+    bs = BatchSolver(probs)
+    r = bs.solve_mixed(tol=1e-10)
+    say(f"[batch] outer={r['outer'].tolist()} relres max={r['relres'].max():.2e}")
+    assert (r["relres"] <= 1e-10).all()
+    for k, pr in enumerate(probs):
+        u = bs.solution(k)
+        res = pr.rhs - oracle.spmv(pr.rowptr, pr.cols, pr.vals, u)
+        fr = pr.free.astype(bool)
+        assert np.linalg.norm(res[fr]) <= 1.0001e-10 * np.linalg.norm(pr.rhs[fr])
+
+
+def test_batch_rejects_coupled_problems(device):
+    from ngsolve_amd import _ngsamd as N
+    rp = np.array([0, 2, 3], dtype=np.int64)
+    cols = np.array([0, 1, 1], dtype=np.int32)   # row 0 couples to problem 1
+    with pytest.raises(RuntimeError):
+        N.CreateBatchCSR(rp, cols, np.ones(3), np.array([0, 1, 2], dtype=np.int64), False)
