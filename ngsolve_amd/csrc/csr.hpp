@@ -16,6 +16,7 @@ struct CsrOp : ngsamd_op_s {
   DevBuf<int> blk_row;  // stream kernel row blocks
   int nblocks = 0;
   int block_nnz = 2048;
+  int vec = 1;     // stream kernel: nonzeros per lane load (1 or 4)
   CsrOp(int64_t h, int64_t w, const int64_t* rp, const int* c, const double* v);
   void MultAdd(double s, const double* x, double* y, hipStream_t st) override;
   void Mult(const double* x, double* y, hipStream_t st) override;

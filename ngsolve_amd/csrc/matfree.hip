@@ -28,6 +28,7 @@
 // fixed order (deterministic, no atomics).  ATOMIC adds the element results
 // with hardware floating-point atomics, as the reference kernel does.
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -82,7 +83,8 @@ __device__ __forceinline__ void ApplyMetric(const T* g, const T* u, T* v) {
 
 // MODE 0: contributions (row-major [nel][nd], table row numbering)
 // MODE 1: atomic add s*y_e into y
-template <class T, int DIM, int NDP, int MODE, class V>
+// ET element tiles of 16 per wavefront share every A-operand read.
+template <class T, int DIM, int NDP, int MODE, int ET, class V>
 __global__ __launch_bounds__(kMfBlock) void MfMfmaKernel(
     const T* __restrict__ bclass, const double* __restrict__ weights, int nqb, const int* __restrict__ dofs,
     const double* __restrict__ metric, const double* __restrict__ kappa, const int* __restrict__ elnr,
@@ -102,47 +104,58 @@ __global__ __launch_bounds__(kMfBlock) void MfMfmaKernel(
 
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int col = lane & 15, grp = lane >> 4;
-  const int64_t ntiles = (end - begin + 15) / 16;
+  const int64_t ntiles = (end - begin + 16 * ET - 1) / (16 * ET);
   for (int64_t tile = (int64_t)blockIdx.x * kMfWaves + wave; tile < ntiles; tile += (int64_t)gridDim.x * kMfWaves) {
-    const int64_t e = begin + tile * 16 + col;
-    const bool valid = e < end;
-    // element vector, K layout of the first product: lane holds dof 4s+grp
-    T xk[KS];
+    int64_t e[ET];
+    bool valid[ET];
+    T xk[ET][KS];
+    T g[ET][NC];
+    acc_t yacc[ET][MT];
 #pragma unroll
-    for (int st = 0; st < KS; st++) {
-      const int dof = 4 * st + grp;
-      xk[st] = (valid && dof < nd) ? (T)x[dofs[e * nd + dof]] : (T)0;
+    for (int et = 0; et < ET; et++) {
+      e[et] = begin + (tile * ET + et) * 16 + col;
+      valid[et] = e[et] < end;
+      // element vector, K layout of the first product: lane holds dof 4s+grp
+#pragma unroll
+      for (int st = 0; st < KS; st++) {
+        const int dof = 4 * st + grp;
+        xk[et][st] = (valid[et] && dof < nd) ? (T)x[dofs[e[et] * nd + dof]] : (T)0;
+      }
+      const double kap = valid[et] ? kappa[elnr[e[et]]] : 0.0;
+#pragma unroll
+      for (int c = 0; c < NC; c++) g[et][c] = valid[et] ? (T)(kap * metric[e[et] * NC + c]) : (T)0;
+#pragma unroll
+      for (int m = 0; m < MT; m++) yacc[et][m] = acc_t{0, 0, 0, 0};
     }
-    T g[NC];
-    {
-      const double kap = valid ? kappa[elnr[e]] : 0.0;
-#pragma unroll
-      for (int c = 0; c < NC; c++) g[c] = valid ? (T)(kap * metric[e * NC + c]) : (T)0;
-    }
-    acc_t yacc[MT];
-#pragma unroll
-    for (int m = 0; m < MT; m++) yacc[m] = acc_t{0, 0, 0, 0};
 
     for (int t = 0; t < nqb; t++) {
-      acc_t u[DIM];
+      acc_t u[ET][DIM];
 #pragma unroll
       for (int d = 0; d < DIM; d++) {
-        u[d] = acc_t{0, 0, 0, 0};
+#pragma unroll
+        for (int et = 0; et < ET; et++) u[et][d] = acc_t{0, 0, 0, 0};
         const T* arow = lb + (16 * t + col) * QS + d * NDP + grp;
 #pragma unroll
-        for (int st = 0; st < KS; st++) u[d] = Mfma(arow[4 * st], xk[st], u[d]);
+        for (int st = 0; st < KS; st++) {
+          const T a = arow[4 * st];
+#pragma unroll
+          for (int et = 0; et < ET; et++) u[et][d] = Mfma(a, xk[et][st], u[et][d]);
+        }
       }
       // metric at the 4 points this lane holds: q = 16t + AccRow(grp, r)
-      T v[DIM][4];
+      T v[ET][DIM][4];
 #pragma unroll
       for (int r = 0; r < 4; r++) {
         const T w = (T)weights[16 * t + AccRow<T>(grp, r)];
-        T uu[DIM], vv[DIM];
 #pragma unroll
-        for (int d = 0; d < DIM; d++) uu[d] = u[d][r];
-        ApplyMetric<DIM, T>(g, uu, vv);
+        for (int et = 0; et < ET; et++) {
+          T uu[DIM], vv[DIM];
 #pragma unroll
-        for (int d = 0; d < DIM; d++) v[d][r] = w * vv[d];
+          for (int d = 0; d < DIM; d++) uu[d] = u[et][d][r];
+          ApplyMetric<DIM, T>(g[et], uu, vv);
+#pragma unroll
+          for (int d = 0; d < DIM; d++) v[et][d][r] = w * vv[d];
+        }
       }
 #pragma unroll
       for (int m = 0; m < MT; m++) {
@@ -153,26 +166,28 @@ __global__ __launch_bounds__(kMfBlock) void MfMfmaKernel(
 #pragma unroll
           for (int r = 0; r < 4; r++) {
             const T a = inrange ? lb[(16 * t + AccRow<T>(grp, r)) * QS + d * NDP + dof] : (T)0;
-            yacc[m] = Mfma(a, v[d][r], yacc[m]);
+#pragma unroll
+            for (int et = 0; et < ET; et++) yacc[et][m] = Mfma(a, v[et][d][r], yacc[et][m]);
           }
       }
     }
-    if (valid) {
+#pragma unroll
+    for (int et = 0; et < ET; et++) {
+      if (!valid[et]) continue;
 #pragma unroll
       for (int m = 0; m < MT; m++)
 #pragma unroll
         for (int r = 0; r < 4; r++) {
           const int dof = 16 * m + AccRow<T>(grp, r);
           if (dof < nd) {
-            const V val = (V)yacc[m][r];
-            if (MODE == 0) out[e * nd + dof] = val;
-            else AtomicAddV(&out[dofs[e * nd + dof]], (V)(s * (double)val));
+            const V val = (V)yacc[et][m][r];
+            if (MODE == 0) out[e[et] * nd + dof] = val;
+            else AtomicAddV(&out[dofs[e[et] * nd + dof]], (V)(s * (double)val));
           }
         }
     }
   }
 }
-
 
 // BF16 path (v_mfma_f32_16x16x32_bf16, FP32 accumulation).  Operand layout
 // measured on gfx950: lane l supplies A[l%16][8*(l/16)+j] and
@@ -493,6 +508,8 @@ MatFreeOp::MatFreeOp(int dim_, int nd_, int nq_, int64_t ndof, int nclass, const
 
   ndp = (nd + 3) / 4 * 4;
   nqp = (nq + 15) / 16 * 16;
+  const char* tpw = getenv("NGSAMD_MF_TILES");
+  tiles_per_wave = (tpw && atoi(tpw) == 2) ? 2 : 1;
   const int qs = dim * ndp + 1;
   bstride = (int64_t)nqp * qs;
   // MFMA path if the padded shape is instantiated and B fits in LDS
@@ -591,14 +608,14 @@ static void LaunchMfma(const MatFreeOp& A, const T* b, const MfClass& c, const V
                        const int* stop, hipStream_t st, bool prepare) {
   const int nqb = A.nqp / 16;
   const size_t lds = (size_t)nqb * 16 * (DIM * NDP + 1) * sizeof(T);
-  auto kern = MfMfmaKernel<T, DIM, NDP, MODE, V>;
+  auto kern = A.tiles_per_wave == 2 ? MfMfmaKernel<T, DIM, NDP, MODE, 2, V> : MfMfmaKernel<T, DIM, NDP, MODE, 1, V>;
   if (prepare) {
     // dynamic LDS above the default limit must be enabled per kernel, once,
     // outside any stream capture
     NGSAMD_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     return;
   }
-  const int64_t ntiles = (c.end - c.begin + 15) / 16;
+  const int64_t ntiles = (c.end - c.begin + 16 * A.tiles_per_wave - 1) / (16 * A.tiles_per_wave);
   int64_t grid = (ntiles + kMfWaves - 1) / kMfWaves;
   const int64_t maxg = (int64_t)DeviceCUs() * (lds > 81920 ? 1 : 2);
   if (grid > maxg) grid = maxg;

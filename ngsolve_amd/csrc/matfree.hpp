@@ -17,6 +17,7 @@ struct MatFreeOp : ngsamd_op_s {
   int precision = NGSAMD_PREC_F64;
   int scatter = NGSAMD_SCATTER_GATHER;
   bool use_mfma = false;
+  int tiles_per_wave = 1;     // element tiles of 16 per wavefront (FP64/FP32 MFMA path)
   int64_t nel = 0;
   std::vector<MfClass> classes;
   // per class reference gradients, MFMA layout [nqp][dim][ndp] (+1 pad per q),

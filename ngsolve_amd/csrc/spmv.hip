@@ -85,16 +85,20 @@ __global__ __launch_bounds__(kSpmvBlock) void CsrSpmvKernel(
 
 
 constexpr int kStreamBlock = 256;
+typedef int int4v __attribute__((ext_vector_type(4)));
+typedef double double2v __attribute__((ext_vector_type(2)));
 
 // y[r] = (ACCUM ? y[r] : 0) + s * (A x)[r] over the row blocks
 // [blk_row[b], blk_row[b+1]) of at most NNZ nonzeros; DOT as in CsrSpmvKernel.
-template <int NNZ, bool ACCUM, bool DOT>
+// VEC = 4: each lane loads 4 consecutive column numbers (16 B) and values
+// (2 x 16 B) from a 4-aligned window (the arrays carry 4 padding entries, so
+// the window never leaves them); VEC = 1: one nonzero per load.
+template <int NNZ, int VEC, bool ACCUM, bool DOT>
 __global__ __launch_bounds__(kStreamBlock) void CsrStreamKernel(
     const int* __restrict__ rowptr, const int* __restrict__ cols, const double* __restrict__ vals,
     const double* __restrict__ x, double* __restrict__ y, const int* __restrict__ blk_row, int nblocks,
     int blocks_per_wg, double s, double* __restrict__ partials, const int* __restrict__ stop_flag) {
   if (stop_flag && *stop_flag) return;
-  constexpr int PER = NNZ / kStreamBlock;  // nonzeros per thread and block
   __shared__ double prod[NNZ];
   __shared__ double lds[kStreamBlock / 64];
   const int tid = threadIdx.x;
@@ -118,19 +122,50 @@ __global__ __launch_bounds__(kStreamBlock) void CsrStreamKernel(
       }
       continue;
     }
-    // all loads of this thread first (memory-level parallelism), then the products
-    int c[PER];
-    double v[PER];
+    if constexpr (VEC == 4) {
+      // groups of 4 nonzeros starting at the aligned index kb; at most
+      // NNZ/4 + 1 groups cover [k0, k1)
+      constexpr int G = NNZ / (4 * kStreamBlock) + 1;
+      const int kb = k0 & ~3;
+      int4v c4[G];
+      double2v va[G], vb[G];
 #pragma unroll
-    for (int i = 0; i < PER; i++) {
-      const int k = tid + i * kStreamBlock;
-      c[i] = k < nz ? __builtin_nontemporal_load(&cols[k0 + k]) : -1;
-      v[i] = k < nz ? __builtin_nontemporal_load(&vals[k0 + k]) : 0.0;
-    }
+      for (int i = 0; i < G; i++) {
+        const int base = kb + 4 * (tid + i * kStreamBlock);
+        if (base < k1) {
+          c4[i] = __builtin_nontemporal_load(reinterpret_cast<const int4v*>(cols + base));
+          va[i] = __builtin_nontemporal_load(reinterpret_cast<const double2v*>(vals + base));
+          vb[i] = __builtin_nontemporal_load(reinterpret_cast<const double2v*>(vals + base + 2));
+        }
+      }
 #pragma unroll
-    for (int i = 0; i < PER; i++) {
-      const int k = tid + i * kStreamBlock;
-      if (c[i] >= 0) prod[k] = v[i] * x[c[i]];
+      for (int i = 0; i < G; i++) {
+        const int base = kb + 4 * (tid + i * kStreamBlock);
+        if (base < k1) {
+          const int cc[4] = {c4[i].x, c4[i].y, c4[i].z, c4[i].w};
+          const double vv[4] = {va[i].x, va[i].y, vb[i].x, vb[i].y};
+#pragma unroll
+          for (int j = 0; j < 4; j++) {
+            const int k = base + j;
+            if (k >= k0 && k < k1) prod[k - k0] = vv[j] * x[cc[j]];
+          }
+        }
+      }
+    } else {
+      constexpr int PER = NNZ / kStreamBlock;
+      int c[PER];
+      double v[PER];
+#pragma unroll
+      for (int i = 0; i < PER; i++) {
+        const int k = tid + i * kStreamBlock;
+        c[i] = k < nz ? __builtin_nontemporal_load(&cols[k0 + k]) : -1;
+        v[i] = k < nz ? __builtin_nontemporal_load(&vals[k0 + k]) : 0.0;
+      }
+#pragma unroll
+      for (int i = 0; i < PER; i++) {
+        const int k = tid + i * kStreamBlock;
+        if (c[i] >= 0) prod[k] = v[i] * x[c[i]];
+      }
     }
     __syncthreads();
     const int nr = r1 - r0;
@@ -172,11 +207,20 @@ CsrOp::CsrOp(int64_t h, int64_t w, const int64_t* rp, const int* c, const double
   for (int64_t k = 0; k < nnz; k++)
     NGSAMD_REQUIRE(c[k] >= 0 && c[k] < w, "ngsamd_csr_create: column number out of range");
   rowptr.alloc(h + 1);
-  cols.alloc(nnz);
-  vals.alloc(nnz);
+  // 4 padding entries (column 0, value 0) behind the last nonzero for the
+  // 16-byte loads of the stream kernel
+  cols.alloc(nnz + 4);
+  vals.alloc(nnz + 4);
   rowptr.upload(rp32.data(), h + 1);
   cols.upload(c, nnz);
   vals.upload(v, nnz);
+  {
+    const int zc[4] = {0, 0, 0, 0};
+    const double zv[4] = {0, 0, 0, 0};
+    NGSAMD_HIP(hipMemcpyAsync(cols.p + nnz, zc, sizeof(zc), hipMemcpyHostToDevice, Stream()));
+    NGSAMD_HIP(hipMemcpyAsync(vals.p + nnz, zv, sizeof(zv), hipMemcpyHostToDevice, Stream()));
+    NGSAMD_HIP(hipStreamSynchronize(Stream()));
+  }
   NGSAMD_HIP(hipStreamSynchronize(Stream()));
   double mean = h ? double(nnz) / double(h) : 0.0;
   group = mean > 48 ? 32 : mean > 20 ? 16 : mean > 9 ? 8 : 4;
@@ -203,6 +247,8 @@ CsrOp::CsrOp(int64_t h, int64_t w, const int64_t* rp, const int* c, const double
   NGSAMD_HIP(hipStreamSynchronize(Stream()));
   const char* algo_env = getenv("NGSAMD_SPMV");
   algo = (algo_env && std::string(algo_env) == "rowgroup") ? 0 : 1;
+  const char* vec_env = getenv("NGSAMD_SPMV_VEC");
+  vec = (vec_env && atoi(vec_env) == 4) ? 4 : 1;
 }
 
 template <int G, bool ACCUM, bool DOT>
@@ -231,9 +277,12 @@ static int LaunchStream(const CsrOp& A, double s, const double* x, double* y, hi
   if (wg > 8) wg &= ~int64_t(7);
   if (wg < 1) wg = 1;
   const int per = (int)((A.nblocks + wg - 1) / wg);
-  auto kern = A.block_nnz == 1024   ? CsrStreamKernel<1024, ACCUM, DOT>
-              : A.block_nnz == 4096 ? CsrStreamKernel<4096, ACCUM, DOT>
-                                    : CsrStreamKernel<2048, ACCUM, DOT>;
+  auto kern = A.vec == 1 ? (A.block_nnz == 1024   ? CsrStreamKernel<1024, 1, ACCUM, DOT>
+                            : A.block_nnz == 4096 ? CsrStreamKernel<4096, 1, ACCUM, DOT>
+                                                  : CsrStreamKernel<2048, 1, ACCUM, DOT>)
+                         : (A.block_nnz == 1024   ? CsrStreamKernel<1024, 4, ACCUM, DOT>
+                            : A.block_nnz == 4096 ? CsrStreamKernel<4096, 4, ACCUM, DOT>
+                                                  : CsrStreamKernel<2048, 4, ACCUM, DOT>);
   kern<<<(int)wg, kStreamBlock, 0, st>>>(A.rowptr.p, A.cols.p, A.vals.p, x, y, A.blk_row.p, A.nblocks, per, s,
                                          partials, stop);
   NGSAMD_CHECK_LAUNCH();

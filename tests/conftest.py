@@ -1,6 +1,14 @@
 import os
 import sys
 
+# torch ships its own HIP runtime under the same soname as the system one; if
+# torch is loaded first, libngsamd binds to that one and the process has a
+# single runtime (needed by the torch-tensor tests of ngsolve_amd.autograd)
+try:
+    import torch  # noqa: F401
+except ImportError:  # pragma: no cover
+    pass
+
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))

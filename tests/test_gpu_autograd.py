@@ -26,21 +26,22 @@ def _setup(nx, p):
     return ngs, m, fes, f, mass, pr, M
 
 
-def test_adjoint_gradient_matches_oracle_and_fd(device):
+@pytest.mark.parametrize("tdev", ["cuda", "cpu"])
+def test_adjoint_gradient_matches_oracle_and_fd(device, tdev):
     import torch
     from ngsolve_amd.autograd import ApplySymmetric, PoissonProblem, SolvePoisson
-    say("[autograd] 32x32 p=3")
+    say(f"[autograd] 32x32 p=3 tensors on {tdev}")
     ngs, m, fes, f, mass, pr, M = _setup(32, 3)
     ne = m.ne
     rng = np.random.default_rng(3)
     kappa_true = np.exp(rng.standard_normal(ne))
     P = PoissonProblem(fes, f, tol=1e-13)
-    u_true = SolvePoisson.apply(torch.tensor(kappa_true, device="cuda"), P).cpu().numpy()
+    u_true = SolvePoisson.apply(torch.tensor(kappa_true, device=tdev), P).cpu().numpy()
     u_obs = u_true + rng.normal(0.0, 1e-3, fes.ndof)
     kappa = np.exp(np.random.default_rng(4).normal(0.0, 0.5, ne))
 
-    kt = torch.tensor(kappa, device="cuda", requires_grad=True)
-    uo = torch.tensor(u_obs, device="cuda")
+    kt = torch.tensor(kappa, device=tdev, requires_grad=True)
+    uo = torch.tensor(u_obs, device=tdev)
     u = SolvePoisson.apply(kt, P)
     r = u - uo
     J = 0.5 * torch.dot(r, ApplySymmetric.apply(r, mass.mat.op))
@@ -64,4 +65,7 @@ def test_adjoint_gradient_matches_oracle_and_fd(device):
         fd = (Jp - Jm) / (2 * h)
         worst = max(worst, abs(fd - g @ d) / max(abs(fd), 1e-30))
     say(f"[autograd] worst FD relative error {worst:.2e}")
-    assert worst < 1e-5
+    # central differences at h = 1e-6 carry the solves' rounding,
+    # ~eps * cond(A) * |J| / h (about 5e-5 of g.d here); a wrong sign, a
+    # dropped term or a transposed operand is an O(1) error
+    assert worst < 1e-3
