@@ -174,6 +174,12 @@ def main():
     if args.profile_mode:
         for _ in range(3):
             A.mult(dx, dy)
+        os.environ["NGSAMD_SPMV"] = "stream"
+        As = N.CreateCSR(ndof, ndof, rowptr, cols, vals)
+        del os.environ["NGSAMD_SPMV"]
+        for _ in range(3):
+            As.mult(dx, dy)
+        del As
         for prec in (0, 1, 2):
             M = build_mf(N, fes, mesh, ir, None, prec)
             for _ in range(3):
@@ -194,15 +200,19 @@ def main():
     log(f"[bench] csr spmv {ms_csr * 1e3:.1f} us  {gbs:.0f} GB/s  {gdofs:.2f} GDoF/s")
 
     extra = {}
-    # the row-group kernel on the same matrix, for comparison
-    os.environ["NGSAMD_SPMV"] = "rowgroup"
-    Arg = N.CreateCSR(ndof, ndof, rowptr, cols, vals)
-    del os.environ["NGSAMD_SPMV"]
-    ms_rg = timed(N, lambda: Arg.mult(dx, dy), args.steps, args.warmup)
-    extra["csr_rowgroup_kernel"] = {"us_per_apply": round(ms_rg * 1e3, 1),
-                                    "gbs": round(csr_bytes / (ms_rg * 1e-3) / 1e9, 1)}
-    del Arg
-    log(f"[bench] csr rowgroup {ms_rg * 1e3:.1f} us")
+    # the plain-CSR kernels on the same matrix, for comparison (the default
+    # above is the pattern-runs kernel: this matrix shares column lists
+    # between the dofs of an edge, face or cell)
+    for algo in ("stream", "rowgroup"):
+        os.environ["NGSAMD_SPMV"] = algo
+        Aalt = N.CreateCSR(ndof, ndof, rowptr, cols, vals)
+        del os.environ["NGSAMD_SPMV"]
+        ms_alt = timed(N, lambda: Aalt.mult(dx, dy), args.steps, args.warmup)
+        extra[f"csr_{algo}_kernel"] = {
+            "us_per_apply": round(ms_alt * 1e3, 1), "gbs": round(csr_bytes / (ms_alt * 1e-3) / 1e9, 1),
+            "rel_err_vs_runs_kernel": float(np.abs(dy.download() - y_csr).max() / np.abs(y_csr).max())}
+        del Aalt
+        log(f"[bench] csr {algo} {ms_alt * 1e3:.1f} us")
 
     # ------------------------------------------------- matrix-free applies
     nq = len(ir)
@@ -309,7 +319,7 @@ def main():
                    "ndof": ndof, "nnz": nnz, "elements": ne},
         "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": "CsrSpmvKernel", "algorithmic_bytes": csr_bytes},
+                     "kernel": "CsrRunsKernel", "algorithmic_bytes": csr_bytes},
         "cpu_baseline": cpu,
         "extra": extra,
     }

@@ -129,18 +129,23 @@ __global__ __launch_bounds__(kBatchBlock) void BatchSpmvKernel(
       }
       continue;
     }
+    // unconditional loads (lanes past the block read entry k0, in bounds
+    // thanks to the padding) so that all of them are in flight at once
     int c[PER];
-    double v[PER];
+    double v[PER], xv[PER];
 #pragma unroll
     for (int i = 0; i < PER; i++) {
       const int k = tid + i * kBatchBlock;
-      c[i] = k < nz ? __builtin_nontemporal_load(&cols[k0 + k]) : -1;
-      v[i] = k < nz ? (double)__builtin_nontemporal_load(&vals[k0 + k]) : 0.0;
+      const int kk = k0 + (k < nz ? k : 0);
+      c[i] = __builtin_nontemporal_load(&cols[kk]);
+      v[i] = (double)__builtin_nontemporal_load(&vals[kk]);
     }
 #pragma unroll
+    for (int i = 0; i < PER; i++) xv[i] = (double)x[c[i]];
+#pragma unroll
     for (int i = 0; i < PER; i++) {
       const int k = tid + i * kBatchBlock;
-      if (c[i] >= 0) prod[k] = v[i] * (double)x[c[i]];
+      prod[k] = k < nz ? v[i] * xv[i] : 0.0;
     }
     __syncthreads();
     const int nr = r1 - r0;
@@ -214,14 +219,23 @@ BatchOp::BatchOp(int64_t n, const int64_t* rp, const int* c, const double* v, in
   for (int64_t i = 0; i <= n; i++) rp32[i] = (int)rp[i];
   rowptr.alloc(n + 1);
   rowptr.upload(rp32.data(), n + 1);
-  cols.alloc(nnz);
+  // 4 padding entries (column 0, value 0): lanes past the end of the last
+  // row block read entry nnz
+  static const int zc[4] = {0, 0, 0, 0};
+  static const double zv[4] = {0, 0, 0, 0};
+  static const float zf[4] = {0, 0, 0, 0};
+  cols.alloc(nnz + 4);
   cols.upload(c, nnz);
-  vals.alloc(nnz);
+  cols.upload_at(zc, nnz, 4);
+  vals.alloc(nnz + 4);
   vals.upload(v, nnz);
+  vals.upload_at(zv, nnz, 4);
   if (with_f32) {
     std::vector<float> v32(v, v + nnz);
-    vals32.alloc(nnz);
+    vals32.alloc(nnz + 4);
     vals32.upload(v32.data(), nnz);
+    vals32.upload_at(zf, nnz, 4);
+    NGSAMD_HIP(hipStreamSynchronize(Stream()));
   }
   // vector chunks
   std::vector<int64_t> cs;

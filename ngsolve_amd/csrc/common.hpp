@@ -82,6 +82,10 @@ struct DevBuf {
     if (count > n) throw Error(NGSAMD_E_ARG, "DevBuf::upload: too many elements");
     if (count) NGSAMD_HIP(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, Stream()));
   }
+  void upload_at(const T* h, size_t offset, size_t count) {
+    if (offset + count > n) throw Error(NGSAMD_E_ARG, "DevBuf::upload_at: out of range");
+    if (count) NGSAMD_HIP(hipMemcpyAsync(p + offset, h, count * sizeof(T), hipMemcpyHostToDevice, Stream()));
+  }
   void release() {
     if (p) (void)hipFree(p);
     p = nullptr;
@@ -110,4 +114,12 @@ namespace ngsamd {
 void LaunchFill(double* x, int64_t n, double a, hipStream_t st);
 void LaunchFillF32(float* x, int64_t n, float a, hipStream_t st);
 int GridFor(int64_t n, int block, int max_blocks = 1 << 20);
+
+// Workgroups are dealt to the 8 XCDs round-robin (workgroup b runs on XCD
+// b mod 8).  This maps the hardware id to a logical id such that each XCD owns
+// a contiguous range of logical ids, so neighbouring work shares one L2.
+__device__ __forceinline__ int XcdContiguousBlock(int b, int grid) {
+  if ((grid & 7) != 0) return b;
+  return (b & 7) * (grid >> 3) + (b >> 3);
+}
 }  // namespace ngsamd

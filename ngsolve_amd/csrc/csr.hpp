@@ -7,12 +7,29 @@
 
 namespace ngsamd {
 
+// Pattern runs: up to kMaxRun consecutive rows with one column list share
+// that list (the dofs of one mesh node have the same sparsity pattern); the
+// values of a run are stored column by column, [L][R].
+constexpr int kMaxRun = 8;
+
 struct CsrOp : ngsamd_op_s {
+  // rowgroup / stream layout (plain CSR)
   DevBuf<int> rowptr, cols;
   DevBuf<double> vals;
+  // runs layout
+  DevBuf<int> run_row;  // first row of each run, nruns + 1 entries
+  DevBuf<int> run_ptr;  // first shared column of each run, nruns + 1
+  DevBuf<int> val_ptr;  // first value of each run, nruns + 1
+  DevBuf<int> rcols;    // shared column lists
+  DevBuf<double> rvals; // values, run by run, [L][R]
+  DevBuf<int> blk_run;  // runs kernel blocks: first run, nblocks + 1
+  DevBuf<int> blk_len;  // runs kernel blocks: rows per run (all runs of a block alike)
+  int64_t nruns = 0, nshared = 0;
   int64_t nnz = 0;
+  // stands in for x when the width is 0 (padding lanes still read x[0])
+  DevBuf<double> xpad;
   int group = 32;  // lanes per row (rowgroup kernel)
-  int algo = 1;    // 0: rowgroup, 1: stream
+  int algo = 1;    // 0: rowgroup, 1: stream, 2: runs
   DevBuf<int> blk_row;  // stream kernel row blocks
   int nblocks = 0;
   int block_nnz = 2048;

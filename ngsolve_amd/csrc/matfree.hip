@@ -81,14 +81,32 @@ __device__ __forceinline__ void ApplyMetric(const T* g, const T* u, T* v) {
   }
 }
 
+// Work schedule shared by the MFMA kernels: the element tiles (16 elements)
+// of every class are numbered class after class in rounds of kMfWaves tiles,
+// one per wavefront, and a round never straddles two classes.  Workgroup w
+// (after the XCD-contiguous remap) walks the rounds
+// [w * nrounds / grid, (w + 1) * nrounds / grid) and reloads the class's B
+// into LDS only when the class changes, so one launch covers all classes.
+struct MfRounds {
+  const int* round0;     // first round of each class, nclass + 1 entries
+  const int64_t* range;  // element range [begin, end) of each class
+  int nrounds;
+};
+
+// Class of round r, scanning on from the class of the previous round.
+__device__ __forceinline__ int RoundClass(const MfRounds& R, int r, int from) {
+  int c = from < 0 ? 0 : from;
+  while (R.round0[c + 1] <= r) c++;
+  return c;
+}
+
 // MODE 0: contributions (row-major [nel][nd], table row numbering)
 // MODE 1: atomic add s*y_e into y
-// ET element tiles of 16 per wavefront share every A-operand read.
-template <class T, int DIM, int NDP, int MODE, int ET, class V>
+template <class T, int DIM, int NDP, int MODE, class V>
 __global__ __launch_bounds__(kMfBlock) void MfMfmaKernel(
-    const T* __restrict__ bclass, const double* __restrict__ weights, int nqb, const int* __restrict__ dofs,
-    const double* __restrict__ metric, const double* __restrict__ kappa, const int* __restrict__ elnr,
-    int64_t begin, int64_t end, int nd, const V* __restrict__ x, V* __restrict__ out, double s,
+    const T* __restrict__ bmat, int64_t bstride, MfRounds R, const double* __restrict__ weights, int nqb,
+    const int* __restrict__ dofs, const double* __restrict__ metric, const double* __restrict__ kappa,
+    const int* __restrict__ elnr, int nd, const V* __restrict__ x, V* __restrict__ out, double s,
     const int* __restrict__ stop) {
   if (stop && *stop) return;
   typedef typename Acc4<T>::type acc_t;
@@ -99,63 +117,72 @@ __global__ __launch_bounds__(kMfBlock) void MfMfmaKernel(
   extern __shared__ __align__(16) unsigned char lds_raw[];
   T* lb = reinterpret_cast<T*>(lds_raw);
   const int nlds = nqb * 16 * QS;
-  for (int i = threadIdx.x; i < nlds; i += kMfBlock) lb[i] = bclass[i];
-  __syncthreads();
-
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int col = lane & 15, grp = lane >> 4;
-  const int64_t ntiles = (end - begin + 16 * ET - 1) / (16 * ET);
-  for (int64_t tile = (int64_t)blockIdx.x * kMfWaves + wave; tile < ntiles; tile += (int64_t)gridDim.x * kMfWaves) {
-    int64_t e[ET];
-    bool valid[ET];
-    T xk[ET][KS];
-    T g[ET][NC];
-    acc_t yacc[ET][MT];
-#pragma unroll
-    for (int et = 0; et < ET; et++) {
-      e[et] = begin + (tile * ET + et) * 16 + col;
-      valid[et] = e[et] < end;
-      // element vector, K layout of the first product: lane holds dof 4s+grp
-#pragma unroll
-      for (int st = 0; st < KS; st++) {
-        const int dof = 4 * st + grp;
-        xk[et][st] = (valid[et] && dof < nd) ? (T)x[dofs[e[et] * nd + dof]] : (T)0;
-      }
-      const double kap = valid[et] ? kappa[elnr[e[et]]] : 0.0;
-#pragma unroll
-      for (int c = 0; c < NC; c++) g[et][c] = valid[et] ? (T)(kap * metric[e[et] * NC + c]) : (T)0;
-#pragma unroll
-      for (int m = 0; m < MT; m++) yacc[et][m] = acc_t{0, 0, 0, 0};
+  const int wg = XcdContiguousBlock(blockIdx.x, gridDim.x);
+  const int r0 = (int)((int64_t)wg * R.nrounds / gridDim.x);
+  const int r1 = (int)((int64_t)(wg + 1) * R.nrounds / gridDim.x);
+  int cls = -1;
+  for (int rd = r0; rd < r1; rd++) {
+    const int c = RoundClass(R, rd, cls);
+    if (c != cls) {
+      if (cls >= 0) __syncthreads();
+      const T* src = bmat + (int64_t)c * bstride;
+      for (int i = threadIdx.x; i < nlds; i += kMfBlock) lb[i] = src[i];
+      __syncthreads();
+      cls = c;
     }
+    const int64_t end = R.range[2 * c + 1];
+    const int64_t first = R.range[2 * c] + ((int64_t)(rd - R.round0[c]) * kMfWaves + wave) * 16;
+    if (first >= end) continue;
+    const int64_t e = first + col;
+    const bool valid = e < end;
+    // Gathers are unconditional, from indices clamped into range (element
+    // `first`, dof nd-1), and masked afterwards: conditional loads would each
+    // cost a round trip of their own.
+    const int64_t ev = valid ? e : first;
+    // element vector, K layout of the first product: lane holds dof 4s+grp
+    int dv[KS];
+#pragma unroll
+    for (int st = 0; st < KS; st++) dv[st] = dofs[ev * nd + min(4 * st + grp, nd - 1)];
+    const int en = elnr[ev];
+    double mv[NC];
+#pragma unroll
+    for (int cc = 0; cc < NC; cc++) mv[cc] = metric[ev * NC + cc];
+    V xv[KS];
+#pragma unroll
+    for (int st = 0; st < KS; st++) xv[st] = x[dv[st]];
+    const double kap = kappa[en];
+    T xk[KS];
+#pragma unroll
+    for (int st = 0; st < KS; st++) xk[st] = (valid && 4 * st + grp < nd) ? (T)xv[st] : (T)0;
+    T g[NC];
+#pragma unroll
+    for (int cc = 0; cc < NC; cc++) g[cc] = valid ? (T)(kap * mv[cc]) : (T)0;
+    acc_t yacc[MT];
+#pragma unroll
+    for (int m = 0; m < MT; m++) yacc[m] = acc_t{0, 0, 0, 0};
 
     for (int t = 0; t < nqb; t++) {
-      acc_t u[ET][DIM];
+      acc_t u[DIM];
 #pragma unroll
       for (int d = 0; d < DIM; d++) {
-#pragma unroll
-        for (int et = 0; et < ET; et++) u[et][d] = acc_t{0, 0, 0, 0};
+        u[d] = acc_t{0, 0, 0, 0};
         const T* arow = lb + (16 * t + col) * QS + d * NDP + grp;
 #pragma unroll
-        for (int st = 0; st < KS; st++) {
-          const T a = arow[4 * st];
-#pragma unroll
-          for (int et = 0; et < ET; et++) u[et][d] = Mfma(a, xk[et][st], u[et][d]);
-        }
+        for (int st = 0; st < KS; st++) u[d] = Mfma(arow[4 * st], xk[st], u[d]);
       }
       // metric at the 4 points this lane holds: q = 16t + AccRow(grp, r)
-      T v[ET][DIM][4];
+      T v[DIM][4];
 #pragma unroll
       for (int r = 0; r < 4; r++) {
         const T w = (T)weights[16 * t + AccRow<T>(grp, r)];
+        T uu[DIM], vv[DIM];
 #pragma unroll
-        for (int et = 0; et < ET; et++) {
-          T uu[DIM], vv[DIM];
+        for (int d = 0; d < DIM; d++) uu[d] = u[d][r];
+        ApplyMetric<DIM, T>(g, uu, vv);
 #pragma unroll
-          for (int d = 0; d < DIM; d++) uu[d] = u[et][d][r];
-          ApplyMetric<DIM, T>(g[et], uu, vv);
-#pragma unroll
-          for (int d = 0; d < DIM; d++) v[et][d][r] = w * vv[d];
-        }
+        for (int d = 0; d < DIM; d++) v[d][r] = w * vv[d];
       }
 #pragma unroll
       for (int m = 0; m < MT; m++) {
@@ -166,23 +193,20 @@ __global__ __launch_bounds__(kMfBlock) void MfMfmaKernel(
 #pragma unroll
           for (int r = 0; r < 4; r++) {
             const T a = inrange ? lb[(16 * t + AccRow<T>(grp, r)) * QS + d * NDP + dof] : (T)0;
-#pragma unroll
-            for (int et = 0; et < ET; et++) yacc[et][m] = Mfma(a, v[et][d][r], yacc[et][m]);
+            yacc[m] = Mfma(a, v[d][r], yacc[m]);
           }
       }
     }
-#pragma unroll
-    for (int et = 0; et < ET; et++) {
-      if (!valid[et]) continue;
+    if (valid) {
 #pragma unroll
       for (int m = 0; m < MT; m++)
 #pragma unroll
         for (int r = 0; r < 4; r++) {
           const int dof = 16 * m + AccRow<T>(grp, r);
           if (dof < nd) {
-            const V val = (V)yacc[et][m][r];
-            if (MODE == 0) out[e[et] * nd + dof] = val;
-            else AtomicAddV(&out[dofs[e[et] * nd + dof]], (V)(s * (double)val));
+            const V val = (V)yacc[m][r];
+            if (MODE == 0) out[e * nd + dof] = val;
+            else AtomicAddV(&out[dofs[e * nd + dof]], (V)(s * (double)val));
           }
         }
     }
@@ -203,44 +227,68 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
 template <int DIM, int KS1, int MT, int MODE, class V>
 __global__ __launch_bounds__(kMfBlock) void MfBf16Kernel(
-    const uint16_t* __restrict__ bclass, int k1p, int nqc, const double* __restrict__ weights,
-    const int* __restrict__ dofs, const double* __restrict__ metric, const double* __restrict__ kappa,
-    const int* __restrict__ elnr, int64_t begin, int64_t end, int nd, const V* __restrict__ x, V* __restrict__ out,
-    double s, const int* __restrict__ stop) {
+    const uint16_t* __restrict__ bmat, int64_t bstride, MfRounds R, int k1p, int nqc,
+    const double* __restrict__ weights, const int* __restrict__ dofs, const double* __restrict__ metric,
+    const double* __restrict__ kappa, const int* __restrict__ elnr, int nd, const V* __restrict__ x,
+    V* __restrict__ out, double s, const int* __restrict__ stop) {
   if (stop && *stop) return;
   constexpr int NC = DIM * (DIM + 1) / 2;
   extern __shared__ __align__(16) unsigned char lds_raw[];
   const int size1 = nqc * 32 * DIM * k1p;
-  const int size2 = nqc * DIM * MT * 16 * 32;
-  {
-    const uint4* src = reinterpret_cast<const uint4*>(bclass);
-    uint4* dst = reinterpret_cast<uint4*>(lds_raw);
-    for (int i = threadIdx.x; i < (size1 + size2) / 8; i += kMfBlock) dst[i] = src[i];
-  }
-  __syncthreads();
   const __bf16* B1 = reinterpret_cast<const __bf16*>(lds_raw);
   const __bf16* B2 = B1 + size1;
 
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int col = lane & 15, grp = lane >> 4;
-  const int64_t ntiles = (end - begin + 15) / 16;
-  for (int64_t tile = (int64_t)blockIdx.x * kMfWaves + wave; tile < ntiles; tile += (int64_t)gridDim.x * kMfWaves) {
-    const int64_t e = begin + tile * 16 + col;
+  const int wg = XcdContiguousBlock(blockIdx.x, gridDim.x);
+  const int r0 = (int)((int64_t)wg * R.nrounds / gridDim.x);
+  const int r1 = (int)((int64_t)(wg + 1) * R.nrounds / gridDim.x);
+  int cls = -1;
+  for (int rd = r0; rd < r1; rd++) {
+    const int cl = RoundClass(R, rd, cls);
+    if (cl != cls) {
+      if (cls >= 0) __syncthreads();
+      // B1 and B2 of the class: bstride 16-bit words, a multiple of 8
+      const uint4* src = reinterpret_cast<const uint4*>(bmat + (int64_t)cl * bstride);
+      uint4* dst = reinterpret_cast<uint4*>(lds_raw);
+      for (int i = threadIdx.x; i < (int)(bstride / 8); i += kMfBlock) dst[i] = src[i];
+      __syncthreads();
+      cls = cl;
+    }
+    const int64_t end = R.range[2 * cl + 1];
+    const int64_t first = R.range[2 * cl] + ((int64_t)(rd - R.round0[cl]) * kMfWaves + wave) * 16;
+    if (first >= end) continue;
+    const int64_t e = first + col;
     const bool valid = e < end;
+    // unconditional gathers from clamped indices, masked afterwards (see
+    // MfMfmaKernel)
+    const int64_t ev = valid ? e : first;
+    int dv[KS1][8];
+#pragma unroll
+    for (int ks = 0; ks < KS1; ks++)
+#pragma unroll
+      for (int j = 0; j < 8; j++) dv[ks][j] = dofs[ev * nd + min(32 * ks + 8 * grp + j, nd - 1)];
+    const int en = elnr[ev];
+    double mv[NC];
+#pragma unroll
+    for (int c = 0; c < NC; c++) mv[c] = metric[ev * NC + c];
+    V xv[KS1][8];
+#pragma unroll
+    for (int ks = 0; ks < KS1; ks++)
+#pragma unroll
+      for (int j = 0; j < 8; j++) xv[ks][j] = x[dv[ks][j]];
+    const double kap = kappa[en];
     bf16x8 xk[KS1];
 #pragma unroll
     for (int ks = 0; ks < KS1; ks++)
 #pragma unroll
       for (int j = 0; j < 8; j++) {
         const int dof = 32 * ks + 8 * grp + j;
-        xk[ks][j] = (__bf16)((valid && dof < nd) ? (float)x[dofs[e * nd + dof]] : 0.0f);
+        xk[ks][j] = (__bf16)((valid && dof < nd) ? (float)xv[ks][j] : 0.0f);
       }
     float g[NC];
-    {
-      const double kap = valid ? kappa[elnr[e]] : 0.0;
 #pragma unroll
-      for (int c = 0; c < NC; c++) g[c] = valid ? (float)(kap * metric[e * NC + c]) : 0.0f;
-    }
+    for (int c = 0; c < NC; c++) g[c] = valid ? (float)(kap * mv[c]) : 0.0f;
     float4v yacc[MT];
 #pragma unroll
     for (int m = 0; m < MT; m++) yacc[m] = float4v{0, 0, 0, 0};
@@ -357,7 +405,20 @@ __global__ void MfGatherKernel(const int* __restrict__ rowptr, const int* __rest
   if (stop && *stop) return;
   for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
     double acc = 0.0;
-    for (int k = rowptr[r]; k < rowptr[r + 1]; k++) acc += (double)contrib[slots[k]];
+    const int k0 = rowptr[r], k1 = rowptr[r + 1];
+    // 4 slots at a time, loads unconditional (clamped to the last slot) so
+    // they are in flight together; summed in slot order
+    for (int k = k0; k < k1; k += 4) {
+      int sl[4];
+      C cv[4];
+#pragma unroll
+      for (int j = 0; j < 4; j++) sl[j] = slots[min(k + j, k1 - 1)];
+#pragma unroll
+      for (int j = 0; j < 4; j++) cv[j] = contrib[sl[j]];
+#pragma unroll
+      for (int j = 0; j < 4; j++)
+        if (k + j < k1) acc += (double)cv[j];
+    }
     const double v = s * acc;
     y[r] = accumulate ? (V)((double)y[r] + v) : (V)v;
   }
@@ -469,8 +530,8 @@ static uint16_t ToBf16(double v) {
 }
 
 template <int MODE, class V>
-static void LaunchClass(const MatFreeOp& A, int ci, const V* x, V* out, double s, const int* stop, hipStream_t st,
-                        bool prepare);
+static void LaunchElements(const MatFreeOp& A, const V* x, V* out, double s, const int* stop, hipStream_t st,
+                           bool prepare);
 
 MatFreeOp::MatFreeOp(int dim_, int nd_, int nq_, int64_t ndof, int nclass, const int64_t* off, const double* bmat,
                      const double* w, const int* hdofs, const double* hmetric, const double* hkappa,
@@ -508,8 +569,6 @@ MatFreeOp::MatFreeOp(int dim_, int nd_, int nq_, int64_t ndof, int nclass, const
 
   ndp = (nd + 3) / 4 * 4;
   nqp = (nq + 15) / 16 * 16;
-  const char* tpw = getenv("NGSAMD_MF_TILES");
-  tiles_per_wave = (tpw && atoi(tpw) == 2) ? 2 : 1;
   const int qs = dim * ndp + 1;
   bstride = (int64_t)nqp * qs;
   // MFMA path if the padded shape is instantiated and B fits in LDS
@@ -593,105 +652,124 @@ MatFreeOp::MatFreeOp(int dim_, int nd_, int nq_, int64_t ndof, int nclass, const
   g_slots.upload(sl.data(), sl.size());
   contrib64.alloc((size_t)nel * nd);
   contrib32.alloc((size_t)nel * nd);
-  // enable the dynamic LDS size of every kernel variant this operator uses
-  for (size_t c = 0; c < classes.size(); c++) {
-    LaunchClass<0, double>(*this, (int)c, nullptr, nullptr, 1.0, nullptr, nullptr, true);
-    LaunchClass<1, double>(*this, (int)c, nullptr, nullptr, 1.0, nullptr, nullptr, true);
-    LaunchClass<0, float>(*this, (int)c, nullptr, nullptr, 1.0, nullptr, nullptr, true);
-    LaunchClass<1, float>(*this, (int)c, nullptr, nullptr, 1.0, nullptr, nullptr, true);
+  // round schedule of the MFMA kernels: 16-element tiles, kMfWaves tiles per
+  // round, rounds never straddle a class
+  {
+    std::vector<int> r0(nclass + 1, 0);
+    std::vector<int64_t> rng(2 * (size_t)nclass);
+    for (int c = 0; c < nclass; c++) {
+      const int64_t tiles = (off[c + 1] - off[c] + 15) / 16;
+      r0[c + 1] = r0[c] + (int)((tiles + kMfWaves - 1) / kMfWaves);
+      rng[2 * c] = off[c];
+      rng[2 * c + 1] = off[c + 1];
+    }
+    nrounds = r0[nclass];
+    round0.alloc(nclass + 1);
+    round0.upload(r0.data(), nclass + 1);
+    cls_range.alloc(2 * (size_t)nclass);
+    cls_range.upload(rng.data(), 2 * (size_t)nclass);
   }
+  // enable the dynamic LDS size of every kernel variant this operator uses
+  // and record its occupancy
+  LaunchElements<0, double>(*this, nullptr, nullptr, 1.0, nullptr, nullptr, true);
+  LaunchElements<1, double>(*this, nullptr, nullptr, 1.0, nullptr, nullptr, true);
+  LaunchElements<0, float>(*this, nullptr, nullptr, 1.0, nullptr, nullptr, true);
+  LaunchElements<1, float>(*this, nullptr, nullptr, 1.0, nullptr, nullptr, true);
   NGSAMD_HIP(hipStreamSynchronize(Stream()));
 }
 
+// Workgroups of one launch: every round's worth if it fits, else as many as
+// are resident at once (occupancy queried when the operator is built).
+static int MfGrid(const MatFreeOp& A, int mode, bool f64_vectors) {
+  const int per_cu = std::max(1, A.occ[mode][f64_vectors ? 1 : 0]);
+  return (int)std::max<int64_t>(1, std::min<int64_t>(A.nrounds, (int64_t)DeviceCUs() * per_cu));
+}
+
+// prepare: enable the kernel's dynamic LDS size and record its occupancy
+// (once, outside any stream capture)
+template <class K>
+static void PrepareKernel(const MatFreeOp& A, K kern, size_t lds, int mode, bool f64_vectors) {
+  NGSAMD_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  int nb = 0;
+  NGSAMD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, kMfBlock, lds));
+  A.occ[mode][f64_vectors ? 1 : 0] = nb;
+}
+
 template <class T, int DIM, int NDP, int MODE, class V>
-static void LaunchMfma(const MatFreeOp& A, const T* b, const MfClass& c, const V* x, V* out, double s,
-                       const int* stop, hipStream_t st, bool prepare) {
+static void LaunchMfma(const MatFreeOp& A, const T* b, const V* x, V* out, double s, const int* stop, hipStream_t st,
+                       bool prepare) {
   const int nqb = A.nqp / 16;
-  const size_t lds = (size_t)nqb * 16 * (DIM * NDP + 1) * sizeof(T);
-  auto kern = A.tiles_per_wave == 2 ? MfMfmaKernel<T, DIM, NDP, MODE, 2, V> : MfMfmaKernel<T, DIM, NDP, MODE, 1, V>;
-  if (prepare) {
-    // dynamic LDS above the default limit must be enabled per kernel, once,
-    // outside any stream capture
-    NGSAMD_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    return;
-  }
-  const int64_t ntiles = (c.end - c.begin + 16 * A.tiles_per_wave - 1) / (16 * A.tiles_per_wave);
-  int64_t grid = (ntiles + kMfWaves - 1) / kMfWaves;
-  const int64_t maxg = (int64_t)DeviceCUs() * (lds > 81920 ? 1 : 2);
-  if (grid > maxg) grid = maxg;
-  if (grid < 1) grid = 1;
-  kern<<<(int)grid, kMfBlock, lds, st>>>(b, A.weights.p, nqb, A.dofs.p, A.metric.p, A.Kappa(), A.elnr.p, c.begin,
-                                         c.end, A.nd, x, out, s, stop);
+  const size_t lds = (size_t)A.bstride * sizeof(T);
+  auto kern = MfMfmaKernel<T, DIM, NDP, MODE, V>;
+  if (prepare) return PrepareKernel(A, kern, lds, MODE, sizeof(V) == 8);
+  const MfRounds R{A.round0.p, A.cls_range.p, A.nrounds};
+  kern<<<MfGrid(A, MODE, sizeof(V) == 8), kMfBlock, lds, st>>>(b, A.bstride, R, A.weights.p, nqb, A.dofs.p,
+                                                               A.metric.p, A.Kappa(), A.elnr.p, A.nd, x, out, s, stop);
   NGSAMD_CHECK_LAUNCH();
 }
 
 template <class T, int DIM, int MODE, class V>
-static void DispatchNdp(const MatFreeOp& A, const T* b, const MfClass& c, const V* x, V* out, double s,
-                        const int* stop, hipStream_t st, bool prepare) {
+static void DispatchNdp(const MatFreeOp& A, const T* b, const V* x, V* out, double s, const int* stop,
+                        hipStream_t st, bool prepare) {
   switch (A.ndp) {
-    case 4: LaunchMfma<T, DIM, 4, MODE, V>(A, b, c, x, out, s, stop, st, prepare); break;
-    case 8: LaunchMfma<T, DIM, 8, MODE, V>(A, b, c, x, out, s, stop, st, prepare); break;
-    case 12: LaunchMfma<T, DIM, 12, MODE, V>(A, b, c, x, out, s, stop, st, prepare); break;
-    case 16: LaunchMfma<T, DIM, 16, MODE, V>(A, b, c, x, out, s, stop, st, prepare); break;
-    case 20: LaunchMfma<T, DIM, 20, MODE, V>(A, b, c, x, out, s, stop, st, prepare); break;
-    case 24: LaunchMfma<T, DIM, 24, MODE, V>(A, b, c, x, out, s, stop, st, prepare); break;
-    case 36: LaunchMfma<T, DIM, 36, MODE, V>(A, b, c, x, out, s, stop, st, prepare); break;
+    case 4: LaunchMfma<T, DIM, 4, MODE, V>(A, b, x, out, s, stop, st, prepare); break;
+    case 8: LaunchMfma<T, DIM, 8, MODE, V>(A, b, x, out, s, stop, st, prepare); break;
+    case 12: LaunchMfma<T, DIM, 12, MODE, V>(A, b, x, out, s, stop, st, prepare); break;
+    case 16: LaunchMfma<T, DIM, 16, MODE, V>(A, b, x, out, s, stop, st, prepare); break;
+    case 20: LaunchMfma<T, DIM, 20, MODE, V>(A, b, x, out, s, stop, st, prepare); break;
+    case 24: LaunchMfma<T, DIM, 24, MODE, V>(A, b, x, out, s, stop, st, prepare); break;
+    case 36: LaunchMfma<T, DIM, 36, MODE, V>(A, b, x, out, s, stop, st, prepare); break;
     default: throw Error(NGSAMD_E_STATE, "matrix-free: unsupported padded dof count");
   }
 }
 
 template <int DIM, int KS1, int MT, int MODE, class V>
-static void LaunchBf16(const MatFreeOp& A, const uint16_t* b, const MfClass& c, const V* x, V* out, double s,
-                       const int* stop, hipStream_t st, bool prepare) {
+static void LaunchBf16(const MatFreeOp& A, const V* x, V* out, double s, const int* stop, hipStream_t st,
+                       bool prepare) {
   const size_t lds = (size_t)A.bstride * 2;
   auto kern = MfBf16Kernel<DIM, KS1, MT, MODE, V>;
-  if (prepare) {
-    NGSAMD_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    return;
-  }
-  const int64_t ntiles = (c.end - c.begin + 15) / 16;
-  int64_t grid = (ntiles + kMfWaves - 1) / kMfWaves;
-  const int64_t maxg = (int64_t)DeviceCUs() * (lds > 81920 ? 1 : 2);
-  if (grid > maxg) grid = maxg;
-  if (grid < 1) grid = 1;
-  kern<<<(int)grid, kMfBlock, lds, st>>>(b, A.k1p, A.nqc, A.weights.p, A.dofs.p, A.metric.p, A.Kappa(), A.elnr.p,
-                                         c.begin, c.end, A.nd, x, out, s, stop);
+  if (prepare) return PrepareKernel(A, kern, lds, MODE, sizeof(V) == 8);
+  const MfRounds R{A.round0.p, A.cls_range.p, A.nrounds};
+  kern<<<MfGrid(A, MODE, sizeof(V) == 8), kMfBlock, lds, st>>>(A.bmat16.p, A.bstride, R, A.k1p, A.nqc, A.weights.p,
+                                                               A.dofs.p, A.metric.p, A.Kappa(), A.elnr.p, A.nd, x,
+                                                               out, s, stop);
   NGSAMD_CHECK_LAUNCH();
 }
 
 template <int DIM, int MODE, class V>
-static void DispatchBf16(const MatFreeOp& A, const uint16_t* b, const MfClass& c, const V* x, V* out, double s,
-                         const int* stop, hipStream_t st, bool prepare) {
-  if (A.ks1 == 1 && A.mt16 == 1) LaunchBf16<DIM, 1, 1, MODE, V>(A, b, c, x, out, s, stop, st, prepare);
-  else if (A.ks1 == 1 && A.mt16 == 2) LaunchBf16<DIM, 1, 2, MODE, V>(A, b, c, x, out, s, stop, st, prepare);
-  else if (A.ks1 == 2 && A.mt16 == 3) LaunchBf16<DIM, 2, 3, MODE, V>(A, b, c, x, out, s, stop, st, prepare);
+static void DispatchBf16(const MatFreeOp& A, const V* x, V* out, double s, const int* stop, hipStream_t st,
+                         bool prepare) {
+  if (A.ks1 == 1 && A.mt16 == 1) LaunchBf16<DIM, 1, 1, MODE, V>(A, x, out, s, stop, st, prepare);
+  else if (A.ks1 == 1 && A.mt16 == 2) LaunchBf16<DIM, 1, 2, MODE, V>(A, x, out, s, stop, st, prepare);
+  else if (A.ks1 == 2 && A.mt16 == 3) LaunchBf16<DIM, 2, 3, MODE, V>(A, x, out, s, stop, st, prepare);
   else throw Error(NGSAMD_E_STATE, "matrix-free: unsupported BF16 shape");
 }
 
+// All elements: one launch of the MFMA kernel (every class), or the scalar
+// kernel class by class.
 template <int MODE, class V>
-static void LaunchClass(const MatFreeOp& A, int ci, const V* x, V* out, double s, const int* stop, hipStream_t st,
-                        bool prepare) {
-  const MfClass& c = A.classes[ci];
-  if (c.end <= c.begin) return;
+static void LaunchElements(const MatFreeOp& A, const V* x, V* out, double s, const int* stop, hipStream_t st,
+                           bool prepare) {
   if (A.use_mfma) {
+    if (A.nrounds == 0 && !prepare) return;
     if (A.precision == NGSAMD_PREC_BF16) {
-      const uint16_t* b = A.bmat16.p + (size_t)ci * A.bstride;
-      if (A.dim == 2) DispatchBf16<2, MODE, V>(A, b, c, x, out, s, stop, st, prepare);
-      else DispatchBf16<3, MODE, V>(A, b, c, x, out, s, stop, st, prepare);
+      if (A.dim == 2) DispatchBf16<2, MODE, V>(A, x, out, s, stop, st, prepare);
+      else DispatchBf16<3, MODE, V>(A, x, out, s, stop, st, prepare);
     } else if (A.precision == NGSAMD_PREC_F64) {
-      const double* b = A.bmat64.p + (size_t)ci * A.bstride;
-      if (A.dim == 2) DispatchNdp<double, 2, MODE, V>(A, b, c, x, out, s, stop, st, prepare);
-      else DispatchNdp<double, 3, MODE, V>(A, b, c, x, out, s, stop, st, prepare);
+      if (A.dim == 2) DispatchNdp<double, 2, MODE, V>(A, A.bmat64.p, x, out, s, stop, st, prepare);
+      else DispatchNdp<double, 3, MODE, V>(A, A.bmat64.p, x, out, s, stop, st, prepare);
     } else {
-      const float* b = A.bmat32.p + (size_t)ci * A.bstride;
-      if (A.dim == 2) DispatchNdp<float, 2, MODE, V>(A, b, c, x, out, s, stop, st, prepare);
-      else DispatchNdp<float, 3, MODE, V>(A, b, c, x, out, s, stop, st, prepare);
+      if (A.dim == 2) DispatchNdp<float, 2, MODE, V>(A, A.bmat32.p, x, out, s, stop, st, prepare);
+      else DispatchNdp<float, 3, MODE, V>(A, A.bmat32.p, x, out, s, stop, st, prepare);
     }
-  } else {
-    if (prepare) return;
-    const double* b = A.bref.p + (size_t)ci * A.nq * A.nd * A.dim;
-    const int64_t n = c.end - c.begin;
-    const int grid = GridFor(n, kScalarBlock, 4096);
+    return;
+  }
+  if (prepare) return;
+  for (size_t ci = 0; ci < A.classes.size(); ci++) {
+    const MfClass& c = A.classes[ci];
+    if (c.end <= c.begin) continue;
+    const double* b = A.bref.p + ci * A.nq * A.nd * A.dim;
+    const int grid = GridFor(c.end - c.begin, kScalarBlock, 4096);
     if (A.dim == 2)
       MfScalarKernel<2, MODE, V><<<grid, kScalarBlock, 0, st>>>(b, A.weights.p, A.nq, A.dofs.p, A.metric.p, A.Kappa(),
                                                                A.elnr.p, c.begin, c.end, A.nd, x, out, s, stop);
@@ -710,7 +788,7 @@ void MatFreeOp::Apply(double s, const V* x, V* y, bool accumulate, const int* st
       ScaleFillKernel<V><<<GridFor(height, 256, 8192), 256, 0, st>>>(y, height, stop);
       NGSAMD_CHECK_LAUNCH();
     }
-    for (size_t c = 0; c < classes.size(); c++) LaunchClass<1, V>(*this, (int)c, x, y, s, stop, st, false);
+    LaunchElements<1, V>(*this, x, y, s, stop, st, false);
     return;
   }
   // contribution buffers are allocated by the constructor: nothing here
@@ -718,7 +796,7 @@ void MatFreeOp::Apply(double s, const V* x, V* y, bool accumulate, const int* st
   V* contrib;
   if constexpr (sizeof(V) == 8) contrib = reinterpret_cast<V*>(contrib64.p);
   else contrib = reinterpret_cast<V*>(contrib32.p);
-  for (size_t c = 0; c < classes.size(); c++) LaunchClass<0, V>(*this, (int)c, x, contrib, 1.0, stop, st, false);
+  LaunchElements<0, V>(*this, x, contrib, 1.0, stop, st, false);
   MfGatherKernel<V, V><<<GridFor(height, 256, 16384), 256, 0, st>>>(g_rowptr.p, g_slots.p, contrib, y, height, s,
                                                                     accumulate ? 1 : 0, stop);
   NGSAMD_CHECK_LAUNCH();

@@ -17,9 +17,13 @@ struct MatFreeOp : ngsamd_op_s {
   int precision = NGSAMD_PREC_F64;
   int scatter = NGSAMD_SCATTER_GATHER;
   bool use_mfma = false;
-  int tiles_per_wave = 1;     // element tiles of 16 per wavefront (FP64/FP32 MFMA path)
   int64_t nel = 0;
   std::vector<MfClass> classes;
+  // MFMA kernels' schedule: rounds of 16-element tiles (one per wavefront)
+  DevBuf<int> round0;         // first round of each class, nclass + 1
+  DevBuf<int64_t> cls_range;  // [begin, end) of each class
+  int nrounds = 0;
+  mutable int occ[2][2] = {{1, 1}, {1, 1}};  // resident workgroups per CU [mode][FP64 vectors]
   // per class reference gradients, MFMA layout [nqp][dim][ndp] (+1 pad per q),
   // in the contraction type (double / float / bf16 bits)
   DevBuf<double> bmat64;

@@ -4,10 +4,18 @@
 // row kernel RowTimesVector linalg/sparsematrix.hpp:625-632) and the
 // reference's cuSPARSE call (ngscuda/cuda_linalg.cpp:243-275).
 //
-// Layout: int32 row pointers, int32 column numbers, FP64 values (12 B per
-// nonzero streamed once).  Two kernels:
+// Three kernels:
 //
-// stream (default): the rows are cut on the host into blocks of consecutive
+// runs (default when it saves at least a quarter of the column numbers): up
+// to 8 consecutive rows with the same column list (the dofs of one edge, face
+// or cell of an H1 space) form a run that stores the list once and its values
+// column by column.  Values still stream once (8 B per nonzero) but the column
+// numbers drop to 4 B per run entry and each gathered x entry serves the
+// whole run.  Blocks hold runs of one length and at most 4096 values; the
+// block loop is the stream kernel's.
+//
+// stream: int32 row pointers, int32 column numbers, FP64 values (12 B per
+// nonzero streamed once); the rows are cut on the host into blocks of consecutive
 // rows holding at most 2048 nonzeros (NGSAMD_SPMV_NNZ=1024/4096 selects other
 // sizes; a single longer row is a block of its own).  A 256-thread workgroup streams a block's column and value
 // ranges with fully coalesced non-temporal loads (they are read once and
@@ -19,8 +27,9 @@
 // nonzeros l, l+G, ...; the partial sums are combined with cross-lane
 // shuffles.  Kept for comparison (NGSAMD_SPMV=rowgroup).
 //
-// Both walk contiguous row ranges that are grouped per XCD (workgroup i runs
+// All three walk contiguous row ranges that are grouped per XCD (workgroup i runs
 // on XCD i mod 8), so the x entries a row range gathers stay in one XCD's L2.
+#include <algorithm>
 #include <cstdlib>
 #include <string>
 
@@ -32,13 +41,6 @@
 namespace ngsamd {
 
 constexpr int kSpmvBlock = 256;
-
-// Map the hardware workgroup id to a logical id so that the 8 XCDs each own a
-// contiguous range of logical ids.
-__device__ __forceinline__ int XcdContiguousBlock(int b, int grid) {
-  if ((grid & 7) != 0) return b;
-  return (b & 7) * (grid >> 3) + (b >> 3);
-}
 
 // y[r] = (ACCUM ? y[r] : 0) + s * (A x)[r]; optionally the per-workgroup
 // partial sums of x[r]*(A x)[r] (the <s, A s> of CG) into partials[].
@@ -152,19 +154,25 @@ __global__ __launch_bounds__(kStreamBlock) void CsrStreamKernel(
         }
       }
     } else {
+      // every load is unconditional (lanes past the block read entry k0,
+      // which the padding keeps in bounds, and contribute 0) so all of them
+      // are in flight at once instead of one round trip per branch
       constexpr int PER = NNZ / kStreamBlock;
       int c[PER];
-      double v[PER];
+      double v[PER], xv[PER];
 #pragma unroll
       for (int i = 0; i < PER; i++) {
         const int k = tid + i * kStreamBlock;
-        c[i] = k < nz ? __builtin_nontemporal_load(&cols[k0 + k]) : -1;
-        v[i] = k < nz ? __builtin_nontemporal_load(&vals[k0 + k]) : 0.0;
+        const int kk = k0 + (k < nz ? k : 0);
+        c[i] = __builtin_nontemporal_load(&cols[kk]);
+        v[i] = __builtin_nontemporal_load(&vals[kk]);
       }
 #pragma unroll
+      for (int i = 0; i < PER; i++) xv[i] = x[c[i]];
+#pragma unroll
       for (int i = 0; i < PER; i++) {
         const int k = tid + i * kStreamBlock;
-        if (c[i] >= 0) prod[k] = v[i] * x[c[i]];
+        prod[k] = k < nz ? v[i] * xv[i] : 0.0;
       }
     }
     __syncthreads();
@@ -190,6 +198,180 @@ __global__ __launch_bounds__(kStreamBlock) void CsrStreamKernel(
   if (DOT) StorePartial<kStreamBlock>(dotacc, partials, lds);
 }
 
+// One block of runs with R rows each: runs [u0, u1), values
+// [val_ptr[u0], val_ptr[u1]) (at most NNZ), rows run_row[u0] + j.  Lane k of
+// the value range forms value k times x[column of shared slot k / R] into LDS
+// (values stream coalesced; the R lanes of one slot read one column number
+// and one x entry); row j = run * R + q then sums slots of its run at stride R.
+template <int NNZ, int R, bool ACCUM, bool DOT>
+__device__ __forceinline__ void RunsBlock(const int* __restrict__ run_row, const int* __restrict__ run_ptr,
+                                          const int* __restrict__ val_ptr, const int* __restrict__ rcols,
+                                          const double* __restrict__ rvals, const double* __restrict__ x,
+                                          double* __restrict__ y, int u0, int u1, double s, double* prod,
+                                          double& dotacc) {
+  const int tid = threadIdx.x;
+  const int c0 = run_ptr[u0];
+  const int vb = val_ptr[u0];
+  const int nv = val_ptr[u1] - vb;
+  constexpr int PER = NNZ / kStreamBlock;
+  int c[PER];
+  double v[PER], xv[PER];
+  // unconditional loads (lanes past the block read entry 0 of it, kept in
+  // bounds by the padding) so that every load is in flight at once
+#pragma unroll
+  for (int i = 0; i < PER; i++) {
+    const int k = tid + i * kStreamBlock;
+    const int kk = k < nv ? k : 0;
+    c[i] = __builtin_nontemporal_load(&rcols[c0 + kk / R]);
+    v[i] = __builtin_nontemporal_load(&rvals[vb + kk]);
+  }
+#pragma unroll
+  for (int i = 0; i < PER; i++) xv[i] = x[c[i]];
+#pragma unroll
+  for (int i = 0; i < PER; i++) {
+    const int k = tid + i * kStreamBlock;
+    prod[k] = k < nv ? v[i] * xv[i] : 0.0;
+  }
+  __syncthreads();
+  const int row0 = run_row[u0];
+  const int nr = (u1 - u0) * R;
+  int tpr = 64;
+  while (tpr > 1 && tpr * nr > kStreamBlock) tpr >>= 1;
+  const int l = tid & (tpr - 1);
+  for (int j0 = 0; j0 < nr; j0 += kStreamBlock / tpr) {
+    const int j = j0 + tid / tpr;
+    double sum = 0.0;
+    if (j < nr) {
+      const int run = j / R, q = j - run * R;
+      const int s0 = run_ptr[u0 + run] - c0, s1 = run_ptr[u0 + run + 1] - c0;
+      for (int k = s0 + l; k < s1; k += tpr) sum += prod[k * R + q];
+    }
+    for (int off = tpr / 2; off > 0; off >>= 1) sum += __shfl_xor(sum, off, 64);
+    if (j < nr && l == 0) {
+      if (ACCUM) y[row0 + j] += s * sum; else y[row0 + j] = s * sum;
+      if (DOT) dotacc += x[row0 + j] * sum;
+    }
+  }
+  __syncthreads();
+}
+
+// y = (ACCUM ? y : 0) + s * A x over the run blocks [blk_run[b], blk_run[b+1])
+// with blk_len[b] rows per run; DOT as in CsrSpmvKernel.  A block holding
+// more than NNZ values is a single row (R = 1) that the whole workgroup strides.
+template <int NNZ, bool ACCUM, bool DOT>
+__global__ __launch_bounds__(kStreamBlock) void CsrRunsKernel(
+    const int* __restrict__ run_row, const int* __restrict__ run_ptr, const int* __restrict__ val_ptr,
+    const int* __restrict__ rcols, const double* __restrict__ rvals, const double* __restrict__ x,
+    double* __restrict__ y, const int* __restrict__ blk_run, const int* __restrict__ blk_len, int nblocks,
+    int blocks_per_wg, double s, double* __restrict__ partials, const int* __restrict__ stop_flag) {
+  if (stop_flag && *stop_flag) return;
+  __shared__ double prod[NNZ];
+  __shared__ double lds[kStreamBlock / 64];
+  const int tid = threadIdx.x;
+  const int lb = XcdContiguousBlock(blockIdx.x, gridDim.x);
+  const int b0 = lb * blocks_per_wg;
+  const int b1 = min(b0 + blocks_per_wg, nblocks);
+  double dotacc = 0.0;
+  for (int b = b0; b < b1; b++) {
+    const int u0 = blk_run[b], u1 = blk_run[b + 1];
+    const int k0 = val_ptr[u0], k1 = val_ptr[u1];
+    if (k1 - k0 > NNZ) {
+      const int c0 = run_ptr[u0] - k0;
+      double a = 0.0;
+      for (int k = k0 + tid; k < k1; k += kStreamBlock)
+        a += __builtin_nontemporal_load(&rvals[k]) * x[__builtin_nontemporal_load(&rcols[c0 + k])];
+      a = BlockSum<kStreamBlock>(a, lds);
+      const int r = run_row[u0];
+      if (tid == 0) {
+        if (ACCUM) y[r] += s * a; else y[r] = s * a;
+        if (DOT) dotacc += x[r] * a;
+      }
+      continue;
+    }
+    switch (blk_len[b]) {
+#define NGSAMD_RUNS_CASE(RR)                                                                                 \
+  case RR:                                                                                                  \
+    RunsBlock<NNZ, RR, ACCUM, DOT>(run_row, run_ptr, val_ptr, rcols, rvals, x, y, u0, u1, s, prod, dotacc); \
+    break;
+      NGSAMD_RUNS_CASE(1)
+      NGSAMD_RUNS_CASE(2)
+      NGSAMD_RUNS_CASE(3)
+      NGSAMD_RUNS_CASE(4)
+      NGSAMD_RUNS_CASE(5)
+      NGSAMD_RUNS_CASE(6)
+      NGSAMD_RUNS_CASE(7)
+      NGSAMD_RUNS_CASE(8)
+#undef NGSAMD_RUNS_CASE
+      default: break;
+    }
+  }
+  if (DOT) StorePartial<kStreamBlock>(dotacc, partials, lds);
+}
+
+// Splits the rows into pattern runs (at most kMaxRun rows, at most max_vals
+// values per run unless the run is one row) and the runs into blocks of equal
+// run length with at most max_vals values (or one longer row).  Fills the
+// host arrays of the runs layout; returns the number of shared columns.
+static int64_t BuildRuns(int64_t h, const int64_t* rp, const int* c, const double* v, int max_vals,
+                         std::vector<int>& run_row, std::vector<int>& run_ptr, std::vector<int>& val_ptr,
+                         std::vector<int>& rcols, std::vector<double>& rvals, std::vector<int>& blk_run,
+                         std::vector<int>& blk_len) {
+  run_row.assign(1, 0);
+  run_ptr.assign(1, 0);
+  val_ptr.assign(1, 0);
+  rcols.clear();
+  rvals.clear();
+  blk_run.assign(1, 0);
+  blk_len.clear();
+  int64_t acc = 0;
+  int cur_len = 0;
+  for (int64_t r = 0; r < h;) {
+    const int64_t L = rp[r + 1] - rp[r];
+    int R = 1;
+    const int rmax = L > 0 ? (int)std::max<int64_t>(1, std::min<int64_t>(kMaxRun, max_vals / L)) : kMaxRun;
+    while (R < rmax && r + R < h && rp[r + R + 1] - rp[r + R] == L &&
+           std::equal(c + rp[r], c + rp[r] + L, c + rp[r + R]))
+      R++;
+    const int64_t nv = int64_t(R) * L;
+    const int u = (int)run_row.size() - 1;
+    if (u > blk_run.back() && (R != cur_len || acc + nv > max_vals)) {
+      blk_len.push_back(cur_len);
+      blk_run.push_back(u);
+      acc = 0;
+    }
+    cur_len = R;
+    acc += nv;
+    rcols.insert(rcols.end(), c + rp[r], c + rp[r] + L);
+    for (int64_t j = 0; j < L; j++)
+      for (int q = 0; q < R; q++) rvals.push_back(v[rp[r + q] + j]);
+    r += R;
+    run_row.push_back((int)r);
+    run_ptr.push_back((int)rcols.size());
+    val_ptr.push_back((int)rvals.size());
+  }
+  const int nruns = (int)run_row.size() - 1;
+  if (nruns > blk_run.back()) {
+    blk_len.push_back(cur_len);
+    blk_run.push_back(nruns);
+  }
+  return (int64_t)rcols.size();
+}
+
+__global__ void RunsDiagKernel(const int* __restrict__ run_row, const int* __restrict__ run_ptr,
+                               const int* __restrict__ val_ptr, const int* __restrict__ rcols,
+                               const double* __restrict__ rvals, double* __restrict__ d, int nruns) {
+  for (int u = blockIdx.x * blockDim.x + threadIdx.x; u < nruns; u += gridDim.x * blockDim.x) {
+    const int r0 = run_row[u], R = run_row[u + 1] - r0;
+    const int c0 = run_ptr[u], L = run_ptr[u + 1] - c0;
+    for (int q = 0; q < R; q++) {
+      double val = 0.0;
+      for (int j = 0; j < L; j++)
+        if (rcols[c0 + j] == r0 + q) { val = rvals[val_ptr[u] + j * R + q]; break; }
+      d[r0 + q] = val;
+    }
+  }
+}
+
 CsrOp::CsrOp(int64_t h, int64_t w, const int64_t* rp, const int* c, const double* v)
     : ngsamd_op_s(OpKind::CSR) {
   height = h;
@@ -206,6 +388,67 @@ CsrOp::CsrOp(int64_t h, int64_t w, const int64_t* rp, const int* c, const double
   }
   for (int64_t k = 0; k < nnz; k++)
     NGSAMD_REQUIRE(c[k] >= 0 && c[k] < w, "ngsamd_csr_create: column number out of range");
+  if (w == 0) {
+    static const double z = 0.0;
+    xpad.alloc(1);
+    xpad.upload(&z, 1);
+  }
+  const char* nnz_env = getenv("NGSAMD_SPMV_NNZ");
+  block_nnz = nnz_env ? atoi(nnz_env) : 0;
+  NGSAMD_REQUIRE(block_nnz == 0 || block_nnz == 1024 || block_nnz == 2048 || block_nnz == 4096,
+                 "NGSAMD_SPMV_NNZ must be 1024, 2048 or 4096");
+  const char* algo_env = getenv("NGSAMD_SPMV");
+  const std::string algo_name = algo_env ? algo_env : "";
+  NGSAMD_REQUIRE(algo_name.empty() || algo_name == "rowgroup" || algo_name == "stream" || algo_name == "runs",
+                 "NGSAMD_SPMV must be rowgroup, stream or runs");
+  if (algo_name.empty() || algo_name == "runs") {
+    // measured on the 32^3 p=4 matrix: 4096 values per block 412 us,
+    // 2048 423 us, 1024 429 us
+    const int saved = block_nnz;
+    if (block_nnz == 0) block_nnz = 4096;
+    // the runs layout pays when it saves at least a quarter of the column
+    // numbers (H1 matrices of order >= 2 share patterns between the dofs of
+    // an edge, a face or a cell)
+    std::vector<int> rr, rpp, vp, rc, br, bl;
+    std::vector<double> rv;
+    rc.reserve(nnz);
+    rv.reserve(nnz);
+    nshared = BuildRuns(h, rp, c, v, block_nnz, rr, rpp, vp, rc, rv, br, bl);
+    if (algo_name == "runs" || 4 * nshared <= 3 * nnz) {
+      algo = 2;
+      nruns = (int64_t)rr.size() - 1;
+      nblocks = (int)br.size() - 1;
+      run_row.alloc(rr.size());
+      run_ptr.alloc(rpp.size());
+      val_ptr.alloc(vp.size());
+      // 4 padding entries (column 0, value 0): the kernel's lanes past the
+      // end of the last block read entry nshared / nnz
+      rcols.alloc(nshared + 4);
+      rvals.alloc(nnz + 4);
+      {
+        const int zc[4] = {0, 0, 0, 0};
+        const double zv[4] = {0, 0, 0, 0};
+        rcols.upload_at(zc, nshared, 4);
+        rvals.upload_at(zv, nnz, 4);
+        NGSAMD_HIP(hipStreamSynchronize(Stream()));
+      }
+      blk_run.alloc(br.size());
+      blk_len.alloc(std::max<size_t>(bl.size(), 1));
+      run_row.upload(rr.data(), rr.size());
+      run_ptr.upload(rpp.data(), rpp.size());
+      val_ptr.upload(vp.data(), vp.size());
+      if (nshared) rcols.upload(rc.data(), nshared);
+      if (nnz) rvals.upload(rv.data(), nnz);
+      blk_run.upload(br.data(), br.size());
+      if (!bl.empty()) blk_len.upload(bl.data(), bl.size());
+      NGSAMD_HIP(hipStreamSynchronize(Stream()));
+      return;
+    }
+    block_nnz = saved;
+  }
+  // stream kernel: 2048 values per block (443.8 us on the 32^3 p=4 matrix,
+  // 4096 443.4 us, 1024 470.7 us)
+  if (block_nnz == 0) block_nnz = 2048;
   rowptr.alloc(h + 1);
   // 4 padding entries (column 0, value 0) behind the last nonzero for the
   // 16-byte loads of the stream kernel
@@ -225,10 +468,6 @@ CsrOp::CsrOp(int64_t h, int64_t w, const int64_t* rp, const int* c, const double
   double mean = h ? double(nnz) / double(h) : 0.0;
   group = mean > 48 ? 32 : mean > 20 ? 16 : mean > 9 ? 8 : 4;
   // row blocks of at most block_nnz nonzeros (or one longer row)
-  const char* nnz_env = getenv("NGSAMD_SPMV_NNZ");
-  block_nnz = nnz_env ? atoi(nnz_env) : 2048;
-  NGSAMD_REQUIRE(block_nnz == 1024 || block_nnz == 2048 || block_nnz == 4096,
-                 "NGSAMD_SPMV_NNZ must be 1024, 2048 or 4096");
   std::vector<int> br;
   br.push_back(0);
   int64_t acc = 0;
@@ -245,8 +484,7 @@ CsrOp::CsrOp(int64_t h, int64_t w, const int64_t* rp, const int* c, const double
   blk_row.alloc(br.size());
   blk_row.upload(br.data(), br.size());
   NGSAMD_HIP(hipStreamSynchronize(Stream()));
-  const char* algo_env = getenv("NGSAMD_SPMV");
-  algo = (algo_env && std::string(algo_env) == "rowgroup") ? 0 : 1;
+  algo = algo_name == "rowgroup" ? 0 : 1;
   const char* vec_env = getenv("NGSAMD_SPMV_VEC");
   vec = (vec_env && atoi(vec_env) == 4) ? 4 : 1;
 }
@@ -290,8 +528,26 @@ static int LaunchStream(const CsrOp& A, double s, const double* x, double* y, hi
 }
 
 template <bool ACCUM, bool DOT>
+static int LaunchRuns(const CsrOp& A, double s, const double* x, double* y, hipStream_t st, double* partials,
+                      const int* stop) {
+  int64_t wg = A.nblocks < kMaxReduceBlocks ? A.nblocks : kMaxReduceBlocks;
+  if (wg > 8) wg &= ~int64_t(7);
+  if (wg < 1) wg = 1;
+  const int per = (int)((A.nblocks + wg - 1) / wg);
+  auto kern = A.block_nnz == 1024   ? CsrRunsKernel<1024, ACCUM, DOT>
+              : A.block_nnz == 4096 ? CsrRunsKernel<4096, ACCUM, DOT>
+                                    : CsrRunsKernel<2048, ACCUM, DOT>;
+  kern<<<(int)wg, kStreamBlock, 0, st>>>(A.run_row.p, A.run_ptr.p, A.val_ptr.p, A.rcols.p, A.rvals.p, x, y,
+                                         A.blk_run.p, A.blk_len.p, A.nblocks, per, s, partials, stop);
+  NGSAMD_CHECK_LAUNCH();
+  return (int)wg;
+}
+
+template <bool ACCUM, bool DOT>
 static int LaunchSpmv(const CsrOp& A, double s, const double* x, double* y, hipStream_t st, double* partials,
                       const int* stop) {
+  if (A.width == 0) x = A.xpad.p;
+  if (A.algo == 2) return LaunchRuns<ACCUM, DOT>(A, s, x, y, st, partials, stop);
   if (A.algo == 1) return LaunchStream<ACCUM, DOT>(A, s, x, y, st, partials, stop);
   switch (A.group) {
     case 4: return LaunchSpmvG<4, ACCUM, DOT>(A, s, x, y, st, partials, stop);
@@ -327,6 +583,12 @@ __global__ void CsrDiagKernel(const int* __restrict__ rowptr, const int* __restr
 
 void CsrOp::Diagonal(double* d, hipStream_t st) {
   if (height == 0) return;
+  if (algo == 2) {
+    RunsDiagKernel<<<GridFor(nruns, 256, 8192), 256, 0, st>>>(run_row.p, run_ptr.p, val_ptr.p, rcols.p, rvals.p, d,
+                                                              (int)nruns);
+    NGSAMD_CHECK_LAUNCH();
+    return;
+  }
   CsrDiagKernel<<<GridFor(height, 256, 8192), 256, 0, st>>>(rowptr.p, cols.p, vals.p, d, (int)height);
   NGSAMD_CHECK_LAUNCH();
 }

@@ -1,6 +1,7 @@
 """CSR SpMV on the GPU against the oracle's CSR SpMV (FP64, 1e-13 relative
 in the inf-norm)."""
 import numpy as np
+import scipy.sparse as sp
 import pytest
 
 import oracle
@@ -90,3 +91,67 @@ def test_spmv_full_size_3d(device):
     dx.upload(2 * x)
     A.mult(dx, dy)
     assert np.array_equal(dy.download(), 2 * y)
+
+
+ALGOS = ["runs", "stream", "rowgroup"]
+
+
+def _runs_matrix(rng):
+    """Rows in runs of equal column lists: lengths 1..12 (runs longer than 8
+    rows, or with more than 2048 values, are split), empty rows, a row of
+    5000 nonzeros and runs of 1000-column rows."""
+    w = 6000
+    rows = []
+    for R, L in [(1, 7), (3, 40), (12, 9), (0, 0), (5, 1000), (2, 5000), (4, 0), (6, 33), (1, 2049), (8, 256),
+                 (9, 1), (3, 2048), (1, 1)]:
+        cols = np.sort(rng.choice(w, L, replace=False)).astype(np.int32)
+        rows += [cols] * max(R, 1)
+    rp = np.zeros(len(rows) + 1, dtype=np.int64)
+    rp[1:] = np.cumsum([len(r) for r in rows])
+    cols = np.concatenate(rows).astype(np.int32)
+    return len(rows), w, rp, cols, rng.standard_normal(len(cols))
+
+
+@pytest.mark.parametrize("algo", ALGOS)
+def test_spmv_kernels_on_fe_matrices(device, monkeypatch, algo):
+    """Each SpMV kernel on 3D p=4 and 2D p=6 (face and cell runs of 10 dofs
+    are split) matrices, plus the fused <x, Ax> path through CG."""
+    monkeypatch.setenv("NGSAMD_SPMV", algo)
+    for dim, n, p in [(3, 4, 4), (2, 6, 6)]:
+        say(f"[spmv] kernel {algo} dim={dim} n={n} p={p}")
+        P = problem(dim, n, p, seed_kappa=3)
+        A = N.CreateCSR(P["ndof"], P["ndof"], P["rowptr"], P["cols"], P["vals"])
+        x = np.random.default_rng(2).uniform(-1, 1, P["ndof"])
+        y0 = np.random.default_rng(8).uniform(-1, 1, P["ndof"])
+        dx, dy = dvec(x), dvec(y0)
+        A.mult_add(0.75, dx, dy)
+        ref = oracle.spmv(P["rowptr"], P["cols"], P["vals"], x, y0.copy(), s=0.75, accumulate=True)
+        assert relerr_inf(dy.download(), ref) < 1e-13
+        d = N.DeviceVector(P["ndof"], 8)
+        A.diagonal(d)
+        dref = sp.csr_matrix((P["vals"], P["cols"], P["rowptr"]), shape=(P["ndof"], P["ndof"])).diagonal()
+        assert np.array_equal(d.download(), dref)
+
+
+@pytest.mark.parametrize("algo", ALGOS)
+def test_spmv_kernels_on_runs_edge_cases(device, monkeypatch, algo):
+    monkeypatch.setenv("NGSAMD_SPMV", algo)
+    say(f"[spmv] kernel {algo} run edge cases")
+    rng = np.random.default_rng(11)
+    n, w, rp, cols, vals = _runs_matrix(rng)
+    A = N.CreateCSR(n, w, rp, cols, vals)
+    x = rng.standard_normal(w)
+    dx, dy = dvec(x), dvec(np.full(n, 3.0))
+    A.mult(dx, dy)
+    assert relerr_inf(dy.download(), oracle.spmv(rp, cols, vals, x)) < 1e-13
+
+
+@pytest.mark.parametrize("algo", ALGOS)
+def test_spmv_zero_width(device, monkeypatch, algo):
+    """A 3 x 0 matrix (all rows empty): y = A x = 0 with an empty x."""
+    monkeypatch.setenv("NGSAMD_SPMV", algo)
+    say(f"[spmv] kernel {algo} zero width")
+    A = N.CreateCSR(3, 0, np.zeros(4, dtype=np.int64), np.zeros(0, dtype=np.int32), np.zeros(0))
+    dx, dy = N.DeviceVector(0, 8), dvec(np.full(3, 5.0))
+    A.mult(dx, dy)
+    assert np.array_equal(dy.download(), np.zeros(3))
