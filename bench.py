@@ -78,7 +78,7 @@ def cpu_baseline(rowptr, cols, vals, x, ndof, budget_s=10.0):
         oracle.spmv(rowptr, cols, vals, x, y)
         reps += 1
         el = time.perf_counter() - t0
-        if el >= budget_s or reps >= 200:
+        if el >= budget_s:
             break
     return {"value": round(ndof * reps / el / 1e9, 4), "unit": "GDoF/s", "cores": oracle.num_threads(),
             "kind": "port",
@@ -143,13 +143,14 @@ def main():
     ap.add_argument("--no-cg", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-problems", type=int, default=512)
+    ap.add_argument("--profile-cg", action="store_true",
+                    help="one FP64 CSR CG solve (kappa = 1, tol 1e-10) without graph capture, for a kernel trace")
     ap.add_argument("--profile-mode", action="store_true",
                     help="short run for rocprofv3 counter passes: a few CSR and matrix-free applies and one "
                          "1 GiB device-to-device copy (the calibration kernel of known size); no JSON line")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path (see DESIGN.md for how the "
-              "operator would shard)", file=sys.stderr)
+        print("bench.py measures one GPU: the operator and solver have no multi-GPU path", file=sys.stderr)
         sys.exit(2)
 
     from ngsolve_amd import _ngsamd as N
@@ -170,6 +171,18 @@ def main():
     x = np.random.default_rng(1).uniform(-1, 1, ndof)
     dx, dy = N.DeviceVector(ndof, 8), N.DeviceVector(ndof, 8)
     dx.upload(x)
+
+    if args.profile_cg:
+        dd = N.DeviceVector(ndof, 8)
+        A.diagonal(dd)
+        C = N.CreateJacobi(dd.download(), fes.freedofs)
+        f = N.AssembleLinearForm(fes, ir, np.ones((ne, len(ir))))
+        df, du = N.DeviceVector(ndof, 8), N.DeviceVector(ndof, 8)
+        df.upload(f)
+        its, _ = N.CGSolve(A, C, df, du, True, 1e-10, -1.0, 100000, 0)
+        N.synchronize()
+        log(f"[bench] profile CG done: {its} iterations")
+        return
 
     if args.profile_mode:
         for _ in range(3):
@@ -303,11 +316,15 @@ def main():
         cpu = cpu_baseline(rowptr, cols, vals, x, ndof)
         log(f"[bench] cpu {cpu['value']} GDoF/s on {cpu['cores']} threads")
 
+    # HBM bytes per launch of the default CSR kernel, from the rocprofv3 PMC
+    # passes summarised under profiles/ (only if they measured this kernel)
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tfile):
         with open(tfile) as fh:
-            traffic = json.load(fh).get("csr_spmv_32cube_p4")
+            tj = json.load(fh)
+        if "CsrRunsKernel" in tj.get("kernel", ""):
+            traffic = tj.get("csr_spmv_32cube_p4")
 
     out = {
         "metric": "GDoF/s apply + time-to-1e-10 CG, mixed vs FP64 on 1 GPU, % HBM/MFMA roofline",

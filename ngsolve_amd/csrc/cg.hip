@@ -13,17 +13,22 @@
 // check: iterations += 1; converged when iterations >= maxiter or
 // residual <= final, final = tol*res0 (max'ed with atol when given).
 //
-// One iteration is five kernels: (1) w = A s fused with the partial sums of
-// <s,w> (CSR) or the operator followed by a partial-dot kernel, (2) a
-// one-workgroup kernel forming alpha, (3) the two updates, the Jacobi apply
-// and the partials of <w,d> fused, (4) a one-workgroup kernel doing the
-// stopping test and beta, (5) s = beta s + w.  Scalars, the iteration count
-// and the residual history stay on the device, so the loop needs no host
-// round trip; the one-workgroup kernels sum the partials in index order, so
-// every run gives the same iterates bit for bit.  `chunk` iterations are
-// captured once into a HIP graph that is replayed until the device flag says
-// done; every kernel returns at once when the flag is set, so the overshoot
-// of the last chunk costs only launch latency.  (Reference: DevCGSolver,
+// One iteration is three kernels after the operator: (1) w = A s with the
+// partial sums of <s,w> (fused into the CSR kernel and into the matrix-free
+// gather kernel), (2) the two updates, the Jacobi apply and the partials of
+// <w,d>, each workgroup first forming alpha itself from the partials of (1),
+// (3) s = beta s + w, each workgroup forming beta from the partials of (2);
+// workgroup 0 of (3) also does the stopping test and keeps the iteration
+// count and the residual history.  Every workgroup sums the same partials
+// in the same order, so all of them hold the same alpha / beta bit for bit
+// and every run gives the same iterates.  <w,d> is kept per iteration parity
+// (wdn[it & 1]): kernel (3) of an even iteration reads wdn[0] and its
+// workgroup 0 writes wdn[1], which the next iteration reads, so no kernel
+// reads a scalar that it also writes.  Scalars stay on the device, so the
+// loop needs no host round trip.  `chunk` (even) iterations are captured once
+// into a HIP graph that is replayed until the device flag says done; every
+// kernel returns at once when the flag is set, so the overshoot of the last
+// chunk costs only launch latency.  (Reference: DevCGSolver,
 // ngscuda/cuda_krylov.cpp:19-203, which uses a CUDA conditional WHILE node.)
 //
 // The vector type V is double for the FP64 solver and float for the inner
@@ -96,7 +101,7 @@ void JacobiOp::Mult(const double* x, double* y, hipStream_t st) {
 // ---------------------------------------------------------- CG device state
 
 struct CGScalars {
-  double wdn;        // current <w,d>
+  double wdn[2];     // <w,d> of the current iteration, by iteration parity
   double alpha, beta;
   double final_res;  // stopping threshold
   double tol, atol;  // tol < 0: not given; atol < 0: not given
@@ -144,33 +149,31 @@ __global__ __launch_bounds__(kCgBlock) void CgInitCheckKernel(const double* __re
   __shared__ double lds[kCgBlock / 64];
   const double wdn = SumPartials<kCgBlock>(partials, count, lds);
   if (threadIdx.x == 0) {
-    S->wdn = wdn;
+    S->wdn[0] = wdn;
     CheckResidual(S, residuals, sqrt(fabs(wdn)));
   }
 }
 
-// one workgroup: as = <s, A s> from the partials; alpha or breakdown
-__global__ __launch_bounds__(kCgBlock) void CgAlphaKernel(const double* __restrict__ partials, int count,
-                                                          CGScalars* S) {
-  if (S->done) return;
-  __shared__ double lds[kCgBlock / 64];
-  const double as_s = SumPartials<kCgBlock>(partials, count, lds);
-  if (threadIdx.x == 0) {
-    const double wd = S->wdn;
-    if (as_s == 0.0 || wd == 0.0) S->done = 2;
-    else S->alpha = wd / as_s;
-  }
-}
-
-// u += alpha s; d -= alpha w; w = C d; partials of <w,d>
+// as = <s, A s> from the partials pa of the operator kernel; breakdown when
+// as == 0 or wd == 0, else alpha = wd / as and u += alpha s; d -= alpha w;
+// w = C d; partials of <w,d> into pw.  P is the iteration parity.
 template <class V>
 __global__ __launch_bounds__(kCgBlock) void CgUpdateKernel(const V* __restrict__ inv, V* __restrict__ u,
                                                            V* __restrict__ d, V* __restrict__ w,
-                                                           const V* __restrict__ s, int64_t n, const CGScalars* S,
-                                                           double* __restrict__ partials) {
+                                                           const V* __restrict__ s, int64_t n, CGScalars* S, int P,
+                                                           const double* __restrict__ pa, int na,
+                                                           double* __restrict__ pw) {
   if (S->done) return;
-  const V alv = (V)S->alpha;
-  __shared__ double lds[kCgBlock / 64];
+  __shared__ double lds[kCgBlock / 64 + 1];
+  const double as_s = SumPartialsAll<kCgBlock>(pa, na, lds);
+  const double wd = S->wdn[P];
+  if (as_s == 0.0 || wd == 0.0) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) S->done = 2;
+    return;
+  }
+  const double alpha = wd / as_s;
+  if (blockIdx.x == 0 && threadIdx.x == 0) S->alpha = alpha;
+  const V alv = (V)alpha;
   double acc = 0.0;
   for (int64_t i = blockIdx.x * (int64_t)kCgBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kCgBlock) {
     const V si = s[i], wi = w[i];
@@ -181,31 +184,29 @@ __global__ __launch_bounds__(kCgBlock) void CgUpdateKernel(const V* __restrict__
     w[i] = zi;
     acc += (double)zi * (double)di;
   }
-  StorePartial<kCgBlock>(acc, partials, lds);
+  StorePartial<kCgBlock>(acc, pw, lds);
 }
 
-// one workgroup: wdn from the partials; check; beta
-__global__ __launch_bounds__(kCgBlock) void CgCheckKernel(const double* __restrict__ partials, int count,
-                                                          CGScalars* S, double* residuals) {
-  if (S->done) return;
-  __shared__ double lds[kCgBlock / 64];
-  const double wdn = SumPartials<kCgBlock>(partials, count, lds);
-  if (threadIdx.x == 0) {
-    const double wd = S->wdn;
-    S->wdn = wdn;
-    CheckResidual(S, residuals, sqrt(fabs(wdn)));
-    S->beta = wdn / wd;
-  }
-}
-
-// s = beta s + w
+// wdn = <w,d> from the partials pw; beta = wdn / wd; s = beta s + w.
+// Workgroup 0 stores wdn for the next iteration and does the stopping test
+// (the other workgroups update s whatever its outcome: s is not used once
+// the loop stops).
 template <class V>
 __global__ __launch_bounds__(kCgBlock) void CgDirectionKernel(V* __restrict__ s, const V* __restrict__ w, int64_t n,
-                                                              const CGScalars* S) {
+                                                              CGScalars* S, int P, const double* __restrict__ pw,
+                                                              int nw, double* residuals) {
   if (S->done) return;
-  const V beta = (V)S->beta;
+  __shared__ double lds[kCgBlock / 64 + 1];
+  const double wdn = SumPartialsAll<kCgBlock>(pw, nw, lds);
+  const double beta = wdn / S->wdn[P];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    S->wdn[P ^ 1] = wdn;
+    S->beta = beta;
+    CheckResidual(S, residuals, sqrt(fabs(wdn)));
+  }
+  const V bv = (V)beta;
   for (int64_t i = blockIdx.x * (int64_t)kCgBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kCgBlock)
-    s[i] = beta * s[i] + w[i];
+    s[i] = bv * s[i] + w[i];
 }
 
 // partials of <x,y> (FP64 accumulation), skipped when *done
@@ -251,41 +252,47 @@ __global__ void AddScaledF32Kernel(double* __restrict__ u, const float* __restri
     u[i] += s * (double)c[i];
 }
 
-static int CgGrid(int64_t n) { return GridFor(n, kCgBlock, 2048); }
+// Reducing grids stay at 1024 workgroups or below: the update and direction
+// kernels read every partial in every workgroup.
+constexpr int kCgMaxGrid = 1024;
+static int CgGrid(int64_t n) { return GridFor(n, kCgBlock, kCgMaxGrid); }
 
-// w = A s and the partials of <s, w>; returns the number of partials
+// Partials of the operator kernel (<s, A s>) and of the update (<w, d>).
+static double* PartialsA() { return State().partials; }
+static double* PartialsW() { return State().partials + kMaxReduceBlocks; }
+
+// w = A s and the partials of <s, w> into PartialsA(); returns their number
 template <class V>
 static int EnqueueMatDot(ngsamd_op_t A, const V* s, V* w, int64_t n, CGScalars* dS, hipStream_t st) {
-  auto& R = State();
   const int* done = &dS->done;
   if constexpr (sizeof(V) == 8) {
-    if (A->kind == OpKind::CSR) return static_cast<CsrOp*>(A)->MultDot(s, w, R.partials, done, st);
-    if (A->kind == OpKind::MatrixFree) static_cast<MatFreeOp*>(A)->MultStop(s, w, done, st);
-    else A->Mult(s, w, st);
+    if (A->kind == OpKind::CSR) return static_cast<CsrOp*>(A)->MultDot(s, w, PartialsA(), done, st);
+    if (A->kind == OpKind::MatrixFree) {
+      const int np = static_cast<MatFreeOp*>(A)->MultDot(s, w, PartialsA(), done, st);
+      if (np > 0) return np;
+    } else {
+      A->Mult(s, w, st);
+    }
   } else {
     NGSAMD_REQUIRE(A->kind == OpKind::MatrixFree, "cg: FP32 vectors need a matrix-free operator");
-    static_cast<MatFreeOp*>(A)->MultF32(s, w, done, st);
+    const int np = static_cast<MatFreeOp*>(A)->MultDotF32(s, w, PartialsA(), done, st);
+    if (np > 0) return np;
   }
   const int g = CgGrid(n);
-  CgDotKernel<V><<<g, kCgBlock, 0, st>>>(s, w, n, done, R.partials);
+  CgDotKernel<V><<<g, kCgBlock, 0, st>>>(s, w, n, done, PartialsA());
   NGSAMD_CHECK_LAUNCH();
   return g;
 }
 
-// One CG iteration on the stream (graph-capturable: kernels only).
+// One CG iteration of parity P on the stream (graph-capturable: kernels only).
 template <class V>
 static void EnqueueIteration(ngsamd_op_t A, const V* inv, V* u, V* d, V* w, V* s, int64_t n, CGScalars* dS,
-                             double* residuals, hipStream_t st) {
-  auto& R = State();
-  const int p1 = EnqueueMatDot<V>(A, s, w, n, dS, st);
-  CgAlphaKernel<<<1, kCgBlock, 0, st>>>(R.partials, p1, dS);
-  NGSAMD_CHECK_LAUNCH();
+                             double* residuals, int P, hipStream_t st) {
+  const int na = EnqueueMatDot<V>(A, s, w, n, dS, st);
   const int g = CgGrid(n);
-  CgUpdateKernel<V><<<g, kCgBlock, 0, st>>>(inv, u, d, w, s, n, dS, R.partials);
+  CgUpdateKernel<V><<<g, kCgBlock, 0, st>>>(inv, u, d, w, s, n, dS, P, PartialsA(), na, PartialsW());
   NGSAMD_CHECK_LAUNCH();
-  CgCheckKernel<<<1, kCgBlock, 0, st>>>(R.partials, g, dS, residuals);
-  NGSAMD_CHECK_LAUNCH();
-  CgDirectionKernel<V><<<g, kCgBlock, 0, st>>>(s, w, n, dS);
+  CgDirectionKernel<V><<<g, kCgBlock, 0, st>>>(s, w, n, dS, P, PartialsW(), g, residuals);
   NGSAMD_CHECK_LAUNCH();
 }
 
@@ -308,16 +315,15 @@ struct CgWork {
 template <class V>
 static CgResult IterateCg(ngsamd_op_t A, const V* inv, V* u, CgWork<V>& W, int64_t n, double tol, double atol,
                           int maxiter, int chunk, hipStream_t st) {
-  auto& R = State();
   CGScalars hS{};
   hS.tol = tol;
   hS.atol = atol;
   hS.maxiter = maxiter;
   NGSAMD_HIP(hipMemcpyAsync(W.S.p, &hS, sizeof(CGScalars), hipMemcpyHostToDevice, st));
   const int g = CgGrid(n);
-  CgInitKernel<V><<<g, kCgBlock, 0, st>>>(inv, W.d.p, W.w.p, W.s.p, n, R.partials);
+  CgInitKernel<V><<<g, kCgBlock, 0, st>>>(inv, W.d.p, W.w.p, W.s.p, n, PartialsW());
   NGSAMD_CHECK_LAUNCH();
-  CgInitCheckKernel<<<1, kCgBlock, 0, st>>>(R.partials, g, W.S.p, W.residuals.p);
+  CgInitCheckKernel<<<1, kCgBlock, 0, st>>>(PartialsW(), g, W.S.p, W.residuals.p);
   NGSAMD_CHECK_LAUNCH();
 
   CGScalars* pin = nullptr;
@@ -329,13 +335,17 @@ static CgResult IterateCg(ngsamd_op_t A, const V* inv, V* u, CgWork<V>& W, int64
 
   hipGraphExec_t exec = nullptr;
   hipGraph_t graph = nullptr;
+  // graphs hold an even number of iterations so that every replay starts
+  // at parity 0
+  if (chunk > 0) chunk += chunk & 1;
   bool use_graph = chunk > 0;
   if (use_graph) {
     hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
     if (e == hipSuccess) {
       bool ok = true;
       try {
-        for (int k = 0; k < chunk; k++) EnqueueIteration<V>(A, inv, u, W.d.p, W.w.p, W.s.p, n, W.S.p, W.residuals.p, st);
+        for (int k = 0; k < chunk; k++)
+          EnqueueIteration<V>(A, inv, u, W.d.p, W.w.p, W.s.p, n, W.S.p, W.residuals.p, k & 1, st);
       } catch (...) {
         ok = false;
       }
@@ -361,9 +371,10 @@ static CgResult IterateCg(ngsamd_op_t A, const V* inv, V* u, CgWork<V>& W, int64
   } graph_guard{graph, exec};
 
   const int per_launch = use_graph ? chunk : 1;
+  long long enqueued = 0;  // iterations enqueued without a graph (parity)
   auto enqueue = [&](int slot) {
     if (use_graph) NGSAMD_HIP(hipGraphLaunch(exec, st));
-    else EnqueueIteration<V>(A, inv, u, W.d.p, W.w.p, W.s.p, n, W.S.p, W.residuals.p, st);
+    else EnqueueIteration<V>(A, inv, u, W.d.p, W.w.p, W.s.p, n, W.S.p, W.residuals.p, (int)(enqueued++ & 1), st);
     NGSAMD_HIP(hipMemcpyAsync(&pin[slot], W.S.p, sizeof(CGScalars), hipMemcpyDeviceToHost, st));
   };
   hipEvent_t ev[2];

@@ -114,6 +114,9 @@ namespace ngsamd {
 void LaunchFill(double* x, int64_t n, double a, hipStream_t st);
 void LaunchFillF32(float* x, int64_t n, float a, hipStream_t st);
 int GridFor(int64_t n, int block, int max_blocks = 1 << 20);
+// device properties of the current device (queried once)
+int DeviceCUs();
+int DeviceMaxLds();
 
 // Workgroups are dealt to the 8 XCDs round-robin (workgroup b runs on XCD
 // b mod 8).  This maps the hardware id to a logical id such that each XCD owns

@@ -34,6 +34,7 @@ struct CsrOp : ngsamd_op_s {
   int nblocks = 0;
   int block_nnz = 2048;
   int vec = 1;     // stream kernel: nonzeros per lane load (1 or 4)
+  bool pipe = false;  // runs kernel: next block's loads in flight during the reduction
   CsrOp(int64_t h, int64_t w, const int64_t* rp, const int* c, const double* v);
   void MultAdd(double s, const double* x, double* y, hipStream_t st) override;
   void Mult(const double* x, double* y, hipStream_t st) override;

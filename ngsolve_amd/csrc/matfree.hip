@@ -34,6 +34,8 @@
 
 #include "common.hpp"
 #include "matfree.hpp"
+#include "reduce.hpp"
+#include "runtime_state.hpp"
 
 namespace ngsamd {
 
@@ -397,13 +399,23 @@ __global__ __launch_bounds__(kScalarBlock) void MfScalarKernel(
   }
 }
 
-// y[r] = (ACC ? y[r] : 0) + s * sum of the contributions of dof r, in slot order.
-template <class C, class V>
-__global__ void MfGatherKernel(const int* __restrict__ rowptr, const int* __restrict__ slots,
-                               const C* __restrict__ contrib, V* __restrict__ y, int64_t n, double s, int accumulate,
-                               const int* __restrict__ stop) {
+// y[r] = (ACC ? y[r] : 0) + s * sum of the contributions of dof r, in slot
+// order; with DOT also the per-workgroup partial sums of x[r] * y[r] (the
+// <s, A s> of CG) into partials[].
+constexpr int kGatherBlock = 256;
+
+template <class C, class V, bool DOT>
+__global__ __launch_bounds__(kGatherBlock) void MfGatherKernel(
+    const int* __restrict__ rowptr, const int* __restrict__ slots, const int* __restrict__ order,
+    const C* __restrict__ contrib, V* __restrict__ y, int64_t n, double s, int accumulate,
+    const int* __restrict__ stop, const V* __restrict__ x, double* __restrict__ partials) {
   if (stop && *stop) return;
-  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+  double dotacc = 0.0;
+  // contiguous ranges of the gather order per XCD
+  const int64_t per = (n + gridDim.x - 1) / gridDim.x;
+  const int64_t r0 = (int64_t)XcdContiguousBlock(blockIdx.x, gridDim.x) * per;
+  const int64_t r1 = min(r0 + per, n);
+  for (int64_t r = r0 + threadIdx.x; r < r1; r += kGatherBlock) {
     double acc = 0.0;
     const int k0 = rowptr[r], k1 = rowptr[r + 1];
     // 4 slots at a time, loads unconditional (clamped to the last slot) so
@@ -419,8 +431,15 @@ __global__ void MfGatherKernel(const int* __restrict__ rowptr, const int* __rest
       for (int j = 0; j < 4; j++)
         if (k + j < k1) acc += (double)cv[j];
     }
+    const int d = order[r];
     const double v = s * acc;
-    y[r] = accumulate ? (V)((double)y[r] + v) : (V)v;
+    const V yv = accumulate ? (V)((double)y[d] + v) : (V)v;
+    y[d] = yv;
+    if (DOT) dotacc += (double)x[d] * (double)yv;
+  }
+  if (DOT) {
+    __shared__ double lds[kGatherBlock / 64];
+    StorePartial<kGatherBlock>(dotacc, partials, lds);
   }
 }
 
@@ -488,30 +507,6 @@ __global__ void MfElementGradKernel(const double* __restrict__ bref, const doubl
 }
 
 // ----------------------------------------------------------------- host side
-
-static int DeviceMaxLds() {
-  static int v = -1;
-  if (v < 0) {
-    int dev = 0;
-    (void)hipGetDevice(&dev);
-    int val = 0;
-    if (hipDeviceGetAttribute(&val, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) val = 65536;
-    v = val;
-  }
-  return v;
-}
-
-static int DeviceCUs() {
-  static int v = -1;
-  if (v < 0) {
-    int dev = 0;
-    (void)hipGetDevice(&dev);
-    int val = 0;
-    if (hipDeviceGetAttribute(&val, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) val = 256;
-    v = val;
-  }
-  return v;
-}
 
 static bool SupportedNdp(int ndp) {
   return ndp == 4 || ndp == 8 || ndp == 12 || ndp == 16 || ndp == 20 || ndp == 24 || ndp == 36;
@@ -640,16 +635,40 @@ MatFreeOp::MatFreeOp(int dim_, int nd_, int nq_, int64_t ndof, int nclass, const
   kappa.alloc(ne_global);
   kappa.upload(kap.data(), ne_global);
 
-  // dof -> contribution slots, ascending slot order
+  // dof -> contribution slots, ascending slot order (the order of each sum)
   std::vector<int> cnt(ndof + 1, 0);
   for (int64_t i = 0; i < nel * nd; i++) cnt[hdofs[i] + 1]++;
   for (int64_t r = 0; r < ndof; r++) cnt[r + 1] += cnt[r];
   std::vector<int> sl(nel * nd), pos(cnt.begin(), cnt.end() - 1);
   for (int64_t i = 0; i < nel * nd; i++) sl[pos[hdofs[i]]++] = (int)i;
-  g_rowptr.alloc(ndof + 1);
-  g_rowptr.upload(cnt.data(), ndof + 1);
-  g_slots.alloc(sl.size());
-  g_slots.upload(sl.data(), sl.size());
+  // Gather order: dofs by the lowest global element number among their
+  // elements (counting sort, stable).  The dofs of one element (vertex, edge,
+  // face and cell dofs lie far apart in the numbering) are then gathered close
+  // together in time and, with the XCD-contiguous grid, on one XCD, so each
+  // element row of the contribution buffer is fetched into L2 about once
+  // instead of once per node that reads it.
+  {
+    std::vector<int> key(ndof, (int)ne_global);
+    for (int64_t i = 0; i < nel * nd; i++) key[hdofs[i]] = std::min(key[hdofs[i]], el[i / nd]);
+    std::vector<int> kc(ne_global + 2, 0);
+    for (int64_t d = 0; d < ndof; d++) kc[key[d] + 1]++;
+    for (int64_t k = 0; k <= ne_global; k++) kc[k + 1] += kc[k];
+    std::vector<int> order(ndof);
+    for (int64_t d = 0; d < ndof; d++) order[kc[key[d]]++] = (int)d;
+    std::vector<int> prp(ndof + 1, 0), psl(sl.size());
+    for (int64_t r = 0; r < ndof; r++) {
+      const int d = order[r];
+      std::copy(sl.begin() + cnt[d], sl.begin() + cnt[d + 1], psl.begin() + prp[r]);
+      prp[r + 1] = prp[r] + (cnt[d + 1] - cnt[d]);
+    }
+    g_rowptr.alloc(ndof + 1);
+    g_rowptr.upload(prp.data(), ndof + 1);
+    g_slots.alloc(std::max<size_t>(psl.size(), 1));
+    if (!psl.empty()) g_slots.upload(psl.data(), psl.size());
+    g_order.alloc(std::max<int64_t>(ndof, 1));
+    if (ndof) g_order.upload(order.data(), ndof);
+    NGSAMD_HIP(hipStreamSynchronize(Stream()));
+  }
   contrib64.alloc((size_t)nel * nd);
   contrib32.alloc((size_t)nel * nd);
   // round schedule of the MFMA kernels: 16-element tiles, kMfWaves tiles per
@@ -781,15 +800,16 @@ static void LaunchElements(const MatFreeOp& A, const V* x, V* out, double s, con
 }
 
 template <class V>
-void MatFreeOp::Apply(double s, const V* x, V* y, bool accumulate, const int* stop, hipStream_t st) {
-  if (height == 0) return;
+int MatFreeOp::Apply(double s, const V* x, V* y, bool accumulate, const int* stop, hipStream_t st,
+                     double* partials) {
+  if (height == 0) return 0;
   if (scatter == NGSAMD_SCATTER_ATOMIC) {
     if (!accumulate) {
       ScaleFillKernel<V><<<GridFor(height, 256, 8192), 256, 0, st>>>(y, height, stop);
       NGSAMD_CHECK_LAUNCH();
     }
     LaunchElements<1, V>(*this, x, y, s, stop, st, false);
-    return;
+    return 0;
   }
   // contribution buffers are allocated by the constructor: nothing here
   // allocates, so the apply can be captured into a graph
@@ -797,9 +817,17 @@ void MatFreeOp::Apply(double s, const V* x, V* y, bool accumulate, const int* st
   if constexpr (sizeof(V) == 8) contrib = reinterpret_cast<V*>(contrib64.p);
   else contrib = reinterpret_cast<V*>(contrib32.p);
   LaunchElements<0, V>(*this, x, contrib, 1.0, stop, st, false);
-  MfGatherKernel<V, V><<<GridFor(height, 256, 16384), 256, 0, st>>>(g_rowptr.p, g_slots.p, contrib, y, height, s,
-                                                                    accumulate ? 1 : 0, stop);
+  if (partials) {
+    const int g = GridFor(height, kGatherBlock, kMaxDotBlocks);
+    MfGatherKernel<V, V, true><<<g, kGatherBlock, 0, st>>>(g_rowptr.p, g_slots.p, g_order.p, contrib, y, height,
+                                                           s, accumulate ? 1 : 0, stop, x, partials);
+    NGSAMD_CHECK_LAUNCH();
+    return g;
+  }
+  MfGatherKernel<V, V, false><<<GridFor(height, kGatherBlock, 16384), kGatherBlock, 0, st>>>(
+      g_rowptr.p, g_slots.p, g_order.p, contrib, y, height, s, accumulate ? 1 : 0, stop, nullptr, nullptr);
   NGSAMD_CHECK_LAUNCH();
+  return 0;
 }
 
 void MatFreeOp::MultAdd(double s, const double* x, double* y, hipStream_t st) { Apply<double>(s, x, y, true, nullptr, st); }
@@ -809,6 +837,12 @@ void MatFreeOp::MultStop(const double* x, double* y, const int* stop, hipStream_
 }
 void MatFreeOp::MultF32(const float* x, float* y, const int* stop, hipStream_t st) {
   Apply<float>(1.0, x, y, false, stop, st);
+}
+int MatFreeOp::MultDot(const double* x, double* y, double* partials, const int* stop, hipStream_t st) {
+  return Apply<double>(1.0, x, y, false, stop, st, partials);
+}
+int MatFreeOp::MultDotF32(const float* x, float* y, double* partials, const int* stop, hipStream_t st) {
+  return Apply<float>(1.0, x, y, false, stop, st, partials);
 }
 
 void MatFreeOp::Diagonal(double* d, hipStream_t st) {
@@ -824,8 +858,8 @@ void MatFreeOp::Diagonal(double* d, hipStream_t st) {
       MfDiagKernel<3><<<grid, 128, 0, st>>>(b, weights.p, nq, metric.p, Kappa(), elnr.p, c.begin, c.end, nd, contrib64.p);
     NGSAMD_CHECK_LAUNCH();
   }
-  MfGatherKernel<double, double><<<GridFor(height, 256, 16384), 256, 0, st>>>(g_rowptr.p, g_slots.p, contrib64.p, d,
-                                                                              height, 1.0, 0, nullptr);
+  MfGatherKernel<double, double, false><<<GridFor(height, kGatherBlock, 16384), kGatherBlock, 0, st>>>(
+      g_rowptr.p, g_slots.p, g_order.p, contrib64.p, d, height, 1.0, 0, nullptr, nullptr, nullptr);
   NGSAMD_CHECK_LAUNCH();
 }
 

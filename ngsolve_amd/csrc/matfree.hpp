@@ -39,8 +39,9 @@ struct MatFreeOp : ngsamd_op_s {
   const double* kappa_ext = nullptr;  // set by ngsamd_mf_set_kappa
   DevBuf<int> elnr;           // [nel] global element numbers
   int64_t ne_global = 0;
-  // gather scatter: dof -> contribution slots (element row * nd + local)
-  DevBuf<int> g_rowptr, g_slots;
+  // gather scatter: dof -> contribution slots (element row * nd + local),
+  // dofs in the gather order g_order (by lowest element number)
+  DevBuf<int> g_rowptr, g_slots, g_order;
   DevBuf<double> contrib64;
   DevBuf<float> contrib32;
 
@@ -53,12 +54,18 @@ struct MatFreeOp : ngsamd_op_s {
   // y = A x, all kernels return immediately when *d_stop != 0
   void MultStop(const double* x, double* y, const int* d_stop, hipStream_t st);
   void MultF32(const float* x, float* y, const int* d_stop, hipStream_t st);
+  // y = A x plus per-workgroup partial sums of <x, A x> in partials[];
+  // returns their number, 0 when the scatter mode has no gather kernel
+  // (then the caller forms the inner product itself)
+  int MultDot(const double* x, double* y, double* partials, const int* d_stop, hipStream_t st);
+  int MultDotF32(const float* x, float* y, double* partials, const int* d_stop, hipStream_t st);
   void Diagonal(double* d, hipStream_t st);
   void ElementGrad(const double* lam, const double* u, double* grad, hipStream_t st);
 
  private:
   template <class V>
-  void Apply(double s, const V* x, V* y, bool accumulate, const int* d_stop, hipStream_t st);
+  int Apply(double s, const V* x, V* y, bool accumulate, const int* d_stop, hipStream_t st,
+            double* partials = nullptr);
 };
 
 }  // namespace ngsamd

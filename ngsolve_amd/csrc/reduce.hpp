@@ -41,11 +41,34 @@ __device__ __forceinline__ void StorePartial(double v, double* partials, double*
 }
 
 // Inside a one-workgroup kernel: deterministic sum of partials[0..count).
+// Loads go 4 at a time (one round trip per 4 partials per thread); the
+// additions keep the order of the plain strided loop.
 template <int BLOCK>
 __device__ __forceinline__ double SumPartials(const double* partials, int count, double* lds) {
   double v = 0.0;
-  for (int i = threadIdx.x; i < count; i += BLOCK) v += partials[i];
+  int i = threadIdx.x;
+  for (; i + 3 * BLOCK < count; i += 4 * BLOCK) {
+    const double a = partials[i], b = partials[i + BLOCK], c = partials[i + 2 * BLOCK], d = partials[i + 3 * BLOCK];
+    v += a;
+    v += b;
+    v += c;
+    v += d;
+  }
+  for (; i < count; i += BLOCK) v += partials[i];
   return BlockSum<BLOCK>(v, lds);
+}
+
+// The same sum, returned to every thread of the workgroup.  Every workgroup
+// of a grid that calls it on the same partials gets the same value bit for
+// bit (the order of the additions depends only on count), so a kernel can
+// form a grid-wide scalar redundantly in each workgroup instead of in a
+// one-workgroup kernel of its own.  lds must hold BLOCK/64 + 1 doubles.
+template <int BLOCK>
+__device__ __forceinline__ double SumPartialsAll(const double* partials, int count, double* lds) {
+  const double v = SumPartials<BLOCK>(partials, count, lds);
+  if (threadIdx.x == 0) lds[BLOCK / 64] = v;
+  __syncthreads();
+  return lds[BLOCK / 64];
 }
 
 }  // namespace ngsamd

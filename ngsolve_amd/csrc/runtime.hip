@@ -28,6 +28,28 @@ int GridFor(int64_t n, int block, int max_blocks) {
   return (int)g;
 }
 
+int DeviceCUs() {
+  static int v = -1;
+  if (v < 0) {
+    int dev = 0, val = 0;
+    (void)hipGetDevice(&dev);
+    if (hipDeviceGetAttribute(&val, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) val = 256;
+    v = val;
+  }
+  return v;
+}
+
+int DeviceMaxLds() {
+  static int v = -1;
+  if (v < 0) {
+    int dev = 0, val = 0;
+    (void)hipGetDevice(&dev);
+    if (hipDeviceGetAttribute(&val, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) val = 65536;
+    v = val;
+  }
+  return v;
+}
+
 // ------------------------------------------------------------ fill kernels
 __global__ void FillKernel(double* __restrict__ x, int64_t n, double a) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
@@ -126,7 +148,7 @@ int ngsamd_init(int device) {
     NGSAMD_HIP(hipSetDevice(device));
     S.device = device;
     NGSAMD_HIP(hipStreamCreateWithFlags(&S.stream, hipStreamNonBlocking));
-    NGSAMD_HIP(hipMalloc((void**)&S.partials, sizeof(double) * kMaxReduceBlocks));
+    NGSAMD_HIP(hipMalloc((void**)&S.partials, sizeof(double) * 2 * kMaxReduceBlocks));
     NGSAMD_HIP(hipMalloc((void**)&S.dev_scalars, sizeof(double) * kNumScalars));
     NGSAMD_HIP(hipHostMalloc((void**)&S.host_scalars, sizeof(double) * kNumScalars, hipHostMallocDefault));
     NGSAMD_HIP(hipDeviceSynchronize());

@@ -8,6 +8,9 @@
 namespace ngsamd {
 
 constexpr int kMaxReduceBlocks = 4096;  // upper bound on workgroups of a reducing kernel
+// grid bound of the matrix-free gather kernel when it also writes partials of
+// <x, A x> for CG (CG's update kernel reads all partials in every workgroup)
+constexpr int kMaxDotBlocks = 1024;
 constexpr int kNumScalars = 64;
 
 struct RuntimeState {
@@ -16,7 +19,7 @@ struct RuntimeState {
   int device = -1;
   hipStream_t stream = nullptr;       // library stream
   hipStream_t user_stream = nullptr;  // stream set by ngsamd_set_stream
-  double* partials = nullptr;         // kMaxReduceBlocks partial sums
+  double* partials = nullptr;         // 2 x kMaxReduceBlocks partial sums
   double* dev_scalars = nullptr;      // kNumScalars
   double* host_scalars = nullptr;     // pinned mirror
 };

@@ -198,73 +198,30 @@ __global__ __launch_bounds__(kStreamBlock) void CsrStreamKernel(
   if (DOT) StorePartial<kStreamBlock>(dotacc, partials, lds);
 }
 
-// One block of runs with R rows each: runs [u0, u1), values
-// [val_ptr[u0], val_ptr[u1]) (at most NNZ), rows run_row[u0] + j.  Lane k of
-// the value range forms value k times x[column of shared slot k / R] into LDS
-// (values stream coalesced; the R lanes of one slot read one column number
-// and one x entry); row j = run * R + q then sums slots of its run at stride R.
-template <int NNZ, int R, bool ACCUM, bool DOT>
-__device__ __forceinline__ void RunsBlock(const int* __restrict__ run_row, const int* __restrict__ run_ptr,
-                                          const int* __restrict__ val_ptr, const int* __restrict__ rcols,
-                                          const double* __restrict__ rvals, const double* __restrict__ x,
-                                          double* __restrict__ y, int u0, int u1, double s, double* prod,
-                                          double& dotacc) {
-  const int tid = threadIdx.x;
-  const int c0 = run_ptr[u0];
-  const int vb = val_ptr[u0];
-  const int nv = val_ptr[u1] - vb;
-  constexpr int PER = NNZ / kStreamBlock;
-  int c[PER];
-  double v[PER], xv[PER];
-  // unconditional loads (lanes past the block read entry 0 of it, kept in
-  // bounds by the padding) so that every load is in flight at once
-#pragma unroll
-  for (int i = 0; i < PER; i++) {
-    const int k = tid + i * kStreamBlock;
-    const int kk = k < nv ? k : 0;
-    c[i] = __builtin_nontemporal_load(&rcols[c0 + kk / R]);
-    v[i] = __builtin_nontemporal_load(&rvals[vb + kk]);
-  }
-#pragma unroll
-  for (int i = 0; i < PER; i++) xv[i] = x[c[i]];
-#pragma unroll
-  for (int i = 0; i < PER; i++) {
-    const int k = tid + i * kStreamBlock;
-    prod[k] = k < nv ? v[i] * xv[i] : 0.0;
-  }
-  __syncthreads();
-  const int row0 = run_row[u0];
-  const int nr = (u1 - u0) * R;
-  int tpr = 64;
-  while (tpr > 1 && tpr * nr > kStreamBlock) tpr >>= 1;
-  const int l = tid & (tpr - 1);
-  for (int j0 = 0; j0 < nr; j0 += kStreamBlock / tpr) {
-    const int j = j0 + tid / tpr;
-    double sum = 0.0;
-    if (j < nr) {
-      const int run = j / R, q = j - run * R;
-      const int s0 = run_ptr[u0 + run] - c0, s1 = run_ptr[u0 + run + 1] - c0;
-      for (int k = s0 + l; k < s1; k += tpr) sum += prod[k * R + q];
-    }
-    for (int off = tpr / 2; off > 0; off >>= 1) sum += __shfl_xor(sum, off, 64);
-    if (j < nr && l == 0) {
-      if (ACCUM) y[row0 + j] += s * sum; else y[row0 + j] = s * sum;
-      if (DOT) dotacc += x[row0 + j] * sum;
-    }
-  }
-  __syncthreads();
-}
+// Exact k / R for 0 <= k < 4096 and 1 <= R <= 8 as (k * ceil(2^16 / R)) >> 16:
+// the rounding error k (ceil(2^16/R) - 2^16/R) / 2^16 stays below 1/16 while
+// the fractional part of k / R is at most 7/8.
+__device__ __forceinline__ int RunDiv(int k, int inv) { return (k * inv) >> 16; }
 
 // y = (ACCUM ? y : 0) + s * A x over the run blocks [blk_run[b], blk_run[b+1])
-// with blk_len[b] rows per run; DOT as in CsrSpmvKernel.  A block holding
-// more than NNZ values is a single row (R = 1) that the whole workgroup strides.
-template <int NNZ, bool ACCUM, bool DOT>
+// with blk_len[b] rows per run (at most NNZ values and NNZ rows per block);
+// DOT as in CsrSpmvKernel.  Block b: lane k of its value range forms value k
+// times x[column of shared slot k / R] into LDS (values stream coalesced, the
+// R lanes of one slot read one column number and one x entry); row j = run R
+// + q then sums the slots of its run at stride R.  PIPE: the loads of block
+// b + 1 are issued before block b is reduced from LDS, so they are in flight
+// meanwhile (at the price of the registers holding them).  A block holding
+// more than NNZ values is a single row (R = 1) that the whole workgroup
+// strides.
+template <int NNZ, bool PIPE, bool ACCUM, bool DOT>
 __global__ __launch_bounds__(kStreamBlock) void CsrRunsKernel(
     const int* __restrict__ run_row, const int* __restrict__ run_ptr, const int* __restrict__ val_ptr,
     const int* __restrict__ rcols, const double* __restrict__ rvals, const double* __restrict__ x,
     double* __restrict__ y, const int* __restrict__ blk_run, const int* __restrict__ blk_len, int nblocks,
     int blocks_per_wg, double s, double* __restrict__ partials, const int* __restrict__ stop_flag) {
+  static_assert(NNZ <= 4096, "RunDiv is exact below 4096");
   if (stop_flag && *stop_flag) return;
+  constexpr int PER = NNZ / kStreamBlock;
   __shared__ double prod[NNZ];
   __shared__ double lds[kStreamBlock / 64];
   const int tid = threadIdx.x;
@@ -272,45 +229,89 @@ __global__ __launch_bounds__(kStreamBlock) void CsrRunsKernel(
   const int b0 = lb * blocks_per_wg;
   const int b1 = min(b0 + blocks_per_wg, nblocks);
   double dotacc = 0.0;
+
+  // header and register-held loads of the block in flight
+  int u0 = 0, u1 = 0, R = 1, inv = 65536, c0 = 0, vb = 0, nv = 0;
+  int c[PER];
+  double v[PER];
+  auto fetch = [&](int b) {
+    u0 = blk_run[b];
+    u1 = blk_run[b + 1];
+    R = blk_len[b];
+    inv = (65536 + R - 1) / R;
+    c0 = run_ptr[u0];
+    vb = val_ptr[u0];
+    nv = val_ptr[u1] - vb;
+    if (nv > NNZ) return;  // long row: streamed when it is reached
+    // unconditional loads (lanes past the block read entry 0 of it, kept in
+    // bounds by the padding) so that every load is in flight at once
+#pragma unroll
+    for (int i = 0; i < PER; i++) {
+      const int k = tid + i * kStreamBlock;
+      const int kk = k < nv ? k : 0;
+      c[i] = __builtin_nontemporal_load(&rcols[c0 + RunDiv(kk, inv)]);
+      v[i] = __builtin_nontemporal_load(&rvals[vb + kk]);
+    }
+  };
+  if (b0 < b1) fetch(b0);
   for (int b = b0; b < b1; b++) {
-    const int u0 = blk_run[b], u1 = blk_run[b + 1];
-    const int k0 = val_ptr[u0], k1 = val_ptr[u1];
-    if (k1 - k0 > NNZ) {
-      const int c0 = run_ptr[u0] - k0;
+    if (nv > NNZ) {
+      // one long row: the whole workgroup strides over it
       double a = 0.0;
-      for (int k = k0 + tid; k < k1; k += kStreamBlock)
-        a += __builtin_nontemporal_load(&rvals[k]) * x[__builtin_nontemporal_load(&rcols[c0 + k])];
+      for (int k = tid; k < nv; k += kStreamBlock)
+        a += __builtin_nontemporal_load(&rvals[vb + k]) * x[__builtin_nontemporal_load(&rcols[c0 + k])];
       a = BlockSum<kStreamBlock>(a, lds);
       const int r = run_row[u0];
       if (tid == 0) {
         if (ACCUM) y[r] += s * a; else y[r] = s * a;
         if (DOT) dotacc += x[r] * a;
       }
+      if (b + 1 < b1) fetch(b + 1);
       continue;
     }
-    switch (blk_len[b]) {
-#define NGSAMD_RUNS_CASE(RR)                                                                                 \
-  case RR:                                                                                                  \
-    RunsBlock<NNZ, RR, ACCUM, DOT>(run_row, run_ptr, val_ptr, rcols, rvals, x, y, u0, u1, s, prod, dotacc); \
-    break;
-      NGSAMD_RUNS_CASE(1)
-      NGSAMD_RUNS_CASE(2)
-      NGSAMD_RUNS_CASE(3)
-      NGSAMD_RUNS_CASE(4)
-      NGSAMD_RUNS_CASE(5)
-      NGSAMD_RUNS_CASE(6)
-      NGSAMD_RUNS_CASE(7)
-      NGSAMD_RUNS_CASE(8)
-#undef NGSAMD_RUNS_CASE
-      default: break;
+    {
+      double xv[PER];
+#pragma unroll
+      for (int i = 0; i < PER; i++) xv[i] = x[c[i]];
+#pragma unroll
+      for (int i = 0; i < PER; i++) {
+        const int k = tid + i * kStreamBlock;
+        prod[k] = k < nv ? v[i] * xv[i] : 0.0;
+      }
     }
+    __syncthreads();
+    // this block's header, then the next block's loads
+    const int cu0 = u0, cR = R, cinv = inv, cc0 = c0;
+    const int row0 = run_row[cu0];
+    const int nr = (u1 - u0) * R;
+    if (PIPE && b + 1 < b1) fetch(b + 1);
+    int tpr = 64;
+    while (tpr > 1 && tpr * nr > kStreamBlock) tpr >>= 1;
+    const int l = tid & (tpr - 1);
+    for (int j0 = 0; j0 < nr; j0 += kStreamBlock / tpr) {
+      const int j = j0 + tid / tpr;
+      double sum = 0.0;
+      if (j < nr) {
+        const int run = RunDiv(j, cinv), q = j - run * cR;
+        const int s0 = run_ptr[cu0 + run] - cc0, s1 = run_ptr[cu0 + run + 1] - cc0;
+        for (int k = s0 + l; k < s1; k += tpr) sum += prod[k * cR + q];
+      }
+      for (int off = tpr / 2; off > 0; off >>= 1) sum += __shfl_xor(sum, off, 64);
+      if (j < nr && l == 0) {
+        if (ACCUM) y[row0 + j] += s * sum; else y[row0 + j] = s * sum;
+        if (DOT) dotacc += x[row0 + j] * sum;
+      }
+    }
+    __syncthreads();
+    if (!PIPE && b + 1 < b1) fetch(b + 1);
   }
   if (DOT) StorePartial<kStreamBlock>(dotacc, partials, lds);
 }
 
 // Splits the rows into pattern runs (at most kMaxRun rows, at most max_vals
 // values per run unless the run is one row) and the runs into blocks of equal
-// run length with at most max_vals values (or one longer row).  Fills the
+// run length with at most max_vals values and max_vals rows (or one longer
+// row).  Fills the
 // host arrays of the runs layout; returns the number of shared columns.
 static int64_t BuildRuns(int64_t h, const int64_t* rp, const int* c, const double* v, int max_vals,
                          std::vector<int>& run_row, std::vector<int>& run_ptr, std::vector<int>& val_ptr,
@@ -323,7 +324,7 @@ static int64_t BuildRuns(int64_t h, const int64_t* rp, const int* c, const doubl
   rvals.clear();
   blk_run.assign(1, 0);
   blk_len.clear();
-  int64_t acc = 0;
+  int64_t acc = 0, rows = 0;
   int cur_len = 0;
   for (int64_t r = 0; r < h;) {
     const int64_t L = rp[r + 1] - rp[r];
@@ -334,13 +335,15 @@ static int64_t BuildRuns(int64_t h, const int64_t* rp, const int* c, const doubl
       R++;
     const int64_t nv = int64_t(R) * L;
     const int u = (int)run_row.size() - 1;
-    if (u > blk_run.back() && (R != cur_len || acc + nv > max_vals)) {
+    if (u > blk_run.back() && (R != cur_len || acc + nv > max_vals || rows + R > max_vals)) {
       blk_len.push_back(cur_len);
       blk_run.push_back(u);
       acc = 0;
+      rows = 0;
     }
     cur_len = R;
     acc += nv;
+    rows += R;
     rcols.insert(rcols.end(), c + rp[r], c + rp[r] + L);
     for (int64_t j = 0; j < L; j++)
       for (int q = 0; q < R; q++) rvals.push_back(v[rp[r + q] + j]);
@@ -416,6 +419,8 @@ CsrOp::CsrOp(int64_t h, int64_t w, const int64_t* rp, const int* c, const double
     nshared = BuildRuns(h, rp, c, v, block_nnz, rr, rpp, vp, rc, rv, br, bl);
     if (algo_name == "runs" || 4 * nshared <= 3 * nnz) {
       algo = 2;
+      const char* pipe_env = getenv("NGSAMD_SPMV_PIPE");
+      pipe = pipe_env && atoi(pipe_env) == 1;
       nruns = (int64_t)rr.size() - 1;
       nblocks = (int)br.size() - 1;
       run_row.alloc(rr.size());
@@ -511,16 +516,19 @@ static int LaunchSpmvG(const CsrOp& A, double s, const double* x, double* y, hip
 template <bool ACCUM, bool DOT>
 static int LaunchStream(const CsrOp& A, double s, const double* x, double* y, hipStream_t st, double* partials,
                         const int* stop) {
-  int64_t wg = A.nblocks < kMaxReduceBlocks ? A.nblocks : kMaxReduceBlocks;
-  if (wg > 8) wg &= ~int64_t(7);
-  if (wg < 1) wg = 1;
-  const int per = (int)((A.nblocks + wg - 1) / wg);
   auto kern = A.vec == 1 ? (A.block_nnz == 1024   ? CsrStreamKernel<1024, 1, ACCUM, DOT>
                             : A.block_nnz == 4096 ? CsrStreamKernel<4096, 1, ACCUM, DOT>
                                                   : CsrStreamKernel<2048, 1, ACCUM, DOT>)
                          : (A.block_nnz == 1024   ? CsrStreamKernel<1024, 4, ACCUM, DOT>
                             : A.block_nnz == 4096 ? CsrStreamKernel<4096, 4, ACCUM, DOT>
                                                   : CsrStreamKernel<2048, 4, ACCUM, DOT>);
+  // a wide grid (many more workgroups than resident slots) lets the hardware
+  // balance blocks of uneven cost: measured faster than one workgroup per
+  // resident slot (412 vs 452 us for the runs kernel at 32^3 p=4)
+  int wg = A.nblocks < kMaxReduceBlocks ? A.nblocks : kMaxReduceBlocks;
+  if (wg > 8) wg &= ~7;
+  if (wg < 1) wg = 1;
+  const int per = (A.nblocks + wg - 1) / wg;
   kern<<<(int)wg, kStreamBlock, 0, st>>>(A.rowptr.p, A.cols.p, A.vals.p, x, y, A.blk_row.p, A.nblocks, per, s,
                                          partials, stop);
   NGSAMD_CHECK_LAUNCH();
@@ -530,13 +538,19 @@ static int LaunchStream(const CsrOp& A, double s, const double* x, double* y, hi
 template <bool ACCUM, bool DOT>
 static int LaunchRuns(const CsrOp& A, double s, const double* x, double* y, hipStream_t st, double* partials,
                       const int* stop) {
-  int64_t wg = A.nblocks < kMaxReduceBlocks ? A.nblocks : kMaxReduceBlocks;
-  if (wg > 8) wg &= ~int64_t(7);
+  auto kern = A.pipe ? (A.block_nnz == 1024   ? CsrRunsKernel<1024, true, ACCUM, DOT>
+                        : A.block_nnz == 4096 ? CsrRunsKernel<4096, true, ACCUM, DOT>
+                                              : CsrRunsKernel<2048, true, ACCUM, DOT>)
+                     : (A.block_nnz == 1024   ? CsrRunsKernel<1024, false, ACCUM, DOT>
+                        : A.block_nnz == 4096 ? CsrRunsKernel<4096, false, ACCUM, DOT>
+                                              : CsrRunsKernel<2048, false, ACCUM, DOT>);
+  // a wide grid (many more workgroups than resident slots) lets the hardware
+  // balance blocks of uneven cost: measured faster than one workgroup per
+  // resident slot (412 vs 452 us for the runs kernel at 32^3 p=4)
+  int wg = A.nblocks < kMaxReduceBlocks ? A.nblocks : kMaxReduceBlocks;
+  if (wg > 8) wg &= ~7;
   if (wg < 1) wg = 1;
-  const int per = (int)((A.nblocks + wg - 1) / wg);
-  auto kern = A.block_nnz == 1024   ? CsrRunsKernel<1024, ACCUM, DOT>
-              : A.block_nnz == 4096 ? CsrRunsKernel<4096, ACCUM, DOT>
-                                    : CsrRunsKernel<2048, ACCUM, DOT>;
+  const int per = (A.nblocks + wg - 1) / wg;
   kern<<<(int)wg, kStreamBlock, 0, st>>>(A.run_row.p, A.run_ptr.p, A.val_ptr.p, A.rcols.p, A.rvals.p, x, y,
                                          A.blk_run.p, A.blk_len.p, A.nblocks, per, s, partials, stop);
   NGSAMD_CHECK_LAUNCH();

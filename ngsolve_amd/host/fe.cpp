@@ -169,7 +169,7 @@ Mesh ReadVolMesh(const std::string& text) {
 // ---------------------------------------------------------------- topology
 // Global edge/face numbers: ascending lexicographic order of the sorted
 // vertex tuples.  (The reference takes them from netgen's mesh topology,
-// which this build does not have; every quantity on the hot path that does
+// which is not available here; every quantity on the hot path that does
 // not depend on the global numbering is independent of this choice.)
 
 void Mesh::BuildTopology() {

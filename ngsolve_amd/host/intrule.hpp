@@ -17,7 +17,7 @@
 // polynomials (Golub-Welsch recurrence coefficients), not with the
 // reference's Newton iteration; both give the unique Gauss-Jacobi rule.
 // Tetrahedron orders 0..6 in the reference use tabulated Keast-type rules;
-// this build uses the collapsed rule for every tetrahedron order (exact to the
+// here the collapsed rule serves every tetrahedron order (exact to the
 // same polynomial degree, more points).
 #pragma once
 #include <vector>

@@ -98,12 +98,13 @@ ALGOS = ["runs", "stream", "rowgroup"]
 
 def _runs_matrix(rng):
     """Rows in runs of equal column lists: lengths 1..12 (runs longer than 8
-    rows, or with more than 2048 values, are split), empty rows, a row of
-    5000 nonzeros and runs of 1000-column rows."""
+    rows, or with more than 2048 values, are split), empty rows (9000 in a
+    row: more rows than a block may hold), a row of 5000 nonzeros and runs
+    of 1000-column rows."""
     w = 6000
     rows = []
     for R, L in [(1, 7), (3, 40), (12, 9), (0, 0), (5, 1000), (2, 5000), (4, 0), (6, 33), (1, 2049), (8, 256),
-                 (9, 1), (3, 2048), (1, 1)]:
+                 (9, 1), (3, 2048), (9000, 0), (1, 1)]:
         cols = np.sort(rng.choice(w, L, replace=False)).astype(np.int32)
         rows += [cols] * max(R, 1)
     rp = np.zeros(len(rows) + 1, dtype=np.int64)
