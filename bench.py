@@ -193,7 +193,7 @@ def main():
         for _ in range(3):
             As.mult(dx, dy)
         del As
-        for prec in (0, 1, 2):
+        for prec in (0, 1, 2, 3):
             M = build_mf(N, fes, mesh, ir, None, prec)
             for _ in range(3):
                 M.mult(dx, dy)
@@ -233,7 +233,7 @@ def main():
     x32, y32 = N.DeviceVector(ndof, 4), N.DeviceVector(ndof, 4)
     x32.upload(x.astype(np.float32))
     for name, prec, peak in [("mf_f64", 0, FP64_PEAK_TFLOPS), ("mf_f32", 1, FP32_PEAK_TFLOPS),
-                             ("mf_bf16", 2, BF16_PEAK_TFLOPS)]:
+                             ("mf_bf16", 2, BF16_PEAK_TFLOPS), ("mf_f16", 3, BF16_PEAK_TFLOPS)]:
         for scatter in (0, 1):
             M = build_mf(N, fes, mesh, ir, None, prec, scatter)
             ms = timed(N, lambda: M.mult(dx, dy), args.steps, args.warmup)
@@ -286,7 +286,7 @@ def main():
         M64 = build_mf(N, fes, mesh, ir, kappa, 0)
         res = {"fp64_csr_cg": {"iterations": its_k, "seconds": round(t_k, 4), "true_rel_residual": true_res(u64)}}
         best = None
-        for pname, prec in [("f32", 1), ("bf16", 2)]:
+        for pname, prec in [("f32", 1), ("bf16", 2), ("f16", 3)]:
             Mlow = build_mf(N, fes, mesh, ir, kappa, prec)
             du2 = N.DeviceVector(ndof, 8)
             t0 = time.perf_counter()

@@ -39,6 +39,7 @@ extern "C" {
 #define NGSAMD_PREC_F64 0
 #define NGSAMD_PREC_F32 1
 #define NGSAMD_PREC_BF16 2
+#define NGSAMD_PREC_F16 3
 
 /* scatter mode of a matrix-free operator */
 #define NGSAMD_SCATTER_GATHER 0   /* deterministic two-pass owner-computes gather */
@@ -116,7 +117,7 @@ int ngsamd_jacobi_create_device(ngsamd_op_t* op, int64_t n, const double* d_diag
  *   h_kappa    [nel] element coefficient or NULL (=1)
  *   h_elnr     [nel] global element number of each table row or NULL (=row)
  *   precision  NGSAMD_PREC_*  (B and the contraction in that type, FP32
- *              accumulation for BF16; input/output vectors stay FP64)
+ *              accumulation for BF16 / FP16; input/output vectors stay FP64)
  *   scatter    NGSAMD_SCATTER_* */
 int ngsamd_mf_create(ngsamd_op_t* op, int dim, int nd, int nq, int64_t ndof, int nclass,
                      const int64_t* h_class_offsets, const double* h_bmat, const double* h_weights,

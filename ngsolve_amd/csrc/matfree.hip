@@ -30,9 +30,12 @@
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
+#include <string>
+#include <type_traits>
 #include <vector>
 
 #include "common.hpp"
+#include "half.hpp"
 #include "matfree.hpp"
 #include "reduce.hpp"
 #include "runtime_state.hpp"
@@ -215,8 +218,10 @@ __global__ __launch_bounds__(kMfBlock) void MfMfmaKernel(
   }
 }
 
-// BF16 path (v_mfma_f32_16x16x32_bf16, FP32 accumulation).  Operand layout
-// measured on gfx950: lane l supplies A[l%16][8*(l/16)+j] and
+// 16-bit paths, BF16 (v_mfma_f32_16x16x32_bf16) and FP16
+// (v_mfma_f32_16x16x32_f16), FP32 accumulation.  Operand layout measured on
+// gfx950 for the BF16 instruction, and the FP16 sibling is checked against it
+// by the precision tests: lane l supplies A[l%16][8*(l/16)+j] and
 // B[8*(l/16)+j][l%16], j = 0..7, and holds result rows 4*(l/16)+r.
 // Quadrature points are taken in chunks of 32 so that the K dimension of the
 // second product (one direction x 32 points) is one MFMA.  B is held in LDS
@@ -226,8 +231,18 @@ __global__ __launch_bounds__(kMfBlock) void MfMfmaKernel(
 //   B2[chunk][d][MT*16][32]  second product, the 32 points of a chunk in the
 //                            order the first product's accumulators hold them
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+template <class H> struct Vec8;
+template <> struct Vec8<__bf16> { typedef bf16x8 type; };
+template <> struct Vec8<_Float16> { typedef half8 type; };
+__device__ __forceinline__ float4v Mfma16(bf16x8 a, bf16x8 b, float4v c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ float4v Mfma16(half8 a, half8 b, float4v c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+}
 
-template <int DIM, int KS1, int MT, int MODE, class V>
+template <class H, int DIM, int KS1, int MT, int MODE, class V>
 __global__ __launch_bounds__(kMfBlock) void MfBf16Kernel(
     const uint16_t* __restrict__ bmat, int64_t bstride, MfRounds R, int k1p, int nqc,
     const double* __restrict__ weights, const int* __restrict__ dofs, const double* __restrict__ metric,
@@ -237,8 +252,9 @@ __global__ __launch_bounds__(kMfBlock) void MfBf16Kernel(
   constexpr int NC = DIM * (DIM + 1) / 2;
   extern __shared__ __align__(16) unsigned char lds_raw[];
   const int size1 = nqc * 32 * DIM * k1p;
-  const __bf16* B1 = reinterpret_cast<const __bf16*>(lds_raw);
-  const __bf16* B2 = B1 + size1;
+  typedef typename Vec8<H>::type h8;
+  const H* B1 = reinterpret_cast<const H*>(lds_raw);
+  const H* B2 = B1 + size1;
 
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int col = lane & 15, grp = lane >> 4;
@@ -280,21 +296,55 @@ __global__ __launch_bounds__(kMfBlock) void MfBf16Kernel(
 #pragma unroll
       for (int j = 0; j < 8; j++) xv[ks][j] = x[dv[ks][j]];
     const double kap = kappa[en];
-    bf16x8 xk[KS1];
+    float xf[KS1][8];
 #pragma unroll
     for (int ks = 0; ks < KS1; ks++)
 #pragma unroll
       for (int j = 0; j < 8; j++) {
         const int dof = 32 * ks + 8 * grp + j;
-        xk[ks][j] = (__bf16)((valid && dof < nd) ? (float)xv[ks][j] : 0.0f);
+        xf[ks][j] = (valid && dof < nd) ? (float)xv[ks][j] : 0.0f;
       }
     float g[NC];
 #pragma unroll
     for (int c = 0; c < NC; c++) g[c] = valid ? (float)(kap * mv[c]) : 0.0f;
+    // FP16 has a 5-bit exponent: the element vector and the metric are
+    // scaled by powers of two (exact) so that their largest entries lie in
+    // [0.5, 1) before rounding, and the result is scaled back in FP32.
+    // Without this the small vectors of a converging inner solve fall into
+    // the FP16 subnormals and lose their significant bits.  BF16 keeps the
+    // FP32 exponent range and needs no scaling.
+    int escale = 0;
+    if constexpr (sizeof(H) == 2 && !std::is_same<H, __bf16>::value) {
+      float xm = 0.0f, gm = 0.0f;
+#pragma unroll
+      for (int ks = 0; ks < KS1; ks++)
+#pragma unroll
+        for (int j = 0; j < 8; j++) xm = fmaxf(xm, fabsf(xf[ks][j]));
+      // the 4 lanes holding one element's dofs: col, col+16, col+32, col+48
+      xm = fmaxf(xm, __shfl_xor(xm, 16, 64));
+      xm = fmaxf(xm, __shfl_xor(xm, 32, 64));
+#pragma unroll
+      for (int c = 0; c < NC; c++) gm = fmaxf(gm, fabsf(g[c]));
+      int ex = 0, eg = 0;
+      (void)frexpf(xm, &ex);
+      (void)frexpf(gm, &eg);
+#pragma unroll
+      for (int ks = 0; ks < KS1; ks++)
+#pragma unroll
+        for (int j = 0; j < 8; j++) xf[ks][j] = ldexpf(xf[ks][j], -ex);
+#pragma unroll
+      for (int c = 0; c < NC; c++) g[c] = ldexpf(g[c], -eg);
+      escale = ex + eg;
+    }
+    h8 xk[KS1];
+#pragma unroll
+    for (int ks = 0; ks < KS1; ks++)
+#pragma unroll
+      for (int j = 0; j < 8; j++) xk[ks][j] = (H)xf[ks][j];
     float4v yacc[MT];
 #pragma unroll
     for (int m = 0; m < MT; m++) yacc[m] = float4v{0, 0, 0, 0};
-    const bf16x8 zero8 = {};
+    const h8 zero8 = {};
 
     for (int c = 0; c < nqc; c++) {
       float4v u[2][DIM];
@@ -306,10 +356,9 @@ __global__ __launch_bounds__(kMfBlock) void MfBf16Kernel(
 #pragma unroll
           for (int ks = 0; ks < KS1; ks++) {
             const int k0 = 32 * ks + 8 * grp;
-            const bf16x8 a = k0 < k1p
-                                 ? *reinterpret_cast<const bf16x8*>(B1 + ((32 * c + 16 * h + col) * DIM + d) * k1p + k0)
-                                 : zero8;
-            u[h][d] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, xk[ks], u[h][d], 0, 0, 0);
+            const h8 a = k0 < k1p ? *reinterpret_cast<const h8*>(B1 + ((32 * c + 16 * h + col) * DIM + d) * k1p + k0)
+                                  : zero8;
+            u[h][d] = Mfma16(a, xk[ks], u[h][d]);
           }
         }
       // metric at the 8 points this lane holds: q = 32c + 16h + 4grp + r
@@ -328,14 +377,13 @@ __global__ __launch_bounds__(kMfBlock) void MfBf16Kernel(
         }
 #pragma unroll
       for (int d = 0; d < DIM; d++) {
-        bf16x8 vb;
+        h8 vb;
 #pragma unroll
-        for (int j = 0; j < 8; j++) vb[j] = (__bf16)v[d][j];
+        for (int j = 0; j < 8; j++) vb[j] = (H)v[d][j];
 #pragma unroll
         for (int m = 0; m < MT; m++) {
-          const bf16x8 a =
-              *reinterpret_cast<const bf16x8*>(B2 + (((c * DIM + d) * MT * 16 + 16 * m + col) * 32 + 8 * grp));
-          yacc[m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, vb, yacc[m], 0, 0, 0);
+          const h8 a = *reinterpret_cast<const h8*>(B2 + (((c * DIM + d) * MT * 16 + 16 * m + col) * 32 + 8 * grp));
+          yacc[m] = Mfma16(a, vb, yacc[m]);
         }
       }
     }
@@ -346,7 +394,230 @@ __global__ __launch_bounds__(kMfBlock) void MfBf16Kernel(
         for (int r = 0; r < 4; r++) {
           const int dof = 16 * m + 4 * grp + r;
           if (dof < nd) {
+            const V val = (V)ldexpf(yacc[m][r], escale);
+            if (MODE == 0) out[e * nd + dof] = val;
+            else AtomicAddV(&out[dofs[e * nd + dof]], (V)(s * (double)val));
+          }
+        }
+    }
+  }
+}
+
+// ------------------------------------------------ reference-matrix kernels
+// For an affine element with one coefficient value the quadrature sum
+// collapses exactly:
+//   sum_q w_q B_q^T (kappa G_e) B_q = sum_c g_{e,c} S_c,
+//   S_(dd) = sum_q w_q b_d b_d^T,  S_(dd') = sum_q w_q (b_d b_d'^T + b_d' b_d^T),
+// with g_e = kappa_e (G_xx, G_yy[, G_zz], G_xy[, G_xz, G_yz]) (the packed
+// metric) and the symmetric S_c (3 in 2D, 6 in 3D) precomputed per class.  The
+// kernel forms y_e = sum_c S_c (g_{e,c} x_e): the metric scales the B
+// operand (one column per element) and every c accumulates into the same
+// MFMA accumulators, so per 16-element tile the work is NC x MT x KS MFMAs
+// (FP64 tet p4: 162 instead of 504 for the quadrature form) and no per-point
+// arithmetic.  LDS holds S[c][i][j] for the rows i < nd (rows beyond nd read
+// as 0), row stride NDP + 1 words (FP64/FP32) or K1P + 8 halves (16-bit).
+
+template <class T, int DIM, int NDP, int MODE, class V>
+__global__ __launch_bounds__(kMfBlock) void MfRefKernel(
+    const T* __restrict__ smat, int64_t sstride, MfRounds R, const int* __restrict__ dofs,
+    const double* __restrict__ metric, const double* __restrict__ kappa, const int* __restrict__ elnr, int nd,
+    const V* __restrict__ x, V* __restrict__ out, double s, const int* __restrict__ stop) {
+  if (stop && *stop) return;
+  typedef typename Acc4<T>::type acc_t;
+  constexpr int NC = DIM * (DIM + 1) / 2;
+  constexpr int KS = NDP / 4;
+  constexpr int MT = (NDP + 15) / 16;
+  constexpr int RS = NDP + 1;
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  T* ls = reinterpret_cast<T*>(lds_raw);
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int col = lane & 15, grp = lane >> 4;
+  const int wg = XcdContiguousBlock(blockIdx.x, gridDim.x);
+  const int r0 = (int)((int64_t)wg * R.nrounds / gridDim.x);
+  const int r1 = (int)((int64_t)(wg + 1) * R.nrounds / gridDim.x);
+  int cls = -1;
+  for (int rd = r0; rd < r1; rd++) {
+    const int c = RoundClass(R, rd, cls);
+    if (c != cls) {
+      if (cls >= 0) __syncthreads();
+      const T* src = smat + (int64_t)c * sstride;
+      for (int i = threadIdx.x; i < (int)sstride; i += kMfBlock) ls[i] = src[i];
+      __syncthreads();
+      cls = c;
+    }
+    const int64_t end = R.range[2 * c + 1];
+    const int64_t first = R.range[2 * c] + ((int64_t)(rd - R.round0[c]) * kMfWaves + wave) * 16;
+    if (first >= end) continue;
+    const int64_t e = first + col;
+    const bool valid = e < end;
+    // unconditional gathers from clamped indices, masked afterwards (see
+    // MfMfmaKernel)
+    const int64_t ev = valid ? e : first;
+    int dv[KS];
+#pragma unroll
+    for (int st = 0; st < KS; st++) dv[st] = dofs[ev * nd + min(4 * st + grp, nd - 1)];
+    const int en = elnr[ev];
+    double mv[NC];
+#pragma unroll
+    for (int cc = 0; cc < NC; cc++) mv[cc] = metric[ev * NC + cc];
+    V xv[KS];
+#pragma unroll
+    for (int st = 0; st < KS; st++) xv[st] = x[dv[st]];
+    const double kap = kappa[en];
+    T xk[KS];
+#pragma unroll
+    for (int st = 0; st < KS; st++) xk[st] = (valid && 4 * st + grp < nd) ? (T)xv[st] : (T)0;
+    acc_t yacc[MT];
+#pragma unroll
+    for (int m = 0; m < MT; m++) yacc[m] = acc_t{0, 0, 0, 0};
+#pragma unroll
+    for (int cc = 0; cc < NC; cc++) {
+      const T gc = valid ? (T)(kap * mv[cc]) : (T)0;
+      T xg[KS];
+#pragma unroll
+      for (int st = 0; st < KS; st++) xg[st] = gc * xk[st];
+      const T* srow = ls + cc * nd * RS;
+#pragma unroll
+      for (int m = 0; m < MT; m++) {
+        const int row = 16 * m + col;
+        const bool inrange = row < nd;
+        const T* sr = srow + (inrange ? row : 0) * RS + grp;
+#pragma unroll
+        for (int st = 0; st < KS; st++) {
+          const T a = inrange ? sr[4 * st] : (T)0;
+          yacc[m] = Mfma(a, xg[st], yacc[m]);
+        }
+      }
+    }
+    if (valid) {
+#pragma unroll
+      for (int m = 0; m < MT; m++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const int dof = 16 * m + AccRow<T>(grp, r);
+          if (dof < nd) {
             const V val = (V)yacc[m][r];
+            if (MODE == 0) out[e * nd + dof] = val;
+            else AtomicAddV(&out[dofs[e * nd + dof]], (V)(s * (double)val));
+          }
+        }
+    }
+  }
+}
+
+// 16-bit operands (BF16 / FP16), FP32 accumulation; operand layout of
+// v_mfma_f32_16x16x32_{bf16,f16} as in MfBf16Kernel.  FP16 uses the same
+// per-element power-of-two scaling of x and g as MfBf16Kernel.
+template <class H, int DIM, int KS1, int MT, int MODE, class V>
+__global__ __launch_bounds__(kMfBlock) void MfRef16Kernel(
+    const uint16_t* __restrict__ smat, int64_t sstride, MfRounds R, const int* __restrict__ dofs,
+    const double* __restrict__ metric, const double* __restrict__ kappa, const int* __restrict__ elnr, int nd,
+    const V* __restrict__ x, V* __restrict__ out, double s, const int* __restrict__ stop) {
+  if (stop && *stop) return;
+  typedef typename Vec8<H>::type h8;
+  constexpr int NC = DIM * (DIM + 1) / 2;
+  constexpr int RS = KS1 * 32 + 8;  // halves per row: 16-byte aligned rows
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  const H* ls = reinterpret_cast<const H*>(lds_raw);
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int col = lane & 15, grp = lane >> 4;
+  const int wg = XcdContiguousBlock(blockIdx.x, gridDim.x);
+  const int r0 = (int)((int64_t)wg * R.nrounds / gridDim.x);
+  const int r1 = (int)((int64_t)(wg + 1) * R.nrounds / gridDim.x);
+  int cls = -1;
+  for (int rd = r0; rd < r1; rd++) {
+    const int cl = RoundClass(R, rd, cls);
+    if (cl != cls) {
+      if (cls >= 0) __syncthreads();
+      // sstride 16-bit words, a multiple of 8
+      const uint4* src = reinterpret_cast<const uint4*>(smat + (int64_t)cl * sstride);
+      uint4* dst = reinterpret_cast<uint4*>(lds_raw);
+      for (int i = threadIdx.x; i < (int)(sstride / 8); i += kMfBlock) dst[i] = src[i];
+      __syncthreads();
+      cls = cl;
+    }
+    const int64_t end = R.range[2 * cl + 1];
+    const int64_t first = R.range[2 * cl] + ((int64_t)(rd - R.round0[cl]) * kMfWaves + wave) * 16;
+    if (first >= end) continue;
+    const int64_t e = first + col;
+    const bool valid = e < end;
+    const int64_t ev = valid ? e : first;
+    int dv[KS1][8];
+#pragma unroll
+    for (int ks = 0; ks < KS1; ks++)
+#pragma unroll
+      for (int j = 0; j < 8; j++) dv[ks][j] = dofs[ev * nd + min(32 * ks + 8 * grp + j, nd - 1)];
+    const int en = elnr[ev];
+    double mv[NC];
+#pragma unroll
+    for (int c = 0; c < NC; c++) mv[c] = metric[ev * NC + c];
+    V xv[KS1][8];
+#pragma unroll
+    for (int ks = 0; ks < KS1; ks++)
+#pragma unroll
+      for (int j = 0; j < 8; j++) xv[ks][j] = x[dv[ks][j]];
+    const double kap = kappa[en];
+    float xf[KS1][8];
+#pragma unroll
+    for (int ks = 0; ks < KS1; ks++)
+#pragma unroll
+      for (int j = 0; j < 8; j++) xf[ks][j] = (valid && 32 * ks + 8 * grp + j < nd) ? (float)xv[ks][j] : 0.0f;
+    float g[NC];
+#pragma unroll
+    for (int c = 0; c < NC; c++) g[c] = valid ? (float)(kap * mv[c]) : 0.0f;
+    int escale = 0;
+    if constexpr (!std::is_same<H, __bf16>::value) {
+      float xm = 0.0f, gm = 0.0f;
+#pragma unroll
+      for (int ks = 0; ks < KS1; ks++)
+#pragma unroll
+        for (int j = 0; j < 8; j++) xm = fmaxf(xm, fabsf(xf[ks][j]));
+      xm = fmaxf(xm, __shfl_xor(xm, 16, 64));
+      xm = fmaxf(xm, __shfl_xor(xm, 32, 64));
+#pragma unroll
+      for (int c = 0; c < NC; c++) gm = fmaxf(gm, fabsf(g[c]));
+      int ex = 0, eg = 0;
+      (void)frexpf(xm, &ex);
+      (void)frexpf(gm, &eg);
+#pragma unroll
+      for (int ks = 0; ks < KS1; ks++)
+#pragma unroll
+        for (int j = 0; j < 8; j++) xf[ks][j] = ldexpf(xf[ks][j], -ex);
+#pragma unroll
+      for (int c = 0; c < NC; c++) g[c] = ldexpf(g[c], -eg);
+      escale = ex + eg;
+    }
+    float4v yacc[MT];
+#pragma unroll
+    for (int m = 0; m < MT; m++) yacc[m] = float4v{0, 0, 0, 0};
+    const h8 zero8 = {};
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+      h8 xg[KS1];
+#pragma unroll
+      for (int ks = 0; ks < KS1; ks++)
+#pragma unroll
+        for (int j = 0; j < 8; j++) xg[ks][j] = (H)(g[c] * xf[ks][j]);
+#pragma unroll
+      for (int m = 0; m < MT; m++) {
+        const int row = 16 * m + col;
+        const bool inrange = row < nd;
+        const H* sr = ls + (c * nd + (inrange ? row : 0)) * RS + 8 * grp;
+#pragma unroll
+        for (int ks = 0; ks < KS1; ks++) {
+          const h8 a = inrange ? *reinterpret_cast<const h8*>(sr + 32 * ks) : zero8;
+          yacc[m] = Mfma16(a, xg[ks], yacc[m]);
+        }
+      }
+    }
+    if (valid) {
+#pragma unroll
+      for (int m = 0; m < MT; m++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const int dof = 16 * m + 4 * grp + r;
+          if (dof < nd) {
+            const V val = (V)ldexpf(yacc[m][r], escale);
             if (MODE == 0) out[e * nd + dof] = val;
             else AtomicAddV(&out[dofs[e * nd + dof]], (V)(s * (double)val));
           }
@@ -516,13 +787,6 @@ static bool SupportedBf16(int ks1, int mt) {
   return (ks1 == 1 && (mt == 1 || mt == 2)) || (ks1 == 2 && mt == 3);
 }
 
-static uint16_t ToBf16(double v) {
-  float f = (float)v;
-  uint32_t u;
-  std::memcpy(&u, &f, 4);
-  uint32_t r = u + 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even
-  return (uint16_t)(r >> 16);
-}
 
 template <int MODE, class V>
 static void LaunchElements(const MatFreeOp& A, const V* x, V* out, double s, const int* stop, hipStream_t st,
@@ -535,7 +799,8 @@ MatFreeOp::MatFreeOp(int dim_, int nd_, int nq_, int64_t ndof, int nclass, const
   NGSAMD_REQUIRE(dim_ == 2 || dim_ == 3, "ngsamd_mf_create: dim must be 2 or 3");
   NGSAMD_REQUIRE(nd_ >= 1 && nd_ <= kScalarMaxNd, "ngsamd_mf_create: nd out of range (1..64)");
   NGSAMD_REQUIRE(nq_ >= 1 && nclass >= 1, "ngsamd_mf_create: empty rule or no classes");
-  NGSAMD_REQUIRE(prec == NGSAMD_PREC_F64 || prec == NGSAMD_PREC_F32 || prec == NGSAMD_PREC_BF16,
+  NGSAMD_REQUIRE(prec == NGSAMD_PREC_F64 || prec == NGSAMD_PREC_F32 || prec == NGSAMD_PREC_BF16 ||
+                     prec == NGSAMD_PREC_F16,
                  "ngsamd_mf_create: unknown precision");
   NGSAMD_REQUIRE(scat == NGSAMD_SCATTER_GATHER || scat == NGSAMD_SCATTER_ATOMIC, "ngsamd_mf_create: unknown scatter");
   NGSAMD_REQUIRE(ndof >= 0 && ndof < (int64_t(1) << 31), "ngsamd_mf_create: ndof out of range");
@@ -567,7 +832,8 @@ MatFreeOp::MatFreeOp(int dim_, int nd_, int nq_, int64_t ndof, int nclass, const
   const int qs = dim * ndp + 1;
   bstride = (int64_t)nqp * qs;
   // MFMA path if the padded shape is instantiated and B fits in LDS
-  if (prec == NGSAMD_PREC_BF16) {
+  const bool half = prec == NGSAMD_PREC_BF16 || prec == NGSAMD_PREC_F16;
+  if (half) {
     nqc = (nq + 31) / 32;
     k1p = (nd + 7) / 8 * 8;
     ks1 = (nd + 31) / 32;
@@ -579,6 +845,72 @@ MatFreeOp::MatFreeOp(int dim_, int nd_, int nq_, int64_t ndof, int nclass, const
     use_mfma = SupportedNdp(ndp) && (size_t)bstride * esize <= (size_t)DeviceMaxLds();
   }
 
+  // Reference-matrix kernels by default (affine elements, one coefficient
+  // value per element: exact); NGSAMD_MF_KERNEL=quad selects the
+  // per-quadrature-point kernels.
+  {
+    const char* kenv = getenv("NGSAMD_MF_KERNEL");
+    NGSAMD_REQUIRE(!kenv || std::string(kenv) == "quad" || std::string(kenv) == "ref",
+                   "NGSAMD_MF_KERNEL must be quad or ref");
+    ref_kernel = use_mfma && !(kenv && std::string(kenv) == "quad");
+  }
+  if (ref_kernel) {
+    const int nc = ncomp;
+    // S[c][k][i][j] in FP64, k over the packed metric components
+    std::vector<double> S((size_t)nclass * nc * nd * nd, 0.0);
+    const int pd[6][2] = {{0, 0}, {1, 1}, {2, 2}, {0, 1}, {0, 2}, {1, 2}};
+    const int pd2[3][2] = {{0, 0}, {1, 1}, {0, 1}};
+    const size_t bsz0 = (size_t)nq * nd * dim;
+    for (int c = 0; c < nclass; c++)
+      for (int k = 0; k < nc; k++) {
+        const int d0 = dim == 3 ? pd[k][0] : pd2[k][0], d1 = dim == 3 ? pd[k][1] : pd2[k][1];
+        double* Sk = S.data() + ((size_t)c * nc + k) * nd * nd;
+        for (int q = 0; q < nq; q++) {
+          const double* bq = bmat + c * bsz0 + (size_t)q * nd * dim;
+          for (int i = 0; i < nd; i++)
+            for (int j = 0; j < nd; j++) {
+              const double v = d0 == d1 ? bq[i * dim + d0] * bq[j * dim + d0]
+                                        : bq[i * dim + d0] * bq[j * dim + d1] + bq[i * dim + d1] * bq[j * dim + d0];
+              Sk[(size_t)i * nd + j] += w[q] * v;
+            }
+        }
+      }
+    if (half) {
+      const int rs = ks1 * 32 + 8;
+      sstride = (int64_t)nc * nd * rs;
+      std::vector<uint16_t> lay((size_t)sstride * nclass, 0);
+      for (int c = 0; c < nclass; c++)
+        for (int k = 0; k < nc; k++)
+          for (int i = 0; i < nd; i++)
+            for (int j = 0; j < nd; j++) {
+              const double v = S[(((size_t)c * nc + k) * nd + i) * nd + j];
+              lay[(size_t)c * sstride + ((size_t)k * nd + i) * rs + j] =
+                  prec == NGSAMD_PREC_F16 ? ToF16(v) : ToBf16(v);
+            }
+      smat16.alloc(lay.size());
+      smat16.upload(lay.data(), lay.size());
+    } else {
+      const int rs = ndp + 1;
+      sstride = (int64_t)nc * nd * rs;
+      std::vector<double> lay((size_t)sstride * nclass, 0.0);
+      for (int c = 0; c < nclass; c++)
+        for (int k = 0; k < nc; k++)
+          for (int i = 0; i < nd; i++)
+            for (int j = 0; j < nd; j++)
+              lay[(size_t)c * sstride + ((size_t)k * nd + i) * rs + j] = S[(((size_t)c * nc + k) * nd + i) * nd + j];
+      if (prec == NGSAMD_PREC_F64) {
+        smat64.alloc(lay.size());
+        smat64.upload(lay.data(), lay.size());
+      } else {
+        std::vector<float> l32(lay.begin(), lay.end());
+        smat32.alloc(l32.size());
+        smat32.upload(l32.data(), l32.size());
+        NGSAMD_HIP(hipStreamSynchronize(Stream()));
+      }
+    }
+    NGSAMD_HIP(hipStreamSynchronize(Stream()));
+  }
+
   // FP64 reference gradients [class][nq][nd][dim] for the scalar kernels
   const size_t bsz = (size_t)nq * nd * dim;
   bref.alloc(bsz * nclass);
@@ -588,7 +920,7 @@ MatFreeOp::MatFreeOp(int dim_, int nd_, int nq_, int64_t ndof, int nclass, const
   for (int q = 0; q < nq; q++) wp[q] = w[q];
   weights.alloc(nw);
   weights.upload(wp.data(), nw);
-  if (use_mfma && prec == NGSAMD_PREC_BF16) {
+  if (use_mfma && half && !ref_kernel) {
     std::vector<uint16_t> lay((size_t)bstride * nclass, 0);
     const int64_t size1 = (int64_t)nqc * 32 * dim * k1p;
     for (int c = 0; c < nclass; c++) {
@@ -597,7 +929,8 @@ MatFreeOp::MatFreeOp(int dim_, int nd_, int nq_, int64_t ndof, int nclass, const
       for (int q = 0; q < nq; q++)
         for (int i = 0; i < nd; i++)
           for (int d = 0; d < dim; d++) {
-            const uint16_t v = ToBf16(bmat[c * bsz + ((size_t)q * nd + i) * dim + d]);
+            const double bv = bmat[c * bsz + ((size_t)q * nd + i) * dim + d];
+            const uint16_t v = prec == NGSAMD_PREC_F16 ? ToF16(bv) : ToBf16(bv);
             b1[((size_t)q * dim + d) * k1p + i] = v;
             // position of point q inside its chunk in the accumulator order
             const int ch = q / 32, ql = q % 32, h = ql / 16, grp = (ql % 16) / 4, r = ql % 4;
@@ -607,7 +940,7 @@ MatFreeOp::MatFreeOp(int dim_, int nd_, int nq_, int64_t ndof, int nclass, const
     }
     bmat16.alloc(lay.size());
     bmat16.upload(lay.data(), lay.size());
-  } else if (use_mfma) {
+  } else if (use_mfma && !ref_kernel) {
     std::vector<double> lay((size_t)bstride * nclass, 0.0);
     for (int c = 0; c < nclass; c++)
       for (int q = 0; q < nq; q++)
@@ -742,11 +1075,11 @@ static void DispatchNdp(const MatFreeOp& A, const T* b, const V* x, V* out, doub
   }
 }
 
-template <int DIM, int KS1, int MT, int MODE, class V>
+template <class H, int DIM, int KS1, int MT, int MODE, class V>
 static void LaunchBf16(const MatFreeOp& A, const V* x, V* out, double s, const int* stop, hipStream_t st,
                        bool prepare) {
   const size_t lds = (size_t)A.bstride * 2;
-  auto kern = MfBf16Kernel<DIM, KS1, MT, MODE, V>;
+  auto kern = MfBf16Kernel<H, DIM, KS1, MT, MODE, V>;
   if (prepare) return PrepareKernel(A, kern, lds, MODE, sizeof(V) == 8);
   const MfRounds R{A.round0.p, A.cls_range.p, A.nrounds};
   kern<<<MfGrid(A, MODE, sizeof(V) == 8), kMfBlock, lds, st>>>(A.bmat16.p, A.bstride, R, A.k1p, A.nqc, A.weights.p,
@@ -755,13 +1088,61 @@ static void LaunchBf16(const MatFreeOp& A, const V* x, V* out, double s, const i
   NGSAMD_CHECK_LAUNCH();
 }
 
-template <int DIM, int MODE, class V>
+template <class H, int DIM, int MODE, class V>
 static void DispatchBf16(const MatFreeOp& A, const V* x, V* out, double s, const int* stop, hipStream_t st,
                          bool prepare) {
-  if (A.ks1 == 1 && A.mt16 == 1) LaunchBf16<DIM, 1, 1, MODE, V>(A, x, out, s, stop, st, prepare);
-  else if (A.ks1 == 1 && A.mt16 == 2) LaunchBf16<DIM, 1, 2, MODE, V>(A, x, out, s, stop, st, prepare);
-  else if (A.ks1 == 2 && A.mt16 == 3) LaunchBf16<DIM, 2, 3, MODE, V>(A, x, out, s, stop, st, prepare);
-  else throw Error(NGSAMD_E_STATE, "matrix-free: unsupported BF16 shape");
+  if (A.ks1 == 1 && A.mt16 == 1) LaunchBf16<H, DIM, 1, 1, MODE, V>(A, x, out, s, stop, st, prepare);
+  else if (A.ks1 == 1 && A.mt16 == 2) LaunchBf16<H, DIM, 1, 2, MODE, V>(A, x, out, s, stop, st, prepare);
+  else if (A.ks1 == 2 && A.mt16 == 3) LaunchBf16<H, DIM, 2, 3, MODE, V>(A, x, out, s, stop, st, prepare);
+  else throw Error(NGSAMD_E_STATE, "matrix-free: unsupported 16-bit shape");
+}
+
+template <class T, int DIM, int NDP, int MODE, class V>
+static void LaunchRef(const MatFreeOp& A, const T* sm, const V* x, V* out, double s, const int* stop,
+                      hipStream_t st, bool prepare) {
+  const size_t lds = (size_t)A.sstride * sizeof(T);
+  auto kern = MfRefKernel<T, DIM, NDP, MODE, V>;
+  if (prepare) return PrepareKernel(A, kern, lds, MODE, sizeof(V) == 8);
+  const MfRounds R{A.round0.p, A.cls_range.p, A.nrounds};
+  kern<<<MfGrid(A, MODE, sizeof(V) == 8), kMfBlock, lds, st>>>(sm, A.sstride, R, A.dofs.p, A.metric.p, A.Kappa(),
+                                                               A.elnr.p, A.nd, x, out, s, stop);
+  NGSAMD_CHECK_LAUNCH();
+}
+
+template <class T, int DIM, int MODE, class V>
+static void DispatchRefNdp(const MatFreeOp& A, const T* sm, const V* x, V* out, double s, const int* stop,
+                           hipStream_t st, bool prepare) {
+  switch (A.ndp) {
+    case 4: LaunchRef<T, DIM, 4, MODE, V>(A, sm, x, out, s, stop, st, prepare); break;
+    case 8: LaunchRef<T, DIM, 8, MODE, V>(A, sm, x, out, s, stop, st, prepare); break;
+    case 12: LaunchRef<T, DIM, 12, MODE, V>(A, sm, x, out, s, stop, st, prepare); break;
+    case 16: LaunchRef<T, DIM, 16, MODE, V>(A, sm, x, out, s, stop, st, prepare); break;
+    case 20: LaunchRef<T, DIM, 20, MODE, V>(A, sm, x, out, s, stop, st, prepare); break;
+    case 24: LaunchRef<T, DIM, 24, MODE, V>(A, sm, x, out, s, stop, st, prepare); break;
+    case 36: LaunchRef<T, DIM, 36, MODE, V>(A, sm, x, out, s, stop, st, prepare); break;
+    default: throw Error(NGSAMD_E_STATE, "matrix-free: unsupported padded dof count");
+  }
+}
+
+template <class H, int DIM, int KS1, int MT, int MODE, class V>
+static void LaunchRef16(const MatFreeOp& A, const V* x, V* out, double s, const int* stop, hipStream_t st,
+                        bool prepare) {
+  const size_t lds = (size_t)A.sstride * 2;
+  auto kern = MfRef16Kernel<H, DIM, KS1, MT, MODE, V>;
+  if (prepare) return PrepareKernel(A, kern, lds, MODE, sizeof(V) == 8);
+  const MfRounds R{A.round0.p, A.cls_range.p, A.nrounds};
+  kern<<<MfGrid(A, MODE, sizeof(V) == 8), kMfBlock, lds, st>>>(A.smat16.p, A.sstride, R, A.dofs.p, A.metric.p,
+                                                               A.Kappa(), A.elnr.p, A.nd, x, out, s, stop);
+  NGSAMD_CHECK_LAUNCH();
+}
+
+template <class H, int DIM, int MODE, class V>
+static void DispatchRef16(const MatFreeOp& A, const V* x, V* out, double s, const int* stop, hipStream_t st,
+                          bool prepare) {
+  if (A.ks1 == 1 && A.mt16 == 1) LaunchRef16<H, DIM, 1, 1, MODE, V>(A, x, out, s, stop, st, prepare);
+  else if (A.ks1 == 1 && A.mt16 == 2) LaunchRef16<H, DIM, 1, 2, MODE, V>(A, x, out, s, stop, st, prepare);
+  else if (A.ks1 == 2 && A.mt16 == 3) LaunchRef16<H, DIM, 2, 3, MODE, V>(A, x, out, s, stop, st, prepare);
+  else throw Error(NGSAMD_E_STATE, "matrix-free: unsupported 16-bit shape");
 }
 
 // All elements: one launch of the MFMA kernel (every class), or the scalar
@@ -769,11 +1150,31 @@ static void DispatchBf16(const MatFreeOp& A, const V* x, V* out, double s, const
 template <int MODE, class V>
 static void LaunchElements(const MatFreeOp& A, const V* x, V* out, double s, const int* stop, hipStream_t st,
                            bool prepare) {
+  if (A.use_mfma && A.ref_kernel) {
+    if (A.nrounds == 0 && !prepare) return;
+    if (A.precision == NGSAMD_PREC_BF16) {
+      if (A.dim == 2) DispatchRef16<__bf16, 2, MODE, V>(A, x, out, s, stop, st, prepare);
+      else DispatchRef16<__bf16, 3, MODE, V>(A, x, out, s, stop, st, prepare);
+    } else if (A.precision == NGSAMD_PREC_F16) {
+      if (A.dim == 2) DispatchRef16<_Float16, 2, MODE, V>(A, x, out, s, stop, st, prepare);
+      else DispatchRef16<_Float16, 3, MODE, V>(A, x, out, s, stop, st, prepare);
+    } else if (A.precision == NGSAMD_PREC_F64) {
+      if (A.dim == 2) DispatchRefNdp<double, 2, MODE, V>(A, A.smat64.p, x, out, s, stop, st, prepare);
+      else DispatchRefNdp<double, 3, MODE, V>(A, A.smat64.p, x, out, s, stop, st, prepare);
+    } else {
+      if (A.dim == 2) DispatchRefNdp<float, 2, MODE, V>(A, A.smat32.p, x, out, s, stop, st, prepare);
+      else DispatchRefNdp<float, 3, MODE, V>(A, A.smat32.p, x, out, s, stop, st, prepare);
+    }
+    return;
+  }
   if (A.use_mfma) {
     if (A.nrounds == 0 && !prepare) return;
     if (A.precision == NGSAMD_PREC_BF16) {
-      if (A.dim == 2) DispatchBf16<2, MODE, V>(A, x, out, s, stop, st, prepare);
-      else DispatchBf16<3, MODE, V>(A, x, out, s, stop, st, prepare);
+      if (A.dim == 2) DispatchBf16<__bf16, 2, MODE, V>(A, x, out, s, stop, st, prepare);
+      else DispatchBf16<__bf16, 3, MODE, V>(A, x, out, s, stop, st, prepare);
+    } else if (A.precision == NGSAMD_PREC_F16) {
+      if (A.dim == 2) DispatchBf16<_Float16, 2, MODE, V>(A, x, out, s, stop, st, prepare);
+      else DispatchBf16<_Float16, 3, MODE, V>(A, x, out, s, stop, st, prepare);
     } else if (A.precision == NGSAMD_PREC_F64) {
       if (A.dim == 2) DispatchNdp<double, 2, MODE, V>(A, A.bmat64.p, x, out, s, stop, st, prepare);
       else DispatchNdp<double, 3, MODE, V>(A, A.bmat64.p, x, out, s, stop, st, prepare);

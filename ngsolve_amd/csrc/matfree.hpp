@@ -17,6 +17,7 @@ struct MatFreeOp : ngsamd_op_s {
   int precision = NGSAMD_PREC_F64;
   int scatter = NGSAMD_SCATTER_GATHER;
   bool use_mfma = false;
+  bool ref_kernel = false;    // reference-matrix MFMA kernels (else per-quadrature-point)
   int64_t nel = 0;
   std::vector<MfClass> classes;
   // MFMA kernels' schedule: rounds of 16-element tiles (one per wavefront)
@@ -31,6 +32,12 @@ struct MatFreeOp : ngsamd_op_s {
   DevBuf<uint16_t> bmat16;
   int64_t bstride = 0;        // elements per class block in bmat*
   int nqc = 0, k1p = 0, ks1 = 0, mt16 = 0;  // BF16 layout: point chunks of 32, padded dofs, k-steps, dof tiles
+  // reference matrices S[class][c][i][j] (c over the packed metric
+  // components), rows padded to ndp + 1 words (FP64/FP32) or ks1*32 + 8 halves
+  DevBuf<double> smat64;
+  DevBuf<float> smat32;
+  DevBuf<uint16_t> smat16;
+  int64_t sstride = 0;        // elements per class block in smat*
   DevBuf<double> weights;     // [nqp], 0 beyond nq
   DevBuf<double> bref;        // [nclass][nq][nd][dim] FP64 (scalar path, diagonal, gradient)
   DevBuf<int> dofs;           // [nel][nd]
