@@ -149,8 +149,11 @@ int ngsamd_cg_solve(ngsamd_op_t mat, ngsamd_op_t pre, const double* d_rhs, doubl
 /* Mixed-precision iterative refinement: outer FP64 residual r = f - A u with
  * `mat`, inner Jacobi-PCG on `inner` (FP32 vectors, FP64 dots and scalars)
  * to relative tolerance inner_tol, u += correction, until
- * ||r||_2 <= tol ||f||_2.  h_stats[0]=outer its, [1]=total inner its,
- * [2]=final relative residual. */
+ * ||r||_2 <= tol ||f||_2 (norms over the free dofs).  inner_tol <= 0 picks a
+ * default by the inner operator's precision (FP32 3e-4, FP16 1e-3, BF16
+ * 3e-2).  h_stats[0]=outer its, [1]=total inner its, [2]=final relative
+ * residual.  No counterpart in the reference (it has no mixed-precision
+ * solver); it sits beside CGSolver (python/krylovspace.py:227-291). */
 int ngsamd_ir_cg_solve(ngsamd_op_t mat, ngsamd_op_t inner, ngsamd_op_t pre, const double* d_rhs,
                        double* d_sol, double tol, double inner_tol, int max_outer, int max_inner,
                        double* h_stats);

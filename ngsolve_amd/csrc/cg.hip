@@ -505,12 +505,18 @@ extern "C" int ngsamd_ir_cg_solve(ngsamd_op_t A, ngsamd_op_t inner, ngsamd_op_t 
     long long inner_total = 0;
     double relres = 1.0;
     if (n) NGSAMD_HIP(hipMemcpyAsync(r.p, f, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+    // default inner tolerance by the inner operator's precision (the
+    // fastest setting measured on the 32^3 p=4 lognormal-kappa problem)
+    if (inner_tol <= 0.0)
+      inner_tol = Ai->precision == NGSAMD_PREC_F32 ? 3e-4 : Ai->precision == NGSAMD_PREC_F16 ? 1e-3 : 3e-2;
+    double hsc[2] = {1.0, 1.0};  // host source of the async copy below: lives until the next sync
     while (true) {
       const double rn = n ? free_norm(r.p) : 0.0;
       relres = fn > 0 ? rn / fn : 0.0;
       if (rn <= tol * fn || outer >= max_outer || rn == 0.0) break;
       // inner solve A c = r / ||r|| in FP32 vectors
-      double hsc[2] = {1.0 / rn, rn};
+      hsc[0] = 1.0 / rn;
+      hsc[1] = rn;
       NGSAMD_HIP(hipMemcpyAsync(scal.p, hsc, sizeof(hsc), hipMemcpyHostToDevice, st));
       ScaledToF32Kernel<<<GridFor(n, 256, 8192), 256, 0, st>>>(r.p, W.d.p, n, scal.p);
       NGSAMD_CHECK_LAUNCH();

@@ -477,17 +477,17 @@ __global__ __launch_bounds__(kMfBlock) void MfRefKernel(
 #pragma unroll
       for (int st = 0; st < KS; st++) xg[st] = gc * xk[st];
       const T* srow = ls + cc * nd * RS;
+      // k-steps outside, dof tiles inside: consecutive MFMAs go to
+      // independent accumulators
 #pragma unroll
-      for (int m = 0; m < MT; m++) {
-        const int row = 16 * m + col;
-        const bool inrange = row < nd;
-        const T* sr = srow + (inrange ? row : 0) * RS + grp;
+      for (int st = 0; st < KS; st++)
 #pragma unroll
-        for (int st = 0; st < KS; st++) {
-          const T a = inrange ? sr[4 * st] : (T)0;
+        for (int m = 0; m < MT; m++) {
+          const int row = 16 * m + col;
+          const bool inrange = row < nd;
+          const T a = inrange ? srow[(inrange ? row : 0) * RS + grp + 4 * st] : (T)0;
           yacc[m] = Mfma(a, xg[st], yacc[m]);
         }
-      }
     }
     if (valid) {
 #pragma unroll
@@ -599,16 +599,15 @@ __global__ __launch_bounds__(kMfBlock) void MfRef16Kernel(
 #pragma unroll
         for (int j = 0; j < 8; j++) xg[ks][j] = (H)(g[c] * xf[ks][j]);
 #pragma unroll
-      for (int m = 0; m < MT; m++) {
-        const int row = 16 * m + col;
-        const bool inrange = row < nd;
-        const H* sr = ls + (c * nd + (inrange ? row : 0)) * RS + 8 * grp;
+      for (int ks = 0; ks < KS1; ks++)
 #pragma unroll
-        for (int ks = 0; ks < KS1; ks++) {
-          const h8 a = inrange ? *reinterpret_cast<const h8*>(sr + 32 * ks) : zero8;
+        for (int m = 0; m < MT; m++) {
+          const int row = 16 * m + col;
+          const bool inrange = row < nd;
+          const h8 a = inrange ? *reinterpret_cast<const h8*>(ls + (c * nd + (inrange ? row : 0)) * RS + 8 * grp + 32 * ks)
+                               : zero8;
           yacc[m] = Mfma16(a, xg[ks], yacc[m]);
         }
-      }
     }
     if (valid) {
 #pragma unroll
@@ -712,6 +711,100 @@ __global__ __launch_bounds__(kGatherBlock) void MfGatherKernel(
     __shared__ double lds[kGatherBlock / 64];
     StorePartial<kGatherBlock>(dotacc, partials, lds);
   }
+}
+
+// The same gather with the rows (dofs in gather order) cut into blocks of at
+// most kGatherNnz slots: a workgroup loads a block's slot numbers coalesced
+// and its contributions (one per lane, all loads in flight), then sums each
+// row from LDS with a power-of-two group of lanes sized to the block's row
+// count.  Every lane stays busy whatever the number of slots per dof (24 for
+// a vertex of a tet mesh, 1 for a cell dof), where the one-thread-per-dof
+// kernel above idles most lanes of a wave on the longest row.
+constexpr int kGatherNnz = 2048;
+
+template <class C, class V, bool DOT>
+__global__ __launch_bounds__(kGatherBlock) void MfGatherBlockKernel(
+    const int* __restrict__ rowptr, const int* __restrict__ slots, const int* __restrict__ order,
+    const int* __restrict__ blk_row, int nblocks, int blocks_per_wg, const C* __restrict__ contrib,
+    V* __restrict__ y, double s, int accumulate, const int* __restrict__ stop, const V* __restrict__ x,
+    double* __restrict__ partials) {
+  if (stop && *stop) return;
+  constexpr int PER = kGatherNnz / kGatherBlock;
+  __shared__ double prod[kGatherNnz];
+  __shared__ double lds[kGatherBlock / 64];
+  const int tid = threadIdx.x;
+  const int lb = XcdContiguousBlock(blockIdx.x, gridDim.x);
+  const int b0 = lb * blocks_per_wg;
+  const int b1 = min(b0 + blocks_per_wg, nblocks);
+  double dotacc = 0.0;
+  for (int b = b0; b < b1; b++) {
+    const int r0 = blk_row[b], r1 = blk_row[b + 1];
+    const int k0 = rowptr[r0], k1 = rowptr[r1];
+    const int nz = k1 - k0;
+    if (nz > kGatherNnz) {
+      // one dof with more slots than a block: the whole workgroup strides
+      double a = 0.0;
+      for (int k = k0 + tid; k < k1; k += kGatherBlock) a += (double)contrib[slots[k]];
+      a = BlockSum<kGatherBlock>(a, lds);
+      if (tid == 0) {
+        const int d = order[r0];
+        const double v = s * a;
+        const V yv = accumulate ? (V)((double)y[d] + v) : (V)v;
+        y[d] = yv;
+        if (DOT) dotacc += (double)x[d] * (double)yv;
+      }
+      continue;
+    }
+    // unconditional loads (lanes past the block read slot entry k0, kept in
+    // bounds by the padding of the slot array)
+    int sl[PER];
+    C cv[PER];
+#pragma unroll
+    for (int i = 0; i < PER; i++) {
+      const int k = tid + i * kGatherBlock;
+      sl[i] = slots[k0 + (k < nz ? k : 0)];
+    }
+#pragma unroll
+    for (int i = 0; i < PER; i++) cv[i] = contrib[sl[i]];
+#pragma unroll
+    for (int i = 0; i < PER; i++) {
+      const int k = tid + i * kGatherBlock;
+      prod[k] = k < nz ? (double)cv[i] : 0.0;
+    }
+    __syncthreads();
+    const int nr = r1 - r0;
+    int tpr = 64;
+    while (tpr > 1 && tpr * nr > kGatherBlock) tpr >>= 1;
+    const int l = tid & (tpr - 1);
+    for (int j0 = 0; j0 < nr; j0 += kGatherBlock / tpr) {
+      const int j = j0 + tid / tpr;
+      double sum = 0.0;
+      if (j < nr) {
+        const int s0 = rowptr[r0 + j] - k0, s1 = rowptr[r0 + j + 1] - k0;
+        // four independent partial sums: the LDS reads of a long row overlap
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+        int k = s0 + l;
+        for (; k + 3 * tpr < s1; k += 4 * tpr) {
+          a0 += prod[k];
+          a1 += prod[k + tpr];
+          a2 += prod[k + 2 * tpr];
+          a3 += prod[k + 3 * tpr];
+        }
+        for (; k < s1; k += tpr) a0 += prod[k];
+        sum = (a0 + a1) + (a2 + a3);
+      }
+      for (int off = tpr / 2; off > 0; off >>= 1) sum += __shfl_xor(sum, off, 64);
+      if (j < nr && l == 0) {
+        const int d = order[r0 + j];
+        const double v = s * sum;
+        const V yv = accumulate ? (V)((double)y[d] + v) : (V)v;
+        y[d] = yv;
+        if (DOT) dotacc += (double)x[d] * (double)yv;
+      }
+    }
+    __syncthreads();
+  }
+  if (DOT) StorePartial<kGatherBlock>(dotacc, partials, lds);
 }
 
 template <class V>
@@ -996,14 +1089,41 @@ MatFreeOp::MatFreeOp(int dim_, int nd_, int nq_, int64_t ndof, int nclass, const
     }
     g_rowptr.alloc(ndof + 1);
     g_rowptr.upload(prp.data(), ndof + 1);
-    g_slots.alloc(std::max<size_t>(psl.size(), 1));
-    if (!psl.empty()) g_slots.upload(psl.data(), psl.size());
+    // 4 padding entries (slot 0): lanes past the last block read entry nslots
+    std::vector<int> psl_pad(psl);
+    psl_pad.resize(psl.size() + 4, 0);
+    g_slots.alloc(psl_pad.size());
+    g_slots.upload(psl_pad.data(), psl_pad.size());
+    // blocks of rows with at most kGatherNnz slots (or one longer row)
+    std::vector<int> gb(1, 0);
+    int64_t acc = 0;
+    for (int64_t r = 0; r < ndof; r++) {
+      const int64_t len = prp[r + 1] - prp[r];
+      if (r > gb.back() && acc + len > kGatherNnz) {
+        gb.push_back((int)r);
+        acc = 0;
+      }
+      acc += len;
+    }
+    if (gb.back() != ndof || ndof == 0) gb.push_back((int)ndof);
+    g_nblocks = (int)gb.size() - 1;
+    const char* genv = getenv("NGSAMD_MF_GATHER");
+    NGSAMD_REQUIRE(!genv || std::string(genv) == "dof" || std::string(genv) == "block",
+                   "NGSAMD_MF_GATHER must be dof or block");
+    // default (-1): one thread per dof for plain applies (a wide grid hides
+    // the latency of each dof's load chain), slot blocks when CG's partial
+    // sums are fused in (that grid is capped at kMaxDotBlocks)
+    gather_kind = !genv ? -1 : std::string(genv) == "dof" ? 0 : 1;
+    g_blk.alloc(gb.size());
+    g_blk.upload(gb.data(), gb.size());
+    NGSAMD_HIP(hipStreamSynchronize(Stream()));
     g_order.alloc(std::max<int64_t>(ndof, 1));
     if (ndof) g_order.upload(order.data(), ndof);
     NGSAMD_HIP(hipStreamSynchronize(Stream()));
   }
-  contrib64.alloc((size_t)nel * nd);
-  contrib32.alloc((size_t)nel * nd);
+  // at least one entry: the gather's padding lanes read contribution 0
+  contrib64.alloc(std::max<size_t>((size_t)nel * nd, 1));
+  contrib32.alloc(std::max<size_t>((size_t)nel * nd, 1));
   // round schedule of the MFMA kernels: 16-element tiles, kMfWaves tiles per
   // round, rounds never straddle a class
   {
@@ -1218,15 +1338,35 @@ int MatFreeOp::Apply(double s, const V* x, V* y, bool accumulate, const int* sto
   if constexpr (sizeof(V) == 8) contrib = reinterpret_cast<V*>(contrib64.p);
   else contrib = reinterpret_cast<V*>(contrib32.p);
   LaunchElements<0, V>(*this, x, contrib, 1.0, stop, st, false);
+  if (gather_kind == 0 || (gather_kind < 0 && !partials)) {
+    // one thread per dof
+    if (partials) {
+      const int gd = GridFor(height, kGatherBlock, kMaxDotBlocks);
+      MfGatherKernel<V, V, true><<<gd, kGatherBlock, 0, st>>>(g_rowptr.p, g_slots.p, g_order.p, contrib, y, height,
+                                                              s, accumulate ? 1 : 0, stop, x, partials);
+      NGSAMD_CHECK_LAUNCH();
+      return gd;
+    }
+    MfGatherKernel<V, V, false><<<GridFor(height, kGatherBlock, 16384), kGatherBlock, 0, st>>>(
+        g_rowptr.p, g_slots.p, g_order.p, contrib, y, height, s, accumulate ? 1 : 0, stop, nullptr, nullptr);
+    NGSAMD_CHECK_LAUNCH();
+    return 0;
+  }
+  int g = g_nblocks < (partials ? kMaxDotBlocks : kMaxReduceBlocks) ? g_nblocks
+                                                                      : (partials ? kMaxDotBlocks : kMaxReduceBlocks);
+  if (g > 8) g &= ~7;
+  if (g < 1) g = 1;
+  const int per = (g_nblocks + g - 1) / g;
   if (partials) {
-    const int g = GridFor(height, kGatherBlock, kMaxDotBlocks);
-    MfGatherKernel<V, V, true><<<g, kGatherBlock, 0, st>>>(g_rowptr.p, g_slots.p, g_order.p, contrib, y, height,
-                                                           s, accumulate ? 1 : 0, stop, x, partials);
+    MfGatherBlockKernel<V, V, true><<<g, kGatherBlock, 0, st>>>(g_rowptr.p, g_slots.p, g_order.p, g_blk.p, g_nblocks,
+                                                                per, contrib, y, s, accumulate ? 1 : 0, stop, x,
+                                                                partials);
     NGSAMD_CHECK_LAUNCH();
     return g;
   }
-  MfGatherKernel<V, V, false><<<GridFor(height, kGatherBlock, 16384), kGatherBlock, 0, st>>>(
-      g_rowptr.p, g_slots.p, g_order.p, contrib, y, height, s, accumulate ? 1 : 0, stop, nullptr, nullptr);
+  MfGatherBlockKernel<V, V, false><<<g, kGatherBlock, 0, st>>>(g_rowptr.p, g_slots.p, g_order.p, g_blk.p, g_nblocks,
+                                                               per, contrib, y, s, accumulate ? 1 : 0, stop, nullptr,
+                                                               nullptr);
   NGSAMD_CHECK_LAUNCH();
   return 0;
 }

@@ -166,12 +166,13 @@ class DevCGSolver(LinearSolver):
 class IRCGSolver(LinearSolver):
     """FP64 iterative refinement around a low-precision matrix-free inner CG.
 
-    `mat` is the FP64 operator for the residual, `inner` the FP32/BF16
-    matrix-free operator; stops when ||f - A u||_free <= tol ||f||_free."""
+    `mat` is the FP64 operator for the residual, `inner` the FP32/FP16/BF16
+    matrix-free operator; stops when ||f - A u||_free <= tol ||f||_free.
+    inner_tol=None uses the solver's default for the inner precision."""
 
     name = "IRCG"
 
-    def __init__(self, mat, inner, pre, tol=1e-10, inner_tol=1e-3, maxouter=50, maxinner=20000):
+    def __init__(self, mat, inner, pre, tol=1e-10, inner_tol=None, maxouter=50, maxinner=20000):
         super().__init__(mat, pre=pre, tol=tol, maxiter=maxouter)
         self.inner = inner
         self.inner_tol = inner_tol
@@ -183,7 +184,8 @@ class IRCGSolver(LinearSolver):
     def _SolveImpl(self, rhs, sol):
         from ._native import N
         outer, inner, rel = N.IRCGSolve(self.mat.op, self.inner.op, self.pre.op, rhs._device(), sol._write_device(),
-                                        float(self.tol), float(self.inner_tol), int(self.maxiter),
+                                        float(self.tol), -1.0 if self.inner_tol is None else float(self.inner_tol),
+                                        int(self.maxiter),
                                         int(self.maxinner))
         self.outer_iterations, self.inner_iterations, self.relative_residual = outer, inner, rel
         self.iterations = outer
