@@ -11,7 +11,7 @@
 // or cell of an H1 space) form a run that stores the list once and its values
 // column by column.  Values still stream once (8 B per nonzero) but the column
 // numbers drop to 4 B per run entry and each gathered x entry serves the
-// whole run.  Blocks hold runs of one length and at most 4096 values; the
+// whole run.  Blocks hold runs of one length and at most 2048 values; the
 // block loop is the stream kernel's.
 //
 // stream: int32 row pointers, int32 column numbers, FP64 values (12 B per
@@ -32,6 +32,7 @@
 #include <algorithm>
 #include <cstdlib>
 #include <string>
+#include <type_traits>
 
 #include "common.hpp"
 #include "csr.hpp"
@@ -185,7 +186,16 @@ __global__ __launch_bounds__(kStreamBlock) void CsrStreamKernel(
       double sum = 0.0;
       if (j < nr) {
         const int s0 = rowptr[r0 + j] - k0, s1 = rowptr[r0 + j + 1] - k0;
-        for (int k = s0 + l; k < s1; k += tpr) sum += prod[k];
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+        int k = s0 + l;
+        for (; k + 3 * tpr < s1; k += 4 * tpr) {
+          a0 += prod[k];
+          a1 += prod[k + tpr];
+          a2 += prod[k + 2 * tpr];
+          a3 += prod[k + 3 * tpr];
+        }
+        for (; k < s1; k += tpr) a0 += prod[k];
+        sum = (a0 + a1) + (a2 + a3);
       }
       for (int off = tpr / 2; off > 0; off >>= 1) sum += __shfl_xor(sum, off, 64);
       if (j < nr && l == 0) {
@@ -198,7 +208,7 @@ __global__ __launch_bounds__(kStreamBlock) void CsrStreamKernel(
   if (DOT) StorePartial<kStreamBlock>(dotacc, partials, lds);
 }
 
-// Exact k / R for 0 <= k < 4096 and 1 <= R <= 8 as (k * ceil(2^16 / R)) >> 16:
+// Exact k / R for 0 <= k <= 4096 and 1 <= R <= 8 as (k * ceil(2^16 / R)) >> 16:
 // the rounding error k (ceil(2^16/R) - 2^16/R) / 2^16 stays below 1/16 while
 // the fractional part of k / R is at most 7/8.
 __device__ __forceinline__ int RunDiv(int k, int inv) { return (k * inv) >> 16; }
@@ -213,7 +223,9 @@ __device__ __forceinline__ int RunDiv(int k, int inv) { return (k * inv) >> 16; 
 // meanwhile (at the price of the registers holding them).  A block holding
 // more than NNZ values is a single row (R = 1) that the whole workgroup
 // strides.
-template <int NNZ, bool PIPE, bool ACCUM, bool DOT>
+typedef double double2u __attribute__((ext_vector_type(2), aligned(8)));
+
+template <int NNZ, bool PIPE, int VEC, bool ACCUM, bool DOT>
 __global__ __launch_bounds__(kStreamBlock) void CsrRunsKernel(
     const int* __restrict__ run_row, const int* __restrict__ run_ptr, const int* __restrict__ val_ptr,
     const int* __restrict__ rcols, const double* __restrict__ rvals, const double* __restrict__ x,
@@ -245,12 +257,27 @@ __global__ __launch_bounds__(kStreamBlock) void CsrRunsKernel(
     if (nv > NNZ) return;  // long row: streamed when it is reached
     // unconditional loads (lanes past the block read entry 0 of it, kept in
     // bounds by the padding) so that every load is in flight at once
+    if constexpr (VEC == 1) {
 #pragma unroll
-    for (int i = 0; i < PER; i++) {
-      const int k = tid + i * kStreamBlock;
-      const int kk = k < nv ? k : 0;
-      c[i] = __builtin_nontemporal_load(&rcols[c0 + RunDiv(kk, inv)]);
-      v[i] = __builtin_nontemporal_load(&rvals[vb + kk]);
+      for (int i = 0; i < PER; i++) {
+        const int k = tid + i * kStreamBlock;
+        const int kk = k < nv ? k : 0;
+        c[i] = __builtin_nontemporal_load(&rcols[c0 + RunDiv(kk, inv)]);
+        v[i] = __builtin_nontemporal_load(&rvals[vb + kk]);
+      }
+    } else {
+      // two consecutive values per lane in one 16-byte load (8-byte
+      // aligned); the second may lie past the block (read, then masked)
+#pragma unroll
+      for (int i = 0; i < PER / 2; i++) {
+        const int k = 2 * (tid + i * kStreamBlock);
+        const int kk = k < nv ? k : 0;
+        const double2u vv = __builtin_nontemporal_load(reinterpret_cast<const double2u*>(rvals + vb + kk));
+        v[2 * i] = vv.x;
+        v[2 * i + 1] = vv.y;
+        c[2 * i] = __builtin_nontemporal_load(&rcols[c0 + RunDiv(kk, inv)]);
+        c[2 * i + 1] = __builtin_nontemporal_load(&rcols[c0 + RunDiv(kk + 1, inv)]);
+      }
     }
   };
   if (b0 < b1) fetch(b0);
@@ -273,10 +300,21 @@ __global__ __launch_bounds__(kStreamBlock) void CsrRunsKernel(
       double xv[PER];
 #pragma unroll
       for (int i = 0; i < PER; i++) xv[i] = x[c[i]];
+      if constexpr (VEC == 1) {
 #pragma unroll
-      for (int i = 0; i < PER; i++) {
-        const int k = tid + i * kStreamBlock;
-        prod[k] = k < nv ? v[i] * xv[i] : 0.0;
+        for (int i = 0; i < PER; i++) {
+          const int k = tid + i * kStreamBlock;
+          prod[k] = k < nv ? v[i] * xv[i] : 0.0;
+        }
+      } else {
+#pragma unroll
+        for (int i = 0; i < PER / 2; i++) {
+          const int k = 2 * (tid + i * kStreamBlock);
+          double2u pv;
+          pv.x = k < nv ? v[2 * i] * xv[2 * i] : 0.0;
+          pv.y = k + 1 < nv ? v[2 * i + 1] * xv[2 * i + 1] : 0.0;
+          *reinterpret_cast<double2u*>(prod + k) = pv;
+        }
       }
     }
     __syncthreads();
@@ -294,7 +332,17 @@ __global__ __launch_bounds__(kStreamBlock) void CsrRunsKernel(
       if (j < nr) {
         const int run = RunDiv(j, cinv), q = j - run * cR;
         const int s0 = run_ptr[cu0 + run] - cc0, s1 = run_ptr[cu0 + run + 1] - cc0;
-        for (int k = s0 + l; k < s1; k += tpr) sum += prod[k * cR + q];
+        // four independent partial sums so that the LDS reads overlap
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+        int k = s0 + l;
+        for (; k + 3 * tpr < s1; k += 4 * tpr) {
+          a0 += prod[k * cR + q];
+          a1 += prod[(k + tpr) * cR + q];
+          a2 += prod[(k + 2 * tpr) * cR + q];
+          a3 += prod[(k + 3 * tpr) * cR + q];
+        }
+        for (; k < s1; k += tpr) a0 += prod[k * cR + q];
+        sum = (a0 + a1) + (a2 + a3);
       }
       for (int off = tpr / 2; off > 0; off >>= 1) sum += __shfl_xor(sum, off, 64);
       if (j < nr && l == 0) {
@@ -400,15 +448,24 @@ CsrOp::CsrOp(int64_t h, int64_t w, const int64_t* rp, const int* c, const double
   block_nnz = nnz_env ? atoi(nnz_env) : 0;
   NGSAMD_REQUIRE(block_nnz == 0 || block_nnz == 1024 || block_nnz == 2048 || block_nnz == 4096,
                  "NGSAMD_SPMV_NNZ must be 1024, 2048 or 4096");
+  // grid of the plain applies (the variant with partial sums stays at
+  // kMaxReduceBlocks): 16384 measured best at 32^3 p=4 (runs kernel 395 us;
+  // 8192 405, 4096 418, 65536 494)
+  max_grid = 16384;
+  const char* grid_env = getenv("NGSAMD_SPMV_GRID");
+  if (grid_env) {
+    max_grid = atoi(grid_env);
+    NGSAMD_REQUIRE(max_grid >= 8 && max_grid <= 65536, "NGSAMD_SPMV_GRID must be 8..65536");
+  }
   const char* algo_env = getenv("NGSAMD_SPMV");
   const std::string algo_name = algo_env ? algo_env : "";
   NGSAMD_REQUIRE(algo_name.empty() || algo_name == "rowgroup" || algo_name == "stream" || algo_name == "runs",
                  "NGSAMD_SPMV must be rowgroup, stream or runs");
   if (algo_name.empty() || algo_name == "runs") {
-    // measured on the 32^3 p=4 matrix: 4096 values per block 412 us,
-    // 2048 423 us, 1024 429 us
+    // measured on the 32^3 p=4 matrix (us per apply, grid 16384 / 4096
+    // workgroups): 2048 values per block 395 / 418, 4096 466 / 423
     const int saved = block_nnz;
-    if (block_nnz == 0) block_nnz = 4096;
+    if (block_nnz == 0) block_nnz = 2048;
     // the runs layout pays when it saves at least a quarter of the column
     // numbers (H1 matrices of order >= 2 share patterns between the dofs of
     // an edge, a face or a cell)
@@ -421,6 +478,8 @@ CsrOp::CsrOp(int64_t h, int64_t w, const int64_t* rp, const int* c, const double
       algo = 2;
       const char* pipe_env = getenv("NGSAMD_SPMV_PIPE");
       pipe = pipe_env && atoi(pipe_env) == 1;
+      const char* rvec_env = getenv("NGSAMD_SPMV_VEC");
+      vec = (rvec_env && atoi(rvec_env) == 2) ? 2 : 1;
       nruns = (int64_t)rr.size() - 1;
       nblocks = (int)br.size() - 1;
       run_row.alloc(rr.size());
@@ -524,8 +583,10 @@ static int LaunchStream(const CsrOp& A, double s, const double* x, double* y, hi
                                                   : CsrStreamKernel<2048, 4, ACCUM, DOT>);
   // a wide grid (many more workgroups than resident slots) lets the hardware
   // balance blocks of uneven cost: measured faster than one workgroup per
-  // resident slot (412 vs 452 us for the runs kernel at 32^3 p=4)
-  int wg = A.nblocks < kMaxReduceBlocks ? A.nblocks : kMaxReduceBlocks;
+  // resident slot (412 vs 452 us for the runs kernel at 32^3 p=4); the
+  // variant with partial sums stays within the partials workspace
+  const int cap = DOT ? kMaxReduceBlocks : A.max_grid;
+  int wg = A.nblocks < cap ? A.nblocks : cap;
   if (wg > 8) wg &= ~7;
   if (wg < 1) wg = 1;
   const int per = (A.nblocks + wg - 1) / wg;
@@ -538,16 +599,22 @@ static int LaunchStream(const CsrOp& A, double s, const double* x, double* y, hi
 template <bool ACCUM, bool DOT>
 static int LaunchRuns(const CsrOp& A, double s, const double* x, double* y, hipStream_t st, double* partials,
                       const int* stop) {
-  auto kern = A.pipe ? (A.block_nnz == 1024   ? CsrRunsKernel<1024, true, ACCUM, DOT>
-                        : A.block_nnz == 4096 ? CsrRunsKernel<4096, true, ACCUM, DOT>
-                                              : CsrRunsKernel<2048, true, ACCUM, DOT>)
-                     : (A.block_nnz == 1024   ? CsrRunsKernel<1024, false, ACCUM, DOT>
-                        : A.block_nnz == 4096 ? CsrRunsKernel<4096, false, ACCUM, DOT>
-                                              : CsrRunsKernel<2048, false, ACCUM, DOT>);
+  auto pick = [&](auto vec) {
+    constexpr int V = decltype(vec)::value;
+    return A.pipe ? (A.block_nnz == 1024   ? CsrRunsKernel<1024, true, V, ACCUM, DOT>
+                     : A.block_nnz == 4096 ? CsrRunsKernel<4096, true, V, ACCUM, DOT>
+                                           : CsrRunsKernel<2048, true, V, ACCUM, DOT>)
+                  : (A.block_nnz == 1024   ? CsrRunsKernel<1024, false, V, ACCUM, DOT>
+                     : A.block_nnz == 4096 ? CsrRunsKernel<4096, false, V, ACCUM, DOT>
+                                           : CsrRunsKernel<2048, false, V, ACCUM, DOT>);
+  };
+  auto kern = A.vec == 2 ? pick(std::integral_constant<int, 2>()) : pick(std::integral_constant<int, 1>());
   // a wide grid (many more workgroups than resident slots) lets the hardware
   // balance blocks of uneven cost: measured faster than one workgroup per
-  // resident slot (412 vs 452 us for the runs kernel at 32^3 p=4)
-  int wg = A.nblocks < kMaxReduceBlocks ? A.nblocks : kMaxReduceBlocks;
+  // resident slot (412 vs 452 us for the runs kernel at 32^3 p=4); the
+  // variant with partial sums stays within the partials workspace
+  const int cap = DOT ? kMaxReduceBlocks : A.max_grid;
+  int wg = A.nblocks < cap ? A.nblocks : cap;
   if (wg > 8) wg &= ~7;
   if (wg < 1) wg = 1;
   const int per = (A.nblocks + wg - 1) / wg;

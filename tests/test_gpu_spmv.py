@@ -93,7 +93,13 @@ def test_spmv_full_size_3d(device):
     assert np.array_equal(dy.download(), 2 * y)
 
 
-ALGOS = ["runs", "stream", "rowgroup"]
+ALGOS = ["runs", "runs-vec2", "stream", "rowgroup"]
+
+
+def _select(monkeypatch, algo):
+    """NGSAMD_SPMV / NGSAMD_SPMV_VEC for a kernel variant name."""
+    monkeypatch.setenv("NGSAMD_SPMV", algo.split("-")[0])
+    monkeypatch.setenv("NGSAMD_SPMV_VEC", "2" if algo.endswith("vec2") else "1")
 
 
 def _runs_matrix(rng):
@@ -117,7 +123,7 @@ def _runs_matrix(rng):
 def test_spmv_kernels_on_fe_matrices(device, monkeypatch, algo):
     """Each SpMV kernel on 3D p=4 and 2D p=6 (face and cell runs of 10 dofs
     are split) matrices, plus the fused <x, Ax> path through CG."""
-    monkeypatch.setenv("NGSAMD_SPMV", algo)
+    _select(monkeypatch, algo)
     for dim, n, p in [(3, 4, 4), (2, 6, 6)]:
         say(f"[spmv] kernel {algo} dim={dim} n={n} p={p}")
         P = problem(dim, n, p, seed_kappa=3)
@@ -136,7 +142,7 @@ def test_spmv_kernels_on_fe_matrices(device, monkeypatch, algo):
 
 @pytest.mark.parametrize("algo", ALGOS)
 def test_spmv_kernels_on_runs_edge_cases(device, monkeypatch, algo):
-    monkeypatch.setenv("NGSAMD_SPMV", algo)
+    _select(monkeypatch, algo)
     say(f"[spmv] kernel {algo} run edge cases")
     rng = np.random.default_rng(11)
     n, w, rp, cols, vals = _runs_matrix(rng)
@@ -150,7 +156,7 @@ def test_spmv_kernels_on_runs_edge_cases(device, monkeypatch, algo):
 @pytest.mark.parametrize("algo", ALGOS)
 def test_spmv_zero_width(device, monkeypatch, algo):
     """A 3 x 0 matrix (all rows empty): y = A x = 0 with an empty x."""
-    monkeypatch.setenv("NGSAMD_SPMV", algo)
+    _select(monkeypatch, algo)
     say(f"[spmv] kernel {algo} zero width")
     A = N.CreateCSR(3, 0, np.zeros(4, dtype=np.int64), np.zeros(0, dtype=np.int32), np.zeros(0))
     dx, dy = N.DeviceVector(0, 8), dvec(np.full(3, 5.0))
