@@ -145,6 +145,9 @@ def main():
     ap.add_argument("--batch-problems", type=int, default=512)
     ap.add_argument("--profile-cg", action="store_true",
                     help="one FP64 CSR CG solve (kappa = 1, tol 1e-10) without graph capture, for a kernel trace")
+    ap.add_argument("--profile-ir", action="store_true",
+                    help="one FP32-inner refinement solve (lognormal kappa, tol 1e-10) without graph capture, "
+                         "for a kernel trace")
     ap.add_argument("--profile-mode", action="store_true",
                     help="short run for rocprofv3 counter passes: a few CSR and matrix-free applies and one "
                          "1 GiB device-to-device copy (the calibration kernel of known size); no JSON line")
@@ -182,6 +185,23 @@ def main():
         its, _ = N.CGSolve(A, C, df, du, True, 1e-10, -1.0, 100000, 0)
         N.synchronize()
         log(f"[bench] profile CG done: {its} iterations")
+        return
+
+    if args.profile_ir:
+        os.environ["NGSAMD_CG_CHUNK"] = "0"
+        kappa = np.exp(np.random.default_rng(2).standard_normal(ne))
+        vals_k = N.AssembleCSR(fes, rowptr, cols, ir, True, kappa, False, None)
+        Ak = N.CreateCSR(ndof, ndof, rowptr, cols, vals_k)
+        dd = N.DeviceVector(ndof, 8)
+        Ak.diagonal(dd)
+        Ck = N.CreateJacobi(dd.download(), fes.freedofs)
+        f = N.AssembleLinearForm(fes, ir, np.ones((ne, len(ir))))
+        df, du = N.DeviceVector(ndof, 8), N.DeviceVector(ndof, 8)
+        df.upload(f)
+        M64, M32 = build_mf(N, fes, mesh, ir, kappa, 0), build_mf(N, fes, mesh, ir, kappa, 1)
+        outer, inner, rel = N.IRCGSolve(M64, M32, Ck, df, du, 1e-10, -1.0, 60, 200000)
+        N.synchronize()
+        log(f"[bench] profile IR done: outer {outer} inner {inner} rel {rel:.1e}")
         return
 
     if args.profile_mode:

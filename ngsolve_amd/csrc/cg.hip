@@ -35,6 +35,7 @@
 // solver of iterative refinement; inner products, alpha, beta and the
 // stopping test are FP64 in both.
 #include <algorithm>
+#include <cstdlib>
 #include <cmath>
 #include <vector>
 
@@ -510,6 +511,10 @@ extern "C" int ngsamd_ir_cg_solve(ngsamd_op_t A, ngsamd_op_t inner, ngsamd_op_t 
     if (inner_tol <= 0.0)
       inner_tol = Ai->precision == NGSAMD_PREC_F32 ? 3e-4 : Ai->precision == NGSAMD_PREC_F16 ? 1e-3 : 3e-2;
     double hsc[2] = {1.0, 1.0};  // host source of the async copy below: lives until the next sync
+    // inner iterations per captured graph; NGSAMD_CG_CHUNK=0 launches every
+    // kernel directly (no graph), e.g. for a kernel trace
+    int chunk = 16;
+    if (const char* ce = getenv("NGSAMD_CG_CHUNK")) chunk = std::max(0, atoi(ce));
     while (true) {
       const double rn = n ? free_norm(r.p) : 0.0;
       relres = fn > 0 ? rn / fn : 0.0;
@@ -524,7 +529,7 @@ extern "C" int ngsamd_ir_cg_solve(ngsamd_op_t A, ngsamd_op_t inner, ngsamd_op_t 
       // the last refinement step only needs to gain the factor still missing
       double itol = inner_tol;
       if (fn > 0) itol = std::max(itol, std::min(0.5, 0.3 * tol * fn / rn));
-      CgResult ir = IterateCg<float>(Ai, inv32.p, c32.p, W, n, itol, -1.0, max_inner, 16, st);
+      CgResult ir = IterateCg<float>(Ai, inv32.p, c32.p, W, n, itol, -1.0, max_inner, chunk, st);
       inner_total += ir.iterations;
       // u += ||r|| c; r = f - A u (FP64)
       AddScaledF32Kernel<<<GridFor(n, 256, 8192), 256, 0, st>>>(u, c32.p, n, scal.p + 1);

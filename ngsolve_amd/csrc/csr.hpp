@@ -34,8 +34,6 @@ struct CsrOp : ngsamd_op_s {
   int nblocks = 0;
   int block_nnz = 2048;
   int max_grid = 16384;  // workgroups of the stream/runs kernels without partial sums
-  int vec = 1;     // values per lane load: stream kernel 1 or 4, runs kernel 1 or 2
-  bool pipe = false;  // runs kernel: next block's loads in flight during the reduction
   CsrOp(int64_t h, int64_t w, const int64_t* rp, const int* c, const double* v);
   void MultAdd(double s, const double* x, double* y, hipStream_t st) override;
   void Mult(const double* x, double* y, hipStream_t st) override;

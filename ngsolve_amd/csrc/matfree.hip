@@ -476,18 +476,37 @@ __global__ __launch_bounds__(kMfBlock) void MfRefKernel(
       T xg[KS];
 #pragma unroll
       for (int st = 0; st < KS; st++) xg[st] = gc * xk[st];
-      const T* srow = ls + cc * nd * RS;
-      // k-steps outside, dof tiles inside: consecutive MFMAs go to
-      // independent accumulators
+      if constexpr (std::is_same<T, float>::value) {
+        // FP32: S in lane order [c][m][grp][col][KSP] (rows zero-padded to
+        // the tile), so a lane's k-steps are contiguous: 16-byte LDS reads,
+        // conflict-free, no row masking
+        constexpr int KSP = (KS + 3) / 4 * 4;
+        const T* lp = ls + ((cc * MT * 4 + grp) * 16 + col) * KSP;
 #pragma unroll
-      for (int st = 0; st < KS; st++)
+        for (int q = 0; q < KSP / 4; q++) {
+          float4v a4[MT];
 #pragma unroll
-        for (int m = 0; m < MT; m++) {
-          const int row = 16 * m + col;
-          const bool inrange = row < nd;
-          const T a = inrange ? srow[(inrange ? row : 0) * RS + grp + 4 * st] : (T)0;
-          yacc[m] = Mfma(a, xg[st], yacc[m]);
+          for (int m = 0; m < MT; m++) a4[m] = *reinterpret_cast<const float4v*>(lp + m * 64 * KSP + 4 * q);
+#pragma unroll
+          for (int j = 0; j < 4; j++)
+            if (4 * q + j < KS)
+#pragma unroll
+              for (int m = 0; m < MT; m++) yacc[m] = Mfma(a4[m][j], xg[4 * q + j], yacc[m]);
         }
+      } else {
+        const T* srow = ls + cc * nd * RS;
+        // k-steps outside, dof tiles inside: consecutive MFMAs go to
+        // independent accumulators.  Rows below NDP - 3 <= nd always exist.
+#pragma unroll
+        for (int st = 0; st < KS; st++)
+#pragma unroll
+          for (int m = 0; m < MT; m++) {
+            const int row = 16 * m + col;
+            const bool inrange = (16 * m + 16 <= NDP - 3) || row < nd;
+            const T a = inrange ? srow[(inrange ? row : 0) * RS + grp + 4 * st] : (T)0;
+            yacc[m] = Mfma(a, xg[st], yacc[m]);
+          }
+      }
     }
     if (valid) {
 #pragma unroll
@@ -983,21 +1002,36 @@ MatFreeOp::MatFreeOp(int dim_, int nd_, int nq_, int64_t ndof, int nclass, const
       smat16.alloc(lay.size());
       smat16.upload(lay.data(), lay.size());
     } else {
-      const int rs = ndp + 1;
-      sstride = (int64_t)nc * nd * rs;
-      std::vector<double> lay((size_t)sstride * nclass, 0.0);
-      for (int c = 0; c < nclass; c++)
-        for (int k = 0; k < nc; k++)
-          for (int i = 0; i < nd; i++)
-            for (int j = 0; j < nd; j++)
-              lay[(size_t)c * sstride + ((size_t)k * nd + i) * rs + j] = S[(((size_t)c * nc + k) * nd + i) * nd + j];
       if (prec == NGSAMD_PREC_F64) {
+        const int rs = ndp + 1;
+        sstride = (int64_t)nc * nd * rs;
+        std::vector<double> lay((size_t)sstride * nclass, 0.0);
+        for (int c = 0; c < nclass; c++)
+          for (int k = 0; k < nc; k++)
+            for (int i = 0; i < nd; i++)
+              for (int j = 0; j < nd; j++)
+                lay[(size_t)c * sstride + ((size_t)k * nd + i) * rs + j] = S[(((size_t)c * nc + k) * nd + i) * nd + j];
         smat64.alloc(lay.size());
         smat64.upload(lay.data(), lay.size());
       } else {
-        std::vector<float> l32(lay.begin(), lay.end());
-        smat32.alloc(l32.size());
-        smat32.upload(l32.data(), l32.size());
+        // lane order [c][m][grp][col][KSP]: lane (col, grp) of dof tile m
+        // reads S[16m + col][4 st + grp] for st < KS contiguously
+        const int ks = ndp / 4, mt = (ndp + 15) / 16, ksp = (ks + 3) / 4 * 4;
+        sstride = (int64_t)nc * mt * 64 * ksp;
+        std::vector<float> lay((size_t)sstride * nclass, 0.0f);
+        for (int c = 0; c < nclass; c++)
+          for (int k = 0; k < nc; k++)
+            for (int m = 0; m < mt; m++)
+              for (int grp = 0; grp < 4; grp++)
+                for (int col = 0; col < 16; col++)
+                  for (int st = 0; st < ks; st++) {
+                    const int i = 16 * m + col, j = 4 * st + grp;
+                    if (i < nd && j < nd)
+                      lay[(size_t)c * sstride + ((((size_t)k * mt + m) * 4 + grp) * 16 + col) * ksp + st] =
+                          (float)S[(((size_t)c * nc + k) * nd + i) * nd + j];
+                  }
+        smat32.alloc(lay.size());
+        smat32.upload(lay.data(), lay.size());
         NGSAMD_HIP(hipStreamSynchronize(Stream()));
       }
     }

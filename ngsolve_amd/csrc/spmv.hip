@@ -88,15 +88,10 @@ __global__ __launch_bounds__(kSpmvBlock) void CsrSpmvKernel(
 
 
 constexpr int kStreamBlock = 256;
-typedef int int4v __attribute__((ext_vector_type(4)));
-typedef double double2v __attribute__((ext_vector_type(2)));
 
 // y[r] = (ACCUM ? y[r] : 0) + s * (A x)[r] over the row blocks
 // [blk_row[b], blk_row[b+1]) of at most NNZ nonzeros; DOT as in CsrSpmvKernel.
-// VEC = 4: each lane loads 4 consecutive column numbers (16 B) and values
-// (2 x 16 B) from a 4-aligned window (the arrays carry 4 padding entries, so
-// the window never leaves them); VEC = 1: one nonzero per load.
-template <int NNZ, int VEC, bool ACCUM, bool DOT>
+template <int NNZ, bool ACCUM, bool DOT>
 __global__ __launch_bounds__(kStreamBlock) void CsrStreamKernel(
     const int* __restrict__ rowptr, const int* __restrict__ cols, const double* __restrict__ vals,
     const double* __restrict__ x, double* __restrict__ y, const int* __restrict__ blk_row, int nblocks,
@@ -125,56 +120,25 @@ __global__ __launch_bounds__(kStreamBlock) void CsrStreamKernel(
       }
       continue;
     }
-    if constexpr (VEC == 4) {
-      // groups of 4 nonzeros starting at the aligned index kb; at most
-      // NNZ/4 + 1 groups cover [k0, k1)
-      constexpr int G = NNZ / (4 * kStreamBlock) + 1;
-      const int kb = k0 & ~3;
-      int4v c4[G];
-      double2v va[G], vb[G];
+    // every load is unconditional (lanes past the block read entry k0,
+    // which the padding keeps in bounds, and contribute 0) so all of them
+    // are in flight at once instead of one round trip per branch
+    constexpr int PER = NNZ / kStreamBlock;
+    int c[PER];
+    double v[PER], xv[PER];
 #pragma unroll
-      for (int i = 0; i < G; i++) {
-        const int base = kb + 4 * (tid + i * kStreamBlock);
-        if (base < k1) {
-          c4[i] = __builtin_nontemporal_load(reinterpret_cast<const int4v*>(cols + base));
-          va[i] = __builtin_nontemporal_load(reinterpret_cast<const double2v*>(vals + base));
-          vb[i] = __builtin_nontemporal_load(reinterpret_cast<const double2v*>(vals + base + 2));
-        }
-      }
+    for (int i = 0; i < PER; i++) {
+      const int k = tid + i * kStreamBlock;
+      const int kk = k0 + (k < nz ? k : 0);
+      c[i] = __builtin_nontemporal_load(&cols[kk]);
+      v[i] = __builtin_nontemporal_load(&vals[kk]);
+    }
 #pragma unroll
-      for (int i = 0; i < G; i++) {
-        const int base = kb + 4 * (tid + i * kStreamBlock);
-        if (base < k1) {
-          const int cc[4] = {c4[i].x, c4[i].y, c4[i].z, c4[i].w};
-          const double vv[4] = {va[i].x, va[i].y, vb[i].x, vb[i].y};
+    for (int i = 0; i < PER; i++) xv[i] = x[c[i]];
 #pragma unroll
-          for (int j = 0; j < 4; j++) {
-            const int k = base + j;
-            if (k >= k0 && k < k1) prod[k - k0] = vv[j] * x[cc[j]];
-          }
-        }
-      }
-    } else {
-      // every load is unconditional (lanes past the block read entry k0,
-      // which the padding keeps in bounds, and contribute 0) so all of them
-      // are in flight at once instead of one round trip per branch
-      constexpr int PER = NNZ / kStreamBlock;
-      int c[PER];
-      double v[PER], xv[PER];
-#pragma unroll
-      for (int i = 0; i < PER; i++) {
-        const int k = tid + i * kStreamBlock;
-        const int kk = k0 + (k < nz ? k : 0);
-        c[i] = __builtin_nontemporal_load(&cols[kk]);
-        v[i] = __builtin_nontemporal_load(&vals[kk]);
-      }
-#pragma unroll
-      for (int i = 0; i < PER; i++) xv[i] = x[c[i]];
-#pragma unroll
-      for (int i = 0; i < PER; i++) {
-        const int k = tid + i * kStreamBlock;
-        prod[k] = k < nz ? v[i] * xv[i] : 0.0;
-      }
+    for (int i = 0; i < PER; i++) {
+      const int k = tid + i * kStreamBlock;
+      prod[k] = k < nz ? v[i] * xv[i] : 0.0;
     }
     __syncthreads();
     const int nr = r1 - r0;
@@ -218,14 +182,9 @@ __device__ __forceinline__ int RunDiv(int k, int inv) { return (k * inv) >> 16; 
 // DOT as in CsrSpmvKernel.  Block b: lane k of its value range forms value k
 // times x[column of shared slot k / R] into LDS (values stream coalesced, the
 // R lanes of one slot read one column number and one x entry); row j = run R
-// + q then sums the slots of its run at stride R.  PIPE: the loads of block
-// b + 1 are issued before block b is reduced from LDS, so they are in flight
-// meanwhile (at the price of the registers holding them).  A block holding
-// more than NNZ values is a single row (R = 1) that the whole workgroup
-// strides.
-typedef double double2u __attribute__((ext_vector_type(2), aligned(8)));
-
-template <int NNZ, bool PIPE, int VEC, bool ACCUM, bool DOT>
+// + q then sums the slots of its run at stride R.  A block holding more
+// than NNZ values is a single row (R = 1) that the whole workgroup strides.
+template <int NNZ, bool ACCUM, bool DOT>
 __global__ __launch_bounds__(kStreamBlock) void CsrRunsKernel(
     const int* __restrict__ run_row, const int* __restrict__ run_ptr, const int* __restrict__ val_ptr,
     const int* __restrict__ rcols, const double* __restrict__ rvals, const double* __restrict__ x,
@@ -257,27 +216,12 @@ __global__ __launch_bounds__(kStreamBlock) void CsrRunsKernel(
     if (nv > NNZ) return;  // long row: streamed when it is reached
     // unconditional loads (lanes past the block read entry 0 of it, kept in
     // bounds by the padding) so that every load is in flight at once
-    if constexpr (VEC == 1) {
 #pragma unroll
-      for (int i = 0; i < PER; i++) {
-        const int k = tid + i * kStreamBlock;
-        const int kk = k < nv ? k : 0;
-        c[i] = __builtin_nontemporal_load(&rcols[c0 + RunDiv(kk, inv)]);
-        v[i] = __builtin_nontemporal_load(&rvals[vb + kk]);
-      }
-    } else {
-      // two consecutive values per lane in one 16-byte load (8-byte
-      // aligned); the second may lie past the block (read, then masked)
-#pragma unroll
-      for (int i = 0; i < PER / 2; i++) {
-        const int k = 2 * (tid + i * kStreamBlock);
-        const int kk = k < nv ? k : 0;
-        const double2u vv = __builtin_nontemporal_load(reinterpret_cast<const double2u*>(rvals + vb + kk));
-        v[2 * i] = vv.x;
-        v[2 * i + 1] = vv.y;
-        c[2 * i] = __builtin_nontemporal_load(&rcols[c0 + RunDiv(kk, inv)]);
-        c[2 * i + 1] = __builtin_nontemporal_load(&rcols[c0 + RunDiv(kk + 1, inv)]);
-      }
+    for (int i = 0; i < PER; i++) {
+      const int k = tid + i * kStreamBlock;
+      const int kk = k < nv ? k : 0;
+      c[i] = __builtin_nontemporal_load(&rcols[c0 + RunDiv(kk, inv)]);
+      v[i] = __builtin_nontemporal_load(&rvals[vb + kk]);
     }
   };
   if (b0 < b1) fetch(b0);
@@ -300,21 +244,10 @@ __global__ __launch_bounds__(kStreamBlock) void CsrRunsKernel(
       double xv[PER];
 #pragma unroll
       for (int i = 0; i < PER; i++) xv[i] = x[c[i]];
-      if constexpr (VEC == 1) {
 #pragma unroll
-        for (int i = 0; i < PER; i++) {
-          const int k = tid + i * kStreamBlock;
-          prod[k] = k < nv ? v[i] * xv[i] : 0.0;
-        }
-      } else {
-#pragma unroll
-        for (int i = 0; i < PER / 2; i++) {
-          const int k = 2 * (tid + i * kStreamBlock);
-          double2u pv;
-          pv.x = k < nv ? v[2 * i] * xv[2 * i] : 0.0;
-          pv.y = k + 1 < nv ? v[2 * i + 1] * xv[2 * i + 1] : 0.0;
-          *reinterpret_cast<double2u*>(prod + k) = pv;
-        }
+      for (int i = 0; i < PER; i++) {
+        const int k = tid + i * kStreamBlock;
+        prod[k] = k < nv ? v[i] * xv[i] : 0.0;
       }
     }
     __syncthreads();
@@ -322,7 +255,6 @@ __global__ __launch_bounds__(kStreamBlock) void CsrRunsKernel(
     const int cu0 = u0, cR = R, cinv = inv, cc0 = c0;
     const int row0 = run_row[cu0];
     const int nr = (u1 - u0) * R;
-    if (PIPE && b + 1 < b1) fetch(b + 1);
     int tpr = 64;
     while (tpr > 1 && tpr * nr > kStreamBlock) tpr >>= 1;
     const int l = tid & (tpr - 1);
@@ -351,7 +283,7 @@ __global__ __launch_bounds__(kStreamBlock) void CsrRunsKernel(
       }
     }
     __syncthreads();
-    if (!PIPE && b + 1 < b1) fetch(b + 1);
+    if (b + 1 < b1) fetch(b + 1);
   }
   if (DOT) StorePartial<kStreamBlock>(dotacc, partials, lds);
 }
@@ -476,10 +408,6 @@ CsrOp::CsrOp(int64_t h, int64_t w, const int64_t* rp, const int* c, const double
     nshared = BuildRuns(h, rp, c, v, block_nnz, rr, rpp, vp, rc, rv, br, bl);
     if (algo_name == "runs" || 4 * nshared <= 3 * nnz) {
       algo = 2;
-      const char* pipe_env = getenv("NGSAMD_SPMV_PIPE");
-      pipe = pipe_env && atoi(pipe_env) == 1;
-      const char* rvec_env = getenv("NGSAMD_SPMV_VEC");
-      vec = (rvec_env && atoi(rvec_env) == 2) ? 2 : 1;
       nruns = (int64_t)rr.size() - 1;
       nblocks = (int)br.size() - 1;
       run_row.alloc(rr.size());
@@ -514,8 +442,8 @@ CsrOp::CsrOp(int64_t h, int64_t w, const int64_t* rp, const int* c, const double
   // 4096 443.4 us, 1024 470.7 us)
   if (block_nnz == 0) block_nnz = 2048;
   rowptr.alloc(h + 1);
-  // 4 padding entries (column 0, value 0) behind the last nonzero for the
-  // 16-byte loads of the stream kernel
+  // 4 padding entries (column 0, value 0) behind the last nonzero: lanes
+  // past the last block read entry nnz
   cols.alloc(nnz + 4);
   vals.alloc(nnz + 4);
   rowptr.upload(rp32.data(), h + 1);
@@ -549,8 +477,6 @@ CsrOp::CsrOp(int64_t h, int64_t w, const int64_t* rp, const int* c, const double
   blk_row.upload(br.data(), br.size());
   NGSAMD_HIP(hipStreamSynchronize(Stream()));
   algo = algo_name == "rowgroup" ? 0 : 1;
-  const char* vec_env = getenv("NGSAMD_SPMV_VEC");
-  vec = (vec_env && atoi(vec_env) == 4) ? 4 : 1;
 }
 
 template <int G, bool ACCUM, bool DOT>
@@ -575,12 +501,9 @@ static int LaunchSpmvG(const CsrOp& A, double s, const double* x, double* y, hip
 template <bool ACCUM, bool DOT>
 static int LaunchStream(const CsrOp& A, double s, const double* x, double* y, hipStream_t st, double* partials,
                         const int* stop) {
-  auto kern = A.vec == 1 ? (A.block_nnz == 1024   ? CsrStreamKernel<1024, 1, ACCUM, DOT>
-                            : A.block_nnz == 4096 ? CsrStreamKernel<4096, 1, ACCUM, DOT>
-                                                  : CsrStreamKernel<2048, 1, ACCUM, DOT>)
-                         : (A.block_nnz == 1024   ? CsrStreamKernel<1024, 4, ACCUM, DOT>
-                            : A.block_nnz == 4096 ? CsrStreamKernel<4096, 4, ACCUM, DOT>
-                                                  : CsrStreamKernel<2048, 4, ACCUM, DOT>);
+  auto kern = A.block_nnz == 1024   ? CsrStreamKernel<1024, ACCUM, DOT>
+              : A.block_nnz == 4096 ? CsrStreamKernel<4096, ACCUM, DOT>
+                                    : CsrStreamKernel<2048, ACCUM, DOT>;
   // a wide grid (many more workgroups than resident slots) lets the hardware
   // balance blocks of uneven cost: measured faster than one workgroup per
   // resident slot (412 vs 452 us for the runs kernel at 32^3 p=4); the
@@ -599,16 +522,9 @@ static int LaunchStream(const CsrOp& A, double s, const double* x, double* y, hi
 template <bool ACCUM, bool DOT>
 static int LaunchRuns(const CsrOp& A, double s, const double* x, double* y, hipStream_t st, double* partials,
                       const int* stop) {
-  auto pick = [&](auto vec) {
-    constexpr int V = decltype(vec)::value;
-    return A.pipe ? (A.block_nnz == 1024   ? CsrRunsKernel<1024, true, V, ACCUM, DOT>
-                     : A.block_nnz == 4096 ? CsrRunsKernel<4096, true, V, ACCUM, DOT>
-                                           : CsrRunsKernel<2048, true, V, ACCUM, DOT>)
-                  : (A.block_nnz == 1024   ? CsrRunsKernel<1024, false, V, ACCUM, DOT>
-                     : A.block_nnz == 4096 ? CsrRunsKernel<4096, false, V, ACCUM, DOT>
-                                           : CsrRunsKernel<2048, false, V, ACCUM, DOT>);
-  };
-  auto kern = A.vec == 2 ? pick(std::integral_constant<int, 2>()) : pick(std::integral_constant<int, 1>());
+  auto kern = A.block_nnz == 1024   ? CsrRunsKernel<1024, ACCUM, DOT>
+              : A.block_nnz == 4096 ? CsrRunsKernel<4096, ACCUM, DOT>
+                                    : CsrRunsKernel<2048, ACCUM, DOT>;
   // a wide grid (many more workgroups than resident slots) lets the hardware
   // balance blocks of uneven cost: measured faster than one workgroup per
   // resident slot (412 vs 452 us for the runs kernel at 32^3 p=4); the

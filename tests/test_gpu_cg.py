@@ -39,12 +39,11 @@ def test_cg_config1(device, chunk):
     assert np.linalg.norm(r[fr]) < 1e-8 * np.linalg.norm(f[fr])
 
 
-@pytest.mark.parametrize("algo", ["runs", "runs-vec2", "stream", "rowgroup"])
+@pytest.mark.parametrize("algo", ["runs", "stream", "rowgroup"])
 def test_cg_kappa_3d(device, monkeypatch, algo):
     """Small 3D p=4 problem with lognormal kappa (the precision configuration's
     shape), with each SpMV kernel's fused <s, As> variant."""
-    monkeypatch.setenv("NGSAMD_SPMV", algo.split("-")[0])
-    monkeypatch.setenv("NGSAMD_SPMV_VEC", "2" if algo.endswith("vec2") else "1")
+    monkeypatch.setenv("NGSAMD_SPMV", algo)
     say(f"[cg] 3D p=4 lognormal kappa, SpMV kernel {algo}")
     P = problem(3, 4, 4, order=8, seed_kappa=2)
     rng = np.random.default_rng(1)

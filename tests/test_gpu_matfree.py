@@ -18,11 +18,15 @@ CASES = [(2, 6, 1, None), (2, 6, 2, None), (2, 8, 3, 6), (2, 5, 4, None), (2, 5,
 KERNELS = ["ref", "quad"]
 
 
+def _select(monkeypatch, kernel):
+    monkeypatch.setenv("NGSAMD_MF_KERNEL", kernel)
+
+
 @pytest.mark.parametrize("dim,n,p,order", CASES)
 @pytest.mark.parametrize("scatter", [0, 1])
 @pytest.mark.parametrize("kernel", KERNELS)
 def test_mf_f64_matches_oracle(device, monkeypatch, dim, n, p, order, scatter, kernel):
-    monkeypatch.setenv("NGSAMD_MF_KERNEL", kernel)
+    _select(monkeypatch, kernel)
     say(f"[mf f64 {kernel}] dim={dim} n={n} p={p} order={order} scatter={scatter}")
     P = problem(dim, n, p, order=order, seed_kappa=3)
     A = mf_operator(P, precision=0, scatter=scatter)
@@ -63,7 +67,7 @@ def test_mf_oracle_btdb_apply(device):
 @pytest.mark.parametrize("scatter", [0, 1])
 @pytest.mark.parametrize("kernel", KERNELS)
 def test_mf_low_precision(device, monkeypatch, precision, tol, dim, n, p, order, scatter, kernel):
-    monkeypatch.setenv("NGSAMD_MF_KERNEL", kernel)
+    _select(monkeypatch, kernel)
     say(f"[mf prec={precision} {kernel}] dim={dim} n={n} p={p} scatter={scatter}")
     P = problem(dim, n, p, order=order, seed_kappa=3)
     A = mf_operator(P, precision=precision, scatter=scatter)

@@ -93,13 +93,11 @@ def test_spmv_full_size_3d(device):
     assert np.array_equal(dy.download(), 2 * y)
 
 
-ALGOS = ["runs", "runs-vec2", "stream", "rowgroup"]
+ALGOS = ["runs", "stream", "rowgroup"]
 
 
 def _select(monkeypatch, algo):
-    """NGSAMD_SPMV / NGSAMD_SPMV_VEC for a kernel variant name."""
-    monkeypatch.setenv("NGSAMD_SPMV", algo.split("-")[0])
-    monkeypatch.setenv("NGSAMD_SPMV_VEC", "2" if algo.endswith("vec2") else "1")
+    monkeypatch.setenv("NGSAMD_SPMV", algo)
 
 
 def _runs_matrix(rng):
