@@ -28,6 +28,7 @@
 #include "common.hpp"
 #include "csr.hpp"
 #include "reduce.hpp"
+#include "runs.hpp"
 #include "runtime_state.hpp"
 
 namespace ngsamd {
@@ -71,6 +72,13 @@ struct BatchOp : ngsamd_op_s {
   DevBuf<int> rowptr, cols;
   DevBuf<double> vals;
   DevBuf<float> vals32;
+  // pattern-runs layout (see runs.hpp), used when it saves >= 25% of the
+  // column numbers; blocks never straddle two problems
+  bool use_runs = false;
+  bool has_f32 = false;
+  DevBuf<int> run_row, run_ptr, val_ptr, rcols, blk_run, blk_len;
+  DevBuf<double> rvals;
+  DevBuf<float> rvals32;
   int nseg = 0;
   std::vector<int64_t> seg_rows;
   int block_nnz = 2048;
@@ -173,9 +181,120 @@ __global__ __launch_bounds__(kBatchBlock) void BatchSpmvKernel(
   }
 }
 
+// Runs-layout form of BatchSpmvKernel (the runs kernel of spmv.hip with the
+// frozen-problem skip and one partial per block).
+template <int NNZ, class V, class M, bool ACCUM, bool DOT>
+__global__ __launch_bounds__(kBatchBlock) void BatchRunsKernel(
+    const int* __restrict__ run_row, const int* __restrict__ run_ptr, const int* __restrict__ val_ptr,
+    const int* __restrict__ rcols, const M* __restrict__ rvals, const V* __restrict__ x, V* __restrict__ y, double s,
+    const int* __restrict__ blk_run, const int* __restrict__ blk_len, const int* __restrict__ blk_seg, int nblocks,
+    int blocks_per_wg, const int* __restrict__ done, double* __restrict__ partials) {
+  static_assert(NNZ <= 4096, "RunDiv is exact below 4096");
+  constexpr int PER = NNZ / kBatchBlock;
+  __shared__ double prod[NNZ];
+  __shared__ double lds[kBatchBlock / 64];
+  const int tid = threadIdx.x;
+  const int lb = XcdBlock(blockIdx.x, gridDim.x);
+  const int b0 = lb * blocks_per_wg;
+  const int b1 = min(b0 + blocks_per_wg, nblocks);
+  for (int b = b0; b < b1; b++) {
+    if (done && done[blk_seg[b]]) {
+      if (DOT && tid == 0) partials[b] = 0.0;
+      continue;
+    }
+    const int u0 = blk_run[b], u1 = blk_run[b + 1];
+    const int R = blk_len[b];
+    const int inv = (65536 + R - 1) / R;
+    const int c0 = run_ptr[u0];
+    const int vb = val_ptr[u0];
+    const int nv = val_ptr[u1] - vb;
+    double dotacc = 0.0;
+    if (nv > NNZ) {
+      // one long row (R = 1): the whole workgroup strides over it
+      double a = 0.0;
+      for (int k = tid; k < nv; k += kBatchBlock)
+        a += (double)__builtin_nontemporal_load(&rvals[vb + k]) * (double)x[__builtin_nontemporal_load(&rcols[c0 + k])];
+      a = BlockSum<kBatchBlock>(a, lds);
+      if (tid == 0) {
+        const int r = run_row[u0];
+        if (ACCUM) y[r] = (V)((double)y[r] + s * a); else y[r] = (V)(s * a);
+        if (DOT) partials[b] = (double)x[r] * a;
+      }
+      continue;
+    }
+    int c[PER];
+    double v[PER], xv[PER];
+#pragma unroll
+    for (int i = 0; i < PER; i++) {
+      const int k = tid + i * kBatchBlock;
+      const int kk = k < nv ? k : 0;
+      c[i] = __builtin_nontemporal_load(&rcols[c0 + RunDiv(kk, inv)]);
+      v[i] = (double)__builtin_nontemporal_load(&rvals[vb + kk]);
+    }
+#pragma unroll
+    for (int i = 0; i < PER; i++) xv[i] = (double)x[c[i]];
+#pragma unroll
+    for (int i = 0; i < PER; i++) {
+      const int k = tid + i * kBatchBlock;
+      prod[k] = k < nv ? v[i] * xv[i] : 0.0;
+    }
+    __syncthreads();
+    const int row0 = run_row[u0];
+    const int nr = (u1 - u0) * R;
+    int tpr = 64;
+    while (tpr > 1 && tpr * nr > kBatchBlock) tpr >>= 1;
+    const int l = tid & (tpr - 1);
+    for (int j0 = 0; j0 < nr; j0 += kBatchBlock / tpr) {
+      const int j = j0 + tid / tpr;
+      double sum = 0.0;
+      if (j < nr) {
+        const int run = RunDiv(j, inv), q = j - run * R;
+        const int s0 = run_ptr[u0 + run] - c0, s1 = run_ptr[u0 + run + 1] - c0;
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+        int k = s0 + l;
+        for (; k + 3 * tpr < s1; k += 4 * tpr) {
+          a0 += prod[k * R + q];
+          a1 += prod[(k + tpr) * R + q];
+          a2 += prod[(k + 2 * tpr) * R + q];
+          a3 += prod[(k + 3 * tpr) * R + q];
+        }
+        for (; k < s1; k += tpr) a0 += prod[k * R + q];
+        sum = (a0 + a1) + (a2 + a3);
+      }
+      for (int off = tpr / 2; off > 0; off >>= 1) sum += __shfl_xor(sum, off, 64);
+      if (j < nr && l == 0) {
+        if (ACCUM) y[row0 + j] = (V)((double)y[row0 + j] + s * sum); else y[row0 + j] = (V)(s * sum);
+        if (DOT) dotacc += (double)x[row0 + j] * sum;
+      }
+    }
+    if (DOT) {
+      dotacc = BlockSum<kBatchBlock>(dotacc, lds);
+      if (tid == 0) partials[b] = dotacc;
+    }
+    __syncthreads();
+  }
+}
+
 template <class V, class M, bool ACCUM, bool DOT>
 static void LaunchBatchSpmv(const BatchOp& A, const M* vals, const V* x, V* y, double s, const int* done,
                             hipStream_t st, int block_nnz) {
+  if (A.use_runs) {
+    // vals selects the value copy: the FP64 CSR values stand for rvals and
+    // the FP32 copy for rvals32
+    const M* rv;
+    if constexpr (sizeof(M) == 8) rv = A.rvals.p;
+    else rv = A.rvals32.p;
+    (void)vals;
+    int64_t wg = A.nblocks < 16384 ? A.nblocks : 16384;
+    if (wg > 8) wg &= ~int64_t(7);
+    if (wg < 1) wg = 1;
+    const int per = (int)((A.nblocks + wg - 1) / wg);
+    BatchRunsKernel<2048, V, M, ACCUM, DOT><<<(int)wg, kBatchBlock, 0, st>>>(
+        A.run_row.p, A.run_ptr.p, A.val_ptr.p, A.rcols.p, rv, x, y, s, A.blk_run.p, A.blk_len.p, A.blk_seg.p,
+        A.nblocks, per, done, A.partials.p);
+    NGSAMD_CHECK_LAUNCH();
+    return;
+  }
   int64_t wg = A.nblocks < 8192 ? A.nblocks : 8192;
   if (wg > 8) wg &= ~int64_t(7);
   if (wg < 1) wg = 1;
@@ -215,27 +334,80 @@ BatchOp::BatchOp(int64_t n, const int64_t* rp, const int* c, const double* v, in
       for (int64_t k = rp[r]; k < rp[r + 1]; k++)
         NGSAMD_REQUIRE(c[k] >= seg[p] && c[k] < seg[p + 1], "ngsamd_batch_csr_create: matrix is not block-diagonal");
   }
-  std::vector<int> rp32(n + 1);
-  for (int64_t i = 0; i <= n; i++) rp32[i] = (int)rp[i];
-  rowptr.alloc(n + 1);
-  rowptr.upload(rp32.data(), n + 1);
+  has_f32 = with_f32;
   // 4 padding entries (column 0, value 0): lanes past the end of the last
   // row block read entry nnz
   static const int zc[4] = {0, 0, 0, 0};
   static const double zv[4] = {0, 0, 0, 0};
   static const float zf[4] = {0, 0, 0, 0};
-  cols.alloc(nnz + 4);
-  cols.upload(c, nnz);
-  cols.upload_at(zc, nnz, 4);
-  vals.alloc(nnz + 4);
-  vals.upload(v, nnz);
-  vals.upload_at(zv, nnz, 4);
-  if (with_f32) {
-    std::vector<float> v32(v, v + nnz);
-    vals32.alloc(nnz + 4);
-    vals32.upload(v32.data(), nnz);
-    vals32.upload_at(zf, nnz, 4);
-    NGSAMD_HIP(hipStreamSynchronize(Stream()));
+  // pattern-runs layout when it saves >= 25% of the column numbers
+  // (NGSAMD_BATCH_RUNS=0 keeps plain CSR)
+  const char* renv = getenv("NGSAMD_BATCH_RUNS");
+  if (!(renv && std::string(renv) == "0") && n > 0) {
+    std::vector<int> rr, rpp, vp, rc, br, bl, bs;
+    std::vector<double> rv;
+    rc.reserve(nnz);
+    rv.reserve(nnz);
+    const int64_t nsh = BuildRuns(n, rp, c, v, 2048, rr, rpp, vp, rc, rv, br, bl, seg, ns, &bs);
+    if (4 * nsh <= 3 * nnz) {
+      use_runs = true;
+      block_nnz = 2048;
+      nblocks = (int)br.size() - 1;
+      run_row.alloc(rr.size());
+      run_row.upload(rr.data(), rr.size());
+      run_ptr.alloc(rpp.size());
+      run_ptr.upload(rpp.data(), rpp.size());
+      val_ptr.alloc(vp.size());
+      val_ptr.upload(vp.data(), vp.size());
+      rcols.alloc(nsh + 4);
+      rcols.upload(rc.data(), nsh);
+      rcols.upload_at(zc, nsh, 4);
+      rvals.alloc(nnz + 4);
+      rvals.upload(rv.data(), nnz);
+      rvals.upload_at(zv, nnz, 4);
+      if (with_f32) {
+        std::vector<float> v32(rv.begin(), rv.end());
+        rvals32.alloc(nnz + 4);
+        rvals32.upload(v32.data(), nnz);
+        rvals32.upload_at(zf, nnz, 4);
+        NGSAMD_HIP(hipStreamSynchronize(Stream()));
+      }
+      blk_run.alloc(br.size());
+      blk_run.upload(br.data(), br.size());
+      blk_len.alloc(std::max<size_t>(bl.size(), 1));
+      blk_len.upload(bl.data(), bl.size());
+      blk_seg.alloc(std::max<size_t>(bs.size(), 1));
+      blk_seg.upload(bs.data(), bs.size());
+      // first block of each problem (blocks are in problem order)
+      std::vector<int> sb(ns + 1);
+      int b = 0;
+      for (int q = 0; q <= ns; q++) {
+        while (b < nblocks && bs[b] < q) b++;
+        sb[q] = b;
+      }
+      seg_blk.alloc(sb.size());
+      seg_blk.upload(sb.data(), sb.size());
+      NGSAMD_HIP(hipStreamSynchronize(Stream()));
+    }
+  }
+  if (!use_runs) {
+    std::vector<int> rp32(n + 1);
+    for (int64_t i = 0; i <= n; i++) rp32[i] = (int)rp[i];
+    rowptr.alloc(n + 1);
+    rowptr.upload(rp32.data(), n + 1);
+    cols.alloc(nnz + 4);
+    cols.upload(c, nnz);
+    cols.upload_at(zc, nnz, 4);
+    vals.alloc(nnz + 4);
+    vals.upload(v, nnz);
+    vals.upload_at(zv, nnz, 4);
+    if (with_f32) {
+      std::vector<float> v32(v, v + nnz);
+      vals32.alloc(nnz + 4);
+      vals32.upload(v32.data(), nnz);
+      vals32.upload_at(zf, nnz, 4);
+      NGSAMD_HIP(hipStreamSynchronize(Stream()));
+    }
   }
   // vector chunks
   std::vector<int64_t> cs;
@@ -257,6 +429,11 @@ BatchOp::BatchOp(int64_t n, const int64_t* rp, const int* c, const double* v, in
   seg_chunk.alloc(sgc.size());
   seg_chunk.upload(sgc.data(), sgc.size());
 
+  if (use_runs) {
+    partials.alloc(std::max<size_t>({(size_t)nblocks, (size_t)nchunks, (size_t)1}));
+    NGSAMD_HIP(hipStreamSynchronize(Stream()));
+    return;
+  }
   // autotune the row-block size: time each candidate on the device
   hipStream_t st = Stream();
   const char* env = getenv("NGSAMD_SPMV_NNZ");
@@ -756,7 +933,7 @@ extern "C" int ngsamd_batch_ir_cg_solve(ngsamd_op_t op, ngsamd_op_t pre, const d
     RequireInit();
     NGSAMD_REQUIRE(op && op->kind == OpKind::Batch, "ngsamd_batch_ir_cg_solve: not a batched operator");
     auto& A = *static_cast<BatchOp*>(op);
-    NGSAMD_REQUIRE(A.vals32.p != nullptr, "ngsamd_batch_ir_cg_solve: operator created without FP32 values");
+    NGSAMD_REQUIRE(A.has_f32, "ngsamd_batch_ir_cg_solve: operator created without FP32 values");
     NGSAMD_REQUIRE(pre && pre->kind == OpKind::Jacobi && pre->height == op->height,
                    "ngsamd_batch_ir_cg_solve: needs a Jacobi preconditioner of the same size");
     hipStream_t st = Stream();

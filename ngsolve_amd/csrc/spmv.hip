@@ -36,6 +36,7 @@
 
 #include "common.hpp"
 #include "csr.hpp"
+#include "runs.hpp"
 #include "reduce.hpp"
 #include "runtime_state.hpp"
 
@@ -172,10 +173,6 @@ __global__ __launch_bounds__(kStreamBlock) void CsrStreamKernel(
   if (DOT) StorePartial<kStreamBlock>(dotacc, partials, lds);
 }
 
-// Exact k / R for 0 <= k <= 4096 and 1 <= R <= 8 as (k * ceil(2^16 / R)) >> 16:
-// the rounding error k (ceil(2^16/R) - 2^16/R) / 2^16 stays below 1/16 while
-// the fractional part of k / R is at most 7/8.
-__device__ __forceinline__ int RunDiv(int k, int inv) { return (k * inv) >> 16; }
 
 // y = (ACCUM ? y : 0) + s * A x over the run blocks [blk_run[b], blk_run[b+1])
 // with blk_len[b] rows per run (at most NNZ values and NNZ rows per block);
@@ -286,58 +283,6 @@ __global__ __launch_bounds__(kStreamBlock) void CsrRunsKernel(
     if (b + 1 < b1) fetch(b + 1);
   }
   if (DOT) StorePartial<kStreamBlock>(dotacc, partials, lds);
-}
-
-// Splits the rows into pattern runs (at most kMaxRun rows, at most max_vals
-// values per run unless the run is one row) and the runs into blocks of equal
-// run length with at most max_vals values and max_vals rows (or one longer
-// row).  Fills the
-// host arrays of the runs layout; returns the number of shared columns.
-static int64_t BuildRuns(int64_t h, const int64_t* rp, const int* c, const double* v, int max_vals,
-                         std::vector<int>& run_row, std::vector<int>& run_ptr, std::vector<int>& val_ptr,
-                         std::vector<int>& rcols, std::vector<double>& rvals, std::vector<int>& blk_run,
-                         std::vector<int>& blk_len) {
-  run_row.assign(1, 0);
-  run_ptr.assign(1, 0);
-  val_ptr.assign(1, 0);
-  rcols.clear();
-  rvals.clear();
-  blk_run.assign(1, 0);
-  blk_len.clear();
-  int64_t acc = 0, rows = 0;
-  int cur_len = 0;
-  for (int64_t r = 0; r < h;) {
-    const int64_t L = rp[r + 1] - rp[r];
-    int R = 1;
-    const int rmax = L > 0 ? (int)std::max<int64_t>(1, std::min<int64_t>(kMaxRun, max_vals / L)) : kMaxRun;
-    while (R < rmax && r + R < h && rp[r + R + 1] - rp[r + R] == L &&
-           std::equal(c + rp[r], c + rp[r] + L, c + rp[r + R]))
-      R++;
-    const int64_t nv = int64_t(R) * L;
-    const int u = (int)run_row.size() - 1;
-    if (u > blk_run.back() && (R != cur_len || acc + nv > max_vals || rows + R > max_vals)) {
-      blk_len.push_back(cur_len);
-      blk_run.push_back(u);
-      acc = 0;
-      rows = 0;
-    }
-    cur_len = R;
-    acc += nv;
-    rows += R;
-    rcols.insert(rcols.end(), c + rp[r], c + rp[r] + L);
-    for (int64_t j = 0; j < L; j++)
-      for (int q = 0; q < R; q++) rvals.push_back(v[rp[r + q] + j]);
-    r += R;
-    run_row.push_back((int)r);
-    run_ptr.push_back((int)rcols.size());
-    val_ptr.push_back((int)rvals.size());
-  }
-  const int nruns = (int)run_row.size() - 1;
-  if (nruns > blk_run.back()) {
-    blk_len.push_back(cur_len);
-    blk_run.push_back(nruns);
-  }
-  return (int64_t)rcols.size();
 }
 
 __global__ void RunsDiagKernel(const int* __restrict__ run_row, const int* __restrict__ run_ptr,

@@ -16,9 +16,11 @@ def _problems():
     return make_problems(24, seed=5, nx_range=(2, 20), p_range=(1, 5))
 
 
-def test_batch_fp64_matches_per_problem_cg(device):
+@pytest.mark.parametrize("runs", ["1", "0"])
+def test_batch_fp64_matches_per_problem_cg(device, monkeypatch, runs):
     from ngsolve_amd.batch import BatchSolver
-    say("[batch] fp64 24 problems")
+    monkeypatch.setenv("NGSAMD_BATCH_RUNS", runs)
+    say(f"[batch] fp64 24 problems, runs layout {runs}")
     probs = _problems()
     bs = BatchSolver(probs)
     r = bs.solve_fp64(tol=1e-10)
@@ -32,9 +34,11 @@ def test_batch_fp64_matches_per_problem_cg(device):
         assert r["finish_ms"][k] >= 0
 
 
-def test_batch_mixed_reaches_1e10(device):
+@pytest.mark.parametrize("runs", ["1", "0"])
+def test_batch_mixed_reaches_1e10(device, monkeypatch, runs):
     from ngsolve_amd.batch import BatchSolver
-    say("[batch] mixed 24 problems")
+    monkeypatch.setenv("NGSAMD_BATCH_RUNS", runs)
+    say(f"[batch] mixed 24 problems, runs layout {runs}")
     probs = _problems()
     bs = BatchSolver(probs)
     r = bs.solve_mixed(tol=1e-10)
