@@ -114,6 +114,9 @@ namespace ngsamd {
 void LaunchFill(double* x, int64_t n, double a, hipStream_t st);
 void LaunchFillF32(float* x, int64_t n, float a, hipStream_t st);
 int GridFor(int64_t n, int block, int max_blocks = 1 << 20);
+// Sums consecutive groups of the n partial sums in[] into the 64 partials
+// out[] (fixed order, deterministic); skipped when *stop != 0.
+void LaunchFoldPartials(const double* in, int n, double* out, const int* stop, hipStream_t st);
 // device properties of the current device (queried once)
 int DeviceCUs();
 int DeviceMaxLds();

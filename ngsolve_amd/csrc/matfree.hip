@@ -1144,10 +1144,7 @@ MatFreeOp::MatFreeOp(int dim_, int nd_, int nq_, int64_t ndof, int nclass, const
     const char* genv = getenv("NGSAMD_MF_GATHER");
     NGSAMD_REQUIRE(!genv || std::string(genv) == "dof" || std::string(genv) == "block",
                    "NGSAMD_MF_GATHER must be dof or block");
-    // default (-1): one thread per dof for plain applies (a wide grid hides
-    // the latency of each dof's load chain), slot blocks when CG's partial
-    // sums are fused in (that grid is capped at kMaxDotBlocks)
-    gather_kind = !genv ? -1 : std::string(genv) == "dof" ? 0 : 1;
+    gather_kind = (genv && std::string(genv) == "block") ? 1 : 0;
     g_blk.alloc(gb.size());
     g_blk.upload(gb.data(), gb.size());
     NGSAMD_HIP(hipStreamSynchronize(Stream()));
@@ -1372,37 +1369,39 @@ int MatFreeOp::Apply(double s, const V* x, V* y, bool accumulate, const int* sto
   if constexpr (sizeof(V) == 8) contrib = reinterpret_cast<V*>(contrib64.p);
   else contrib = reinterpret_cast<V*>(contrib32.p);
   LaunchElements<0, V>(*this, x, contrib, 1.0, stop, st, false);
-  if (gather_kind == 0 || (gather_kind < 0 && !partials)) {
-    // one thread per dof
-    if (partials) {
-      const int gd = GridFor(height, kGatherBlock, kMaxDotBlocks);
-      MfGatherKernel<V, V, true><<<gd, kGatherBlock, 0, st>>>(g_rowptr.p, g_slots.p, g_order.p, contrib, y, height,
-                                                              s, accumulate ? 1 : 0, stop, x, partials);
-      NGSAMD_CHECK_LAUNCH();
-      return gd;
-    }
-    MfGatherKernel<V, V, false><<<GridFor(height, kGatherBlock, 16384), kGatherBlock, 0, st>>>(
-        g_rowptr.p, g_slots.p, g_order.p, contrib, y, height, s, accumulate ? 1 : 0, stop, nullptr, nullptr);
+  // with partials (CG): the gather on its wide grid writes one partial per
+  // workgroup, folded to kFoldedPartials for CG's kernels
+  double* dotp = partials ? State().wide_partials : nullptr;
+  int np = 0;
+  if (gather_kind != 1) {
+    // one thread per dof (default): a wide grid hides each dof's load chain
+    np = GridFor(height, kGatherBlock, kMaxWideBlocks);
+    if (partials)
+      MfGatherKernel<V, V, true><<<np, kGatherBlock, 0, st>>>(g_rowptr.p, g_slots.p, g_order.p, contrib, y, height, s,
+                                                              accumulate ? 1 : 0, stop, x, dotp);
+    else
+      MfGatherKernel<V, V, false><<<np, kGatherBlock, 0, st>>>(g_rowptr.p, g_slots.p, g_order.p, contrib, y, height,
+                                                               s, accumulate ? 1 : 0, stop, nullptr, nullptr);
     NGSAMD_CHECK_LAUNCH();
-    return 0;
-  }
-  int g = g_nblocks < (partials ? kMaxDotBlocks : kMaxReduceBlocks) ? g_nblocks
-                                                                      : (partials ? kMaxDotBlocks : kMaxReduceBlocks);
-  if (g > 8) g &= ~7;
-  if (g < 1) g = 1;
-  const int per = (g_nblocks + g - 1) / g;
-  if (partials) {
-    MfGatherBlockKernel<V, V, true><<<g, kGatherBlock, 0, st>>>(g_rowptr.p, g_slots.p, g_order.p, g_blk.p, g_nblocks,
-                                                                per, contrib, y, s, accumulate ? 1 : 0, stop, x,
-                                                                partials);
+  } else {
+    // slot blocks
+    np = g_nblocks < kMaxWideBlocks ? g_nblocks : kMaxWideBlocks;
+    if (np > 8) np &= ~7;
+    if (np < 1) np = 1;
+    const int per = (g_nblocks + np - 1) / np;
+    if (partials)
+      MfGatherBlockKernel<V, V, true><<<np, kGatherBlock, 0, st>>>(g_rowptr.p, g_slots.p, g_order.p, g_blk.p,
+                                                                   g_nblocks, per, contrib, y, s, accumulate ? 1 : 0,
+                                                                   stop, x, dotp);
+    else
+      MfGatherBlockKernel<V, V, false><<<np, kGatherBlock, 0, st>>>(g_rowptr.p, g_slots.p, g_order.p, g_blk.p,
+                                                                    g_nblocks, per, contrib, y, s,
+                                                                    accumulate ? 1 : 0, stop, nullptr, nullptr);
     NGSAMD_CHECK_LAUNCH();
-    return g;
   }
-  MfGatherBlockKernel<V, V, false><<<g, kGatherBlock, 0, st>>>(g_rowptr.p, g_slots.p, g_order.p, g_blk.p, g_nblocks,
-                                                               per, contrib, y, s, accumulate ? 1 : 0, stop, nullptr,
-                                                               nullptr);
-  NGSAMD_CHECK_LAUNCH();
-  return 0;
+  if (!partials) return 0;
+  LaunchFoldPartials(dotp, np, partials, stop, st);
+  return kFoldedPartials;
 }
 
 void MatFreeOp::MultAdd(double s, const double* x, double* y, hipStream_t st) { Apply<double>(s, x, y, true, nullptr, st); }

@@ -51,7 +51,7 @@ struct MatFreeOp : ngsamd_op_s {
   DevBuf<int> g_rowptr, g_slots, g_order;
   DevBuf<int> g_blk;          // gather blocks: first row of each, g_nblocks + 1
   int g_nblocks = 0;
-  int gather_kind = -1;       // 0: one thread per dof, 1: slot blocks, -1: by variant
+  int gather_kind = 0;        // 0: one thread per dof, 1: slot blocks
   DevBuf<double> contrib64;
   DevBuf<float> contrib32;
 

@@ -28,6 +28,22 @@ int GridFor(int64_t n, int block, int max_blocks) {
   return (int)g;
 }
 
+__global__ __launch_bounds__(256) void FoldPartialsKernel(const double* __restrict__ in, int n,
+                                                          double* __restrict__ out, const int* __restrict__ stop) {
+  if (stop && *stop) return;
+  __shared__ double lds[256 / 64];
+  const int per = (n + kFoldedPartials - 1) / kFoldedPartials;
+  const int b = blockIdx.x * per;
+  const int cnt = max(0, min(per, n - b));
+  const double v = SumPartials<256>(in + b, cnt, lds);
+  if (threadIdx.x == 0) out[blockIdx.x] = v;
+}
+
+void LaunchFoldPartials(const double* in, int n, double* out, const int* stop, hipStream_t st) {
+  FoldPartialsKernel<<<kFoldedPartials, 256, 0, st>>>(in, n, out, stop);
+  NGSAMD_CHECK_LAUNCH();
+}
+
 int DeviceCUs() {
   static int v = -1;
   if (v < 0) {
@@ -149,6 +165,7 @@ int ngsamd_init(int device) {
     S.device = device;
     NGSAMD_HIP(hipStreamCreateWithFlags(&S.stream, hipStreamNonBlocking));
     NGSAMD_HIP(hipMalloc((void**)&S.partials, sizeof(double) * 2 * kMaxReduceBlocks));
+    NGSAMD_HIP(hipMalloc((void**)&S.wide_partials, sizeof(double) * kMaxWideBlocks));
     NGSAMD_HIP(hipMalloc((void**)&S.dev_scalars, sizeof(double) * kNumScalars));
     NGSAMD_HIP(hipHostMalloc((void**)&S.host_scalars, sizeof(double) * kNumScalars, hipHostMallocDefault));
     NGSAMD_HIP(hipDeviceSynchronize());

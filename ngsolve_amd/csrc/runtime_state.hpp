@@ -8,9 +8,10 @@
 namespace ngsamd {
 
 constexpr int kMaxReduceBlocks = 4096;  // upper bound on workgroups of a reducing kernel
-// grid bound of the matrix-free gather kernel when it also writes partials of
-// <x, A x> for CG (CG's update kernel reads all partials in every workgroup)
-constexpr int kMaxDotBlocks = 1024;
+// wide-grid kernels (the CSR operator's CG variant) write up to this many
+// partial sums, folded to kFoldedPartials before CG's kernels read them
+constexpr int kMaxWideBlocks = 16384;
+constexpr int kFoldedPartials = 64;
 constexpr int kNumScalars = 64;
 
 struct RuntimeState {
@@ -20,6 +21,7 @@ struct RuntimeState {
   hipStream_t stream = nullptr;       // library stream
   hipStream_t user_stream = nullptr;  // stream set by ngsamd_set_stream
   double* partials = nullptr;         // 2 x kMaxReduceBlocks partial sums
+  double* wide_partials = nullptr;    // kMaxWideBlocks partial sums of wide-grid kernels
   double* dev_scalars = nullptr;      // kNumScalars
   double* host_scalars = nullptr;     // pinned mirror
 };

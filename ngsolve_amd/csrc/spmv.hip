@@ -453,7 +453,7 @@ static int LaunchStream(const CsrOp& A, double s, const double* x, double* y, hi
   // balance blocks of uneven cost: measured faster than one workgroup per
   // resident slot (412 vs 452 us for the runs kernel at 32^3 p=4); the
   // variant with partial sums stays within the partials workspace
-  const int cap = DOT ? kMaxReduceBlocks : A.max_grid;
+  const int cap = DOT ? std::min(A.max_grid, kMaxWideBlocks) : A.max_grid;
   int wg = A.nblocks < cap ? A.nblocks : cap;
   if (wg > 8) wg &= ~7;
   if (wg < 1) wg = 1;
@@ -474,7 +474,7 @@ static int LaunchRuns(const CsrOp& A, double s, const double* x, double* y, hipS
   // balance blocks of uneven cost: measured faster than one workgroup per
   // resident slot (412 vs 452 us for the runs kernel at 32^3 p=4); the
   // variant with partial sums stays within the partials workspace
-  const int cap = DOT ? kMaxReduceBlocks : A.max_grid;
+  const int cap = DOT ? std::min(A.max_grid, kMaxWideBlocks) : A.max_grid;
   int wg = A.nblocks < cap ? A.nblocks : cap;
   if (wg > 8) wg &= ~7;
   if (wg < 1) wg = 1;
@@ -510,7 +510,12 @@ void CsrOp::Mult(const double* x, double* y, hipStream_t st) {
 }
 
 int CsrOp::MultDot(const double* x, double* y, double* partials, const int* d_stop, hipStream_t st) {
-  return LaunchSpmv<false, true>(*this, 1.0, x, y, st, partials, d_stop);
+  // the operator on its wide grid writes one partial per workgroup; a
+  // 64-workgroup kernel folds them so that CG's kernels read 64 values
+  double* wide = State().wide_partials;
+  const int np = LaunchSpmv<false, true>(*this, 1.0, x, y, st, wide, d_stop);
+  LaunchFoldPartials(wide, np, partials, d_stop, st);
+  return kFoldedPartials;
 }
 
 __global__ void CsrDiagKernel(const int* __restrict__ rowptr, const int* __restrict__ cols,
