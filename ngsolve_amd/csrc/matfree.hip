@@ -96,7 +96,31 @@ struct MfRounds {
   const int* round0;     // first round of each class, nclass + 1 entries
   const int64_t* range;  // element range [begin, end) of each class
   int nrounds;
+  int nclass;
 };
+
+// classes whose schedule the reference-matrix kernels keep in LDS
+constexpr int kMaxLdsClasses = 64;
+
+// Copy n 16-byte words of a class's matrices from global memory into LDS.
+// Each thread issues 4 loads before its first LDS write (from clamped
+// indices, so no load sits behind a branch): a plain load-then-store loop
+// waits for every load in turn.
+__device__ __forceinline__ void CopyToLds(void* dst, const void* src, int n) {
+  constexpr int U = 4;
+  const uint4* s = reinterpret_cast<const uint4*>(src);
+  uint4* d = reinterpret_cast<uint4*>(dst);
+  for (int base = 0; base < n; base += kMfBlock * U) {
+    uint4 v[U];
+#pragma unroll
+    for (int j = 0; j < U; j++) v[j] = s[min(base + j * kMfBlock + (int)threadIdx.x, n - 1)];
+#pragma unroll
+    for (int j = 0; j < U; j++) {
+      const int i = base + j * kMfBlock + (int)threadIdx.x;
+      if (i < n) d[i] = v[j];
+    }
+  }
+}
 
 // Class of round r, scanning on from the class of the previous round.
 __device__ __forceinline__ int RoundClass(const MfRounds& R, int r, int from) {
@@ -132,8 +156,8 @@ __global__ __launch_bounds__(kMfBlock) void MfMfmaKernel(
     const int c = RoundClass(R, rd, cls);
     if (c != cls) {
       if (cls >= 0) __syncthreads();
-      const T* src = bmat + (int64_t)c * bstride;
-      for (int i = threadIdx.x; i < nlds; i += kMfBlock) lb[i] = src[i];
+      // nlds * sizeof(T) and the class offset are multiples of 64 bytes
+      CopyToLds(lb, bmat + (int64_t)c * bstride, (int)(nlds * sizeof(T) / 16));
       __syncthreads();
       cls = c;
     }
@@ -267,9 +291,7 @@ __global__ __launch_bounds__(kMfBlock) void MfBf16Kernel(
     if (cl != cls) {
       if (cls >= 0) __syncthreads();
       // B1 and B2 of the class: bstride 16-bit words, a multiple of 8
-      const uint4* src = reinterpret_cast<const uint4*>(bmat + (int64_t)cl * bstride);
-      uint4* dst = reinterpret_cast<uint4*>(lds_raw);
-      for (int i = threadIdx.x; i < (int)(bstride / 8); i += kMfBlock) dst[i] = src[i];
+      CopyToLds(lds_raw, bmat + (int64_t)cl * bstride, (int)(bstride / 8));
       __syncthreads();
       cls = cl;
     }
@@ -417,7 +439,7 @@ __global__ __launch_bounds__(kMfBlock) void MfBf16Kernel(
 // arithmetic.  LDS holds S[c][i][j] for the rows i < nd (rows beyond nd read
 // as 0), row stride NDP + 1 words (FP64/FP32) or K1P + 8 halves (16-bit).
 
-template <class T, int DIM, int NDP, int MODE, class V>
+template <class T, int DIM, int NDP, int MODE, class V, bool LSCHED>
 __global__ __launch_bounds__(kMfBlock) void MfRefKernel(
     const T* __restrict__ smat, int64_t sstride, MfRounds R, const int* __restrict__ dofs,
     const double* __restrict__ metric, const double* __restrict__ kappa, const int* __restrict__ elnr, int nd,
@@ -435,18 +457,31 @@ __global__ __launch_bounds__(kMfBlock) void MfRefKernel(
   const int wg = XcdContiguousBlock(blockIdx.x, gridDim.x);
   const int r0 = (int)((int64_t)wg * R.nrounds / gridDim.x);
   const int r1 = (int)((int64_t)(wg + 1) * R.nrounds / gridDim.x);
+  // LSCHED: the class schedule is copied to LDS once (looked up in global
+  // memory, each round's lookup waits for every memory operation in flight,
+  // the previous round's stores included)
+  __shared__ int s_round0[LSCHED ? kMaxLdsClasses + 1 : 1];
+  __shared__ int64_t s_range[LSCHED ? 2 * kMaxLdsClasses : 1];
+  if constexpr (LSCHED) {
+    for (int i = threadIdx.x; i <= R.nclass; i += kMfBlock) s_round0[i] = R.round0[i];
+    for (int i = threadIdx.x; i < 2 * R.nclass; i += kMfBlock) s_range[i] = R.range[i];
+    __syncthreads();
+  }
+  const int* sched_r0 = LSCHED ? s_round0 : R.round0;
+  const int64_t* sched_rng = LSCHED ? s_range : R.range;
   int cls = -1;
   for (int rd = r0; rd < r1; rd++) {
-    const int c = RoundClass(R, rd, cls);
+    int c = cls < 0 ? 0 : cls;
+    while (sched_r0[c + 1] <= rd) c++;
     if (c != cls) {
       if (cls >= 0) __syncthreads();
-      const T* src = smat + (int64_t)c * sstride;
-      for (int i = threadIdx.x; i < (int)sstride; i += kMfBlock) ls[i] = src[i];
+      // sstride * sizeof(T) is a multiple of 16 bytes
+      CopyToLds(ls, smat + (int64_t)c * sstride, (int)(sstride * sizeof(T) / 16));
       __syncthreads();
       cls = c;
     }
-    const int64_t end = R.range[2 * c + 1];
-    const int64_t first = R.range[2 * c] + ((int64_t)(rd - R.round0[c]) * kMfWaves + wave) * 16;
+    const int64_t end = sched_rng[2 * c + 1];
+    const int64_t first = sched_rng[2 * c] + ((int64_t)(rd - sched_r0[c]) * kMfWaves + wave) * 16;
     if (first >= end) continue;
     const int64_t e = first + col;
     const bool valid = e < end;
@@ -527,7 +562,7 @@ __global__ __launch_bounds__(kMfBlock) void MfRefKernel(
 // 16-bit operands (BF16 / FP16), FP32 accumulation; operand layout of
 // v_mfma_f32_16x16x32_{bf16,f16} as in MfBf16Kernel.  FP16 uses the same
 // per-element power-of-two scaling of x and g as MfBf16Kernel.
-template <class H, int DIM, int KS1, int MT, int MODE, class V>
+template <class H, int DIM, int KS1, int MT, int MODE, class V, bool LSCHED>
 __global__ __launch_bounds__(kMfBlock) void MfRef16Kernel(
     const uint16_t* __restrict__ smat, int64_t sstride, MfRounds R, const int* __restrict__ dofs,
     const double* __restrict__ metric, const double* __restrict__ kappa, const int* __restrict__ elnr, int nd,
@@ -543,20 +578,29 @@ __global__ __launch_bounds__(kMfBlock) void MfRef16Kernel(
   const int wg = XcdContiguousBlock(blockIdx.x, gridDim.x);
   const int r0 = (int)((int64_t)wg * R.nrounds / gridDim.x);
   const int r1 = (int)((int64_t)(wg + 1) * R.nrounds / gridDim.x);
+  // LSCHED: the class schedule in LDS, as in MfRefKernel
+  __shared__ int s_round0[LSCHED ? kMaxLdsClasses + 1 : 1];
+  __shared__ int64_t s_range[LSCHED ? 2 * kMaxLdsClasses : 1];
+  if constexpr (LSCHED) {
+    for (int i = threadIdx.x; i <= R.nclass; i += kMfBlock) s_round0[i] = R.round0[i];
+    for (int i = threadIdx.x; i < 2 * R.nclass; i += kMfBlock) s_range[i] = R.range[i];
+    __syncthreads();
+  }
+  const int* sched_r0 = LSCHED ? s_round0 : R.round0;
+  const int64_t* sched_rng = LSCHED ? s_range : R.range;
   int cls = -1;
   for (int rd = r0; rd < r1; rd++) {
-    const int cl = RoundClass(R, rd, cls);
+    int cl = cls < 0 ? 0 : cls;
+    while (sched_r0[cl + 1] <= rd) cl++;
     if (cl != cls) {
       if (cls >= 0) __syncthreads();
       // sstride 16-bit words, a multiple of 8
-      const uint4* src = reinterpret_cast<const uint4*>(smat + (int64_t)cl * sstride);
-      uint4* dst = reinterpret_cast<uint4*>(lds_raw);
-      for (int i = threadIdx.x; i < (int)(sstride / 8); i += kMfBlock) dst[i] = src[i];
+      CopyToLds(lds_raw, smat + (int64_t)cl * sstride, (int)(sstride / 8));
       __syncthreads();
       cls = cl;
     }
-    const int64_t end = R.range[2 * cl + 1];
-    const int64_t first = R.range[2 * cl] + ((int64_t)(rd - R.round0[cl]) * kMfWaves + wave) * 16;
+    const int64_t end = sched_rng[2 * cl + 1];
+    const int64_t first = sched_rng[2 * cl] + ((int64_t)(rd - sched_r0[cl]) * kMfWaves + wave) * 16;
     if (first >= end) continue;
     const int64_t e = first + col;
     const bool valid = e < end;
@@ -1004,7 +1048,7 @@ MatFreeOp::MatFreeOp(int dim_, int nd_, int nq_, int64_t ndof, int nclass, const
     } else {
       if (prec == NGSAMD_PREC_F64) {
         const int rs = ndp + 1;
-        sstride = (int64_t)nc * nd * rs;
+        sstride = ((int64_t)nc * nd * rs + 1) / 2 * 2;  // whole 16-byte words
         std::vector<double> lay((size_t)sstride * nclass, 0.0);
         for (int c = 0; c < nclass; c++)
           for (int k = 0; k < nc; k++)
@@ -1205,7 +1249,7 @@ static void LaunchMfma(const MatFreeOp& A, const T* b, const V* x, V* out, doubl
   const size_t lds = (size_t)A.bstride * sizeof(T);
   auto kern = MfMfmaKernel<T, DIM, NDP, MODE, V>;
   if (prepare) return PrepareKernel(A, kern, lds, MODE, sizeof(V) == 8);
-  const MfRounds R{A.round0.p, A.cls_range.p, A.nrounds};
+  const MfRounds R{A.round0.p, A.cls_range.p, A.nrounds, (int)A.classes.size()};
   kern<<<MfGrid(A, MODE, sizeof(V) == 8), kMfBlock, lds, st>>>(b, A.bstride, R, A.weights.p, nqb, A.dofs.p,
                                                                A.metric.p, A.Kappa(), A.elnr.p, A.nd, x, out, s, stop);
   NGSAMD_CHECK_LAUNCH();
@@ -1232,7 +1276,7 @@ static void LaunchBf16(const MatFreeOp& A, const V* x, V* out, double s, const i
   const size_t lds = (size_t)A.bstride * 2;
   auto kern = MfBf16Kernel<H, DIM, KS1, MT, MODE, V>;
   if (prepare) return PrepareKernel(A, kern, lds, MODE, sizeof(V) == 8);
-  const MfRounds R{A.round0.p, A.cls_range.p, A.nrounds};
+  const MfRounds R{A.round0.p, A.cls_range.p, A.nrounds, (int)A.classes.size()};
   kern<<<MfGrid(A, MODE, sizeof(V) == 8), kMfBlock, lds, st>>>(A.bmat16.p, A.bstride, R, A.k1p, A.nqc, A.weights.p,
                                                                A.dofs.p, A.metric.p, A.Kappa(), A.elnr.p, A.nd, x,
                                                                out, s, stop);
@@ -1252,9 +1296,10 @@ template <class T, int DIM, int NDP, int MODE, class V>
 static void LaunchRef(const MatFreeOp& A, const T* sm, const V* x, V* out, double s, const int* stop,
                       hipStream_t st, bool prepare) {
   const size_t lds = (size_t)A.sstride * sizeof(T);
-  auto kern = MfRefKernel<T, DIM, NDP, MODE, V>;
+  auto kern = A.classes.size() <= (size_t)kMaxLdsClasses ? MfRefKernel<T, DIM, NDP, MODE, V, true>
+                                                         : MfRefKernel<T, DIM, NDP, MODE, V, false>;
   if (prepare) return PrepareKernel(A, kern, lds, MODE, sizeof(V) == 8);
-  const MfRounds R{A.round0.p, A.cls_range.p, A.nrounds};
+  const MfRounds R{A.round0.p, A.cls_range.p, A.nrounds, (int)A.classes.size()};
   kern<<<MfGrid(A, MODE, sizeof(V) == 8), kMfBlock, lds, st>>>(sm, A.sstride, R, A.dofs.p, A.metric.p, A.Kappa(),
                                                                A.elnr.p, A.nd, x, out, s, stop);
   NGSAMD_CHECK_LAUNCH();
@@ -1279,9 +1324,10 @@ template <class H, int DIM, int KS1, int MT, int MODE, class V>
 static void LaunchRef16(const MatFreeOp& A, const V* x, V* out, double s, const int* stop, hipStream_t st,
                         bool prepare) {
   const size_t lds = (size_t)A.sstride * 2;
-  auto kern = MfRef16Kernel<H, DIM, KS1, MT, MODE, V>;
+  auto kern = A.classes.size() <= (size_t)kMaxLdsClasses ? MfRef16Kernel<H, DIM, KS1, MT, MODE, V, true>
+                                                         : MfRef16Kernel<H, DIM, KS1, MT, MODE, V, false>;
   if (prepare) return PrepareKernel(A, kern, lds, MODE, sizeof(V) == 8);
-  const MfRounds R{A.round0.p, A.cls_range.p, A.nrounds};
+  const MfRounds R{A.round0.p, A.cls_range.p, A.nrounds, (int)A.classes.size()};
   kern<<<MfGrid(A, MODE, sizeof(V) == 8), kMfBlock, lds, st>>>(A.smat16.p, A.sstride, R, A.dofs.p, A.metric.p,
                                                                A.Kappa(), A.elnr.p, A.nd, x, out, s, stop);
   NGSAMD_CHECK_LAUNCH();
