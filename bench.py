@@ -142,6 +142,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-cg", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-assembly", action="store_true")
     ap.add_argument("--batch-problems", type=int, default=512)
     ap.add_argument("--profile-cg", action="store_true",
                     help="one FP64 CSR CG solve (kappa = 1, tol 1e-10) without graph capture, for a kernel trace")
@@ -275,6 +276,26 @@ def main():
                 extra[key]["us_per_apply_f32_vectors"] = round(ms32 * 1e3, 1)
             log(f"[bench] {key} {ms * 1e3:.1f} us  {tf:.1f} TFLOP/s  err {err:.1e}")
             del M
+
+    # ------------------------------------------- CSR assembly on the device
+    if not args.no_assembly:
+        t0 = time.perf_counter()
+        vals_h = N.AssembleCSR(fes, rowptr, cols, ir, True, None, False, None)
+        t_host = time.perf_counter() - t0
+        M64 = build_mf(N, fes, mesh, ir, None, 0)
+        N.synchronize()
+        t0 = time.perf_counter()
+        Adev = N.AssembleCSRDevice(M64, rowptr, cols)
+        N.synchronize()
+        t_dev = time.perf_counter() - t0
+        d_asm = float(np.abs(Adev.csr_values() - vals_h).max() / np.abs(vals_h).max())
+        del Adev, M64, vals_h
+        extra["csr_assembly"] = {
+            "host_seconds": round(t_host, 4), "host_threads": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count())),
+            "device_seconds_with_host_structure": round(t_dev, 4), "rel_diff_vs_host": d_asm,
+            "note": "device: pattern-runs structure built on the host, values by AssembleCsrKernel "
+                    "(kernel time in the rocprof kernel trace)"}
+        log(f"[bench] csr assembly host {t_host:.3f}s device {t_dev:.3f}s diff {d_asm:.1e}")
 
     # ----------------------------------------------------------------- CG
     if not args.no_cg:

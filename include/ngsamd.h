@@ -94,6 +94,10 @@ int ngsamd_csr_create(ngsamd_op_t* op, int64_t height, int64_t width,
                       const int64_t* h_rowptr, const int* h_cols, const double* h_vals);
 /* diagonal of a CSR operator (0 where the row has no diagonal entry) */
 int ngsamd_csr_diagonal(ngsamd_op_t op, double* d_diag);
+/* Copies the values to the host in CSR order (row by row, columns as at
+ * creation): the value vector of the matrix, BaseSparseMatrix::AsVector
+ * (linalg/sparsematrix.hpp:312). */
+int ngsamd_csr_values(ngsamd_op_t op, double* h_vals);
 
 /* ---------------------------------------------------------------- Jacobi */
 /* y += s * diag^{-1} x on free dofs, 0 on the others (linalg/jacobi.cpp:41-90).
@@ -132,6 +136,17 @@ int ngsamd_mf_diagonal(ngsamd_op_t op, double* d_diag);
 /* g_e = -int_e grad(lam) . grad(u) = -(lam_e^T K_e u_e)/kappa_e, indexed by
  * global element number (adjoint gradient of an element coefficient). */
 int ngsamd_mf_element_grad(ngsamd_op_t op, const double* d_lam, const double* d_u, double* d_grad);
+/* Assembles the stiffness matrix of a matrix-free operator's form into a new
+ * CSR operator on the device: A[r][c] = sum over the elements e holding dofs
+ * r = dofs[e][i] and c = dofs[e][j] of K_e[i][j], with K_e = kappa_e times
+ * the packed |det F| F^-1 F^-T contracted with the reference matrices, into
+ * the sparsity pattern (h_rowptr, h_cols) (int64 row pointers, sorted int32
+ * columns), which must hold every such (r, c).  Each value is summed in a
+ * fixed order (deterministic).  Replaces BilinearForm::Assemble
+ * (comp/bilinearform.cpp:832) -> S_BilinearForm::DoAssemble (:1563-2215) ->
+ * SparseMatrixTM::AddElementMatrix (linalg/sparsematrix_impl.hpp:150-189)
+ * for the grad-grad form. */
+int ngsamd_mf_assemble_csr(ngsamd_op_t mf, const int64_t* h_rowptr, const int* h_cols, ngsamd_op_t* csr);
 
 /* ----------------------------------------------------------------- CG */
 /* Preconditioned CG with the recurrence and stopping rule of the reference's

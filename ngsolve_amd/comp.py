@@ -282,11 +282,15 @@ class DevMatrixFree(DeviceMatrix):
 
 
 class BilinearForm:
-    def __init__(self, space, integrals=None, mf=None, matrix_free_bdb=False, **kwargs):
+    def __init__(self, space, integrals=None, mf=None, matrix_free_bdb=False, device_assembly=False, **kwargs):
+        """device_assembly=True assembles the CSR values on the device
+        (ngsamd_mf_assemble_csr; one grad(u)*grad(v) term); the pattern is
+        built on the host as before."""
         if isinstance(space, _Integral):  # BilinearForm(grad(u)*grad(v)*dx)
             integrals, space = space, None
         self.integrals = []
         self.space = space
+        self.device_assembly = device_assembly
         self.mf = mf if mf is not None else (MFOpts() if matrix_free_bdb else None)
         self.mat = None
         if integrals is not None:
@@ -325,6 +329,14 @@ class BilinearForm:
                                      self.mf.atomic)
             return self
         rowptr, cols = N.BuildPattern(fes.space)
+        if self.device_assembly:
+            if len(self.integrals) != 1 or self.integrals[0][0].kind != "laplace":
+                raise ValueError("device assembly supports one grad(u)*grad(v) term")
+            term, meas = self.integrals[0]
+            mf = DevMatrixFree(fes, self._order(term, meas), self._coef(term), "f64")
+            op = N.AssembleCSRDevice(mf.op, rowptr, cols)
+            self.mat = DevSparseMatrix(fes.ndof, fes.ndof, rowptr, cols, None, symmetric=True, op=op)
+            return self
         vals = np.zeros(len(cols))
         for term, meas in self.integrals:
             ir = fes.integration_rule(self._order(term, meas))

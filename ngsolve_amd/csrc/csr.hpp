@@ -24,7 +24,9 @@ struct CsrOp : ngsamd_op_s {
   DevBuf<double> rvals; // values, run by run, [L][R]
   DevBuf<int> blk_run;  // runs kernel blocks: first run, nblocks + 1
   DevBuf<int> blk_len;  // runs kernel blocks: rows per run (all runs of a block alike)
+  DevBuf<int> row_run;  // run of each row (structure-only operators, runs layout)
   int64_t nruns = 0, nshared = 0;
+  int64_t max_row_len = 0;
   int64_t nnz = 0;
   // stands in for x when the width is 0 (padding lanes still read x[0])
   DevBuf<double> xpad;
@@ -33,7 +35,8 @@ struct CsrOp : ngsamd_op_s {
   DevBuf<int> blk_row;  // stream kernel row blocks
   int nblocks = 0;
   int block_nnz = 2048;
-  int max_grid = 16384;  // workgroups of the stream/runs kernels without partial sums
+  int max_grid = 16384;  // workgroups of the stream/runs kernels
+  // v null: the structure only, values zero until filled on the device
   CsrOp(int64_t h, int64_t w, const int64_t* rp, const int* c, const double* v);
   void MultAdd(double s, const double* x, double* y, hipStream_t st) override;
   void Mult(const double* x, double* y, hipStream_t st) override;
@@ -41,6 +44,8 @@ struct CsrOp : ngsamd_op_s {
   // returns the number of partials; skipped entirely when *d_stop != 0
   int MultDot(const double* x, double* y, double* partials, const int* d_stop, hipStream_t st);
   void Diagonal(double* d, hipStream_t st);
+  // the values in CSR order (row by row, columns as given at creation)
+  void Values(double* host_vals) const;
 };
 
 struct JacobiOp : ngsamd_op_s {

@@ -30,6 +30,7 @@
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -1010,10 +1011,12 @@ MatFreeOp::MatFreeOp(int dim_, int nd_, int nq_, int64_t ndof, int nclass, const
                    "NGSAMD_MF_KERNEL must be quad or ref");
     ref_kernel = use_mfma && !(kenv && std::string(kenv) == "quad");
   }
-  if (ref_kernel) {
-    const int nc = ncomp;
-    // S[c][k][i][j] in FP64, k over the packed metric components
-    std::vector<double> S((size_t)nclass * nc * nd * nd, 0.0);
+  // S[c][k][i][j] in FP64, k over the packed metric components: the operand
+  // of the reference-matrix kernels (in their layouts below) and of the CSR
+  // assembly (smat_asm, as is)
+  const int nc = ncomp;
+  std::vector<double> S((size_t)nclass * nc * nd * nd, 0.0);
+  {
     const int pd[6][2] = {{0, 0}, {1, 1}, {2, 2}, {0, 1}, {0, 2}, {1, 2}};
     const int pd2[3][2] = {{0, 0}, {1, 1}, {0, 1}};
     const size_t bsz0 = (size_t)nq * nd * dim;
@@ -1031,6 +1034,11 @@ MatFreeOp::MatFreeOp(int dim_, int nd_, int nq_, int64_t ndof, int nclass, const
             }
         }
       }
+  }
+  smat_asm.alloc(std::max<size_t>(S.size(), 1));
+  if (!S.empty()) smat_asm.upload(S.data(), S.size());
+  NGSAMD_HIP(hipStreamSynchronize(Stream()));
+  if (ref_kernel) {
     if (half) {
       const int rs = ks1 * 32 + 8;
       sstride = (int64_t)nc * nd * rs;
@@ -1483,6 +1491,106 @@ void MatFreeOp::Diagonal(double* d, hipStream_t st) {
   NGSAMD_CHECK_LAUNCH();
 }
 
+// ------------------------------------------------------ CSR assembly on device
+// One 64-lane workgroup per row (dofs in gather order, grid-stride): the row's
+// column list into LDS; then, for each contribution (e, i) of the row in slot
+// order, lane j adds K_e[i][j] = sum_k g_{e,k} S_k[i][j] (k over the packed
+// metric components, g_e = kappa_e times the packed |det F| F^-1 F^-T) at the
+// position of dofs[e][j] in the list (binary search); the row's values are
+// written once at the end.  Deterministic: every value is summed in slot
+// order.  RUNS: the target is the runs layout (the row's list is its run's
+// shared list, values at val_ptr[u] + position * R + row in run).  A column
+// missing from the pattern sets *bad.
+template <int DIM, bool RUNS>
+__global__ __launch_bounds__(64) void AssembleCsrKernel(
+    const int* __restrict__ g_rowptr, const int* __restrict__ g_slots, const int* __restrict__ g_order,
+    int64_t nrows, const int* __restrict__ dofs, const double* __restrict__ metric,
+    const double* __restrict__ kappa, const int* __restrict__ elnr, int nd, const double* __restrict__ smat,
+    const int64_t* __restrict__ cls_range, int nclass, const int* __restrict__ lptr,
+    const int* __restrict__ lcols, const int* __restrict__ row_run, const int* __restrict__ run_row,
+    const int* __restrict__ val_ptr, double* __restrict__ vals, int max_len, int* __restrict__ bad) {
+  constexpr int NC = DIM * (DIM + 1) / 2;
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  double* lval = reinterpret_cast<double*>(lds_raw);
+  int* lcol = reinterpret_cast<int*>(lds_raw + sizeof(double) * (size_t)max_len);
+  const int lane = threadIdx.x;
+  for (int64_t t = blockIdx.x; t < nrows; t += gridDim.x) {
+    const int r = g_order[t];
+    int base, L, R = 1;
+    int64_t obase;
+    if constexpr (RUNS) {
+      const int u = row_run[r];
+      base = lptr[u];
+      L = lptr[u + 1] - base;
+      R = run_row[u + 1] - run_row[u];
+      obase = (int64_t)val_ptr[u] + (r - run_row[u]);
+    } else {
+      base = lptr[r];
+      L = lptr[r + 1] - base;
+      obase = base;
+    }
+    for (int k = lane; k < L; k += 64) {
+      lcol[k] = lcols[base + k];
+      lval[k] = 0.0;
+    }
+    __syncthreads();
+    const int s1 = g_rowptr[t + 1];
+    for (int si = g_rowptr[t]; si < s1; si++) {
+      const int slot = g_slots[si];
+      const int e = slot / nd, i = slot - e * nd;
+      int c = 0;
+      while (c + 1 < nclass && cls_range[2 * c + 1] <= e) c++;
+      const double kap = kappa[elnr[e]];
+      double g[NC];
+#pragma unroll
+      for (int k = 0; k < NC; k++) g[k] = kap * metric[(int64_t)e * NC + k];
+      for (int j = lane; j < nd; j += 64) {
+        const double* sp = smat + ((int64_t)c * NC * nd + i) * nd + j;
+        double v = 0.0;
+#pragma unroll
+        for (int k = 0; k < NC; k++) v += g[k] * sp[(int64_t)k * nd * nd];
+        const int col = dofs[(int64_t)e * nd + j];
+        int lo = 0, hi = L;  // first position whose column is >= col
+        while (lo < hi) {
+          const int mid = (lo + hi) >> 1;
+          if (lcol[mid] < col) lo = mid + 1;
+          else hi = mid;
+        }
+        if (lo < L && lcol[lo] == col) lval[lo] += v;
+        else *bad = 1;
+      }
+      // the next contribution may add to the same positions
+      __syncthreads();
+    }
+    for (int k = lane; k < L; k += 64) vals[obase + (int64_t)k * R] = lval[k];
+    __syncthreads();
+  }
+}
+
+void MatFreeOp::AssembleCsr(CsrOp& A, hipStream_t st) {
+  NGSAMD_REQUIRE(A.height == height && A.width == width, "ngsamd_mf_assemble_csr: pattern size differs");
+  if (height == 0 || nel == 0 || A.nnz == 0) return;
+  const size_t lds = (size_t)A.max_row_len * (sizeof(double) + sizeof(int));
+  NGSAMD_REQUIRE(lds <= (size_t)DeviceMaxLds(), "ngsamd_mf_assemble_csr: a row is too long for LDS");
+  DevBuf<int> bad;
+  bad.alloc(1);
+  NGSAMD_HIP(hipMemsetAsync(bad.p, 0, sizeof(int), st));
+  const bool runs = A.algo == 2;
+  auto kern = dim == 2 ? (runs ? AssembleCsrKernel<2, true> : AssembleCsrKernel<2, false>)
+                       : (runs ? AssembleCsrKernel<3, true> : AssembleCsrKernel<3, false>);
+  NGSAMD_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  const int grid = (int)std::min<int64_t>(height, 65536);
+  kern<<<grid, 64, lds, st>>>(g_rowptr.p, g_slots.p, g_order.p, height, dofs.p, metric.p, Kappa(), elnr.p, nd,
+                              smat_asm.p, cls_range.p, (int)classes.size(), runs ? A.run_ptr.p : A.rowptr.p,
+                              runs ? A.rcols.p : A.cols.p, A.row_run.p, A.run_row.p, A.val_ptr.p,
+                              runs ? A.rvals.p : A.vals.p, (int)A.max_row_len, bad.p);
+  NGSAMD_CHECK_LAUNCH();
+  int hbad = 0;
+  NGSAMD_HIP(hipMemcpyAsync(&hbad, bad.p, sizeof(int), hipMemcpyDeviceToHost, st));
+  NGSAMD_HIP(hipStreamSynchronize(st));
+  NGSAMD_REQUIRE(!hbad, "ngsamd_mf_assemble_csr: the pattern lacks a pair of dofs of one element");
+}
+
 void MatFreeOp::ElementGrad(const double* lam, const double* u, double* grad, hipStream_t st) {
   for (size_t ci = 0; ci < classes.size(); ci++) {
     const MfClass& c = classes[ci];
@@ -1535,6 +1643,20 @@ extern "C" int ngsamd_mf_diagonal(ngsamd_op_t op, double* d) {
     RequireInit();
     NGSAMD_REQUIRE(op && op->kind == OpKind::MatrixFree, "ngsamd_mf_diagonal: not a matrix-free operator");
     static_cast<MatFreeOp*>(op)->Diagonal(d, Stream());
+  });
+}
+
+extern "C" int ngsamd_mf_assemble_csr(ngsamd_op_t mf, const int64_t* rowptr, const int* cols, ngsamd_op_t* csr) {
+  return Guard([&] {
+    RequireInit();
+    NGSAMD_REQUIRE(mf && mf->kind == OpKind::MatrixFree, "ngsamd_mf_assemble_csr: not a matrix-free operator");
+    NGSAMD_REQUIRE(csr && rowptr, "ngsamd_mf_assemble_csr: null argument");
+    *csr = nullptr;
+    MatFreeOp* M = static_cast<MatFreeOp*>(mf);
+    NGSAMD_REQUIRE(rowptr[M->height] == 0 || cols, "ngsamd_mf_assemble_csr: null argument");
+    std::unique_ptr<CsrOp> A(new CsrOp(M->height, M->width, rowptr, cols, nullptr));
+    M->AssembleCsr(*A, Stream());
+    *csr = A.release();
   });
 }
 

@@ -3,6 +3,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "csr.hpp"
 
 namespace ngsamd {
 
@@ -35,6 +36,7 @@ struct MatFreeOp : ngsamd_op_s {
   // reference matrices S[class][c][i][j] (c over the packed metric
   // components), rows padded to ndp + 1 words (FP64/FP32) or ks1*32 + 8 halves
   DevBuf<double> smat64;
+  DevBuf<double> smat_asm;    // FP64 S[class][k][i][j], for the CSR assembly
   DevBuf<float> smat32;
   DevBuf<uint16_t> smat16;
   int64_t sstride = 0;        // elements per class block in smat*
@@ -71,6 +73,8 @@ struct MatFreeOp : ngsamd_op_s {
   int MultDotF32(const float* x, float* y, double* partials, const int* d_stop, hipStream_t st);
   void Diagonal(double* d, hipStream_t st);
   void ElementGrad(const double* lam, const double* u, double* grad, hipStream_t st);
+  // fills the values of A (same size, pattern holding every element pair)
+  void AssembleCsr(CsrOp& A, hipStream_t st);
 
  private:
   template <class V>

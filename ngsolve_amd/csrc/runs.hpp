@@ -23,6 +23,8 @@ __device__ __forceinline__ int RunDiv(int k, int inv) { return (k * inv) >> 16; 
 // run length with at most max_vals values and max_vals rows (or one longer
 // row).  Fills the
 // host arrays of the runs layout; returns the number of shared columns.
+// With v null only the structure is built (rvals stays empty; val_ptr still
+// counts the values), for values filled in on the device.
 // With seg (nseg + 1 row offsets of independent problems), runs and blocks
 // never cross a problem boundary and blk_seg receives each block's problem.
 inline int64_t BuildRuns(int64_t h, const int64_t* rp, const int* c, const double* v, int max_vals,
@@ -38,7 +40,7 @@ inline int64_t BuildRuns(int64_t h, const int64_t* rp, const int* c, const doubl
   blk_run.assign(1, 0);
   blk_len.clear();
   if (blk_seg) blk_seg->clear();
-  int64_t acc = 0, rows = 0;
+  int64_t acc = 0, rows = 0, nval = 0;
   int cur_len = 0;
   int p = 0, cur_seg = 0;  // problem of row r, problem of the open block
   for (int64_t r = 0; r < h;) {
@@ -64,12 +66,14 @@ inline int64_t BuildRuns(int64_t h, const int64_t* rp, const int* c, const doubl
     acc += nv;
     rows += R;
     rcols.insert(rcols.end(), c + rp[r], c + rp[r] + L);
-    for (int64_t j = 0; j < L; j++)
-      for (int q = 0; q < R; q++) rvals.push_back(v[rp[r + q] + j]);
+    if (v)
+      for (int64_t j = 0; j < L; j++)
+        for (int q = 0; q < R; q++) rvals.push_back(v[rp[r + q] + j]);
+    nval += nv;
     r += R;
     run_row.push_back((int)r);
     run_ptr.push_back((int)rcols.size());
-    val_ptr.push_back((int)rvals.size());
+    val_ptr.push_back((int)nval);
   }
   const int nruns = (int)run_row.size() - 1;
   if (nruns > blk_run.back()) {

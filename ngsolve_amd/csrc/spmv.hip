@@ -316,6 +316,7 @@ CsrOp::CsrOp(int64_t h, int64_t w, const int64_t* rp, const int* c, const double
   }
   for (int64_t k = 0; k < nnz; k++)
     NGSAMD_REQUIRE(c[k] >= 0 && c[k] < w, "ngsamd_csr_create: column number out of range");
+  for (int64_t i = 0; i < h; i++) max_row_len = std::max<int64_t>(max_row_len, rp[i + 1] - rp[i]);
   if (w == 0) {
     static const double z = 0.0;
     xpad.alloc(1);
@@ -325,9 +326,8 @@ CsrOp::CsrOp(int64_t h, int64_t w, const int64_t* rp, const int* c, const double
   block_nnz = nnz_env ? atoi(nnz_env) : 0;
   NGSAMD_REQUIRE(block_nnz == 0 || block_nnz == 1024 || block_nnz == 2048 || block_nnz == 4096,
                  "NGSAMD_SPMV_NNZ must be 1024, 2048 or 4096");
-  // grid of the plain applies (the variant with partial sums stays at
-  // kMaxReduceBlocks): 16384 measured best at 32^3 p=4 (runs kernel 395 us;
-  // 8192 405, 4096 418, 65536 494)
+  // grid of the stream/runs applies: 16384 measured best at 32^3 p=4 (runs
+  // kernel 395 us; 8192 405, 4096 418, 65536 494)
   max_grid = 16384;
   const char* grid_env = getenv("NGSAMD_SPMV_GRID");
   if (grid_env) {
@@ -375,7 +375,16 @@ CsrOp::CsrOp(int64_t h, int64_t w, const int64_t* rp, const int* c, const double
       run_ptr.upload(rpp.data(), rpp.size());
       val_ptr.upload(vp.data(), vp.size());
       if (nshared) rcols.upload(rc.data(), nshared);
-      if (nnz) rvals.upload(rv.data(), nnz);
+      if (nnz && v) rvals.upload(rv.data(), nnz);
+      if (nnz && !v) NGSAMD_HIP(hipMemsetAsync(rvals.p, 0, sizeof(double) * nnz, Stream()));
+      if (!v) {
+        std::vector<int> rrun(std::max<int64_t>(h, 1), 0);
+        for (int64_t u = 0; u < nruns; u++)
+          for (int r = rr[u]; r < rr[u + 1]; r++) rrun[r] = (int)u;
+        row_run.alloc(rrun.size());
+        row_run.upload(rrun.data(), rrun.size());
+        NGSAMD_HIP(hipStreamSynchronize(Stream()));
+      }
       blk_run.upload(br.data(), br.size());
       if (!bl.empty()) blk_len.upload(bl.data(), bl.size());
       NGSAMD_HIP(hipStreamSynchronize(Stream()));
@@ -393,7 +402,8 @@ CsrOp::CsrOp(int64_t h, int64_t w, const int64_t* rp, const int* c, const double
   vals.alloc(nnz + 4);
   rowptr.upload(rp32.data(), h + 1);
   cols.upload(c, nnz);
-  vals.upload(v, nnz);
+  if (v) vals.upload(v, nnz);
+  else if (nnz) NGSAMD_HIP(hipMemsetAsync(vals.p, 0, sizeof(double) * nnz, Stream()));
   {
     const int zc[4] = {0, 0, 0, 0};
     const double zv[4] = {0, 0, 0, 0};
@@ -422,6 +432,28 @@ CsrOp::CsrOp(int64_t h, int64_t w, const int64_t* rp, const int* c, const double
   blk_row.upload(br.data(), br.size());
   NGSAMD_HIP(hipStreamSynchronize(Stream()));
   algo = algo_name == "rowgroup" ? 0 : 1;
+}
+
+void CsrOp::Values(double* host_vals) const {
+  NGSAMD_HIP(hipStreamSynchronize(Stream()));
+  if (nnz == 0) return;
+  if (algo != 2) {
+    NGSAMD_HIP(hipMemcpy(host_vals, vals.p, sizeof(double) * nnz, hipMemcpyDeviceToHost));
+    return;
+  }
+  // runs layout: run u holds rows run_row[u].. with values [L][R]
+  std::vector<int> rr(nruns + 1), vp(nruns + 1);
+  std::vector<double> rv(nnz);
+  NGSAMD_HIP(hipMemcpy(rr.data(), run_row.p, sizeof(int) * (nruns + 1), hipMemcpyDeviceToHost));
+  NGSAMD_HIP(hipMemcpy(vp.data(), val_ptr.p, sizeof(int) * (nruns + 1), hipMemcpyDeviceToHost));
+  NGSAMD_HIP(hipMemcpy(rv.data(), rvals.p, sizeof(double) * nnz, hipMemcpyDeviceToHost));
+  int64_t k = 0;  // CSR position: rows in order
+  for (int64_t u = 0; u < nruns; u++) {
+    const int R = rr[u + 1] - rr[u];
+    const int64_t L = R ? (vp[u + 1] - vp[u]) / R : 0;
+    for (int q = 0; q < R; q++)
+      for (int64_t j = 0; j < L; j++) host_vals[k++] = rv[vp[u] + j * R + q];
+  }
 }
 
 template <int G, bool ACCUM, bool DOT>
@@ -551,6 +583,16 @@ extern "C" int ngsamd_csr_create(ngsamd_op_t* op, int64_t height, int64_t width,
     NGSAMD_REQUIRE(op && rp && (rp[height] == 0 || (cols && vals)), "ngsamd_csr_create: null argument");
     *op = nullptr;
     *op = new CsrOp(height, width, rp, cols, vals);
+  });
+}
+
+extern "C" int ngsamd_csr_values(ngsamd_op_t op, double* h_vals) {
+  return Guard([&] {
+    RequireInit();
+    NGSAMD_REQUIRE(op && op->kind == OpKind::CSR, "ngsamd_csr_values: not a CSR operator");
+    CsrOp* A = static_cast<CsrOp*>(op);
+    NGSAMD_REQUIRE(h_vals || A->nnz == 0, "ngsamd_csr_values: null argument");
+    A->Values(h_vals);
   });
 }
 

@@ -54,6 +54,7 @@ struct Operator {
   ngsamd_op_t op = nullptr;
   std::string kind;
   int64_t height = 0, width = 0;
+  int64_t nnz = 0;  // CSR operators: number of stored values
   Operator(ngsamd_op_t o, std::string k) : op(o), kind(std::move(k)) {
     Check(ngsamd_op_height(op, &height));
     Check(ngsamd_op_width(op, &width));
@@ -408,10 +409,18 @@ PYBIND11_MODULE(_ngsamd, m) {
              if (k->itemsize != 8) throw std::runtime_error("set_kappa: float64 vector expected");
              Check(ngsamd_mf_set_kappa(A.op, k->d()));
            })
-      .def("element_grad", [](Operator& A, DeviceVector& lam, DeviceVector& u, DeviceVector& g) {
-        RequireF64(lam, A.height, "element_grad");
-        RequireF64(u, A.height, "element_grad");
-        Check(ngsamd_mf_element_grad(A.op, lam.d(), u.d(), g.d()));
+      .def("element_grad",
+           [](Operator& A, DeviceVector& lam, DeviceVector& u, DeviceVector& g) {
+             RequireF64(lam, A.height, "element_grad");
+             RequireF64(u, A.height, "element_grad");
+             Check(ngsamd_mf_element_grad(A.op, lam.d(), u.d(), g.d()));
+           })
+      .def_readonly("nnz", &Operator::nnz)
+      .def("csr_values", [](Operator& A) {
+        if (A.kind != "csr") throw std::runtime_error("csr_values: not a CSR operator");
+        py::array_t<double> v(A.nnz);
+        Check(ngsamd_csr_values(A.op, v.mutable_data()));
+        return v;
       });
 
   m.def("CreateCSR",
@@ -420,8 +429,20 @@ PYBIND11_MODULE(_ngsamd, m) {
             throw std::runtime_error("CreateCSR: inconsistent CSR arrays");
           ngsamd_op_t op = nullptr;
           Check(ngsamd_csr_create(&op, h, w, rowptr.data(), cols.data(), vals.data()));
-          return std::make_shared<Operator>(op, "csr");
+          auto A = std::make_shared<Operator>(op, "csr");
+          A->nnz = (int64_t)cols.size();
+          return A;
         });
+  m.def("AssembleCSRDevice", [](Operator& mf, Arr<int64_t> rowptr, Arr<int> cols) {
+    if (mf.kind != "matfree") throw std::runtime_error("AssembleCSRDevice: matrix-free operator expected");
+    if (rowptr.size() != mf.height + 1 || rowptr.data()[mf.height] != (int64_t)cols.size())
+      throw std::runtime_error("AssembleCSRDevice: inconsistent CSR arrays");
+    ngsamd_op_t op = nullptr;
+    Check(ngsamd_mf_assemble_csr(mf.op, rowptr.data(), cols.data(), &op));
+    auto A = std::make_shared<Operator>(op, "csr");
+    A->nnz = (int64_t)cols.size();
+    return A;
+  });
   m.def("CreateJacobi", [](Arr<double> diag, py::object freedofs) {
     ngsamd_op_t op = nullptr;
     std::vector<uint8_t> fr;

@@ -560,12 +560,23 @@ class DeviceMatrix(BaseMatrix):
 class DevSparseMatrix(DeviceMatrix):
     """CSR matrix on the device (SparseMatrix<double> + DevSparseMatrix)."""
 
-    def __init__(self, height, width, rowptr, cols, vals, symmetric=True):
+    def __init__(self, height, width, rowptr, cols, vals, symmetric=True, op=None):
+        """From host CSR arrays, or (vals None, op given) around a CSR operator
+        whose values were assembled on the device."""
         N = ensure_device()
-        super().__init__(N.CreateCSR(int(height), int(width), np.ascontiguousarray(rowptr, dtype=np.int64),
-                                     np.ascontiguousarray(cols, dtype=np.int32),
-                                     np.ascontiguousarray(vals, dtype=np.float64)), symmetric)
-        self.rowptr, self.cols, self.vals = rowptr, cols, vals
+        if op is None:
+            op = N.CreateCSR(int(height), int(width), np.ascontiguousarray(rowptr, dtype=np.int64),
+                             np.ascontiguousarray(cols, dtype=np.int32), np.ascontiguousarray(vals, dtype=np.float64))
+        super().__init__(op, symmetric)
+        self.rowptr, self.cols, self._vals = rowptr, cols, vals
+
+    @property
+    def vals(self):
+        """The values in CSR order (copied from the device when they were
+        assembled there)."""
+        if self._vals is None:
+            self._vals = self.op.csr_values()
+        return self._vals
 
     @property
     def nze(self):

@@ -1,0 +1,87 @@
+"""CSR assembly on the device (ngsamd_mf_assemble_csr) against the oracle's
+assembly (oracle/fe_oracle.py Problem.assemble: per-element matrices from
+the per-point F^-T F^-1 form, scipy COO -> CSR): values within 1e-13 of the
+largest entry, in every CSR layout the operator may choose (runs, stream,
+rowgroup).  Also against the product's host assembly on the same pattern,
+the assembled operator's apply against the oracle SpMV, determinism (two
+assemblies are bit-identical) and the refusal of a pattern that lacks a
+pair of dofs of one element."""
+import numpy as np
+import pytest
+
+import oracle
+from oracle import fe_oracle as O
+from helpers import N, dvec, mf_operator, problem, relerr_inf, say
+
+pytestmark = pytest.mark.gpu
+
+CASES = [(2, 6, 1, None), (2, 8, 3, 6), (2, 5, 5, None), (3, 3, 1, None), (3, 3, 2, None), (3, 3, 4, 8),
+         (3, 2, 5, None)]
+ALGOS = ["runs", "stream", "rowgroup"]
+
+
+@pytest.mark.parametrize("dim,n,p,order", CASES)
+@pytest.mark.parametrize("algo", ALGOS)
+def test_assembly_matches_oracle(device, monkeypatch, dim, n, p, order, algo):
+    monkeypatch.setenv("NGSAMD_SPMV", algo)
+    say(f"[assemble {algo}] dim={dim} n={n} p={p} order={order}")
+    P = problem(dim, n, p, order=order, seed_kappa=3)
+    A = N.AssembleCSRDevice(mf_operator(P), P["rowptr"], P["cols"])
+    vals = A.csr_values()
+    pr = O.Problem(P["omesh"], p, intorder=P["order"])
+    orp, ocols, ovals = pr.assemble(kappa=P["kappa"])
+    assert np.array_equal(orp, P["rowptr"]) and np.array_equal(ocols, P["cols"])
+    scale = np.abs(ovals).max()
+    assert np.abs(vals - ovals).max() / scale < 1e-13
+    assert np.abs(vals - P["vals"]).max() / scale < 1e-13
+    x = np.random.default_rng(1).uniform(-1, 1, P["ndof"])
+    dx, dy = dvec(x), N.DeviceVector(P["ndof"], 8)
+    A.mult(dx, dy)
+    assert relerr_inf(dy.download(), oracle.spmv(orp, ocols, ovals, x)) < 1e-13
+
+
+def test_assembly_deterministic_and_kappa_one(device):
+    say("[assemble] determinism, kappa = 1")
+    P = problem(3, 4, 3)
+    M = mf_operator(P)
+    v1 = N.AssembleCSRDevice(M, P["rowptr"], P["cols"]).csr_values()
+    v2 = N.AssembleCSRDevice(M, P["rowptr"], P["cols"]).csr_values()
+    assert np.array_equal(v1, v2)
+    # A 1 = 0 without boundary conditions: the constant 1 is the sum of the
+    # vertex functions (dofs [0, nv)), every other dof coefficient 0
+    one = np.zeros(P["ndof"])
+    one[:P["mesh"].nv] = 1.0
+    rows = np.repeat(np.arange(P["ndof"]), np.diff(P["rowptr"]))
+    a1 = np.bincount(rows, weights=v1 * one[P["cols"]], minlength=P["ndof"])
+    assert np.abs(a1).max() < 1e-13 * np.abs(v1).max()
+
+
+def test_assembly_rejects_short_pattern(device):
+    say("[assemble] pattern without an element pair")
+    P = problem(2, 3, 2)
+    rp, cols = P["rowptr"].copy(), P["cols"].copy()
+    # drop the last column of row 0 (an element neighbour of dof 0)
+    keep = np.ones(len(cols), dtype=bool)
+    keep[rp[1] - 1] = False
+    cols2 = cols[keep]
+    rp2 = rp.copy()
+    rp2[1:] -= 1
+    with pytest.raises(RuntimeError, match="lacks a pair"):
+        N.AssembleCSRDevice(mf_operator(P), rp2, cols2)
+
+
+def test_bilinearform_device_assembly(device):
+    """BilinearForm(..., device_assembly=True) gives the host assembly's matrix."""
+    import ngsolve_amd as ngs
+    say("[assemble] BilinearForm device_assembly")
+    m = ngs.MakeStructured3DMesh(nx=3)
+    fes = ngs.H1(m, order=3, dirichlet=".*")
+    u, v = fes.TnT()
+    kappa = ngs.ElementCoefficient(np.exp(np.random.default_rng(2).standard_normal(m.ne)))
+    form = kappa * ngs.grad(u) * ngs.grad(v) * ngs.dx(bonus_intorder=2)
+    a_host = ngs.BilinearForm(form).Assemble()
+    a_dev = ngs.BilinearForm(form, device_assembly=True).Assemble()
+    vh, vd = np.asarray(a_host.mat.vals), np.asarray(a_dev.mat.vals)
+    assert vh.shape == vd.shape
+    assert np.abs(vh - vd).max() < 1e-13 * np.abs(vh).max()
+    assert abs(a_dev.mat[0, 0] - a_host.mat[0, 0]) < 1e-13 * abs(a_host.mat[0, 0])
