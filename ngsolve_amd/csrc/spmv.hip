@@ -181,8 +181,10 @@ __global__ __launch_bounds__(kStreamBlock) void CsrStreamKernel(
 // R lanes of one slot read one column number and one x entry); row j = run R
 // + q then sums the slots of its run at stride R.  A block holding more
 // than NNZ values is a single row (R = 1) that the whole workgroup strides.
+// NNZ <= 2048: at most 64 VGPRs, so 8 workgroups fit a CU (72 registers
+// allowed 7; 394 vs 398 us at 32^3 p=4)
 template <int NNZ, bool ACCUM, bool DOT>
-__global__ __launch_bounds__(kStreamBlock) void CsrRunsKernel(
+__global__ __launch_bounds__(kStreamBlock, NNZ <= 2048 ? 8 : 4) void CsrRunsKernel(
     const int* __restrict__ run_row, const int* __restrict__ run_ptr, const int* __restrict__ val_ptr,
     const int* __restrict__ rcols, const double* __restrict__ rvals, const double* __restrict__ x,
     double* __restrict__ y, const int* __restrict__ blk_run, const int* __restrict__ blk_len, int nblocks,
