@@ -2,7 +2,7 @@
 
 bmat.npz     reference gradients of class 0 for trig p=3 (12-point rule) and
              tet p=4 (125-point rule): [Q][nd][dim]
-config1.npz  config-1 CSR pattern (64x64 trigs, p=3) and y = A x for
+trig64_p3.npz  CSR pattern of the 64x64-square, 2-trigs-per-square, p=3 space and y = A x for
              x ~ U[-1,1] from numpy's default_rng(0)
 
 Run from the repository root:  python tests/golden/make_golden.py
@@ -34,7 +34,7 @@ def main():
     x = np.random.default_rng(0).uniform(-1, 1, pr.ndof)
     import scipy.sparse as sp
     y = sp.csr_matrix((v, c, rp), shape=(pr.ndof, pr.ndof)) @ x
-    np.savez_compressed(os.path.join(HERE, "config1.npz"), rowptr=rp, cols=c, y=y)
+    np.savez_compressed(os.path.join(HERE, "trig64_p3.npz"), rowptr=rp, cols=c, y=y)
 
 
 if __name__ == "__main__":

@@ -17,9 +17,9 @@ def _jacobi(P):
 
 
 @pytest.mark.parametrize("chunk", [16, 0])
-def test_cg_config1(device, chunk):
+def test_cg_trig64_p3(device, chunk):
     """64x64 trigs, p=3, kappa=1, f=1, Jacobi CG to 1e-10."""
-    say(f"[cg] config 1 chunk={chunk}")
+    say(f"[cg] 64x64 trigs p=3 chunk={chunk}")
     P = problem(2, 64, 3, order=6)
     f = N.AssembleLinearForm(P["fes"], P["ir"], np.ones((P["mesh"].ne, len(P["ir"]))))
     inv, diag = _jacobi(P)

@@ -29,8 +29,8 @@ def test_spmv_matches_oracle(device, dim, n, p):
 
 
 def test_spmv_oracle_assembled_matrix(device):
-    """The device SpMV applied to the oracle's own assembly (config-1 mesh)."""
-    say("[spmv] oracle-assembled config 1")
+    """The device SpMV applied to the oracle's own assembly (64x64 trigs, p=3)."""
+    say("[spmv] oracle-assembled 64x64 trigs p=3")
     pr = O.Problem(O.structured_2d(64), 3, intorder=6)
     rp, c, v = pr.assemble()
     A = N.CreateCSR(pr.ndof, pr.ndof, rp, c, v)

@@ -76,9 +76,9 @@ def test_golden_reference_gradients():
         assert np.abs(c0["bmat"] - g[key]).max() < 1e-13
 
 
-def test_golden_config1_pattern_and_apply():
-    """Config 1 (64x64 trigs, p=3): pattern digest, values sample and y = A x."""
-    g = np.load(os.path.join(GOLDEN, "config1.npz"))
+def test_golden_trig64_p3_pattern_and_apply():
+    """64x64 trigs, p=3: pattern digest, values sample and y = A x."""
+    g = np.load(os.path.join(GOLDEN, "trig64_p3.npz"))
     m = N.MakeStructured2DMesh(64, 64)
     fes = N.H1Space(m, 3, ALL)
     assert fes.ndof == 37249 and int(fes.freedofs.sum()) == 36481
