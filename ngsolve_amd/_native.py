@@ -10,7 +10,7 @@ try:
 except ImportError as e:  # pragma: no cover - exercised only when unbuilt
     raise ImportError(
         "ngsolve_amd native modules are not built (libngsamd.so / _ngsamd*.so); "
-        "run `python -m ngsolve_amd.build` first"
+        "run `python ngsolve_amd/build.py` first"
     ) from e
 
 LIBRARY_PATH = os.path.join(_HERE, "libngsamd.so")

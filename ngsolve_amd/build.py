@@ -3,7 +3,8 @@
 libngsamd.so    HIP kernels + C ABI (hipcc, gfx950)
 _ngsamd*.so     pybind11 host module (g++): FE setup + device handles
 
-Run:  python -m ngsolve_amd.build   (or __graft_entry__.build())
+Run:  python ngsolve_amd/build.py   (or __graft_entry__.build()); not as
+`python -m ngsolve_amd.build`, which would import the unbuilt package first.
 """
 import os
 import subprocess
