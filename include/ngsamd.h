@@ -104,6 +104,17 @@ int ngsamd_csr_values(ngsamd_op_t op, double* h_vals);
  * h_freedofs may be NULL (all free). */
 int ngsamd_jacobi_create(ngsamd_op_t* op, int64_t n, const double* h_diag, const uint8_t* h_freedofs);
 int ngsamd_jacobi_create_device(ngsamd_op_t* op, int64_t n, const double* d_diag, const uint8_t* h_freedofs);
+/* Block-Jacobi preconditioner y += s sum_b P_b^T inv(A_bb) P_b x over the
+ * blocks b (h_block_ptr[nblocks + 1] offsets into h_block_dofs; each block's
+ * dofs are sorted, blocks may overlap).  A_bb is taken from the host CSR
+ * (int64 row pointers, sorted int32 columns, FP64 values; 0 where the
+ * pattern has no entry) and inverted on the host; the apply runs on the
+ * device without atomics.  BlockJacobiPrecond (linalg/blockjacobi.cpp:390,
+ * 404, 498, 596-632) and its device copy DevBlockJacobiMatrix
+ * (ngscuda/dev_blockjacobi.cpp:92-131).  Accepted by ngsamd_cg_solve. */
+int ngsamd_blockjacobi_create(ngsamd_op_t* op, int64_t n, const int64_t* h_rowptr, const int* h_cols,
+                              const double* h_vals, int64_t nblocks, const int64_t* h_block_ptr,
+                              const int* h_block_dofs);
 
 /* ------------------------------------------------- matrix-free H1 Laplace */
 /* Fused element-batched operator y += s * sum_e P_e^T B^T D_e B P_e x with
@@ -153,7 +164,8 @@ int ngsamd_mf_assemble_csr(ngsamd_op_t mf, const int64_t* h_rowptr, const int* h
  * CGSolver (python/krylovspace.py:227-291): stop when
  * sqrt|<w,d>| <= max(tol*sqrt|<w0,d0>|, atol) or after maxiter checks; the
  * iteration count includes the initial check.  pre must be a Jacobi
- * operator (fused) or any operator (generic path).  Scalars, dots and
+ * operator (applied inside the update kernel) or a block-Jacobi operator
+ * (applied between the update and the direction kernels).  Scalars, dots and
  * convergence tests stay on the device; the iteration is replayed from a
  * captured HIP graph in chunks of `chunk` iterations (0 = no graph).
  * h_residuals (may be NULL) receives up to maxiter residuals. */

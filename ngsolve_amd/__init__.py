@@ -11,14 +11,14 @@ from ._native import ensure_device
 from .comp import (H1, BilinearForm, CoefficientFunction, DevMatrixFree, ElementCoefficient, GridFunction,
                    LinearForm, Mesh, MFOpts, dx, grad, x, y, z)
 from .krylovspace import CG, CGSolver, DevCGSolver, IRCGSolver, LinearSolver
-from .la import (BaseMatrix, BaseVector, BitArray, CreateVVector, DeviceMatrix, DevSparseMatrix, InnerProduct,
-                 JacobiPrecond, Norm, Projector)
+from .la import (BaseMatrix, BaseVector, BitArray, BlockJacobiPrecond, CreateVVector, DeviceMatrix, DevSparseMatrix,
+                 InnerProduct, JacobiPrecond, Norm, Projector)
 from .meshes import MakeStructured2DMesh, MakeStructured3DMesh
 
 __all__ = [
     "H1", "BilinearForm", "CoefficientFunction", "DevMatrixFree", "ElementCoefficient", "GridFunction",
     "LinearForm", "Mesh", "MFOpts", "dx", "grad", "x", "y", "z", "CG", "CGSolver", "DevCGSolver", "IRCGSolver",
     "LinearSolver", "BaseMatrix", "BaseVector", "BitArray", "CreateVVector", "DeviceMatrix", "DevSparseMatrix",
-    "InnerProduct", "JacobiPrecond", "Norm", "Projector", "MakeStructured2DMesh", "MakeStructured3DMesh",
+    "InnerProduct", "JacobiPrecond", "BlockJacobiPrecond", "Norm", "Projector", "MakeStructured2DMesh", "MakeStructured3DMesh",
     "ensure_device",
 ]

@@ -221,6 +221,28 @@ class H1:
     def GetDofNrs(self, el):
         return tuple(int(d) for d in self.space.el_dofs[el])
 
+    def CreateSmoothingBlocks(self, blocktype=None):
+        """Per-node smoothing blocks of the free dofs for BlockJacobiPrecond:
+        H1HighOrderFESpace::CreateSmoothingBlocks (comp/h1hofespace.cpp:1478-1530)
+        block types 1 (2D: one block per vertex, edge and element interior)
+        and 3 (3D: vertex, edge, face and cell); non-free dofs are filtered
+        out, so blocks on the Dirichlet boundary may be empty."""
+        dim = self.mesh.ngmesh.dim
+        if blocktype is None:
+            blocktype = 1 if dim == 2 else 3
+        if (dim, blocktype) not in ((2, 1), (3, 3)):
+            raise ValueError("only the per-node block types 1 (2D) and 3 (3D) are provided")
+        sp = self.space
+        nv = self.mesh.ngmesh.nv
+        bounds = [np.arange(nv + 1, dtype=np.int64)]
+        for first in ((sp.first_edge_dof, sp.first_el_dof) if dim == 2 else
+                      (sp.first_edge_dof, sp.first_face_dof, sp.first_el_dof)):
+            bounds.append(np.asarray(first, dtype=np.int64)[1:])
+        starts = np.concatenate(bounds)
+        free = sp.freedofs.astype(bool)
+        dofs = np.arange(self.ndof, dtype=np.int32)
+        return [b[free[b]] for b in np.split(dofs, starts[1:-1])]
+
     def TrialFunction(self):
         return _Proxy(self, True)
 

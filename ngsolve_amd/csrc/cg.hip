@@ -127,7 +127,7 @@ __device__ __forceinline__ void CheckResidual(CGScalars* S, double* residuals, d
   if (S->it >= S->maxiter || res <= S->final_res) S->done = 1;
 }
 
-// w = C d; s = w; partials of <w,d>
+// w = C d (inv null: w already holds C d); s = w; partials of <w,d>
 template <class V>
 __global__ __launch_bounds__(kCgBlock) void CgInitKernel(const V* __restrict__ inv, const V* __restrict__ d,
                                                          V* __restrict__ w, V* __restrict__ s, int64_t n,
@@ -136,7 +136,7 @@ __global__ __launch_bounds__(kCgBlock) void CgInitKernel(const V* __restrict__ i
   double acc = 0.0;
   for (int64_t i = blockIdx.x * (int64_t)kCgBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kCgBlock) {
     const V di = d[i];
-    const V wi = inv[i] * di;
+    const V wi = inv ? inv[i] * di : w[i];
     w[i] = wi;
     s[i] = wi;
     acc += (double)wi * (double)di;
@@ -157,7 +157,9 @@ __global__ __launch_bounds__(kCgBlock) void CgInitCheckKernel(const double* __re
 
 // as = <s, A s> from the partials pa of the operator kernel; breakdown when
 // as == 0 or wd == 0, else alpha = wd / as and u += alpha s; d -= alpha w;
-// w = C d; partials of <w,d> into pw.  P is the iteration parity.
+// w = C d; partials of <w,d> into pw.  P is the iteration parity.  inv null
+// (a preconditioner other than Jacobi): only the two updates; C d and its
+// partials follow in their own kernels.
 template <class V>
 __global__ __launch_bounds__(kCgBlock) void CgUpdateKernel(const V* __restrict__ inv, V* __restrict__ u,
                                                            V* __restrict__ d, V* __restrict__ w,
@@ -181,11 +183,13 @@ __global__ __launch_bounds__(kCgBlock) void CgUpdateKernel(const V* __restrict__
     u[i] += alv * si;
     const V di = d[i] - alv * wi;
     d[i] = di;
-    const V zi = inv[i] * di;
-    w[i] = zi;
-    acc += (double)zi * (double)di;
+    if (inv) {
+      const V zi = inv[i] * di;
+      w[i] = zi;
+      acc += (double)zi * (double)di;
+    }
   }
-  StorePartial<kCgBlock>(acc, pw, lds);
+  if (inv) StorePartial<kCgBlock>(acc, pw, lds);
 }
 
 // wdn = <w,d> from the partials pw; beta = wdn / wd; s = beta s + w.
@@ -285,14 +289,32 @@ static int EnqueueMatDot(ngsamd_op_t A, const V* s, V* w, int64_t n, CGScalars* 
   return g;
 }
 
-// One CG iteration of parity P on the stream (graph-capturable: kernels only).
+// w = C d for a preconditioner other than Jacobi (block Jacobi); its kernels
+// return at once when *stop != 0
 template <class V>
-static void EnqueueIteration(ngsamd_op_t A, const V* inv, V* u, V* d, V* w, V* s, int64_t n, CGScalars* dS,
-                             double* residuals, int P, hipStream_t st) {
+static void EnqueuePrecond(ngsamd_op_t pre, const V* d, V* w, const int* stop, hipStream_t st) {
+  if constexpr (sizeof(V) == 8) {
+    NGSAMD_REQUIRE(pre->kind == OpKind::BlockJacobi, "cg: unsupported preconditioner");
+    static_cast<BlockJacobiOp*>(pre)->MultStop(d, w, stop, st);
+  } else {
+    throw Error(NGSAMD_E_ARG, "cg: FP32 vectors need a Jacobi preconditioner");
+  }
+}
+
+// One CG iteration of parity P on the stream (graph-capturable: kernels only).
+// inv null: pre is applied between the update and the direction kernels.
+template <class V>
+static void EnqueueIteration(ngsamd_op_t A, const V* inv, ngsamd_op_t pre, V* u, V* d, V* w, V* s, int64_t n,
+                             CGScalars* dS, double* residuals, int P, hipStream_t st) {
   const int na = EnqueueMatDot<V>(A, s, w, n, dS, st);
   const int g = CgGrid(n);
   CgUpdateKernel<V><<<g, kCgBlock, 0, st>>>(inv, u, d, w, s, n, dS, P, PartialsA(), na, PartialsW());
   NGSAMD_CHECK_LAUNCH();
+  if (!inv) {
+    EnqueuePrecond<V>(pre, d, w, &dS->done, st);
+    CgDotKernel<V><<<g, kCgBlock, 0, st>>>(w, d, n, &dS->done, PartialsW());
+    NGSAMD_CHECK_LAUNCH();
+  }
   CgDirectionKernel<V><<<g, kCgBlock, 0, st>>>(s, w, n, dS, P, PartialsW(), g, residuals);
   NGSAMD_CHECK_LAUNCH();
 }
@@ -314,14 +336,15 @@ struct CgWork {
 // Iterates from an initialised state (d = f - A u already set, u set) until
 // the device flag is raised.  Uses a graph of `chunk` iterations if possible.
 template <class V>
-static CgResult IterateCg(ngsamd_op_t A, const V* inv, V* u, CgWork<V>& W, int64_t n, double tol, double atol,
-                          int maxiter, int chunk, hipStream_t st) {
+static CgResult IterateCg(ngsamd_op_t A, const V* inv, ngsamd_op_t pre, V* u, CgWork<V>& W, int64_t n, double tol,
+                          double atol, int maxiter, int chunk, hipStream_t st) {
   CGScalars hS{};
   hS.tol = tol;
   hS.atol = atol;
   hS.maxiter = maxiter;
   NGSAMD_HIP(hipMemcpyAsync(W.S.p, &hS, sizeof(CGScalars), hipMemcpyHostToDevice, st));
   const int g = CgGrid(n);
+  if (!inv) EnqueuePrecond<V>(pre, W.d.p, W.w.p, nullptr, st);
   CgInitKernel<V><<<g, kCgBlock, 0, st>>>(inv, W.d.p, W.w.p, W.s.p, n, PartialsW());
   NGSAMD_CHECK_LAUNCH();
   CgInitCheckKernel<<<1, kCgBlock, 0, st>>>(PartialsW(), g, W.S.p, W.residuals.p);
@@ -346,7 +369,7 @@ static CgResult IterateCg(ngsamd_op_t A, const V* inv, V* u, CgWork<V>& W, int64
       bool ok = true;
       try {
         for (int k = 0; k < chunk; k++)
-          EnqueueIteration<V>(A, inv, u, W.d.p, W.w.p, W.s.p, n, W.S.p, W.residuals.p, k & 1, st);
+          EnqueueIteration<V>(A, inv, pre, u, W.d.p, W.w.p, W.s.p, n, W.S.p, W.residuals.p, k & 1, st);
       } catch (...) {
         ok = false;
       }
@@ -375,7 +398,8 @@ static CgResult IterateCg(ngsamd_op_t A, const V* inv, V* u, CgWork<V>& W, int64
   long long enqueued = 0;  // iterations enqueued without a graph (parity)
   auto enqueue = [&](int slot) {
     if (use_graph) NGSAMD_HIP(hipGraphLaunch(exec, st));
-    else EnqueueIteration<V>(A, inv, u, W.d.p, W.w.p, W.s.p, n, W.S.p, W.residuals.p, (int)(enqueued++ & 1), st);
+    else
+      EnqueueIteration<V>(A, inv, pre, u, W.d.p, W.w.p, W.s.p, n, W.S.p, W.residuals.p, (int)(enqueued++ & 1), st);
     NGSAMD_HIP(hipMemcpyAsync(&pin[slot], W.S.p, sizeof(CGScalars), hipMemcpyDeviceToHost, st));
   };
   hipEvent_t ev[2];
@@ -413,9 +437,11 @@ static CgResult IterateCg(ngsamd_op_t A, const V* inv, V* u, CgWork<V>& W, int64
   return res;
 }
 
-static void CheckCgOperators(ngsamd_op_t A, ngsamd_op_t pre) {
+static void CheckCgOperators(ngsamd_op_t A, ngsamd_op_t pre, bool jacobi_only) {
   NGSAMD_REQUIRE(A && pre, "cg: null operator");
-  NGSAMD_REQUIRE(pre->kind == OpKind::Jacobi, "cg: the device CG needs a Jacobi preconditioner");
+  NGSAMD_REQUIRE(pre->kind == OpKind::Jacobi || (!jacobi_only && pre->kind == OpKind::BlockJacobi),
+                 jacobi_only ? "cg: this solver needs a Jacobi preconditioner"
+                             : "cg: the device CG needs a Jacobi or block-Jacobi preconditioner");
   NGSAMD_REQUIRE(A->height == A->width && pre->height == A->height, "cg: operator shapes do not match");
 }
 
@@ -447,7 +473,7 @@ extern "C" int ngsamd_cg_solve(ngsamd_op_t A, ngsamd_op_t pre, const double* f, 
                                double* h_residuals) {
   return Guard([&] {
     RequireInit();
-    CheckCgOperators(A, pre);
+    CheckCgOperators(A, pre, false);
     NGSAMD_REQUIRE(maxiter >= 1, "cg: maxiter must be >= 1");
     const int64_t n = A->height;
     hipStream_t st = Stream();
@@ -456,8 +482,8 @@ extern "C" int ngsamd_cg_solve(ngsamd_op_t A, ngsamd_op_t pre, const double* f, 
     if (initialize) LaunchFill(u, n, 0.0, st);
     if (n) NGSAMD_HIP(hipMemcpyAsync(W.d.p, f, n * sizeof(double), hipMemcpyDeviceToDevice, st));
     if (!initialize) A->MultAdd(-1.0, u, W.d.p, st);
-    const double* inv = static_cast<JacobiOp*>(pre)->invdiag.p;
-    CgResult r = IterateCg<double>(A, inv, u, W, n, tol, atol, maxiter, chunk, st);
+    const double* inv = pre->kind == OpKind::Jacobi ? static_cast<JacobiOp*>(pre)->invdiag.p : nullptr;
+    CgResult r = IterateCg<double>(A, inv, pre, u, W, n, tol, atol, maxiter, chunk, st);
     if (h_iterations) *h_iterations = r.iterations;
     if (h_residuals && r.iterations > 0) {
       const int cnt = r.iterations < maxiter ? r.iterations : maxiter;
@@ -471,7 +497,7 @@ extern "C" int ngsamd_ir_cg_solve(ngsamd_op_t A, ngsamd_op_t inner, ngsamd_op_t 
                                   double tol, double inner_tol, int max_outer, int max_inner, double* h_stats) {
   return Guard([&] {
     RequireInit();
-    CheckCgOperators(A, pre);
+    CheckCgOperators(A, pre, true);
     NGSAMD_REQUIRE(inner && inner->kind == OpKind::MatrixFree && inner->height == A->height,
                    "ir_cg: inner operator must be matrix-free of the same size");
     NGSAMD_REQUIRE(max_outer >= 1 && max_inner >= 1, "ir_cg: iteration limits must be >= 1");
@@ -529,7 +555,7 @@ extern "C" int ngsamd_ir_cg_solve(ngsamd_op_t A, ngsamd_op_t inner, ngsamd_op_t 
       // the last refinement step only needs to gain the factor still missing
       double itol = inner_tol;
       if (fn > 0) itol = std::max(itol, std::min(0.5, 0.3 * tol * fn / rn));
-      CgResult ir = IterateCg<float>(Ai, inv32.p, c32.p, W, n, itol, -1.0, max_inner, chunk, st);
+      CgResult ir = IterateCg<float>(Ai, inv32.p, pre, c32.p, W, n, itol, -1.0, max_inner, chunk, st);
       inner_total += ir.iterations;
       // u += ||r|| c; r = f - A u (FP64)
       AddScaledF32Kernel<<<GridFor(n, 256, 8192), 256, 0, st>>>(u, c32.p, n, scal.p + 1);

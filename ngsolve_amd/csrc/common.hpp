@@ -93,7 +93,7 @@ struct DevBuf {
   }
 };
 
-enum class OpKind { CSR, Jacobi, MatrixFree, Batch };
+enum class OpKind { CSR, Jacobi, MatrixFree, Batch, BlockJacobi };
 
 }  // namespace ngsamd
 

@@ -1,5 +1,5 @@
-// Device CSR operator and Jacobi preconditioner types shared by the SpMV and
-// CG translation units.
+// Device CSR operator and Jacobi / block-Jacobi preconditioner types shared
+// by the SpMV, preconditioner and CG translation units.
 #pragma once
 #include <vector>
 
@@ -53,6 +53,29 @@ struct JacobiOp : ngsamd_op_s {
   JacobiOp(int64_t n, const double* diag, bool diag_on_device, const uint8_t* freedofs);
   void MultAdd(double s, const double* x, double* y, hipStream_t st) override;
   void Mult(const double* x, double* y, hipStream_t st) override;
+};
+
+// Block Jacobi: y += s sum_b P_b^T inv(A_bb) P_b x.  Block rows are numbered
+// block after block (each block's dofs sorted); the inverses are stored row
+// by row.  The apply forms z[t] = (inv_b x_b)[r] for every block row t, then
+// each dof sums the z of its rows in row order (blocks may overlap).
+struct BlockJacobiOp : ngsamd_op_s {
+  int64_t nblocks = 0, nrows = 0;
+  DevBuf<int64_t> bptr;   // first block row of each block, nblocks + 1
+  DevBuf<int64_t> vptr;   // first inverse entry of each block, nblocks + 1
+  DevBuf<int> bdofs;      // dof of each block row
+  DevBuf<int> row_block;  // block of each block row
+  DevBuf<double> inv;     // inverses, row-major per block
+  DevBuf<int> gptr;       // per dof: its block rows [gptr[d], gptr[d+1]) of grows
+  DevBuf<int> grows;
+  DevBuf<double> z;       // block-row results (preallocated: the apply allocates nothing)
+  BlockJacobiOp(int64_t n, const int64_t* rp, const int* cols, const double* vals, int64_t nblocks,
+                const int64_t* block_ptr, const int* block_dofs);
+  void MultAdd(double s, const double* x, double* y, hipStream_t st) override;
+  void Mult(const double* x, double* y, hipStream_t st) override;
+  // y = C x; every kernel returns at once when *stop != 0 (CG after convergence)
+  void MultStop(const double* x, double* y, const int* stop, hipStream_t st);
+  void Apply(double s, const double* x, double* y, bool accumulate, const int* stop, hipStream_t st);
 };
 
 }  // namespace ngsamd

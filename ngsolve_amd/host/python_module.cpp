@@ -454,6 +454,18 @@ PYBIND11_MODULE(_ngsamd, m) {
     Check(ngsamd_jacobi_create(&op, diag.size(), diag.data(), fr.empty() ? nullptr : fr.data()));
     return std::make_shared<Operator>(op, "jacobi");
   });
+  m.def("CreateBlockJacobi", [](int64_t n, Arr<int64_t> rowptr, Arr<int> cols, Arr<double> vals,
+                                 Arr<int64_t> block_ptr, Arr<int> block_dofs) {
+    if (rowptr.size() != n + 1 || cols.size() != vals.size() || rowptr.data()[n] != (int64_t)cols.size())
+      throw std::runtime_error("CreateBlockJacobi: inconsistent CSR arrays");
+    const int64_t nb = (int64_t)block_ptr.size() - 1;
+    if (nb < 0 || block_ptr.data()[nb] != (int64_t)block_dofs.size())
+      throw std::runtime_error("CreateBlockJacobi: inconsistent block arrays");
+    ngsamd_op_t op = nullptr;
+    Check(ngsamd_blockjacobi_create(&op, n, rowptr.data(), cols.data(), vals.data(), nb, block_ptr.data(),
+                                    block_dofs.data()));
+    return std::make_shared<Operator>(op, "blockjacobi");
+  });
   m.def("CreateJacobiDevice", [](DeviceVector& diag, py::object freedofs) {
     ngsamd_op_t op = nullptr;
     std::vector<uint8_t> fr;

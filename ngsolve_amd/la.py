@@ -598,6 +598,10 @@ class DevSparseMatrix(DeviceMatrix):
     def CreateSmoother(self, freedofs=None):
         return JacobiPrecond(self.Diagonal(), freedofs)
 
+    def CreateBlockSmoother(self, blocks):
+        """BaseSparseMatrix::CreateBlockJacobiPrecond (Python CreateBlockSmoother)."""
+        return BlockJacobiPrecond(self, blocks)
+
 
 class JacobiPrecond(DeviceMatrix):
     """JacobiPrecond (linalg/jacobi.cpp:41-90): y += s * diag^{-1} x on free dofs."""
@@ -610,6 +614,26 @@ class JacobiPrecond(DeviceMatrix):
         else:
             op = N.CreateJacobi(np.ascontiguousarray(diag, dtype=np.float64), fr)
         super().__init__(op, True)
+
+
+class BlockJacobiPrecond(DeviceMatrix):
+    """BlockJacobiPrecond (linalg/blockjacobi.cpp) on the device:
+    y += s * sum_b P_b^T inv(A_bb) P_b x for the blocks (lists of dof
+    numbers; sorted, may overlap).  The blocks are inverted on the host, the
+    apply runs on the device (DevBlockJacobiMatrix,
+    ngscuda/dev_blockjacobi.cpp)."""
+
+    def __init__(self, mat, blocks):
+        N = ensure_device()
+        blocks = [np.asarray(b, dtype=np.int32).ravel() for b in blocks]
+        bptr = np.zeros(len(blocks) + 1, dtype=np.int64)
+        bptr[1:] = np.cumsum([len(b) for b in blocks])
+        bd = np.concatenate(blocks).astype(np.int32) if blocks else np.zeros(0, dtype=np.int32)
+        op = N.CreateBlockJacobi(int(mat.Height()), np.ascontiguousarray(mat.rowptr, dtype=np.int64),
+                                 np.ascontiguousarray(mat.cols, dtype=np.int32),
+                                 np.ascontiguousarray(mat.vals, dtype=np.float64), bptr, bd)
+        super().__init__(op, True)
+        self.blocks = blocks
 
 
 class Projector(BaseMatrix):

@@ -15,6 +15,9 @@ This module restates, in numpy, the reference's algorithms for:
   CSR pattern + assembly   comp/bilinearform.cpp:552-825, linalg/sparsematrix.cpp:450-975,
                            linalg/sparsematrix_impl.hpp:150-189
   linear form              fem/symbolicintegrator.cpp:1036-1110
+  Jacobi-PCG               python/krylovspace.py:263-291 (Jacobi or any
+                           preconditioner)
+  block Jacobi             linalg/blockjacobi.cpp:390, 404, 498, 596-632
 
 It is deliberately written differently from the product code: shapes are
 evaluated per element (vectorised over elements and points, orientation per
@@ -575,12 +578,41 @@ def jacobi_invdiag(rowptr, cols, vals, free):
     return diag, inv
 
 
+def block_jacobi(rowptr, cols, vals, blocks):
+    """BlockJacobiPrecond (linalg/blockjacobi.cpp): each block's dofs sorted
+    (:390), its diagonal block A[b][b] taken from the matrix, 0 where the
+    pattern has no entry (:404), inverted (:498); MultAdd y += s sum_b
+    P_b^T inv_b P_b x (:596-632).  Returns (inverses, apply) with
+    apply(x) = sum_b P_b^T inv_b P_b x."""
+    blocks = [np.sort(np.asarray(b, dtype=np.int64)) for b in blocks]
+    invs = []
+    for b in blocks:
+        m = np.zeros((len(b), len(b)))
+        for j, r in enumerate(b):
+            s0, e0 = rowptr[r], rowptr[r + 1]
+            k = np.searchsorted(cols[s0:e0], b)
+            ok = (k < e0 - s0)
+            ok[ok] = cols[s0:e0][k[ok]] == b[ok]
+            m[j, ok] = vals[s0:e0][k[ok]]
+        invs.append(np.linalg.inv(m) if len(b) else m)
+
+    def apply(x):
+        y = np.zeros_like(x)
+        for b, inv in zip(blocks, invs):
+            y[b] += inv @ x[b]
+        return y
+
+    return invs, apply
+
+
 def pcg(matvec, invdiag, f, tol=1e-12, maxiter=100, atol=None):
     """NGSolve CGSolver recurrence (python/krylovspace.py:263-291), FP64,
-    pure numpy.  Returns (u, iterations, residuals)."""
+    pure numpy.  invdiag: the Jacobi inverse diagonal, or a callable
+    preconditioner w = C d.  Returns (u, iterations, residuals)."""
+    pre = invdiag if callable(invdiag) else (lambda v: invdiag * v)
     u = np.zeros_like(f)
     d = f - matvec(u)
-    w = invdiag * d
+    w = pre(d)
     s = w.copy()
     wdn = float(w @ d)
     res = []
@@ -603,7 +635,7 @@ def pcg(matvec, invdiag, f, tol=1e-12, maxiter=100, atol=None):
         alpha = wd / as_s
         u += alpha * s
         d -= alpha * w
-        w = invdiag * d
+        w = pre(d)
         wdn = float(w @ d)
         if check(math.sqrt(abs(wdn))):
             break

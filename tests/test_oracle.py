@@ -257,3 +257,28 @@ def test_cg_maxiter_and_zero_rhs():
     assert its == 1 and not u.any()
     u, its, res = oracle.pcg_csr(rp, c, eig, np.ones(50), np.ones(50), tol=1e-30, maxiter=7)
     assert its == 7 and len(res) == 7
+
+
+def test_block_jacobi_special_cases():
+    """Block Jacobi with singleton blocks is Jacobi; one block holding every
+    dof is the exact inverse (A apply(x) = x, and PCG with it stops after one
+    step)."""
+    pr = O.Problem(O.structured_2d(4), 2)
+    rp, c, v = pr.assemble()
+    n = pr.ndof
+    free = np.array(pr.free, dtype=np.uint8)
+    _, inv = O.jacobi_invdiag(rp, c, v, free)
+    fd = np.flatnonzero(free)
+    _, apply1 = O.block_jacobi(rp, c, v, [[d] for d in fd])
+    x = np.random.default_rng(0).uniform(-1, 1, n)
+    assert np.allclose(apply1(x), inv * x, rtol=1e-14, atol=0)
+    A = sp.csr_matrix((v, c, rp), shape=(n, n))
+    # the free-free block, in shuffled order (the oracle sorts each block)
+    _, applyall = O.block_jacobi(rp, c, v, [np.random.default_rng(1).permutation(fd)])
+    z = applyall(x)
+    assert np.allclose((A @ z)[fd], x[fd], rtol=0, atol=1e-12 * np.abs(x).max())
+    assert np.all(z[free == 0] == 0)
+    f = np.zeros(n)
+    f[fd] = 1.0
+    u, its, _ = O.pcg(lambda t: A @ t, applyall, f, tol=1e-12, maxiter=10)
+    assert its == 2 and np.allclose((A @ u)[fd], f[fd], atol=1e-12)
