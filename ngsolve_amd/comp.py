@@ -193,6 +193,59 @@ class _Integral:
 
 # ------------------------------------------------------------------- space
 
+def smoothing_blocks(m, sp, blocktype=4):
+    """Smoothing blocks of the free dofs of the H1 space sp on the mesh m
+    (native objects) for BlockJacobiPrecond, as
+    H1HighOrderFESpace::CreateSmoothingBlocks (comp/h1hofespace.cpp:1478-1620)
+    builds them (non-free dofs filtered out, so some blocks are empty):
+      1  2D: V + E + I   one block per vertex, edge and element interior
+      2  2D: VE + I      per vertex its dof and the dofs of its edges; interiors
+      3  V + E + F + I   one block per node
+      4  VE + F + I      (the default) vertex-edge patches, faces, cells
+    """
+    dim = m.dim
+    if blocktype not in (1, 2, 3, 4) or (dim == 3 and blocktype in (1, 2)):
+        raise ValueError("block types 1 and 2 (2D), 3 and 4 are provided")
+    nv, ned = m.nv, m.nedges
+    fe = np.asarray(sp.first_edge_dof, dtype=np.int64)
+    fi = np.asarray(sp.first_el_dof, dtype=np.int64)
+    ff = np.asarray(sp.first_face_dof, dtype=np.int64) if dim == 3 else np.zeros(1, dtype=np.int64)
+    nfa = len(ff) - 1
+    ni = len(fi) - 1
+
+    def ranges(first, base):
+        """(block, dof) pairs for blocks base + i holding [first[i], first[i+1])."""
+        cnt = np.diff(first)
+        blk = np.repeat(np.arange(len(cnt), dtype=np.int64) + base, cnt)
+        return blk, np.arange(first[0], first[-1], dtype=np.int64)
+
+    parts = []
+    vb, vd = np.arange(nv, dtype=np.int64), np.arange(nv, dtype=np.int64)
+    if blocktype in (2, 4):
+        ev = np.asarray(m.edges, dtype=np.int64).reshape(-1, 2)
+        eb, ed = ranges(fe, 0)  # edge number, dof
+        parts += [(vb, vd), (ev[eb, 0], ed), (ev[eb, 1], ed)]
+        nblk_ve = nv if blocktype == 4 else nv + ned
+    else:
+        eb, ed = ranges(fe, nv)
+        parts += [(vb, vd), (eb, ed)]
+        nblk_ve = nv + ned
+    if dim == 3:
+        fb, fd = ranges(ff, nblk_ve)
+        parts.append((fb, fd))
+    ib, idof = ranges(fi, nblk_ve + (nfa if dim == 3 else 0))
+    parts.append((ib, idof))
+    nblocks = nblk_ve + (nfa if dim == 3 else 0) + ni
+    blk = np.concatenate([p[0] for p in parts])
+    dof = np.concatenate([p[1] for p in parts])
+    keep = sp.freedofs.astype(bool)[dof]
+    blk, dof = blk[keep], dof[keep]
+    order = np.argsort(blk, kind="stable")
+    blk, dof = blk[order], dof[order].astype(np.int32)
+    bounds = np.searchsorted(blk, np.arange(1, nblocks))
+    return np.split(dof, bounds)
+
+
 class H1:
     def __init__(self, mesh, order=1, dirichlet=None, **kwargs):
         if not isinstance(mesh, Mesh):
@@ -221,27 +274,9 @@ class H1:
     def GetDofNrs(self, el):
         return tuple(int(d) for d in self.space.el_dofs[el])
 
-    def CreateSmoothingBlocks(self, blocktype=None):
-        """Per-node smoothing blocks of the free dofs for BlockJacobiPrecond:
-        H1HighOrderFESpace::CreateSmoothingBlocks (comp/h1hofespace.cpp:1478-1530)
-        block types 1 (2D: one block per vertex, edge and element interior)
-        and 3 (3D: vertex, edge, face and cell); non-free dofs are filtered
-        out, so blocks on the Dirichlet boundary may be empty."""
-        dim = self.mesh.ngmesh.dim
-        if blocktype is None:
-            blocktype = 1 if dim == 2 else 3
-        if (dim, blocktype) not in ((2, 1), (3, 3)):
-            raise ValueError("only the per-node block types 1 (2D) and 3 (3D) are provided")
-        sp = self.space
-        nv = self.mesh.ngmesh.nv
-        bounds = [np.arange(nv + 1, dtype=np.int64)]
-        for first in ((sp.first_edge_dof, sp.first_el_dof) if dim == 2 else
-                      (sp.first_edge_dof, sp.first_face_dof, sp.first_el_dof)):
-            bounds.append(np.asarray(first, dtype=np.int64)[1:])
-        starts = np.concatenate(bounds)
-        free = sp.freedofs.astype(bool)
-        dofs = np.arange(self.ndof, dtype=np.int32)
-        return [b[free[b]] for b in np.split(dofs, starts[1:-1])]
+    def CreateSmoothingBlocks(self, blocktype=4):
+        """Blocks of free dofs for BlockJacobiPrecond (smoothing_blocks)."""
+        return smoothing_blocks(self.mesh.ngmesh, self.space, blocktype)
 
     def TrialFunction(self):
         return _Proxy(self, True)

@@ -26,7 +26,8 @@ constexpr int kBjBlock = 256;
 
 // z[t] = sum_c inv_b[r][c] x[dof(b, c)] for block row t = row r of block b;
 // 4 columns at a time from clamped indices so the loads are in flight
-// together
+// together.  Inverses are stored column by column, so the rows of one block
+// (neighbouring threads) read neighbouring words.
 __global__ __launch_bounds__(kBjBlock) void BjRowsKernel(const int64_t* __restrict__ bptr,
                                                          const int64_t* __restrict__ vptr,
                                                          const int* __restrict__ bdofs,
@@ -39,7 +40,7 @@ __global__ __launch_bounds__(kBjBlock) void BjRowsKernel(const int64_t* __restri
     const int b = row_block[t];
     const int64_t b0 = bptr[b];
     const int sz = (int)(bptr[b + 1] - b0);
-    const double* ir = inv + vptr[b] + (t - b0) * sz;
+    const double* ir = inv + vptr[b] + (t - b0);
     const int* bd = bdofs + b0;
     double acc = 0.0;
     for (int c = 0; c < sz; c += 4) {
@@ -47,7 +48,7 @@ __global__ __launch_bounds__(kBjBlock) void BjRowsKernel(const int64_t* __restri
 #pragma unroll
       for (int j = 0; j < 4; j++) {
         const int cc = min(c + j, sz - 1);
-        a[j] = ir[cc];
+        a[j] = ir[(int64_t)cc * sz];
         xv[j] = x[bd[cc]];
       }
 #pragma unroll
@@ -112,6 +113,9 @@ BlockJacobiOp::BlockJacobiOp(int64_t n, const int64_t* rp, const int* cols, cons
       }
     }
     if (sz && !InvertInPlace(m, sz)) singular = 1;
+    // stored column by column (the apply's access order)
+    for (int j = 0; j < sz; j++)
+      for (int k = j + 1; k < sz; k++) std::swap(m[(size_t)j * sz + k], m[(size_t)k * sz + j]);
   }
   NGSAMD_REQUIRE(!singular, "ngsamd_blockjacobi_create: a diagonal block is singular");
   std::vector<int> rb(std::max<int64_t>(nrows, 1), 0);

@@ -56,8 +56,8 @@ struct JacobiOp : ngsamd_op_s {
 };
 
 // Block Jacobi: y += s sum_b P_b^T inv(A_bb) P_b x.  Block rows are numbered
-// block after block (each block's dofs sorted); the inverses are stored row
-// by row.  The apply forms z[t] = (inv_b x_b)[r] for every block row t, then
+// block after block (each block's dofs sorted); the inverses are stored
+// column by column.  The apply forms z[t] = (inv_b x_b)[r] for every block row t, then
 // each dof sums the z of its rows in row order (blocks may overlap).
 struct BlockJacobiOp : ngsamd_op_s {
   int64_t nblocks = 0, nrows = 0;
@@ -65,7 +65,7 @@ struct BlockJacobiOp : ngsamd_op_s {
   DevBuf<int64_t> vptr;   // first inverse entry of each block, nblocks + 1
   DevBuf<int> bdofs;      // dof of each block row
   DevBuf<int> row_block;  // block of each block row
-  DevBuf<double> inv;     // inverses, row-major per block
+  DevBuf<double> inv;     // inverses, column-major per block
   DevBuf<int> gptr;       // per dof: its block rows [gptr[d], gptr[d+1]) of grows
   DevBuf<int> grows;
   DevBuf<double> z;       // block-row results (preallocated: the apply allocates nothing)

@@ -1,11 +1,12 @@
 """Block-Jacobi preconditioner on the device (ngsamd_blockjacobi_create)
 against the oracle's restatement of BlockJacobiPrecond
 (oracle/fe_oracle.py block_jacobi): the apply within 1e-12 of the largest
-entry for per-node blocks (the reference's H1 smoothing blocks), overlapping
-blocks and singleton blocks (= Jacobi); Mult / MultAdd semantics; the device
-CG with the block preconditioner against the oracle's PCG with the same
-preconditioner (iterations within 1, solution within 1e-10); the Python
-CGSolver with it; singular blocks refused."""
+entry for per-node blocks, random overlapping blocks and singleton blocks
+(= Jacobi); Mult / MultAdd semantics; the device CG with the block
+preconditioner (per-node blocks and the reference's H1 smoothing block types)
+against the oracle's PCG with the same preconditioner (iterations within 1,
+solution within 1e-10); the Python CGSolver with it; singular blocks
+refused."""
 import numpy as np
 import pytest
 
@@ -90,6 +91,29 @@ def test_devcg_blockjacobi_matches_oracle(device, dim, n, p, chunk):
     assert abs(its - its_ref) <= 1
     u = du.download()
     assert np.abs(u - u_ref).max() <= 1e-10 * np.abs(u_ref).max()
+
+
+@pytest.mark.parametrize("dim,p,blocktype", [(2, 4, 1), (2, 4, 2), (2, 3, 4), (3, 3, 3), (3, 4, 4)])
+def test_devcg_smoothing_blocks_match_oracle(device, dim, p, blocktype):
+    """The reference's H1 smoothing blocks (type 4, its default, overlaps:
+    vertex-edge patches) in the device CG against the oracle's PCG."""
+    say(f"[blockjacobi cg] smoothing blocks type {blocktype} dim={dim} p={p}")
+    m = ngs.MakeStructured2DMesh(nx=12, ny=12) if dim == 2 else ngs.MakeStructured3DMesh(nx=4)
+    fes = ngs.H1(m, order=p, dirichlet=".*")
+    u, v = fes.TnT()
+    kappa = ngs.ElementCoefficient(np.exp(np.random.default_rng(4).standard_normal(m.ne)))
+    a = ngs.BilinearForm(kappa * ngs.grad(u) * ngs.grad(v) * ngs.dx).Assemble()
+    blocks = fes.CreateSmoothingBlocks(blocktype)
+    pre = a.mat.CreateBlockSmoother(blocks)
+    free = fes.space.freedofs.astype(bool)
+    f = np.where(free, np.random.default_rng(3).uniform(0, 1, fes.ndof), 0.0)
+    df, du = dvec(f), N.DeviceVector(fes.ndof, 8)
+    its, _ = N.CGSolve(a.mat.op, pre.op, df, du, True, 1e-10, -1.0, 3000, 16)
+    rp, c, vals = a.mat.rowptr, a.mat.cols, a.mat.vals
+    _, apply = O.block_jacobi(rp, c, vals, blocks)
+    u_ref, its_ref, _ = O.pcg(lambda t: oracle.spmv(rp, c, vals, t), apply, f, tol=1e-10, maxiter=3000)
+    assert abs(its - its_ref) <= 1
+    assert np.abs(du.download() - u_ref).max() <= 1e-10 * np.abs(u_ref).max()
 
 
 def test_python_cgsolver_with_block_smoother(device):

@@ -1,0 +1,49 @@
+"""H1.CreateSmoothingBlocks against a direct restatement of the block types
+of H1HighOrderFESpace::CreateSmoothingBlocks (comp/h1hofespace.cpp:1478-1620)
+on small meshes: every block's dof set, the block count (empty blocks
+included) and the free-dof filtering."""
+import numpy as np
+import pytest
+
+import ngsolve_amd as ngs
+
+
+def _expected(fes, blocktype):
+    m, sp = fes.mesh.ngmesh, fes.space
+    free = sp.freedofs.astype(bool)
+    fe, fi = np.asarray(sp.first_edge_dof), np.asarray(sp.first_el_dof)
+    ff = np.asarray(sp.first_face_dof) if m.dim == 3 else np.zeros(1, dtype=np.int64)
+    edges = np.asarray(m.edges).reshape(-1, 2)
+    edge_dofs = [list(range(fe[e], fe[e + 1])) for e in range(m.nedges)]
+    vblocks = []
+    for v in range(m.nv):
+        b = [v]
+        if blocktype in (2, 4):
+            for e in range(m.nedges):
+                if v in edges[e]:
+                    b += edge_dofs[e]
+        vblocks.append(b)
+    blocks = list(vblocks)
+    if blocktype in (1, 3):
+        blocks += edge_dofs
+    elif blocktype == 2:
+        blocks += [[] for _ in range(m.nedges)]
+    if m.dim == 3:
+        blocks += [list(range(ff[f], ff[f + 1])) for f in range(len(ff) - 1)]
+    blocks += [list(range(fi[i], fi[i + 1])) for i in range(len(fi) - 1)]
+    return [sorted(d for d in b if free[d]) for b in blocks]
+
+
+@pytest.mark.parametrize("dim,blocktype", [(2, 1), (2, 2), (2, 3), (2, 4), (3, 3), (3, 4)])
+@pytest.mark.parametrize("p", [1, 3, 4])
+def test_blocks_match_restatement(dim, blocktype, p):
+    m = ngs.MakeStructured2DMesh(nx=3, ny=2) if dim == 2 else ngs.MakeStructured3DMesh(nx=2)
+    fes = ngs.H1(m, order=p, dirichlet="left|bottom" if dim == 2 else ".*")
+    got = [sorted(b.tolist()) for b in fes.CreateSmoothingBlocks(blocktype)]
+    assert got == _expected(fes, blocktype)
+
+
+def test_unknown_blocktype_refused():
+    fes = ngs.H1(ngs.MakeStructured3DMesh(nx=1), order=2)
+    with pytest.raises(ValueError):
+        fes.CreateSmoothingBlocks(1)
