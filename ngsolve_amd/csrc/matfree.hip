@@ -1514,6 +1514,8 @@ __global__ __launch_bounds__(64) void AssembleCsrKernel(
   extern __shared__ __align__(16) unsigned char lds_raw[];
   double* lval = reinterpret_cast<double*>(lds_raw);
   int* lcol = reinterpret_cast<int*>(lds_raw + sizeof(double) * (size_t)max_len);
+  __shared__ double cg[64 * NC];  // per contribution of a chunk: kappa_e times the packed metric
+  __shared__ int ce[64], ci[64];  // its element (table row) and local dof | class << 16
   const int lane = threadIdx.x;
   for (int64_t t = blockIdx.x; t < nrows; t += gridDim.x) {
     const int r = g_order[t];
@@ -1535,33 +1537,60 @@ __global__ __launch_bounds__(64) void AssembleCsrKernel(
       lval[k] = 0.0;
     }
     __syncthreads();
-    const int s1 = g_rowptr[t + 1];
-    for (int si = g_rowptr[t]; si < s1; si++) {
-      const int slot = g_slots[si];
-      const int e = slot / nd, i = slot - e * nd;
-      int c = 0;
-      while (c + 1 < nclass && cls_range[2 * c + 1] <= e) c++;
-      const double kap = kappa[elnr[e]];
-      double g[NC];
+    // contributions of the row in chunks of 64: lane q fetches contribution
+    // q's element data (slot, class, kappa, metric) into LDS, all lanes at
+    // once; then the chunk's contributions are added one after another
+    const int s0 = g_rowptr[t], s1 = g_rowptr[t + 1];
+    for (int c0 = s0; c0 < s1; c0 += 64) {
+      const int cn = min(64, s1 - c0);
+      if (lane < cn) {
+        const int slot = g_slots[c0 + lane];
+        const int e = slot / nd;
+        int c = 0;
+        while (c + 1 < nclass && cls_range[2 * c + 1] <= e) c++;
+        const double kap = kappa[elnr[e]];
+        ce[lane] = e;
+        ci[lane] = (slot - e * nd) | (c << 16);
 #pragma unroll
-      for (int k = 0; k < NC; k++) g[k] = kap * metric[(int64_t)e * NC + k];
-      for (int j = lane; j < nd; j += 64) {
-        const double* sp = smat + ((int64_t)c * NC * nd + i) * nd + j;
-        double v = 0.0;
-#pragma unroll
-        for (int k = 0; k < NC; k++) v += g[k] * sp[(int64_t)k * nd * nd];
-        const int col = dofs[(int64_t)e * nd + j];
-        int lo = 0, hi = L;  // first position whose column is >= col
-        while (lo < hi) {
-          const int mid = (lo + hi) >> 1;
-          if (lcol[mid] < col) lo = mid + 1;
-          else hi = mid;
-        }
-        if (lo < L && lcol[lo] == col) lval[lo] += v;
-        else *bad = 1;
+        for (int k = 0; k < NC; k++) cg[lane * NC + k] = kap * metric[(int64_t)e * NC + k];
       }
-      // the next contribution may add to the same positions
       __syncthreads();
+      // lane j < nd (nd <= 64) adds entry j of each contribution's row; the
+      // global loads of contribution q + 1 (S rows, dof number) are issued
+      // before contribution q is added
+      const int j = min(lane, nd - 1);
+      double sv[NC];
+      int col = 0;
+      auto fetch = [&](int q) {
+        const int e = ce[q], i = ci[q] & 0xFFFF, c = ci[q] >> 16;
+        const double* sp = smat + ((int64_t)c * NC * nd + i) * nd + j;
+#pragma unroll
+        for (int k = 0; k < NC; k++) sv[k] = sp[(int64_t)k * nd * nd];
+        col = dofs[(int64_t)e * nd + j];
+      };
+      fetch(0);
+      for (int q = 0; q < cn; q++) {
+        double cur[NC];
+#pragma unroll
+        for (int k = 0; k < NC; k++) cur[k] = sv[k];
+        const int ccol = col;
+        if (q + 1 < cn) fetch(q + 1);
+        if (lane < nd) {
+          double v = 0.0;
+#pragma unroll
+          for (int k = 0; k < NC; k++) v += cg[q * NC + k] * cur[k];
+          int lo = 0, hi = L;  // first position whose column is >= ccol
+          while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (lcol[mid] < ccol) lo = mid + 1;
+            else hi = mid;
+          }
+          if (lo < L && lcol[lo] == ccol) lval[lo] += v;
+          else *bad = 1;
+        }
+        // the next contribution may add to the same positions
+        __syncthreads();
+      }
     }
     for (int k = lane; k < L; k += 64) vals[obase + (int64_t)k * R] = lval[k];
     __syncthreads();
@@ -1572,7 +1601,8 @@ void MatFreeOp::AssembleCsr(CsrOp& A, hipStream_t st) {
   NGSAMD_REQUIRE(A.height == height && A.width == width, "ngsamd_mf_assemble_csr: pattern size differs");
   if (height == 0 || nel == 0 || A.nnz == 0) return;
   const size_t lds = (size_t)A.max_row_len * (sizeof(double) + sizeof(int));
-  NGSAMD_REQUIRE(lds <= (size_t)DeviceMaxLds(), "ngsamd_mf_assemble_csr: a row is too long for LDS");
+  // plus the kernel's static LDS (contribution data of one chunk, <= 4 KB)
+  NGSAMD_REQUIRE(lds + 4096 <= (size_t)DeviceMaxLds(), "ngsamd_mf_assemble_csr: a row is too long for LDS");
   DevBuf<int> bad;
   bad.alloc(1);
   NGSAMD_HIP(hipMemsetAsync(bad.p, 0, sizeof(int), st));
