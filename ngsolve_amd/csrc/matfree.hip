@@ -778,100 +778,6 @@ __global__ __launch_bounds__(kGatherBlock) void MfGatherKernel(
   }
 }
 
-// The same gather with the rows (dofs in gather order) cut into blocks of at
-// most kGatherNnz slots: a workgroup loads a block's slot numbers coalesced
-// and its contributions (one per lane, all loads in flight), then sums each
-// row from LDS with a power-of-two group of lanes sized to the block's row
-// count.  Every lane stays busy whatever the number of slots per dof (24 for
-// a vertex of a tet mesh, 1 for a cell dof), where the one-thread-per-dof
-// kernel above idles most lanes of a wave on the longest row.
-constexpr int kGatherNnz = 2048;
-
-template <class C, class V, bool DOT>
-__global__ __launch_bounds__(kGatherBlock) void MfGatherBlockKernel(
-    const int* __restrict__ rowptr, const int* __restrict__ slots, const int* __restrict__ order,
-    const int* __restrict__ blk_row, int nblocks, int blocks_per_wg, const C* __restrict__ contrib,
-    V* __restrict__ y, double s, int accumulate, const int* __restrict__ stop, const V* __restrict__ x,
-    double* __restrict__ partials) {
-  if (stop && *stop) return;
-  constexpr int PER = kGatherNnz / kGatherBlock;
-  __shared__ double prod[kGatherNnz];
-  __shared__ double lds[kGatherBlock / 64];
-  const int tid = threadIdx.x;
-  const int lb = XcdContiguousBlock(blockIdx.x, gridDim.x);
-  const int b0 = lb * blocks_per_wg;
-  const int b1 = min(b0 + blocks_per_wg, nblocks);
-  double dotacc = 0.0;
-  for (int b = b0; b < b1; b++) {
-    const int r0 = blk_row[b], r1 = blk_row[b + 1];
-    const int k0 = rowptr[r0], k1 = rowptr[r1];
-    const int nz = k1 - k0;
-    if (nz > kGatherNnz) {
-      // one dof with more slots than a block: the whole workgroup strides
-      double a = 0.0;
-      for (int k = k0 + tid; k < k1; k += kGatherBlock) a += (double)contrib[slots[k]];
-      a = BlockSum<kGatherBlock>(a, lds);
-      if (tid == 0) {
-        const int d = order[r0];
-        const double v = s * a;
-        const V yv = accumulate ? (V)((double)y[d] + v) : (V)v;
-        y[d] = yv;
-        if (DOT) dotacc += (double)x[d] * (double)yv;
-      }
-      continue;
-    }
-    // unconditional loads (lanes past the block read slot entry k0, kept in
-    // bounds by the padding of the slot array)
-    int sl[PER];
-    C cv[PER];
-#pragma unroll
-    for (int i = 0; i < PER; i++) {
-      const int k = tid + i * kGatherBlock;
-      sl[i] = slots[k0 + (k < nz ? k : 0)];
-    }
-#pragma unroll
-    for (int i = 0; i < PER; i++) cv[i] = contrib[sl[i]];
-#pragma unroll
-    for (int i = 0; i < PER; i++) {
-      const int k = tid + i * kGatherBlock;
-      prod[k] = k < nz ? (double)cv[i] : 0.0;
-    }
-    __syncthreads();
-    const int nr = r1 - r0;
-    int tpr = 64;
-    while (tpr > 1 && tpr * nr > kGatherBlock) tpr >>= 1;
-    const int l = tid & (tpr - 1);
-    for (int j0 = 0; j0 < nr; j0 += kGatherBlock / tpr) {
-      const int j = j0 + tid / tpr;
-      double sum = 0.0;
-      if (j < nr) {
-        const int s0 = rowptr[r0 + j] - k0, s1 = rowptr[r0 + j + 1] - k0;
-        // four independent partial sums: the LDS reads of a long row overlap
-        double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-        int k = s0 + l;
-        for (; k + 3 * tpr < s1; k += 4 * tpr) {
-          a0 += prod[k];
-          a1 += prod[k + tpr];
-          a2 += prod[k + 2 * tpr];
-          a3 += prod[k + 3 * tpr];
-        }
-        for (; k < s1; k += tpr) a0 += prod[k];
-        sum = (a0 + a1) + (a2 + a3);
-      }
-      for (int off = tpr / 2; off > 0; off >>= 1) sum += __shfl_xor(sum, off, 64);
-      if (j < nr && l == 0) {
-        const int d = order[r0 + j];
-        const double v = s * sum;
-        const V yv = accumulate ? (V)((double)y[d] + v) : (V)v;
-        y[d] = yv;
-        if (DOT) dotacc += (double)x[d] * (double)yv;
-      }
-    }
-    __syncthreads();
-  }
-  if (DOT) StorePartial<kGatherBlock>(dotacc, partials, lds);
-}
-
 template <class V>
 __global__ void ScaleFillKernel(V* __restrict__ y, int64_t n, const int* __restrict__ stop) {
   if (stop && *stop) return;
@@ -1176,36 +1082,14 @@ MatFreeOp::MatFreeOp(int dim_, int nd_, int nq_, int64_t ndof, int nclass, const
     }
     g_rowptr.alloc(ndof + 1);
     g_rowptr.upload(prp.data(), ndof + 1);
-    // 4 padding entries (slot 0): lanes past the last block read entry nslots
-    std::vector<int> psl_pad(psl);
-    psl_pad.resize(psl.size() + 4, 0);
-    g_slots.alloc(psl_pad.size());
-    g_slots.upload(psl_pad.data(), psl_pad.size());
-    // blocks of rows with at most kGatherNnz slots (or one longer row)
-    std::vector<int> gb(1, 0);
-    int64_t acc = 0;
-    for (int64_t r = 0; r < ndof; r++) {
-      const int64_t len = prp[r + 1] - prp[r];
-      if (r > gb.back() && acc + len > kGatherNnz) {
-        gb.push_back((int)r);
-        acc = 0;
-      }
-      acc += len;
-    }
-    if (gb.back() != ndof || ndof == 0) gb.push_back((int)ndof);
-    g_nblocks = (int)gb.size() - 1;
-    const char* genv = getenv("NGSAMD_MF_GATHER");
-    NGSAMD_REQUIRE(!genv || std::string(genv) == "dof" || std::string(genv) == "block",
-                   "NGSAMD_MF_GATHER must be dof or block");
-    gather_kind = (genv && std::string(genv) == "block") ? 1 : 0;
-    g_blk.alloc(gb.size());
-    g_blk.upload(gb.data(), gb.size());
+    g_slots.alloc(std::max<size_t>(psl.size(), 1));
+    if (!psl.empty()) g_slots.upload(psl.data(), psl.size());
     NGSAMD_HIP(hipStreamSynchronize(Stream()));
     g_order.alloc(std::max<int64_t>(ndof, 1));
     if (ndof) g_order.upload(order.data(), ndof);
     NGSAMD_HIP(hipStreamSynchronize(Stream()));
   }
-  // at least one entry: the gather's padding lanes read contribution 0
+  // at least one entry (operators without elements)
   contrib64.alloc(std::max<size_t>((size_t)nel * nd, 1));
   contrib32.alloc(std::max<size_t>((size_t)nel * nd, 1));
   // round schedule of the MFMA kernels: 16-element tiles, kMfWaves tiles per
@@ -1427,33 +1311,15 @@ int MatFreeOp::Apply(double s, const V* x, V* y, bool accumulate, const int* sto
   // with partials (CG): the gather on its wide grid writes one partial per
   // workgroup, folded to kFoldedPartials for CG's kernels
   double* dotp = partials ? State().wide_partials : nullptr;
-  int np = 0;
-  if (gather_kind != 1) {
-    // one thread per dof (default): a wide grid hides each dof's load chain
-    np = GridFor(height, kGatherBlock, kMaxWideBlocks);
-    if (partials)
-      MfGatherKernel<V, V, true><<<np, kGatherBlock, 0, st>>>(g_rowptr.p, g_slots.p, g_order.p, contrib, y, height, s,
-                                                              accumulate ? 1 : 0, stop, x, dotp);
-    else
-      MfGatherKernel<V, V, false><<<np, kGatherBlock, 0, st>>>(g_rowptr.p, g_slots.p, g_order.p, contrib, y, height,
-                                                               s, accumulate ? 1 : 0, stop, nullptr, nullptr);
-    NGSAMD_CHECK_LAUNCH();
-  } else {
-    // slot blocks
-    np = g_nblocks < kMaxWideBlocks ? g_nblocks : kMaxWideBlocks;
-    if (np > 8) np &= ~7;
-    if (np < 1) np = 1;
-    const int per = (g_nblocks + np - 1) / np;
-    if (partials)
-      MfGatherBlockKernel<V, V, true><<<np, kGatherBlock, 0, st>>>(g_rowptr.p, g_slots.p, g_order.p, g_blk.p,
-                                                                   g_nblocks, per, contrib, y, s, accumulate ? 1 : 0,
-                                                                   stop, x, dotp);
-    else
-      MfGatherBlockKernel<V, V, false><<<np, kGatherBlock, 0, st>>>(g_rowptr.p, g_slots.p, g_order.p, g_blk.p,
-                                                                    g_nblocks, per, contrib, y, s,
-                                                                    accumulate ? 1 : 0, stop, nullptr, nullptr);
-    NGSAMD_CHECK_LAUNCH();
-  }
+  // one thread per dof: a wide grid hides each dof's load chain
+  const int np = GridFor(height, kGatherBlock, kMaxWideBlocks);
+  if (partials)
+    MfGatherKernel<V, V, true><<<np, kGatherBlock, 0, st>>>(g_rowptr.p, g_slots.p, g_order.p, contrib, y, height, s,
+                                                            accumulate ? 1 : 0, stop, x, dotp);
+  else
+    MfGatherKernel<V, V, false><<<np, kGatherBlock, 0, st>>>(g_rowptr.p, g_slots.p, g_order.p, contrib, y, height,
+                                                             s, accumulate ? 1 : 0, stop, nullptr, nullptr);
+  NGSAMD_CHECK_LAUNCH();
   if (!partials) return 0;
   LaunchFoldPartials(dotp, np, partials, stop, st);
   return kFoldedPartials;

@@ -51,9 +51,6 @@ struct MatFreeOp : ngsamd_op_s {
   // gather scatter: dof -> contribution slots (element row * nd + local),
   // dofs in the gather order g_order (by lowest element number)
   DevBuf<int> g_rowptr, g_slots, g_order;
-  DevBuf<int> g_blk;          // gather blocks: first row of each, g_nblocks + 1
-  int g_nblocks = 0;
-  int gather_kind = 0;        // 0: one thread per dof, 1: slot blocks
   DevBuf<double> contrib64;
   DevBuf<float> contrib32;
 
