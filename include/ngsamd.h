@@ -158,6 +158,13 @@ int ngsamd_mf_element_grad(ngsamd_op_t op, const double* d_lam, const double* d_
  * SparseMatrixTM::AddElementMatrix (linalg/sparsematrix_impl.hpp:150-189)
  * for the grad-grad form. */
 int ngsamd_mf_assemble_csr(ngsamd_op_t mf, const int64_t* h_rowptr, const int* h_cols, ngsamd_op_t* csr);
+/* Sparsity pattern of a matrix-free operator's form, built on the device:
+ * row r is the sorted union of the dofs of the elements holding dof r, as
+ * BilinearForm::GetGraph (comp/bilinearform.cpp:552-825) -> MatrixGraph
+ * (linalg/sparsematrix.cpp:450-975) builds it.  Two calls: with h_cols NULL
+ * it fills h_rowptr (height + 1 entries); then with h_cols of h_rowptr[height]
+ * entries it fills the columns. */
+int ngsamd_mf_pattern(ngsamd_op_t mf, int64_t* h_rowptr, int* h_cols);
 
 /* ----------------------------------------------------------------- CG */
 /* Preconditioned CG with the recurrence and stopping rule of the reference's

@@ -340,9 +340,9 @@ class DevMatrixFree(DeviceMatrix):
 
 class BilinearForm:
     def __init__(self, space, integrals=None, mf=None, matrix_free_bdb=False, device_assembly=False, **kwargs):
-        """device_assembly=True assembles the CSR values on the device
-        (ngsamd_mf_assemble_csr; one grad(u)*grad(v) term); the pattern is
-        built on the host as before."""
+        """device_assembly=True builds the sparsity pattern and the CSR
+        values on the device (ngsamd_mf_pattern, ngsamd_mf_assemble_csr; one
+        grad(u)*grad(v) term)."""
         if isinstance(space, _Integral):  # BilinearForm(grad(u)*grad(v)*dx)
             integrals, space = space, None
         self.integrals = []
@@ -385,15 +385,16 @@ class BilinearForm:
             self.mat = DevMatrixFree(fes, self._order(term, meas), self._coef(term), self.mf.precision,
                                      self.mf.atomic)
             return self
-        rowptr, cols = N.BuildPattern(fes.space)
         if self.device_assembly:
             if len(self.integrals) != 1 or self.integrals[0][0].kind != "laplace":
                 raise ValueError("device assembly supports one grad(u)*grad(v) term")
             term, meas = self.integrals[0]
             mf = DevMatrixFree(fes, self._order(term, meas), self._coef(term), "f64")
+            rowptr, cols = N.PatternDevice(mf.op)
             op = N.AssembleCSRDevice(mf.op, rowptr, cols)
             self.mat = DevSparseMatrix(fes.ndof, fes.ndof, rowptr, cols, None, symmetric=True, op=op)
             return self
+        rowptr, cols = N.BuildPattern(fes.space)
         vals = np.zeros(len(cols))
         for term, meas in self.integrals:
             ir = fes.integration_rule(self._order(term, meas))

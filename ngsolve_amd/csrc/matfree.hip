@@ -28,6 +28,7 @@
 // fixed order (deterministic, no atomics).  ATOMIC adds the element results
 // with hardware floating-point atomics, as the reference kernel does.
 #include <algorithm>
+#include <climits>
 #include <cstdlib>
 #include <cstring>
 #include <memory>
@@ -1082,6 +1083,7 @@ MatFreeOp::MatFreeOp(int dim_, int nd_, int nq_, int64_t ndof, int nclass, const
     }
     g_rowptr.alloc(ndof + 1);
     g_rowptr.upload(prp.data(), ndof + 1);
+    for (int64_t r = 0; r < ndof; r++) max_row_slots = std::max(max_row_slots, prp[r + 1] - prp[r]);
     g_slots.alloc(std::max<size_t>(psl.size(), 1));
     if (!psl.empty()) g_slots.upload(psl.data(), psl.size());
     NGSAMD_HIP(hipStreamSynchronize(Stream()));
@@ -1488,6 +1490,126 @@ void MatFreeOp::AssembleCsr(CsrOp& A, hipStream_t st) {
   NGSAMD_REQUIRE(!hbad, "ngsamd_mf_assemble_csr: the pattern lacks a pair of dofs of one element");
 }
 
+// ------------------------------------------------- sparsity pattern on device
+// One workgroup per dof r (gather order, grid-stride): the dofs of every
+// element holding r go to LDS (padded with INT_MAX to a power of two, at
+// most cap), a bitonic sort orders them, and the distinct ones are row r of
+// the pattern
+// (MatrixGraph: the sorted union of the element dofs).  MODE 0 stores the
+// row length in len[r]; MODE 1 writes the columns at cols[rowptr[r]...]
+// (each thread counts its contiguous chunk, a serial scan of the 256 counts
+// gives the offsets, so the order is the sorted order).
+constexpr int kPatBlock = 256;
+
+template <int MODE>
+__global__ __launch_bounds__(kPatBlock) void PatternKernel(const int* __restrict__ g_rowptr,
+                                                           const int* __restrict__ g_slots,
+                                                           const int* __restrict__ g_order, int64_t nrows,
+                                                           const int* __restrict__ dofs, int nd, int cap,
+                                                           int* __restrict__ len,
+                                                           const int64_t* __restrict__ rowptr,
+                                                           int* __restrict__ cols) {
+  extern __shared__ int buf[];
+  __shared__ int cnt[kPatBlock + 1];
+  const int tid = threadIdx.x;
+  for (int64_t t = blockIdx.x; t < nrows; t += gridDim.x) {
+    const int r = g_order[t];
+    const int s0 = g_rowptr[t];
+    const int m = (g_rowptr[t + 1] - s0) * nd;
+    int rc = 2;  // this row's sort length: a power of two >= m, <= cap
+    while (rc < m) rc <<= 1;
+    rc = min(rc, cap);  // m <= cap holds: cap covers the most slots of any dof
+    const int per = (rc + kPatBlock - 1) / kPatBlock;
+    for (int k = tid; k < rc; k += kPatBlock) {
+      int v = INT_MAX;
+      if (k < m) {
+        const int q = k / nd, j = k - q * nd;
+        v = dofs[(int64_t)(g_slots[s0 + q] / nd) * nd + j];
+      }
+      buf[k] = v;
+    }
+    __syncthreads();
+    for (int size = 2; size <= rc; size <<= 1)
+      for (int stride = size >> 1; stride > 0; stride >>= 1) {
+        for (int k = tid; k < rc / 2; k += kPatBlock) {
+          const int i = 2 * k - (k & (stride - 1));
+          const int a = buf[i], b = buf[i + stride];
+          if ((a > b) == ((i & size) == 0)) {
+            buf[i] = b;
+            buf[i + stride] = a;
+          }
+        }
+        __syncthreads();
+      }
+    // distinct entries of this thread's chunk [tid * per, (tid + 1) * per)
+    const int k0 = min(tid * per, rc), k1 = min(k0 + per, rc);
+    int c = 0;
+    for (int k = k0; k < k1; k++) c += (buf[k] != INT_MAX && (k == 0 || buf[k] != buf[k - 1])) ? 1 : 0;
+    cnt[tid] = c;
+    __syncthreads();
+    if (tid == 0) {
+      int acc = 0;
+      for (int i = 0; i < kPatBlock; i++) {
+        const int v = cnt[i];
+        cnt[i] = acc;
+        acc += v;
+      }
+      cnt[kPatBlock] = acc;
+    }
+    __syncthreads();
+    if (MODE == 0) {
+      if (tid == 0) len[r] = cnt[kPatBlock];
+    } else {
+      int* out = cols + rowptr[r] + cnt[tid];
+      for (int k = k0; k < k1; k++)
+        if (buf[k] != INT_MAX && (k == 0 || buf[k] != buf[k - 1])) *out++ = buf[k];
+    }
+    __syncthreads();
+  }
+}
+
+void MatFreeOp::Pattern(int64_t* h_rowptr, int* h_cols, hipStream_t st) {
+  const int64_t n = height;
+  if (!h_cols) {
+    std::vector<int64_t> rp(n + 1, 0);
+    if (n > 0 && nel > 0) {
+      int cap = 2;
+      while (cap < max_row_slots * nd) cap <<= 1;
+      NGSAMD_REQUIRE((size_t)cap * sizeof(int) + 2048 <= (size_t)DeviceMaxLds(),
+                     "ngsamd_mf_pattern: too many element dofs around one dof for LDS");
+      DevBuf<int> len(n);
+      NGSAMD_HIP(hipFuncSetAttribute((const void*)PatternKernel<0>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     cap * (int)sizeof(int)));
+      PatternKernel<0><<<(int)std::min<int64_t>(n, 8192), kPatBlock, cap * sizeof(int), st>>>(
+          g_rowptr.p, g_slots.p, g_order.p, n, dofs.p, nd, cap, len.p, nullptr, nullptr);
+      NGSAMD_CHECK_LAUNCH();
+      std::vector<int> hl(n);
+      NGSAMD_HIP(hipMemcpyAsync(hl.data(), len.p, sizeof(int) * n, hipMemcpyDeviceToHost, st));
+      NGSAMD_HIP(hipStreamSynchronize(st));
+      for (int64_t r = 0; r < n; r++) rp[r + 1] = rp[r] + hl[r];
+    }
+    pat_rowptr.alloc(n + 1);
+    pat_rowptr.upload(rp.data(), n + 1);
+    NGSAMD_HIP(hipStreamSynchronize(st));
+    std::copy(rp.begin(), rp.end(), h_rowptr);
+    return;
+  }
+  NGSAMD_REQUIRE(pat_rowptr.n == (size_t)(n + 1), "ngsamd_mf_pattern: call with h_cols NULL first");
+  int64_t nnz = 0;
+  NGSAMD_HIP(hipMemcpy(&nnz, pat_rowptr.p + n, sizeof(int64_t), hipMemcpyDeviceToHost));
+  if (nnz == 0) return;
+  int cap = 2;
+  while (cap < max_row_slots * nd) cap <<= 1;
+  DevBuf<int> dcols(nnz);
+  NGSAMD_HIP(hipFuncSetAttribute((const void*)PatternKernel<1>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 cap * (int)sizeof(int)));
+  PatternKernel<1><<<(int)std::min<int64_t>(n, 8192), kPatBlock, cap * sizeof(int), st>>>(
+      g_rowptr.p, g_slots.p, g_order.p, n, dofs.p, nd, cap, nullptr, pat_rowptr.p, dcols.p);
+  NGSAMD_CHECK_LAUNCH();
+  NGSAMD_HIP(hipMemcpyAsync(h_cols, dcols.p, sizeof(int) * nnz, hipMemcpyDeviceToHost, st));
+  NGSAMD_HIP(hipStreamSynchronize(st));
+}
+
 void MatFreeOp::ElementGrad(const double* lam, const double* u, double* grad, hipStream_t st) {
   for (size_t ci = 0; ci < classes.size(); ci++) {
     const MfClass& c = classes[ci];
@@ -1554,6 +1676,15 @@ extern "C" int ngsamd_mf_assemble_csr(ngsamd_op_t mf, const int64_t* rowptr, con
     std::unique_ptr<CsrOp> A(new CsrOp(M->height, M->width, rowptr, cols, nullptr));
     M->AssembleCsr(*A, Stream());
     *csr = A.release();
+  });
+}
+
+extern "C" int ngsamd_mf_pattern(ngsamd_op_t mf, int64_t* h_rowptr, int* h_cols) {
+  return Guard([&] {
+    RequireInit();
+    NGSAMD_REQUIRE(mf && mf->kind == OpKind::MatrixFree, "ngsamd_mf_pattern: not a matrix-free operator");
+    NGSAMD_REQUIRE(h_rowptr, "ngsamd_mf_pattern: null argument");
+    static_cast<MatFreeOp*>(mf)->Pattern(h_rowptr, h_cols, Stream());
   });
 }
 

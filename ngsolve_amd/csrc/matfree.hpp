@@ -72,6 +72,11 @@ struct MatFreeOp : ngsamd_op_s {
   void ElementGrad(const double* lam, const double* u, double* grad, hipStream_t st);
   // fills the values of A (same size, pattern holding every element pair)
   void AssembleCsr(CsrOp& A, hipStream_t st);
+  // sparsity pattern of the form: h_cols null fills h_rowptr (height + 1);
+  // then h_cols (h_rowptr[height] entries) receives the columns
+  void Pattern(int64_t* h_rowptr, int* h_cols, hipStream_t st);
+  int max_row_slots = 0;      // most element contributions of one dof
+  DevBuf<int64_t> pat_rowptr; // row pointers of the last Pattern call
 
  private:
   template <class V>

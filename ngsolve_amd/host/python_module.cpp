@@ -433,6 +433,14 @@ PYBIND11_MODULE(_ngsamd, m) {
           A->nnz = (int64_t)cols.size();
           return A;
         });
+  m.def("PatternDevice", [](Operator& mf) {
+    if (mf.kind != "matfree") throw std::runtime_error("PatternDevice: matrix-free operator expected");
+    py::array_t<int64_t> rp(mf.height + 1);
+    Check(ngsamd_mf_pattern(mf.op, rp.mutable_data(), nullptr));
+    py::array_t<int> cols(rp.data()[mf.height]);
+    Check(ngsamd_mf_pattern(mf.op, rp.mutable_data(), cols.mutable_data()));
+    return py::make_tuple(rp, cols);
+  });
   m.def("AssembleCSRDevice", [](Operator& mf, Arr<int64_t> rowptr, Arr<int> cols) {
     if (mf.kind != "matfree") throw std::runtime_error("AssembleCSRDevice: matrix-free operator expected");
     if (rowptr.size() != mf.height + 1 || rowptr.data()[mf.height] != (int64_t)cols.size())

@@ -1,4 +1,7 @@
-"""CSR assembly on the device (ngsamd_mf_assemble_csr) against the oracle's
+"""Sparsity pattern on the device (ngsamd_mf_pattern): bit-exact against
+the oracle's pattern (numpy.unique of all element dof pairs) and the
+product's host pattern, 2D and 3D, p = 1..5, structured and unstructured.
+CSR assembly on the device (ngsamd_mf_assemble_csr) against the oracle's
 assembly (oracle/fe_oracle.py Problem.assemble: per-element matrices from
 the per-point F^-T F^-1 form, scipy COO -> CSR): values within 1e-13 of the
 largest entry, in every CSR layout the operator may choose (runs, stream,
@@ -38,6 +41,37 @@ def test_assembly_matches_oracle(device, monkeypatch, dim, n, p, order, algo):
     dx, dy = dvec(x), N.DeviceVector(P["ndof"], 8)
     A.mult(dx, dy)
     assert relerr_inf(dy.download(), oracle.spmv(orp, ocols, ovals, x)) < 1e-13
+
+
+PATTERN_CASES = [(2, 7, 1), (2, 6, 3), (2, 4, 5), (3, 3, 1), (3, 3, 2), (3, 3, 4), (3, 2, 5)]
+
+
+@pytest.mark.parametrize("dim,n,p", PATTERN_CASES)
+def test_pattern_matches_oracle(device, dim, n, p):
+    say(f"[pattern] dim={dim} n={n} p={p}")
+    P = problem(dim, n, p)
+    rp, cols = N.PatternDevice(mf_operator(P))
+    orp, ocols = O.Problem(P["omesh"], p).pattern()
+    assert np.array_equal(rp, orp) and np.array_equal(cols, ocols)
+    assert np.array_equal(rp, P["rowptr"]) and np.array_equal(cols, P["cols"])
+
+
+@pytest.mark.parametrize("name", ["square.vol.gz", "cube.vol.gz"])
+@pytest.mark.parametrize("p", [1, 3, 4])
+def test_pattern_on_vol_mesh(device, name, p):
+    import gzip
+    import os
+    say(f"[pattern] {name} p={p}")
+    text = gzip.open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", name), "rt").read()
+    m = N.ReadVolMesh(text)
+    fes = N.H1Space(m, p, 0)
+    ir = N.TrigRule(2 * p) if m.dim == 2 else N.TetRule(2 * p)
+    P = dict(mesh=m, fes=fes, ir=ir, dim=m.dim, kappa=None)
+    rp, cols = N.PatternDevice(mf_operator(P))
+    hrp, hcols = N.BuildPattern(fes)
+    orp, ocols = O.Problem(O.read_vol(text), p).pattern()
+    assert np.array_equal(rp, hrp) and np.array_equal(cols, hcols)
+    assert np.array_equal(rp, orp) and np.array_equal(cols, ocols)
 
 
 def test_assembly_deterministic_and_kappa_one(device):
