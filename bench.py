@@ -389,6 +389,22 @@ def main():
                 "true_rel_residual": true_res(u_b),
                 "rel_diff_vs_jacobi_solution": float(np.linalg.norm(u_b - u64) / np.linalg.norm(u64))}
             del Cb
+        if not args.no_cpu:
+            # the same solve with the oracle's C Jacobi-PCG (OpenMP, all host threads)
+            import oracle
+            dk = np.empty(ndof)
+            Ak.diagonal(dd)
+            dk[:] = dd.download()
+            invk = np.where(free, 1.0 / np.where(dk == 0, 1.0, dk), 0.0)
+            t0 = time.perf_counter()
+            u_cpu, its_cpu, _ = oracle.pcg_csr(rowptr, cols, vals_k, invk, f, tol=1e-10, maxiter=200000)
+            t_cpu = time.perf_counter() - t0
+            log(f"[bench] CPU oracle PCG lognormal: {its_cpu} its {t_cpu:.2f}s on {oracle.num_threads()} threads")
+            res["cpu_oracle_pcg"] = {
+                "iterations": its_cpu, "seconds": round(t_cpu, 3), "threads": oracle.num_threads(),
+                "true_rel_residual": true_res(u_cpu),
+                "gpu_fp64_csr_cg_speedup": round(t_cpu / t_k, 1),
+                "gpu_best_mixed_speedup": round(t_cpu / best[1], 1) if best else None}
         extra["cg_lognormal_kappa"] = res
 
     if not args.no_batch:
