@@ -19,7 +19,11 @@
 // <w,d>, each workgroup first forming alpha itself from the partials of (1),
 // (3) s = beta s + w, each workgroup forming beta from the partials of (2);
 // workgroup 0 of (3) also does the stopping test and keeps the iteration
-// count and the residual history.  Every workgroup sums the same partials
+// count and the residual history.  The operator kernels write one partial
+// per workgroup of a wide grid; a small kernel folds them to 64 before (2).
+// With a block-Jacobi preconditioner, (2) does only the two updates and the
+// preconditioner's two kernels and a dot kernel for <w,d> follow it.  Every
+// workgroup sums the same partials
 // in the same order, so all of them hold the same alpha / beta bit for bit
 // and every run gives the same iterates.  <w,d> is kept per iteration parity
 // (wdn[it & 1]): kernel (3) of an even iteration reads wdn[0] and its
