@@ -214,10 +214,14 @@ def main():
         for _ in range(3):
             As.mult(dx, dy)
         del As
+        x32, y32 = N.DeviceVector(ndof, 4), N.DeviceVector(ndof, 4)
+        x32.upload(x.astype(np.float32))
         for prec in (0, 1, 2, 3):
             M = build_mf(N, fes, mesh, ir, None, prec)
             for _ in range(3):
                 M.mult(dx, dy)
+                if prec:
+                    M.mult_f32(x32, y32)
             del M
         big_a, big_b = N.DeviceVector(1 << 27, 8), N.DeviceVector(1 << 27, 8)  # 1 GiB each
         big_b.copy_from(big_a)
