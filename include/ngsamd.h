@@ -165,6 +165,18 @@ int ngsamd_mf_assemble_csr(ngsamd_op_t mf, const int64_t* h_rowptr, const int* h
  * it fills h_rowptr (height + 1 entries); then with h_cols of h_rowptr[height]
  * entries it fills the columns. */
 int ngsamd_mf_pattern(ngsamd_op_t mf, int64_t* h_rowptr, int* h_cols);
+/* Linear form f_i = sum_e |det F_e| sum_q w_q f(e, q) phi_i(x_q) on the
+ * device, into d_out (height entries, overwritten), each dof's element
+ * contributions summed in a fixed order through the operator's gather.
+ * nq, h_weights: the form's own rule; h_shape: phi per class [nclass][nq][nd]
+ * in the class order given at creation (nclass and nd as at creation);
+ * h_detj: det F per global element (ne of them, as at creation); d_fvals: f
+ * at the rule's points, [global element][nq], on the device.
+ * SymbolicLinearFormIntegrator::T_CalcElementVector
+ * (fem/symbolicintegrator.cpp:1036-1110) with LinearForm::Assemble's
+ * scatter. */
+int ngsamd_mf_linear_form(ngsamd_op_t mf, int nclass, int nq, int nd, int64_t ne, const double* h_weights,
+                          const double* h_shape, const double* h_detj, const double* d_fvals, double* d_out);
 
 /* ----------------------------------------------------------------- CG */
 /* Preconditioned CG with the recurrence and stopping rule of the reference's

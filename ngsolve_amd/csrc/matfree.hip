@@ -1360,6 +1360,46 @@ void MatFreeOp::Diagonal(double* d, hipStream_t st) {
   NGSAMD_CHECK_LAUNCH();
 }
 
+// f_e[i] = |det F_e| sum_q w_q f(e, q) phi_i(x_q) for the elements of one
+// class (table rows begin..end), into the contribution buffer; the gather
+// then sums each dof's contributions in slot order.
+__global__ void MfLinearFormKernel(const double* __restrict__ shape, const double* __restrict__ w, int nq,
+                                   const double* __restrict__ detj, const double* __restrict__ fvals,
+                                   const int* __restrict__ elnr, int64_t begin, int64_t end, int nd,
+                                   double* __restrict__ contrib) {
+  const int64_t total = (end - begin) * nd;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t e = begin + t / nd;
+    const int i = (int)(t % nd);
+    const int g = elnr[e];
+    const double* fe = fvals + (int64_t)g * nq;
+    double acc = 0.0;
+    for (int q = 0; q < nq; q++) acc += w[q] * fe[q] * shape[(size_t)q * nd + i];
+    contrib[e * nd + i] = fabs(detj[g]) * acc;
+  }
+}
+
+void MatFreeOp::LinearForm(int nq_lf, const double* h_w, const double* h_shape, const double* h_detj,
+                           const double* d_fvals, double* d_out, hipStream_t st) {
+  if (height == 0) return;
+  NGSAMD_REQUIRE(nq_lf >= 1, "ngsamd_mf_linear_form: empty rule");
+  DevBuf<double> w(nq_lf), sh((size_t)classes.size() * nq_lf * nd), dj(std::max<int64_t>(ne_global, 1));
+  w.upload(h_w, nq_lf);
+  sh.upload(h_shape, (size_t)classes.size() * nq_lf * nd);
+  if (ne_global) dj.upload(h_detj, ne_global);
+  for (size_t ci = 0; ci < classes.size(); ci++) {
+    const MfClass& c = classes[ci];
+    if (c.end <= c.begin) continue;
+    MfLinearFormKernel<<<GridFor((c.end - c.begin) * nd, 256, 8192), 256, 0, st>>>(
+        sh.p + ci * (size_t)nq_lf * nd, w.p, nq_lf, dj.p, d_fvals, elnr.p, c.begin, c.end, nd, contrib64.p);
+    NGSAMD_CHECK_LAUNCH();
+  }
+  MfGatherKernel<double, double, false><<<GridFor(height, kGatherBlock, 16384), kGatherBlock, 0, st>>>(
+      g_rowptr.p, g_slots.p, g_order.p, contrib64.p, d_out, height, 1.0, 0, nullptr, nullptr, nullptr);
+  NGSAMD_CHECK_LAUNCH();
+  NGSAMD_HIP(hipStreamSynchronize(st));  // the uploaded tables are freed on return
+}
+
 // ------------------------------------------------------ CSR assembly on device
 // One 64-lane workgroup per row (dofs in gather order, grid-stride): the row's
 // column list into LDS; then, for each contribution (e, i) of the row in slot
@@ -1685,6 +1725,21 @@ extern "C" int ngsamd_mf_pattern(ngsamd_op_t mf, int64_t* h_rowptr, int* h_cols)
     NGSAMD_REQUIRE(mf && mf->kind == OpKind::MatrixFree, "ngsamd_mf_pattern: not a matrix-free operator");
     NGSAMD_REQUIRE(h_rowptr, "ngsamd_mf_pattern: null argument");
     static_cast<MatFreeOp*>(mf)->Pattern(h_rowptr, h_cols, Stream());
+  });
+}
+
+extern "C" int ngsamd_mf_linear_form(ngsamd_op_t mf, int nclass, int nq, int nd, int64_t ne, const double* h_weights,
+                                     const double* h_shape, const double* h_detj, const double* d_fvals,
+                                     double* d_out) {
+  return Guard([&] {
+    RequireInit();
+    NGSAMD_REQUIRE(mf && mf->kind == OpKind::MatrixFree, "ngsamd_mf_linear_form: not a matrix-free operator");
+    NGSAMD_REQUIRE(h_weights && h_shape && h_detj && d_out && (d_fvals || mf->height == 0),
+                   "ngsamd_mf_linear_form: null argument");
+    MatFreeOp* M = static_cast<MatFreeOp*>(mf);
+    NGSAMD_REQUIRE(nclass == (int)M->classes.size() && nd == M->nd && ne == M->ne_global,
+                   "ngsamd_mf_linear_form: shape or element count does not match the operator");
+    M->LinearForm(nq, h_weights, h_shape, h_detj, d_fvals, d_out, Stream());
   });
 }
 

@@ -72,6 +72,9 @@ struct MatFreeOp : ngsamd_op_s {
   void ElementGrad(const double* lam, const double* u, double* grad, hipStream_t st);
   // fills the values of A (same size, pattern holding every element pair)
   void AssembleCsr(CsrOp& A, hipStream_t st);
+  // linear form of element quadrature values into d_out (overwritten)
+  void LinearForm(int nq_lf, const double* h_w, const double* h_shape, const double* h_detj, const double* d_fvals,
+                  double* d_out, hipStream_t st);
   // sparsity pattern of the form: h_cols null fills h_rowptr (height + 1);
   // then h_cols (h_rowptr[height] entries) receives the columns
   void Pattern(int64_t* h_rowptr, int* h_cols, hipStream_t st);

@@ -433,6 +433,18 @@ PYBIND11_MODULE(_ngsamd, m) {
           A->nnz = (int64_t)cols.size();
           return A;
         });
+  m.def("LinearFormDevice", [](Operator& mf, Arr<double> weights, Arr<double> shape, Arr<double> detj,
+                                DeviceVector& fvals, DeviceVector& out) {
+    if (mf.kind != "matfree") throw std::runtime_error("LinearFormDevice: matrix-free operator expected");
+    const int nq = (int)weights.size();
+    if (shape.ndim() != 3 || shape.shape(1) != nq)
+      throw std::runtime_error("LinearFormDevice: shape must be [nclass][nq][nd] with nq = len(weights)");
+    if (fvals.itemsize != 8 || fvals.size != (int64_t)detj.size() * nq)
+      throw std::runtime_error("LinearFormDevice: fvals must be float64 [ne][nq] with ne = len(detj)");
+    RequireF64(out, mf.height, "LinearFormDevice");
+    Check(ngsamd_mf_linear_form(mf.op, (int)shape.shape(0), nq, (int)shape.shape(2), (int64_t)detj.size(),
+                                weights.data(), shape.data(), detj.data(), fvals.d(), out.d()));
+  });
   m.def("PatternDevice", [](Operator& mf) {
     if (mf.kind != "matfree") throw std::runtime_error("PatternDevice: matrix-free operator expected");
     py::array_t<int64_t> rp(mf.height + 1);

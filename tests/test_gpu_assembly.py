@@ -119,3 +119,27 @@ def test_bilinearform_device_assembly(device):
     assert vh.shape == vd.shape
     assert np.abs(vh - vd).max() < 1e-13 * np.abs(vh).max()
     assert abs(a_dev.mat[0, 0] - a_host.mat[0, 0]) < 1e-13 * abs(a_host.mat[0, 0])
+
+
+@pytest.mark.parametrize("dim,n,p", [(2, 6, 3), (2, 4, 5), (3, 3, 2), (3, 3, 4)])
+def test_linear_form_matches_oracle(device, dim, n, p):
+    """Device linear form (ngsamd_mf_linear_form) with its own rule (order
+    2p) against the oracle's linear form and the host one, f varying per
+    element and point."""
+    say(f"[linear form] dim={dim} n={n} p={p}")
+    P = problem(dim, n, p)
+    M = mf_operator(P)
+    order = 2 * p
+    ir = N.TrigRule(order) if dim == 2 else N.TetRule(order)
+    classes = N.BuildClasses(P["fes"], ir)
+    shape = np.ascontiguousarray(np.stack([c["shape"] for c in classes]))
+    detj, _ = N.ComputeGeometry(P["mesh"])
+    ne, nq = P["mesh"].ne, len(ir)
+    fvals = np.random.default_rng(9).uniform(-1, 2, (ne, nq))
+    dfv, dout = dvec(fvals.ravel()), N.DeviceVector(P["ndof"], 8)
+    N.LinearFormDevice(M, np.ascontiguousarray(ir.weights), shape, np.ascontiguousarray(detj), dfv, dout)
+    got = dout.download()
+    pr = O.Problem(P["omesh"], p, intorder=order)
+    ref = pr.linear_form(fvals)
+    assert relerr_inf(got, ref) < 1e-13
+    assert relerr_inf(got, N.AssembleLinearForm(P["fes"], ir, fvals)) < 1e-13
