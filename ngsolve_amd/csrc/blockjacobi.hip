@@ -175,6 +175,7 @@ extern "C" int ngsamd_blockjacobi_create(ngsamd_op_t* op, int64_t n, const int64
   return Guard([&] {
     RequireInit();
     NGSAMD_REQUIRE(op && rowptr && block_ptr, "ngsamd_blockjacobi_create: null argument");
+    NGSAMD_REQUIRE(n >= 0 && nblocks >= 0, "ngsamd_blockjacobi_create: bad sizes");
     NGSAMD_REQUIRE(rowptr[n] == 0 || (cols && vals), "ngsamd_blockjacobi_create: null argument");
     NGSAMD_REQUIRE(block_ptr[nblocks] == 0 || block_dofs, "ngsamd_blockjacobi_create: null argument");
     *op = nullptr;
