@@ -312,6 +312,30 @@ def main():
         log(f"[bench] csr pattern host {t_pat_host:.3f}s device {t_pat_dev:.3f}s identical {same_pattern}; "
             f"values host {t_host:.3f}s device {t_dev:.3f}s diff {d_asm:.1e}")
 
+        # topology + H1 dof tables: host setup vs ngsamd_h1_create (boundary
+        # found as the faces of one element in both)
+        els = np.ascontiguousarray(mesh.elements, dtype=np.int32)
+        nob, nobi = np.zeros((0, 3), np.int32), np.zeros(0, np.int32)
+        all_bits = (1 << 64) - 1
+        t0 = time.perf_counter()
+        hm = N.BuildTopologyHost(3, mesh.nv, els, nob, nobi)
+        hf = N.H1Space(hm, p, all_bits)
+        t_topo_host = time.perf_counter() - t0
+        N.H1TablesDevice(3, mesh.nv, els, nob, nobi, p, all_bits)  # first call loads the kernels
+        N.synchronize()
+        t0 = time.perf_counter()
+        D = N.H1TablesDevice(3, mesh.nv, els, nob, nobi, p, all_bits)
+        t_topo_dev = time.perf_counter() - t0
+        same_tables = bool(np.array_equal(D["el_dofs"], hf.el_dofs) and np.array_equal(D["freedofs"], hf.freedofs)
+                           and np.array_equal(D["faces"], hm.faces) and np.array_equal(D["edges"], hm.edges))
+        del D, hf, hm
+        extra["h1_setup"] = {
+            "host_seconds": round(t_topo_host, 4), "device_seconds": round(t_topo_dev, 4),
+            "identical": same_tables,
+            "note": "edge/face numbering, boundary, element dof tables, free dofs, element classes; device time "
+                    "includes uploading the element table and downloading every table"}
+        log(f"[bench] topology + H1 dofs host {t_topo_host:.3f}s device {t_topo_dev:.3f}s identical {same_tables}")
+
     # ----------------------------------------------------------------- CG
     if not args.no_cg:
         diag = np.empty(ndof)

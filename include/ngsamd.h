@@ -178,6 +178,38 @@ int ngsamd_mf_pattern(ngsamd_op_t mf, int64_t* h_rowptr, int* h_cols);
 int ngsamd_mf_linear_form(ngsamd_op_t mf, int nclass, int nq, int nd, int64_t ne, const double* h_weights,
                           const double* h_shape, const double* h_detj, const double* d_fvals, double* d_out);
 
+/* ---------------------------------------------- mesh topology, H1 dof tables */
+typedef struct ngsamd_h1_s* ngsamd_h1_t;
+/* Builds on the device the edge and face numbering of a simplex mesh
+ * (h_els: ne x (dim+1) vertex numbers) and the H1 space of the given order
+ * on it: global edges / faces in ascending lexicographic order of their
+ * sorted vertex tuples, per-element edge / face numbers, the edges (and in
+ * 3D the face) of each boundary facet (h_bnd: nbnd x dim vertex numbers,
+ * h_bnd_index: 1-based region; in 3D with nbnd = 0 the faces held by one
+ * element become the boundary, region 1, in face order), the dof layout
+ * (vertex dofs, p-1 per edge, (p-1)(p-2)/2 per face, cell dofs), the
+ * element dof tables, the free-dof mask (dofs on facets of the regions
+ * whose bit r-1 is set in dirichlet_mask are not free) and each element's
+ * vertex-order class.  Replaces netgen's topology tables as NGSolve's
+ * MeshAccess exposes them (comp/meshaccess.cpp), H1HighOrderFESpace::Update
+ * (comp/h1hofespace.cpp:823-950), GetDofNrs (:1389-1413) and the free-dof
+ * mask of FESpace::Update (comp/fespace.cpp:765-810). */
+int ngsamd_h1_create(ngsamd_h1_t* h, int dim, int64_t nv, int64_t ne, const int* h_els, int64_t nbnd,
+                     const int* h_bnd, const int* h_bnd_index, int order, uint64_t dirichlet_mask);
+/* h_sizes[6] = {nedges, nfaces, nbnd, ndof, dofs per element, ne} */
+int ngsamd_h1_sizes(ngsamd_h1_t h, int64_t* h_sizes);
+/* copies the topology tables to the host (any pointer may be NULL):
+ * edges [nedges][2], faces [nfaces][3], element edges [ne][3 or 6],
+ * element faces [ne][4], boundary facets [nbnd][dim], their regions,
+ * their edges [nbnd][1 or 3] and faces [nbnd] */
+int ngsamd_h1_topology(ngsamd_h1_t h, int* h_edges, int* h_faces, int* h_el_edges, int* h_el_faces, int* h_bnd,
+                       int* h_bnd_index, int* h_bnd_edges, int* h_bnd_faces);
+/* copies the element dof tables [ne][nd], the free-dof mask [ndof] and the
+ * element classes [ne] to the host (any may be NULL); *d_el_dofs (if not
+ * NULL) receives the device copy of the dof tables, valid until destroy */
+int ngsamd_h1_dofs(ngsamd_h1_t h, int* h_el_dofs, unsigned char* h_freedofs, int* h_classnr, const int** d_el_dofs);
+int ngsamd_h1_destroy(ngsamd_h1_t h);
+
 /* ----------------------------------------------------------------- CG */
 /* Preconditioned CG with the recurrence and stopping rule of the reference's
  * CGSolver (python/krylovspace.py:227-291): stop when

@@ -105,7 +105,11 @@ PYBIND11_MODULE(_ngsamd, m) {
       })
       .def_property_readonly("element_faces", [](const Mesh& mm) {
         return ToNumpy(mm.el_faces).reshape({(py::ssize_t)mm.NE(), (py::ssize_t)4});
-      });
+      })
+      .def_property_readonly("boundary_edges", [](const Mesh& mm) {
+        return ToNumpy(mm.bnd_edges).reshape({(py::ssize_t)mm.NBnd(), (py::ssize_t)(mm.dim == 2 ? 1 : 3)});
+      })
+      .def_property_readonly("boundary_faces", [](const Mesh& mm) { return ToNumpy(mm.bnd_faces); });
   m.def("MakeStructured2DMesh", [](int nx, int ny) { return std::make_shared<Mesh>(MakeStructured2DMesh(nx, ny)); },
         py::arg("nx"), py::arg("ny"), py::call_guard<py::gil_scoped_release>());
   m.def("MakeStructured3DMesh",
@@ -444,6 +448,54 @@ PYBIND11_MODULE(_ngsamd, m) {
     RequireF64(out, mf.height, "LinearFormDevice");
     Check(ngsamd_mf_linear_form(mf.op, (int)shape.shape(0), nq, (int)shape.shape(2), (int64_t)detj.size(),
                                 weights.data(), shape.data(), detj.data(), fvals.d(), out.d()));
+  });
+  m.def("H1TablesDevice", [](int dim, int64_t nv, Arr<int> els, Arr<int> bnd, Arr<int> bnd_index, int order,
+                             uint64_t dirichlet_mask) {
+    if (dim != 2 && dim != 3) throw std::runtime_error("H1TablesDevice: dim must be 2 or 3");
+    if (els.size() % (dim + 1) || bnd.size() % dim || bnd_index.size() * dim != bnd.size())
+      throw std::runtime_error("H1TablesDevice: els must be [ne][dim+1], bnd [nbnd][dim], bnd_index [nbnd]");
+    const int64_t ne = els.size() / (dim + 1), nb = bnd_index.size();
+    ngsamd_h1_t h = nullptr;
+    Check(ngsamd_h1_create(&h, dim, nv, ne, els.data(), nb, bnd.data(), bnd_index.data(), order, dirichlet_mask));
+    std::unique_ptr<ngsamd_h1_s, int (*)(ngsamd_h1_t)> guard(h, ngsamd_h1_destroy);
+    int64_t s[6];
+    Check(ngsamd_h1_sizes(h, s));
+    const int64_t ned = s[0], nfa = s[1], nbo = s[2], ndof = s[3], nd = s[4];
+    const int nle = dim == 2 ? 3 : 6, nbe = dim == 2 ? 1 : 3;
+    py::array_t<int> edges({ned, (int64_t)2}), faces({nfa, (int64_t)3}), el_edges({ne, (int64_t)nle}),
+        el_faces({dim == 3 ? ne : 0, (int64_t)4}), obnd({nbo, (int64_t)dim}), obi(nbo),
+        bnd_edges({nbo, (int64_t)nbe}), bnd_faces(dim == 3 ? nbo : 0), el_dofs({ne, nd}), classnr(ne);
+    py::array_t<uint8_t> freedofs(ndof);
+    Check(ngsamd_h1_topology(h, edges.mutable_data(), faces.mutable_data(), el_edges.mutable_data(),
+                             el_faces.mutable_data(), obnd.mutable_data(), obi.mutable_data(),
+                             bnd_edges.mutable_data(), bnd_faces.mutable_data()));
+    Check(ngsamd_h1_dofs(h, el_dofs.mutable_data(), freedofs.mutable_data(), classnr.mutable_data(), nullptr));
+    py::dict d;
+    d["ndof"] = ndof;
+    d["edges"] = edges;
+    d["faces"] = faces;
+    d["element_edges"] = el_edges;
+    d["element_faces"] = el_faces;
+    d["boundary"] = obnd;
+    d["boundary_index"] = obi;
+    d["boundary_edges"] = bnd_edges;
+    d["boundary_faces"] = bnd_faces;
+    d["el_dofs"] = el_dofs;
+    d["freedofs"] = freedofs;
+    d["classnr"] = classnr;
+    return d;
+  });
+  m.def("BuildTopologyHost", [](int dim, int64_t nv, Arr<int> els, Arr<int> bnd, Arr<int> bnd_index) {
+    // the host setup's topology from the same arrays (for comparison)
+    auto mesh = std::make_shared<Mesh>();
+    mesh->dim = dim;
+    mesh->nv = (int)nv;
+    mesh->coords.assign((size_t)nv * dim, 0.0);
+    mesh->els.assign(els.data(), els.data() + els.size());
+    mesh->bnd.assign(bnd.data(), bnd.data() + bnd.size());
+    mesh->bnd_index.assign(bnd_index.data(), bnd_index.data() + bnd_index.size());
+    mesh->BuildTopology();
+    return mesh;
   });
   m.def("PatternDevice", [](Operator& mf) {
     if (mf.kind != "matfree") throw std::runtime_error("PatternDevice: matrix-free operator expected");
