@@ -18,6 +18,11 @@
  *               UnifiedVector Add/Scale/SetScalar      ngscuda/unifiedvector.cpp:136-212,437-515
  *   CG          CGSolver._SolveImpl                    python/krylovspace.py:263-291
  *               DevCGSolver (graph-captured loop)      ngscuda/cuda_krylov.cpp:19-203
+ *   block Jac.  BlockJacobiPrecond                     linalg/blockjacobi.cpp:390-632
+ *   assembly    S_BilinearForm::DoAssemble             comp/bilinearform.cpp:1563-2215
+ *               BilinearForm::GetGraph / MatrixGraph   comp/bilinearform.cpp:552-825
+ *               SymbolicLinearFormIntegrator           fem/symbolicintegrator.cpp:1036-1110
+ *   setup       H1HighOrderFESpace::Update, GetDofNrs  comp/h1hofespace.cpp:823-950,1389-1413
  */
 #ifndef NGSAMD_H
 #define NGSAMD_H
