@@ -163,6 +163,17 @@ int ngsamd_mf_element_grad(ngsamd_op_t op, const double* d_lam, const double* d_
  * SparseMatrixTM::AddElementMatrix (linalg/sparsematrix_impl.hpp:150-189)
  * for the grad-grad form. */
 int ngsamd_mf_assemble_csr(ngsamd_op_t mf, const int64_t* h_rowptr, const int* h_cols, ngsamd_op_t* csr);
+/* Assembles the mass form M[r][c] = sum_e h_scale[e] sum_q w_q phi_i(x_q)
+ * phi_j(x_q) (r = dofs[e][i], c = dofs[e][j]) on the mesh and dof tables of
+ * a matrix-free operator into a new CSR operator with the pattern (h_rowptr,
+ * h_cols); h_scale per global element (rho_e |det F_e|, ne as at creation),
+ * nq / h_weights the form's rule, h_shape phi per class [nclass][nq][nd] in
+ * the creation's class order.  Each value is summed in a fixed order.
+ * S_BilinearForm::DoAssemble (comp/bilinearform.cpp:1563-2215) for rho*u*v*dx
+ * (SymbolicBilinearFormIntegrator, fem/symbolicintegrator.cpp:1549-1860). */
+int ngsamd_mf_assemble_mass_csr(ngsamd_op_t mf, int nclass, int nq, int nd, int64_t ne, const double* h_weights,
+                                const double* h_shape, const double* h_scale, const int64_t* h_rowptr,
+                                const int* h_cols, ngsamd_op_t* csr);
 /* Sparsity pattern of a matrix-free operator's form, built on the device:
  * row r is the sorted union of the dofs of the elements holding dof r, as
  * BilinearForm::GetGraph (comp/bilinearform.cpp:552-825) -> MatrixGraph

@@ -341,8 +341,8 @@ class DevMatrixFree(DeviceMatrix):
 class BilinearForm:
     def __init__(self, space, integrals=None, mf=None, matrix_free_bdb=False, device_assembly=False, **kwargs):
         """device_assembly=True builds the sparsity pattern and the CSR
-        values on the device (ngsamd_mf_pattern, ngsamd_mf_assemble_csr; one
-        grad(u)*grad(v) term)."""
+        values on the device (ngsamd_mf_pattern, ngsamd_mf_assemble_csr /
+        ngsamd_mf_assemble_mass_csr; one grad(u)*grad(v) or u*v term)."""
         if isinstance(space, _Integral):  # BilinearForm(grad(u)*grad(v)*dx)
             integrals, space = space, None
         self.integrals = []
@@ -386,12 +386,26 @@ class BilinearForm:
                                      self.mf.atomic)
             return self
         if self.device_assembly:
-            if len(self.integrals) != 1 or self.integrals[0][0].kind != "laplace":
-                raise ValueError("device assembly supports one grad(u)*grad(v) term")
+            if len(self.integrals) != 1:
+                raise ValueError("device assembly supports one grad(u)*grad(v) or u*v term")
             term, meas = self.integrals[0]
-            mf = DevMatrixFree(fes, self._order(term, meas), self._coef(term), "f64")
-            rowptr, cols = N.PatternDevice(mf.op)
-            op = N.AssembleCSRDevice(mf.op, rowptr, cols)
+            order = self._order(term, meas)
+            if term.kind == "laplace":
+                mf = DevMatrixFree(fes, order, self._coef(term), "f64")
+                rowptr, cols = N.PatternDevice(mf.op)
+                op = N.AssembleCSRDevice(mf.op, rowptr, cols)
+            else:
+                # the mesh and dof tables of a matrix-free operator, the mass
+                # form's own rule and coefficient
+                mf = DevMatrixFree(fes, max(2 * fes.order - 2, 1), None, "f64")
+                rowptr, cols = N.PatternDevice(mf.op)
+                ir = fes.integration_rule(order)
+                shape = np.ascontiguousarray(np.stack([c["shape"] for c in N.BuildClasses(fes.space, ir)]))
+                detj, _ = N.ComputeGeometry(fes.mesh.ngmesh)
+                rho = self._coef(term)
+                scale = np.abs(detj) if rho is None else np.asarray(rho, dtype=np.float64) * np.abs(detj)
+                op = N.AssembleMassCSRDevice(mf.op, np.ascontiguousarray(ir.weights), shape,
+                                             np.ascontiguousarray(scale), rowptr, cols)
             self.mat = DevSparseMatrix(fes.ndof, fes.ndof, rowptr, cols, None, symmetric=True, op=op)
             return self
         rowptr, cols = N.BuildPattern(fes.space)

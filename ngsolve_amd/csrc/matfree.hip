@@ -1409,8 +1409,10 @@ void MatFreeOp::LinearForm(int nq_lf, const double* h_w, const double* h_shape, 
 // written once at the end.  Deterministic: every value is summed in slot
 // order.  RUNS: the target is the runs layout (the row's list is its run's
 // shared list, values at val_ptr[u] + position * R + row in run).  A column
-// missing from the pattern sets *bad.
-template <int DIM, bool RUNS>
+// missing from the pattern sets *bad.  NC = 1 with metric NULL is the mass
+// form: K_e = kappa_e S (S the class's weighted shape products, kappa_e the
+// element's coefficient times |det F|).
+template <int NC, bool RUNS>
 __global__ __launch_bounds__(64) void AssembleCsrKernel(
     const int* __restrict__ g_rowptr, const int* __restrict__ g_slots, const int* __restrict__ g_order,
     int64_t nrows, const int* __restrict__ dofs, const double* __restrict__ metric,
@@ -1418,7 +1420,6 @@ __global__ __launch_bounds__(64) void AssembleCsrKernel(
     const int64_t* __restrict__ cls_range, int nclass, const int* __restrict__ lptr,
     const int* __restrict__ lcols, const int* __restrict__ row_run, const int* __restrict__ run_row,
     const int* __restrict__ val_ptr, double* __restrict__ vals, int max_len, int* __restrict__ bad) {
-  constexpr int NC = DIM * (DIM + 1) / 2;
   extern __shared__ __align__(16) unsigned char lds_raw[];
   double* lval = reinterpret_cast<double*>(lds_raw);
   int* lcol = reinterpret_cast<int*>(lds_raw + sizeof(double) * (size_t)max_len);
@@ -1460,7 +1461,7 @@ __global__ __launch_bounds__(64) void AssembleCsrKernel(
         ce[lane] = e;
         ci[lane] = (slot - e * nd) | (c << 16);
 #pragma unroll
-        for (int k = 0; k < NC; k++) cg[lane * NC + k] = kap * metric[(int64_t)e * NC + k];
+        for (int k = 0; k < NC; k++) cg[lane * NC + k] = metric ? kap * metric[(int64_t)e * NC + k] : kap;
       }
       __syncthreads();
       // lane j < nd (nd <= 64) adds entry j of each contribution's row; the
@@ -1506,6 +1507,36 @@ __global__ __launch_bounds__(64) void AssembleCsrKernel(
 }
 
 void MatFreeOp::AssembleCsr(CsrOp& A, hipStream_t st) {
+  AssembleCsrWith(A, smat_asm.p, metric.p, Kappa(), dim * (dim + 1) / 2, st);
+}
+
+// Mass form rho_e |det F_e| sum_q w_q phi_i phi_j: the class matrices
+// S_c[i][j] = sum_q w_q phi_c[q][i] phi_c[q][j] (host, FP64, q ascending)
+// and the element factors go through the same assembly kernel with one
+// component.
+void MatFreeOp::AssembleMassCsr(CsrOp& A, int nq_m, const double* h_w, const double* h_shape,
+                                const double* h_scale, hipStream_t st) {
+  NGSAMD_REQUIRE(nq_m >= 1, "ngsamd_mf_assemble_mass_csr: empty rule");
+  const size_t nc = classes.size();
+  std::vector<double> sm(std::max<size_t>(nc * nd * nd, 1), 0.0);
+  for (size_t c = 0; c < nc; c++) {
+    const double* sh = h_shape + c * (size_t)nq_m * nd;
+    double* S = sm.data() + c * (size_t)nd * nd;
+    for (int i = 0; i < nd; i++)
+      for (int j = 0; j < nd; j++) {
+        double v = 0.0;
+        for (int q = 0; q < nq_m; q++) v += h_w[q] * sh[(size_t)q * nd + i] * sh[(size_t)q * nd + j];
+        S[(size_t)i * nd + j] = v;
+      }
+  }
+  DevBuf<double> smass(sm.size()), scale(std::max<int64_t>(ne_global, 1));
+  smass.upload(sm.data(), sm.size());
+  if (ne_global) scale.upload(h_scale, ne_global);
+  AssembleCsrWith(A, smass.p, nullptr, scale.p, 1, st);  // synchronises before the buffers are freed
+}
+
+void MatFreeOp::AssembleCsrWith(CsrOp& A, const double* smat, const double* met, const double* kap, int nc,
+                                hipStream_t st) {
   NGSAMD_REQUIRE(A.height == height && A.width == width, "ngsamd_mf_assemble_csr: pattern size differs");
   if (height == 0 || nel == 0 || A.nnz == 0) return;
   const size_t lds = (size_t)A.max_row_len * (sizeof(double) + sizeof(int));
@@ -1515,12 +1546,13 @@ void MatFreeOp::AssembleCsr(CsrOp& A, hipStream_t st) {
   bad.alloc(1);
   NGSAMD_HIP(hipMemsetAsync(bad.p, 0, sizeof(int), st));
   const bool runs = A.algo == 2;
-  auto kern = dim == 2 ? (runs ? AssembleCsrKernel<2, true> : AssembleCsrKernel<2, false>)
-                       : (runs ? AssembleCsrKernel<3, true> : AssembleCsrKernel<3, false>);
+  auto kern = nc == 1   ? (runs ? AssembleCsrKernel<1, true> : AssembleCsrKernel<1, false>)
+              : nc == 3 ? (runs ? AssembleCsrKernel<3, true> : AssembleCsrKernel<3, false>)
+                        : (runs ? AssembleCsrKernel<6, true> : AssembleCsrKernel<6, false>);
   NGSAMD_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   const int grid = (int)std::min<int64_t>(height, 65536);
-  kern<<<grid, 64, lds, st>>>(g_rowptr.p, g_slots.p, g_order.p, height, dofs.p, metric.p, Kappa(), elnr.p, nd,
-                              smat_asm.p, cls_range.p, (int)classes.size(), runs ? A.run_ptr.p : A.rowptr.p,
+  kern<<<grid, 64, lds, st>>>(g_rowptr.p, g_slots.p, g_order.p, height, dofs.p, met, kap, elnr.p, nd,
+                              smat, cls_range.p, (int)classes.size(), runs ? A.run_ptr.p : A.rowptr.p,
                               runs ? A.rcols.p : A.cols.p, A.row_run.p, A.run_row.p, A.val_ptr.p,
                               runs ? A.rvals.p : A.vals.p, (int)A.max_row_len, bad.p);
   NGSAMD_CHECK_LAUNCH();
@@ -1715,6 +1747,26 @@ extern "C" int ngsamd_mf_assemble_csr(ngsamd_op_t mf, const int64_t* rowptr, con
     NGSAMD_REQUIRE(rowptr[M->height] == 0 || cols, "ngsamd_mf_assemble_csr: null argument");
     std::unique_ptr<CsrOp> A(new CsrOp(M->height, M->width, rowptr, cols, nullptr));
     M->AssembleCsr(*A, Stream());
+    *csr = A.release();
+  });
+}
+
+extern "C" int ngsamd_mf_assemble_mass_csr(ngsamd_op_t mf, int nclass, int nq, int nd, int64_t ne,
+                                           const double* h_weights, const double* h_shape, const double* h_scale,
+                                           const int64_t* rowptr, const int* cols, ngsamd_op_t* csr) {
+  return Guard([&] {
+    RequireInit();
+    NGSAMD_REQUIRE(mf && mf->kind == OpKind::MatrixFree,
+                   "ngsamd_mf_assemble_mass_csr: not a matrix-free operator");
+    NGSAMD_REQUIRE(csr && rowptr && h_weights && h_shape && (h_scale || ne == 0),
+                   "ngsamd_mf_assemble_mass_csr: null argument");
+    *csr = nullptr;
+    MatFreeOp* M = static_cast<MatFreeOp*>(mf);
+    NGSAMD_REQUIRE(nclass == (int)M->classes.size() && nd == M->nd && ne == M->ne_global,
+                   "ngsamd_mf_assemble_mass_csr: shape or element count does not match the operator");
+    NGSAMD_REQUIRE(rowptr[M->height] == 0 || cols, "ngsamd_mf_assemble_mass_csr: null argument");
+    std::unique_ptr<CsrOp> A(new CsrOp(M->height, M->width, rowptr, cols, nullptr));
+    M->AssembleMassCsr(*A, nq, h_weights, h_shape, h_scale, Stream());
     *csr = A.release();
   });
 }

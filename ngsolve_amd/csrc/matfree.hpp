@@ -72,6 +72,10 @@ struct MatFreeOp : ngsamd_op_s {
   void ElementGrad(const double* lam, const double* u, double* grad, hipStream_t st);
   // fills the values of A (same size, pattern holding every element pair)
   void AssembleCsr(CsrOp& A, hipStream_t st);
+  void AssembleMassCsr(CsrOp& A, int nq_m, const double* h_w, const double* h_shape, const double* h_scale,
+                       hipStream_t st);
+  void AssembleCsrWith(CsrOp& A, const double* smat, const double* met, const double* kap, int nc,
+                       hipStream_t st);
   // linear form of element quadrature values into d_out (overwritten)
   void LinearForm(int nq_lf, const double* h_w, const double* h_shape, const double* h_detj, const double* d_fvals,
                   double* d_out, hipStream_t st);

@@ -449,6 +449,21 @@ PYBIND11_MODULE(_ngsamd, m) {
     Check(ngsamd_mf_linear_form(mf.op, (int)shape.shape(0), nq, (int)shape.shape(2), (int64_t)detj.size(),
                                 weights.data(), shape.data(), detj.data(), fvals.d(), out.d()));
   });
+  m.def("AssembleMassCSRDevice", [](Operator& mf, Arr<double> weights, Arr<double> shape, Arr<double> scale,
+                                     Arr<int64_t> rowptr, Arr<int> cols) {
+    if (mf.kind != "matfree") throw std::runtime_error("AssembleMassCSRDevice: matrix-free operator expected");
+    const int nq = (int)weights.size();
+    if (shape.ndim() != 3 || shape.shape(1) != nq)
+      throw std::runtime_error("AssembleMassCSRDevice: shape must be [nclass][nq][nd] with nq = len(weights)");
+    if (rowptr.size() != mf.height + 1 || rowptr.data()[mf.height] != (int64_t)cols.size())
+      throw std::runtime_error("AssembleMassCSRDevice: inconsistent CSR arrays");
+    ngsamd_op_t op = nullptr;
+    Check(ngsamd_mf_assemble_mass_csr(mf.op, (int)shape.shape(0), nq, (int)shape.shape(2), (int64_t)scale.size(),
+                                      weights.data(), shape.data(), scale.data(), rowptr.data(), cols.data(), &op));
+    auto A = std::make_shared<Operator>(op, "csr");
+    A->nnz = (int64_t)cols.size();
+    return A;
+  });
   m.def("H1TablesDevice", [](int dim, int64_t nv, Arr<int> els, Arr<int> bnd, Arr<int> bnd_index, int order,
                              uint64_t dirichlet_mask) {
     if (dim != 2 && dim != 3) throw std::runtime_error("H1TablesDevice: dim must be 2 or 3");

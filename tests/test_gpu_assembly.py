@@ -8,7 +8,9 @@ largest entry, in every CSR layout the operator may choose (runs, stream,
 rowgroup).  Also against the product's host assembly on the same pattern,
 the assembled operator's apply against the oracle SpMV, determinism (two
 assemblies are bit-identical) and the refusal of a pattern that lacks a
-pair of dofs of one element."""
+pair of dofs of one element.  Mass assembly on the device
+(ngsamd_mf_assemble_mass_csr) against the oracle's mass matrix (pinned by
+the reference's known mass entry), with an element coefficient."""
 import numpy as np
 import pytest
 
@@ -143,3 +145,55 @@ def test_linear_form_matches_oracle(device, dim, n, p):
     ref = pr.linear_form(fvals)
     assert relerr_inf(got, ref) < 1e-13
     assert relerr_inf(got, N.AssembleLinearForm(P["fes"], ir, fvals)) < 1e-13
+
+
+MASS_CASES = [(2, 6, 1), (2, 5, 3), (2, 4, 5), (3, 3, 1), (3, 3, 2), (3, 2, 4)]
+
+
+@pytest.mark.parametrize("dim,n,p", MASS_CASES)
+@pytest.mark.parametrize("algo", ["runs", "stream"])
+def test_mass_assembly_matches_oracle(device, monkeypatch, dim, n, p, algo):
+    """rho * u * v * dx with its own rule (order 2p) and a lognormal element
+    coefficient, against the oracle's mass matrix (1e-13 of the largest
+    entry) and its SpMV."""
+    monkeypatch.setenv("NGSAMD_SPMV", algo)
+    say(f"[mass {algo}] dim={dim} n={n} p={p}")
+    P = problem(dim, n, p)
+    M = mf_operator(P)
+    order = 2 * p
+    ir = N.TrigRule(order) if dim == 2 else N.TetRule(order)
+    shape = np.ascontiguousarray(np.stack([c["shape"] for c in N.BuildClasses(P["fes"], ir)]))
+    detj, _ = N.ComputeGeometry(P["mesh"])
+    rho = np.exp(np.random.default_rng(11).standard_normal(P["mesh"].ne))
+    A = N.AssembleMassCSRDevice(M, np.ascontiguousarray(ir.weights), shape, rho * np.abs(detj), P["rowptr"],
+                                P["cols"])
+    pr = O.Problem(P["omesh"], p, intorder=order)
+    orp, ocols, ovals = pr.assemble(stiffness=False, mass=True, rho=rho)
+    assert np.array_equal(orp, P["rowptr"]) and np.array_equal(ocols, P["cols"])
+    vals = A.csr_values()
+    assert np.abs(vals - ovals).max() < 1e-13 * np.abs(ovals).max()
+    x = np.random.default_rng(1).uniform(-1, 1, P["ndof"])
+    dx, dy = dvec(x), N.DeviceVector(P["ndof"], 8)
+    A.mult(dx, dy)
+    assert relerr_inf(dy.download(), oracle.spmv(orp, ocols, ovals, x)) < 1e-13
+
+
+def test_mass_bilinear_form_on_device(device):
+    """BilinearForm(rho*u*v*dx, device_assembly=True) against the host
+    assembly; c^T M c = integral of rho for the coefficients c of the
+    constant 1."""
+    import ngsolve_amd as ngs
+    say("[mass] BilinearForm(device_assembly=True)")
+    m = ngs.MakeStructured3DMesh(nx=3)
+    fes = ngs.H1(m, order=3)
+    u, v = fes.TnT()
+    rho = np.exp(np.random.default_rng(2).standard_normal(m.ne))
+    c = ngs.ElementCoefficient(rho)
+    a_dev = ngs.BilinearForm(c * u * v * ngs.dx, device_assembly=True).Assemble()
+    a_host = ngs.BilinearForm(c * u * v * ngs.dx).Assemble()
+    vh, vd = a_host.mat.vals, a_dev.mat.vals
+    assert np.abs(vh - vd).max() < 1e-13 * np.abs(vh).max()
+    one = np.zeros(fes.ndof)
+    one[:m.ngmesh.nv] = 1.0  # the constant 1 is the sum of the vertex functions (the first nv dofs)
+    detj, _ = N.ComputeGeometry(m.ngmesh)
+    assert abs(one @ oracle.spmv(a_dev.mat.rowptr, a_dev.mat.cols, vd, one) - np.sum(rho * np.abs(detj)) / 6) < 1e-12
