@@ -366,12 +366,14 @@ def main():
             return float(np.linalg.norm(r[free]) / np.linalg.norm(f[free]))
 
         t0 = time.perf_counter()
-        its_k, _ = N.CGSolve(Ak, Ck, df, du, True, 1e-10, -1.0, 200000, 16)
+        its_k, hist_k = N.CGSolve(Ak, Ck, df, du, True, 1e-10, -1.0, 200000, 16)
         t_k = time.perf_counter() - t0
         u64 = du.download()
         log(f"[bench] FP64 CSR CG lognormal: {its_k} its {t_k:.3f}s")
         M64 = build_mf(N, fes, mesh, ir, kappa, 0)
-        res = {"fp64_csr_cg": {"iterations": its_k, "seconds": round(t_k, 4), "true_rel_residual": true_res(u64)}}
+        res = {"fp64_csr_cg": {"iterations": its_k, "seconds": round(t_k, 4), "true_rel_residual": true_res(u64),
+                               # the reference's stopping quantity sqrt|w.d|, last over first
+                               "stopping_ratio": float(hist_k[-1] / hist_k[0]) if len(hist_k) > 1 else None}}
         # the same FP64 CG on the FP64 matrix-free operator
         du_mf = N.DeviceVector(ndof, 8)
         t0 = time.perf_counter()
