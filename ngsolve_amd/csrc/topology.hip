@@ -85,23 +85,49 @@ __global__ __launch_bounds__(kTopoBlock) void BucketFillKernel(const int* __rest
   }
 }
 
-// One thread per vertex: sort its bucket (insertion sort; buckets hold the
-// few dozen edges or faces around a vertex), keep the distinct keys at the
-// front with their multiplicities, count them.
+// Ascending heap sort of k[0..m): O(m log m) for the rare large bucket (a
+// vertex shared by very many elements).
+__device__ void HeapSort(uint64_t* k, int m) {
+  auto sift = [k](int i, int n) {
+    const uint64_t x = k[i];
+    for (int c = 2 * i + 1; c < n; c = 2 * i + 1) {
+      if (c + 1 < n && k[c + 1] > k[c]) c++;
+      if (k[c] <= x) break;
+      k[i] = k[c];
+      i = c;
+    }
+    k[i] = x;
+  };
+  for (int i = m / 2 - 1; i >= 0; i--) sift(i, m);
+  for (int n = m - 1; n > 0; n--) {
+    const uint64_t t = k[0];
+    k[0] = k[n];
+    k[n] = t;
+    sift(0, n);
+  }
+}
+
+// One thread per vertex: sort its bucket (insertion sort for the usual few
+// dozen edges or faces around a vertex, heap sort above 64), keep the
+// distinct keys at the front with their multiplicities, count them.
 __global__ __launch_bounds__(kTopoBlock) void BucketUniqueKernel(const int* __restrict__ off, int nv,
                                                                  uint64_t* __restrict__ keys,
                                                                  int* __restrict__ mult, int* __restrict__ ucnt) {
   for (int a = blockIdx.x * kTopoBlock + threadIdx.x; a < nv; a += gridDim.x * kTopoBlock) {
     uint64_t* k = keys + off[a];
     const int m = off[a + 1] - off[a];
-    for (int i = 1; i < m; i++) {
-      const uint64_t x = k[i];
-      int j = i - 1;
-      while (j >= 0 && k[j] > x) {
-        k[j + 1] = k[j];
-        j--;
+    if (m > 64) {
+      HeapSort(k, m);
+    } else {
+      for (int i = 1; i < m; i++) {
+        const uint64_t x = k[i];
+        int j = i - 1;
+        while (j >= 0 && k[j] > x) {
+          k[j + 1] = k[j];
+          j--;
+        }
+        k[j + 1] = x;
       }
-      k[j + 1] = x;
     }
     int* mu = mult + off[a];
     int u = 0;

@@ -110,6 +110,21 @@ def test_h1_tables_large_3d(device):
     _check_against_host(D, m, fes)
 
 
+@pytest.mark.parametrize("nfan", [8, 40, 1000])
+def test_fan_mesh_large_bucket(device, nfan):
+    """A disc of nfan triangles around vertex 0: its bucket holds 2 nfan
+    edge entries (heap-sorted above 64), device against host."""
+    say(f"[h1 tables] fan of {nfan} triangles")
+    i = np.arange(1, nfan + 1)
+    els = np.stack([np.zeros(nfan, dtype=np.int32), i, i % nfan + 1], axis=1).astype(np.int32)
+    bnd = np.stack([i, i % nfan + 1], axis=1).astype(np.int32)
+    bi = np.ones(nfan, dtype=np.int32)
+    m = N.BuildTopologyHost(2, nfan + 1, els, bnd, bi)
+    for p in (1, 3):
+        D = N.H1TablesDevice(2, nfan + 1, els, bnd, bi, p, ALL)
+        _check_against_host(D, m, N.H1Space(m, p, ALL))
+
+
 def test_bad_input_refused(device):
     say("[h1 tables] bad input")
     m, _ = _mesh("2d")
