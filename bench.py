@@ -133,6 +133,63 @@ def bench_batch(N, args):
     return out
 
 
+def bench_2d(N, args):
+    """The reference's CPU-runnable case: 2D Poisson on 64^2 squares x 2
+    triangles, H1 p=3 (37,249 dofs, 628,609 nonzeros), kappa = 1, Dirichlet on
+    every side, x ~ U[-1,1] (seed 0), f = 1, Jacobi-CG to 1e-10.  Its 8.3 MB
+    per apply stay in the Infinity Cache, so the apply is bound by launch
+    latency, not by HBM; the oracle's CSR SpMV and PCG time the same work on
+    the host."""
+    m = N.MakeStructured2DMesh(64, 64)
+    fes = N.H1Space(m, 3, 0xFFFFFFFFFFFFFFFF)
+    ir = N.TrigRule(4)
+    rp, cols = N.BuildPattern(fes)
+    vals = N.AssembleCSR(fes, rp, cols, ir, True, None, False, None)
+    n = fes.ndof
+    A = N.CreateCSR(n, n, rp, cols, vals)
+    x = np.random.default_rng(0).uniform(-1, 1, n)
+    dx, dy = N.DeviceVector(n, 8), N.DeviceVector(n, 8)
+    dx.upload(x)
+    ms = timed(N, lambda: A.mult(dx, dy), 200, 20)
+    out = {"ndof": n, "nnz": int(rp[-1]), "csr_apply_us": round(ms * 1e3, 2),
+           "csr_gdofs": round(n / (ms * 1e-3) / 1e9, 3),
+           "csr_algorithmic_gbs": round((int(rp[-1]) * 12 + (n + 1) * 4 + 2 * n * 8) / (ms * 1e-3) / 1e9, 1)}
+    dd = N.DeviceVector(n, 8)
+    A.diagonal(dd)
+    diag = dd.download()
+    C = N.CreateJacobi(diag, fes.freedofs)
+    f = N.AssembleLinearForm(fes, N.TrigRule(6), np.ones((m.ne, len(N.TrigRule(6)))))
+    df, du = N.DeviceVector(n, 8), N.DeviceVector(n, 8)
+    df.upload(f)
+    N.CGSolve(A, C, df, du, True, 1e-10, -1.0, 100000, 16)  # warm-up (graph build)
+    N.synchronize()
+    t0 = time.perf_counter()
+    its, _ = N.CGSolve(A, C, df, du, True, 1e-10, -1.0, 100000, 16)
+    t_cg = time.perf_counter() - t0
+    out.update({"cg_iterations": its, "cg_seconds": round(t_cg, 4),
+                "cg_us_per_iteration": round(1e6 * t_cg / max(its - 1, 1), 2)})
+    log(f"[bench] 2D 64^2 p=3: apply {ms * 1e3:.1f} us, CG {its} its {t_cg * 1e3:.1f} ms")
+    if not args.no_cpu:
+        import oracle
+        y = np.empty(n)
+        oracle.spmv(rp, cols, vals, x, y)
+        t0 = time.perf_counter()
+        reps = 0
+        while time.perf_counter() - t0 < 2.0:
+            oracle.spmv(rp, cols, vals, x, y)
+            reps += 1
+        out["cpu_oracle_apply_us"] = round(1e6 * (time.perf_counter() - t0) / reps, 2)
+        free = fes.freedofs.astype(bool)
+        inv = np.where(free, 1.0 / np.where(diag == 0, 1.0, diag), 0.0)
+        t0 = time.perf_counter()
+        _, its_cpu, _ = oracle.pcg_csr(rp, cols, vals, inv, f, tol=1e-10, maxiter=100000)
+        out.update({"cpu_oracle_pcg_iterations": its_cpu, "cpu_oracle_pcg_seconds": round(time.perf_counter() - t0, 4),
+                    "cpu_threads": oracle.num_threads()})
+        log(f"[bench] 2D 64^2 p=3 CPU oracle: apply {out['cpu_oracle_apply_us']} us, PCG {its_cpu} its "
+            f"{out['cpu_oracle_pcg_seconds']} s")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -450,6 +507,7 @@ def main():
 
     if not args.no_batch:
         extra["batch_512_problems"] = bench_batch(N, args)
+    extra["poisson_2d_64sq_p3"] = bench_2d(N, args)
 
     cpu = None
     if not args.no_cpu:
