@@ -18,7 +18,7 @@ import re
 import numpy as np
 
 from ._native import N, PREC, SCATTER, ensure_device
-from .la import BaseVector, BitArray, DeviceMatrix, DevSparseMatrix, JacobiPrecond
+from .la import BaseVector, BitArray, BlockTable, DeviceMatrix, DevSparseMatrix, JacobiPrecond
 
 
 # ------------------------------------------------------------------- mesh
@@ -197,7 +197,8 @@ def smoothing_blocks(m, sp, blocktype=4):
     """Smoothing blocks of the free dofs of the H1 space sp on the mesh m
     (native objects) for BlockJacobiPrecond, as
     H1HighOrderFESpace::CreateSmoothingBlocks (comp/h1hofespace.cpp:1478-1620)
-    builds them (non-free dofs filtered out, so some blocks are empty):
+    builds them (non-free dofs filtered out, so some blocks are empty), as a
+    BlockTable (flat pointer and dof arrays, iterable as a list of blocks):
       1  2D: V + E + I   one block per vertex, edge and element interior
       2  2D: VE + I      per vertex its dof and the dofs of its edges; interiors
       3  V + E + F + I   one block per node
@@ -242,8 +243,8 @@ def smoothing_blocks(m, sp, blocktype=4):
     blk, dof = blk[keep], dof[keep]
     order = np.argsort(blk, kind="stable")
     blk, dof = blk[order], dof[order].astype(np.int32)
-    bounds = np.searchsorted(blk, np.arange(1, nblocks))
-    return np.split(dof, bounds)
+    ptr = np.searchsorted(blk, np.arange(nblocks + 1))
+    return BlockTable(ptr, dof)
 
 
 class H1:

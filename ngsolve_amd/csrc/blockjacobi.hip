@@ -13,6 +13,9 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
+#include <memory>
+#include <thread>
 #include <vector>
 
 #include "common.hpp"
@@ -23,6 +26,14 @@
 namespace ngsamd {
 
 constexpr int kBjBlock = 256;
+
+// Host threads for the set-up: OMP_NUM_THREADS if set, else the hardware's,
+// at most 64.
+static int HostThreads() {
+  const char* e = std::getenv("OMP_NUM_THREADS");
+  int n = e ? std::atoi(e) : (int)std::thread::hardware_concurrency();
+  return std::max(1, std::min(n, 64));
+}
 
 // z[t] = sum_c inv_b[r][c] x[dof(b, c)] for block row t = row r of block b;
 // 4 columns at a time from clamped indices so the loads are in flight
@@ -96,28 +107,52 @@ BlockJacobiOp::BlockJacobiOp(int64_t n, const int64_t* rp, const int* cols, cons
     }
     vp[b + 1] = vp[b] + sz * sz;
   }
-  // diagonal blocks from the matrix, inverted
-  std::vector<double> hinv(std::max<int64_t>(vp[nb], 1), 0.0);
-  int singular = 0;
-  for (int64_t b = 0; b < nb; b++) {
-    const int64_t b0 = block_ptr[b];
-    const int sz = (int)(block_ptr[b + 1] - b0);
-    double* m = hinv.data() + vp[b];
-    for (int j = 0; j < sz; j++) {
-      const int r = bd[b0 + j];
-      const int* rc = cols + rp[r];
-      const int* re = cols + rp[r + 1];
-      for (int k = 0; k < sz; k++) {
-        const int* pos = std::lower_bound(rc, re, bd[b0 + k]);
-        m[(size_t)j * sz + k] = (pos != re && *pos == bd[b0 + k]) ? vals[rp[r] + (pos - rc)] : 0.0;
+  // diagonal blocks from the matrix, inverted; blocks are independent, so
+  // host threads take contiguous block ranges (same result for any count).
+  // The buffer is pinned host memory (the upload of the inverses, often
+  // hundreds of MB, then runs at DMA rate) and left uninitialised: every
+  // entry is written by the thread that owns its block.
+  const size_t ninv = (size_t)std::max<int64_t>(vp[nb], 1);
+  double* hinv_p = nullptr;
+  NGSAMD_HIP(hipHostMalloc((void**)&hinv_p, ninv * sizeof(double), hipHostMallocDefault));
+  std::unique_ptr<double, hipError_t (*)(void*)> hinv(hinv_p, hipHostFree);
+  auto invert_range = [&](int64_t lo, int64_t hi) -> bool {
+    bool ok = true;
+    for (int64_t b = lo; b < hi; b++) {
+      const int64_t b0 = block_ptr[b];
+      const int sz = (int)(block_ptr[b + 1] - b0);
+      double* m = hinv.get() + vp[b];
+      for (int j = 0; j < sz; j++) {
+        const int r = bd[b0 + j];
+        const int* rc = cols + rp[r];
+        const int* re = cols + rp[r + 1];
+        for (int k = 0; k < sz; k++) {
+          const int* pos = std::lower_bound(rc, re, bd[b0 + k]);
+          m[(size_t)j * sz + k] = (pos != re && *pos == bd[b0 + k]) ? vals[rp[r] + (pos - rc)] : 0.0;
+        }
       }
+      if (sz && !InvertInPlace(m, sz)) ok = false;
+      // stored column by column (the apply's access order)
+      for (int j = 0; j < sz; j++)
+        for (int k = j + 1; k < sz; k++) std::swap(m[(size_t)j * sz + k], m[(size_t)k * sz + j]);
     }
-    if (sz && !InvertInPlace(m, sz)) singular = 1;
-    // stored column by column (the apply's access order)
-    for (int j = 0; j < sz; j++)
-      for (int k = j + 1; k < sz; k++) std::swap(m[(size_t)j * sz + k], m[(size_t)k * sz + j]);
+    return ok;
+  };
+  // ranges of equal work (sum of sz^2 entries; the large vertex-patch
+  // blocks come first in the usual block lists)
+  const int nthreads = (int)std::min<int64_t>(HostThreads(), std::max<int64_t>(1, vp[nb] / 100000));
+  std::vector<int64_t> cut(nthreads + 1, nb);
+  for (int t = 0; t < nthreads; t++)
+    cut[t] = std::lower_bound(vp.begin(), vp.end() - 1, vp[nb] / nthreads * t) - vp.begin();
+  std::vector<char> ok(nthreads, 1);
+  {
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nthreads; t++) pool.emplace_back([&, t] { ok[t] = invert_range(cut[t], cut[t + 1]); });
+    ok[0] = invert_range(cut[0], cut[1]);
+    for (auto& th : pool) th.join();
   }
-  NGSAMD_REQUIRE(!singular, "ngsamd_blockjacobi_create: a diagonal block is singular");
+  NGSAMD_REQUIRE(std::all_of(ok.begin(), ok.end(), [](char c) { return c != 0; }),
+                 "ngsamd_blockjacobi_create: a diagonal block is singular");
   std::vector<int> rb(std::max<int64_t>(nrows, 1), 0);
   for (int64_t b = 0; b < nb; b++)
     for (int64_t k = block_ptr[b]; k < block_ptr[b + 1]; k++) rb[k] = (int)b;
@@ -137,8 +172,9 @@ BlockJacobiOp::BlockJacobiOp(int64_t n, const int64_t* rp, const int* cols, cons
   if (nrows) bdofs.upload(bd.data(), nrows);
   row_block.alloc(rb.size());
   row_block.upload(rb.data(), rb.size());
-  inv.alloc(hinv.size());
-  inv.upload(hinv.data(), hinv.size());
+  if (vp[nb] == 0) hinv.get()[0] = 0.0;
+  inv.alloc(ninv);
+  inv.upload(hinv.get(), ninv);
   gptr.alloc(n + 1);
   gptr.upload(gp.data(), n + 1);
   grows.alloc(gr.size());

@@ -616,6 +616,32 @@ class JacobiPrecond(DeviceMatrix):
         super().__init__(op, True)
 
 
+class BlockTable:
+    """Blocks of dof numbers as two flat arrays (block b holds
+    dofs[ptr[b]:ptr[b+1]]), the form NGSolve's Table<int> has; indexing and
+    iteration give the blocks as arrays."""
+
+    def __init__(self, ptr, dofs):
+        self.ptr = np.ascontiguousarray(ptr, dtype=np.int64)
+        self.dofs = np.ascontiguousarray(dofs, dtype=np.int32)
+        if self.ptr.ndim != 1 or len(self.ptr) < 1 or self.ptr[0] != 0 or self.ptr[-1] != len(self.dofs) \
+                or np.any(np.diff(self.ptr) < 0):
+            raise ValueError("BlockTable: ptr must rise from 0 to len(dofs)")
+
+    def __len__(self):
+        return len(self.ptr) - 1
+
+    def __getitem__(self, b):
+        if b < 0:
+            b += len(self)
+        if not 0 <= b < len(self):
+            raise IndexError("block index out of range")
+        return self.dofs[self.ptr[b]:self.ptr[b + 1]]
+
+    def __iter__(self):
+        return (self.dofs[self.ptr[b]:self.ptr[b + 1]] for b in range(len(self)))
+
+
 class BlockJacobiPrecond(DeviceMatrix):
     """BlockJacobiPrecond (linalg/blockjacobi.cpp) on the device:
     y += s * sum_b P_b^T inv(A_bb) P_b x for the blocks (lists of dof
@@ -625,10 +651,12 @@ class BlockJacobiPrecond(DeviceMatrix):
 
     def __init__(self, mat, blocks):
         N = ensure_device()
-        blocks = [np.asarray(b, dtype=np.int32).ravel() for b in blocks]
-        bptr = np.zeros(len(blocks) + 1, dtype=np.int64)
-        bptr[1:] = np.cumsum([len(b) for b in blocks])
-        bd = np.concatenate(blocks).astype(np.int32) if blocks else np.zeros(0, dtype=np.int32)
+        if not isinstance(blocks, BlockTable):
+            blocks = [np.asarray(b, dtype=np.int32).ravel() for b in blocks]
+            bptr = np.zeros(len(blocks) + 1, dtype=np.int64)
+            bptr[1:] = np.cumsum([len(b) for b in blocks])
+            blocks = BlockTable(bptr, np.concatenate(blocks) if len(blocks) else np.zeros(0, dtype=np.int32))
+        bptr, bd = blocks.ptr, blocks.dofs
         op = N.CreateBlockJacobi(int(mat.Height()), np.ascontiguousarray(mat.rowptr, dtype=np.int64),
                                  np.ascontiguousarray(mat.cols, dtype=np.int32),
                                  np.ascontiguousarray(mat.vals, dtype=np.float64), bptr, bd)

@@ -47,3 +47,16 @@ def test_unknown_blocktype_refused():
     fes = ngs.H1(ngs.MakeStructured3DMesh(nx=1), order=2)
     with pytest.raises(ValueError):
         fes.CreateSmoothingBlocks(1)
+
+
+def test_block_table():
+    from ngsolve_amd.la import BlockTable
+    t = BlockTable([0, 2, 2, 5], [4, 7, 1, 2, 3])
+    assert len(t) == 3
+    assert [b.tolist() for b in t] == [[4, 7], [], [1, 2, 3]]
+    assert t[-1].tolist() == [1, 2, 3] and t[1].size == 0
+    with pytest.raises(IndexError):
+        t[3]
+    for bad in ([1, 2], [0, 3], [0, 2, 1, 5]):
+        with pytest.raises(ValueError):
+            BlockTable(bad, [4, 7, 1, 2, 3])
