@@ -410,6 +410,40 @@ def main():
                                        "ms_per_iteration": round(1e3 * t_cg / max(its - 1, 1), 4)}
         log(f"[bench] FP64 CSR CG kappa=1: {its} its {t_cg:.3f}s")
 
+        # K right-hand sides of the same matrix: the multi-vector apply reads
+        # the matrix once for all K; the lockstep CG (CGSolver::MultiMult)
+        # against K single solves one after the other
+        K = 4
+        Xk = np.random.default_rng(7).uniform(-1, 1, (ndof, K))
+        dXk, dYk = N.DeviceVector(ndof * K, 8), N.DeviceVector(ndof * K, 8)
+        dXk.upload(Xk.ravel())
+        ms_multi = timed(N, lambda: N.CSRMultMulti(A, dXk, dYk, K), args.steps, args.warmup)
+        Fk = np.random.default_rng(8).uniform(0, 1, (ndof, K)) * fes.freedofs[:, None]
+        dFk, dUk = N.DeviceVector(ndof * K, 8), N.DeviceVector(ndof * K, 8)
+        dFk.upload(Fk.ravel())
+        N.synchronize()
+        t0 = time.perf_counter()
+        steps_k, _ = N.CGSolveMulti(A, C, dFk, dUk, K, True, 1e-10, False, 100000)
+        t_multi = time.perf_counter() - t0
+        t_seq, its_seq = 0.0, []
+        for j in range(K):
+            df.upload(np.ascontiguousarray(Fk[:, j]))
+            N.synchronize()
+            t0 = time.perf_counter()
+            its_j, _ = N.CGSolve(A, C, df, du, True, 1e-10, -1.0, 100000, 16)
+            t_seq += time.perf_counter() - t0
+            its_seq.append(its_j)
+        df.upload(f)
+        extra["multi_rhs_4"] = {
+            "apply_us": round(ms_multi * 1e3, 1), "single_apply_us": round(ms_csr * 1e3, 1),
+            "apply_speedup_vs_4_singles": round(4 * ms_csr / ms_multi, 2),
+            "lockstep_cg_steps": steps_k, "lockstep_cg_seconds": round(t_multi, 4),
+            "single_cg_iterations": its_seq, "single_cg_seconds_total": round(t_seq, 4),
+            "cg_speedup": round(t_seq / t_multi, 2)}
+        log(f"[bench] 4 right-hand sides: apply {ms_multi * 1e3:.1f} us (single {ms_csr * 1e3:.1f}); lockstep CG "
+            f"{steps_k} steps {t_multi:.3f}s vs 4 single solves {t_seq:.3f}s")
+        del dXk, dYk, dFk, dUk
+
         # lognormal kappa variant: FP64 CSR CG vs mixed-precision IR
         kappa = np.exp(np.random.default_rng(2).standard_normal(ne))
         vals_k = N.AssembleCSR(fes, rowptr, cols, ir, True, kappa, False, None)

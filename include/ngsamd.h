@@ -252,6 +252,21 @@ int ngsamd_ir_cg_solve(ngsamd_op_t mat, ngsamd_op_t inner, ngsamd_op_t pre, cons
                        double* d_sol, double tol, double inner_tol, int max_outer, int max_inner,
                        double* h_stats);
 
+/* Lockstep CG for nrhs (1..8) right-hand sides of one CSR matrix, vectors
+ * with nrhs components per dof (entry j of dof i at d_x[i * nrhs + j]), Jacobi
+ * preconditioner or none (pre NULL): CGSolver<double>::MultiMult
+ * (linalg/cg.cpp:132-250) -- per component alpha and beta, one stopping test
+ * on the norm of the vector of w.d values (relative to its start, or tol^2
+ * absolute), the matrix read once per iteration for all components.
+ * *h_steps = the reference's `steps`; h_residuals (maxiter + 1 entries, may
+ * be NULL) = sqrt(||wdn||) at the start and after each iteration.  A component
+ * whose w.d or <s, A s> is 0 gets alpha = beta = 0 (the reference: 0/0). */
+int ngsamd_cg_solve_multi(ngsamd_op_t mat, ngsamd_op_t pre, int nrhs, const double* d_rhs, double* d_sol,
+                          int initialize, double tol, int absolute, int maxiter, int* h_steps, double* h_residuals);
+/* Y = A X for nrhs vectors stored as above (CSR operators); the matrix is
+ * read once for all of them.  Each component equals a single apply. */
+int ngsamd_csr_mult_multi(ngsamd_op_t op, int nrhs, const double* d_x, double* d_y);
+
 /* ------------------------------------------------ batched independent solves */
 /* Many independent problems as one block-diagonal CSR system: problem p owns
  * rows/columns [h_seg[p], h_seg[p+1]).  with_f32 keeps an FP32 copy of the

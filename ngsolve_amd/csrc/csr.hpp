@@ -43,6 +43,10 @@ struct CsrOp : ngsamd_op_s {
   // y = A x plus per-workgroup partial sums of <x, A x> in partials[];
   // returns the number of partials; skipped entirely when *d_stop != 0
   int MultDot(const double* x, double* y, double* partials, const int* d_stop, hipStream_t st);
+  // Y = A X for K vectors stored interleaved ([n][K]); the matrix is read
+  // once for all K in the runs layout
+  void MultMulti(int K, const double* x, double* y, hipStream_t st);
+  DevBuf<double> xcomp, ycomp;  // one component, for the other layouts
   void Diagonal(double* d, hipStream_t st);
   // the values in CSR order (row by row, columns as given at creation)
   void Values(double* host_vals) const;

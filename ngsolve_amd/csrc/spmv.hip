@@ -287,6 +287,162 @@ __global__ __launch_bounds__(kStreamBlock, NNZ <= 2048 ? 8 : 4) void CsrRunsKern
   if (DOT) StorePartial<kStreamBlock>(dotacc, partials, lds);
 }
 
+// Y = A X for K vectors stored interleaved (entry j of dof i at i K + j, the
+// layout of a BaseVector with K components): each block's values and column
+// numbers are loaded once and used for all K vectors, G at a time (G = 2:
+// one 16-byte load of two x entries per value).  Per vector the products and
+// row sums follow CsrRunsKernel's order exactly, so each column of Y is
+// bit-identical to a single apply.
+template <int NNZ, int G>
+__global__ __launch_bounds__(kStreamBlock, NNZ <= 2048 ? 8 : 4) void CsrRunsMultiKernel(
+    const int* __restrict__ run_row, const int* __restrict__ run_ptr, const int* __restrict__ val_ptr,
+    const int* __restrict__ rcols, const double* __restrict__ rvals, const double* __restrict__ x,
+    double* __restrict__ y, int K, const int* __restrict__ blk_run, const int* __restrict__ blk_len, int nblocks,
+    int blocks_per_wg) {
+  constexpr int PER = NNZ / kStreamBlock;
+  __shared__ double prod[NNZ * G];
+  __shared__ double lds[kStreamBlock / 64];
+  const int tid = threadIdx.x;
+  const int lb = XcdContiguousBlock(blockIdx.x, gridDim.x);
+  const int b0 = lb * blocks_per_wg;
+  const int b1 = min(b0 + blocks_per_wg, nblocks);
+  int u0 = 0, u1 = 0, R = 1, inv = 65536, c0 = 0, vb = 0, nv = 0;
+  int c[PER];
+  double v[PER];
+  auto fetch = [&](int b) {
+    u0 = blk_run[b];
+    u1 = blk_run[b + 1];
+    R = blk_len[b];
+    inv = (65536 + R - 1) / R;
+    c0 = run_ptr[u0];
+    vb = val_ptr[u0];
+    nv = val_ptr[u1] - vb;
+    if (nv > NNZ) return;
+#pragma unroll
+    for (int i = 0; i < PER; i++) {
+      const int k = tid + i * kStreamBlock;
+      const int kk = k < nv ? k : 0;
+      c[i] = __builtin_nontemporal_load(&rcols[c0 + RunDiv(kk, inv)]);
+      v[i] = __builtin_nontemporal_load(&rvals[vb + kk]);
+    }
+  };
+  if (b0 < b1) fetch(b0);
+  for (int b = b0; b < b1; b++) {
+    if (nv > NNZ) {
+      const int r = run_row[u0];
+      for (int j = 0; j < K; j++) {
+        double a = 0.0;
+        for (int k = tid; k < nv; k += kStreamBlock)
+          a += rvals[vb + k] * x[(int64_t)rcols[c0 + k] * K + j];
+        a = BlockSum<kStreamBlock>(a, lds);
+        if (tid == 0) y[(int64_t)r * K + j] = a;
+      }
+      if (b + 1 < b1) fetch(b + 1);
+      continue;
+    }
+    const int row0 = run_row[u0];
+    const int nr = (u1 - u0) * R;
+    int tpr = 64;
+    while (tpr > 1 && tpr * nr > kStreamBlock) tpr >>= 1;
+    const int l = tid & (tpr - 1);
+    for (int j0 = 0; j0 < K; j0 += G) {
+#pragma unroll
+      for (int i = 0; i < PER; i++) {
+        const int k = tid + i * kStreamBlock;
+        const double* xp = x + (int64_t)c[i] * K + j0;
+        if constexpr (G == 2) {
+          const double2 xv = *reinterpret_cast<const double2*>(xp);
+          prod[2 * k] = k < nv ? v[i] * xv.x : 0.0;
+          prod[2 * k + 1] = k < nv ? v[i] * xv.y : 0.0;
+        } else {
+          prod[k] = k < nv ? v[i] * xp[0] : 0.0;
+        }
+      }
+      __syncthreads();
+      for (int jr = 0; jr < nr; jr += kStreamBlock / tpr) {
+        const int j = jr + tid / tpr;
+        double sum[G];
+#pragma unroll
+        for (int g = 0; g < G; g++) sum[g] = 0.0;
+        if (j < nr) {
+          const int run = RunDiv(j, inv), q = j - run * R;
+          const int s0 = run_ptr[u0 + run] - c0, s1 = run_ptr[u0 + run + 1] - c0;
+#pragma unroll
+          for (int g = 0; g < G; g++) {
+            double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+            int k = s0 + l;
+            for (; k + 3 * tpr < s1; k += 4 * tpr) {
+              a0 += prod[(k * R + q) * G + g];
+              a1 += prod[((k + tpr) * R + q) * G + g];
+              a2 += prod[((k + 2 * tpr) * R + q) * G + g];
+              a3 += prod[((k + 3 * tpr) * R + q) * G + g];
+            }
+            for (; k < s1; k += tpr) a0 += prod[(k * R + q) * G + g];
+            sum[g] = (a0 + a1) + (a2 + a3);
+          }
+        }
+#pragma unroll
+        for (int g = 0; g < G; g++)
+          for (int off = tpr / 2; off > 0; off >>= 1) sum[g] += __shfl_xor(sum[g], off, 64);
+        if (j < nr && l == 0)
+#pragma unroll
+          for (int g = 0; g < G; g++) y[(int64_t)(row0 + j) * K + j0 + g] = sum[g];
+      }
+      __syncthreads();
+    }
+    if (b + 1 < b1) fetch(b + 1);
+  }
+}
+
+// Component j of an interleaved K-vector to a plain vector and back (the
+// other layouts apply the single-vector kernel per component).
+__global__ void GatherComponentKernel(const double* __restrict__ x, int K, int j, double* __restrict__ out,
+                                      int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = x[i * K + j];
+}
+__global__ void ScatterComponentKernel(const double* __restrict__ in, int K, int j, double* __restrict__ y,
+                                       int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    y[i * K + j] = in[i];
+}
+
+void CsrOp::MultMulti(int K, const double* x, double* y, hipStream_t st) {
+  NGSAMD_REQUIRE(K >= 1 && K <= 64, "multi-vector apply: 1 <= K <= 64 vectors");
+  if (height == 0) return;
+  if (width == 0) {
+    // no columns: Y = 0
+    LaunchFill(y, height * K, 0.0, st);
+    return;
+  }
+  if (algo == 2) {
+    auto pick = [&](auto g) {
+      constexpr int Gc = decltype(g)::value;
+      return block_nnz == 1024 ? CsrRunsMultiKernel<1024, Gc>
+             : block_nnz == 4096 ? CsrRunsMultiKernel<4096, Gc>
+                                 : CsrRunsMultiKernel<2048, Gc>;
+    };
+    auto kern = (K % 2 == 0) ? pick(std::integral_constant<int, 2>{}) : pick(std::integral_constant<int, 1>{});
+    int wg = nblocks < max_grid ? nblocks : max_grid;
+    if (wg > 8) wg &= ~7;
+    if (wg < 1) wg = 1;
+    const int per = (nblocks + wg - 1) / wg;
+    kern<<<wg, kStreamBlock, 0, st>>>(run_row.p, run_ptr.p, val_ptr.p, rcols.p, rvals.p, x, y, K, blk_run.p,
+                                      blk_len.p, nblocks, per);
+    NGSAMD_CHECK_LAUNCH();
+    return;
+  }
+  if (xcomp.n < (size_t)width) xcomp.alloc(width);
+  if (ycomp.n < (size_t)height) ycomp.alloc(height);
+  for (int j = 0; j < K; j++) {
+    GatherComponentKernel<<<GridFor(width, 256, 8192), 256, 0, st>>>(x, K, j, xcomp.p, width);
+    NGSAMD_CHECK_LAUNCH();
+    Mult(xcomp.p, ycomp.p, st);
+    ScatterComponentKernel<<<GridFor(height, 256, 8192), 256, 0, st>>>(ycomp.p, K, j, y, height);
+    NGSAMD_CHECK_LAUNCH();
+  }
+}
+
 __global__ void RunsDiagKernel(const int* __restrict__ run_row, const int* __restrict__ run_ptr,
                                const int* __restrict__ val_ptr, const int* __restrict__ rcols,
                                const double* __restrict__ rvals, double* __restrict__ d, int nruns) {

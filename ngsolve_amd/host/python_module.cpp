@@ -613,6 +613,29 @@ PYBIND11_MODULE(_ngsamd, m) {
         py::arg("mat"), py::arg("pre"), py::arg("rhs"), py::arg("sol"), py::arg("initialize") = true,
         py::arg("tol") = 1e-12, py::arg("atol") = -1.0, py::arg("maxiter") = 100, py::arg("chunk") = 16);
 
+  m.def("CGSolveMulti",
+        [](Operator& A, py::object pre, DeviceVector& f, DeviceVector& u, int nrhs, bool initialize, double tol,
+           bool absolute, int maxiter) {
+          if (f.itemsize != 8 || u.itemsize != 8 || f.size != A.height * nrhs || u.size != A.height * nrhs)
+            throw std::runtime_error("CGSolveMulti: float64 vectors of height * nrhs entries expected");
+          ngsamd_op_t p = pre.is_none() ? nullptr : pre.cast<Operator&>().op;
+          int steps = 0;
+          std::vector<double> res(std::max(maxiter, 0) + 1);
+          {
+            py::gil_scoped_release rel;
+            Check(ngsamd_cg_solve_multi(A.op, p, nrhs, f.d(), u.d(), initialize ? 1 : 0, tol, absolute ? 1 : 0,
+                                        maxiter, &steps, res.data()));
+          }
+          res.resize(std::min(steps, maxiter + 1));
+          return py::make_tuple(steps, ToNumpy(res));
+        },
+        py::arg("mat"), py::arg("pre"), py::arg("rhs"), py::arg("sol"), py::arg("nrhs"), py::arg("initialize") = true,
+        py::arg("tol") = 1e-8, py::arg("absolute") = false, py::arg("maxiter") = 200);
+  m.def("CSRMultMulti", [](Operator& A, DeviceVector& x, DeviceVector& y, int nrhs) {
+    if (x.itemsize != 8 || y.itemsize != 8 || x.size != A.width * nrhs || y.size != A.height * nrhs)
+      throw std::runtime_error("CSRMultMulti: float64 vectors of size * nrhs entries expected");
+    Check(ngsamd_csr_mult_multi(A.op, nrhs, x.d(), y.d()));
+  });
   m.def("CreateBatchCSR",
         [](Arr<int64_t> rowptr, Arr<int> cols, Arr<double> vals, Arr<int64_t> seg, bool with_f32) {
           const int64_t n = rowptr.size() - 1;

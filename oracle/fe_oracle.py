@@ -18,6 +18,7 @@ This module restates, in numpy, the reference's algorithms for:
   Jacobi-PCG               python/krylovspace.py:263-291 (Jacobi or any
                            preconditioner)
   block Jacobi             linalg/blockjacobi.cpp:390, 404, 498, 596-632
+  lockstep multi-RHS CG    linalg/cg.cpp:132-250 (CGSolver::MultiMult)
 
 It is deliberately written differently from the product code: shapes are
 evaluated per element (vectorised over elements and points, orientation per
@@ -642,6 +643,60 @@ def pcg(matvec, invdiag, f, tol=1e-12, maxiter=100, atol=None):
         beta = wdn / wd
         s = beta * s + w
     return u, len(res), res
+
+
+def cg_multi(matvec, invdiag, F, tol=1e-8, maxiter=200, U0=None, absolute=False):
+    """Lockstep CG for the K columns of F (n x K), CGSolver<double>::MultiMult
+    (linalg/cg.cpp:132-250): per column alpha_j = wd_j / <s_j, A s_j> and
+    beta_j = wdn_j / wd_j, one stopping test on the 2-norm of the vector of
+    w.d values (loop while n++ < maxiter and ||wdn|| > tol^2 ||wdn_0||, or
+    tol^2 if absolute; ||wdn_0|| = 0 -> wdn = 1), stop when ||kss|| = 0.  A
+    column with wd_j or kss_j = 0 gets alpha_j = beta_j = 0 (the reference
+    divides 0 by 0 there).  matvec maps an n x K array to A times it;
+    invdiag: Jacobi inverse diagonal or None.  Returns (U, steps, history of
+    sqrt(||wdn||))."""
+    F = np.asarray(F, dtype=float)
+    pre = (lambda V: V.copy()) if invdiag is None else (lambda V: V * np.asarray(invdiag)[:, None])
+
+    def dots(A, B):
+        return np.einsum("ij,ij->j", A, B)
+
+    def safe(a, b):
+        return np.where(b != 0, a / np.where(b != 0, b, 1.0), 0.0)
+
+    if U0 is None:
+        U = np.zeros_like(F)
+        D = F.copy()
+    else:
+        U = np.array(U0, dtype=float)
+        D = F - matvec(U)
+    W = pre(D)
+    S = W.copy()
+    wdn = dots(W, D)
+    if np.linalg.norm(wdn) == 0.0:
+        wdn = np.ones_like(wdn)
+    err = tol * tol if absolute else tol * tol * np.linalg.norm(wdn)
+    hist = [math.sqrt(np.linalg.norm(wdn))]
+    n = 0
+    while True:
+        go = n < maxiter
+        n += 1
+        if not (go and np.linalg.norm(wdn) > err):
+            break
+        W = matvec(S)
+        wd = wdn
+        kss = dots(S, W)
+        if np.linalg.norm(kss) == 0.0:
+            break
+        al = safe(wd, kss)
+        U += al * S
+        D -= al * W
+        W = pre(D)
+        wdn = dots(W, D)
+        be = safe(wdn, wd)
+        S = S * be + W
+        hist.append(math.sqrt(np.linalg.norm(wdn)))
+    return U, n, np.array(hist)
 
 
 def adjoint_gradient(pr, kappa, f, u_obs, M):

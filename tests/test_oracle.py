@@ -282,3 +282,49 @@ def test_block_jacobi_special_cases():
     f[fd] = 1.0
     u, its, _ = O.pcg(lambda t: A @ t, applyall, f, tol=1e-12, maxiter=10)
     assert its == 2 and np.allclose((A @ u)[fd], f[fd], atol=1e-12)
+
+
+def _spd(n, seed):
+    rng = np.random.default_rng(seed)
+    Q = rng.standard_normal((n, n))
+    return Q @ Q.T + n * np.eye(n)
+
+
+def test_cg_multi_single_column_is_pcg():
+    """One column: the lockstep recurrence and stopping rule are those of the
+    single CG (same iterates, step count within the counting convention)."""
+    A = _spd(40, 1)
+    f = np.random.default_rng(2).standard_normal(40)
+    inv = 1.0 / np.diag(A)
+    U, steps, hist = O.cg_multi(lambda V: A @ V, inv, f[:, None], tol=1e-10, maxiter=500)
+    u, its, res = O.pcg(lambda v: A @ v, inv, f, tol=1e-10, maxiter=500)
+    assert abs(steps - its) <= 1
+    assert np.abs(U[:, 0] - u).max() <= 1e-12 * np.abs(u).max()
+    assert np.allclose(hist, res[:len(hist)], rtol=1e-10)
+
+
+def test_cg_multi_columns():
+    """Identical columns take the steps of one; every column solves its own
+    system; k distinct eigenvalues give convergence within k iterations;
+    a zero column stays zero; maxiter bounds the steps."""
+    A = _spd(30, 3)
+    rng = np.random.default_rng(4)
+    f = rng.standard_normal(30)
+    inv = 1.0 / np.diag(A)
+    mv = lambda V: A @ V
+    U1, s1, _ = O.cg_multi(mv, inv, f[:, None], tol=1e-10, maxiter=500)
+    U3, s3, _ = O.cg_multi(mv, inv, np.stack([f, f, f], 1), tol=1e-10, maxiter=500)
+    assert s3 == s1 and np.array_equal(U3[:, 0], U3[:, 2]) and np.allclose(U3[:, 1], U1[:, 0], rtol=0, atol=1e-14)
+    F = rng.standard_normal((30, 4))
+    U, steps, _ = O.cg_multi(mv, inv, F, tol=1e-11, maxiter=500)
+    X = np.linalg.solve(A, F)
+    assert np.abs(U - X).max() <= 1e-8 * np.abs(X).max()
+    # diagonal matrix with 3 distinct eigenvalues, no preconditioner: 3 iterations (4 tests)
+    Dg = np.diag(np.repeat([1.0, 4.0, 9.0], 5))
+    _, sd, _ = O.cg_multi(lambda V: Dg @ V, None, rng.standard_normal((15, 3)), tol=1e-12, maxiter=50)
+    assert sd == 4
+    Fz = np.stack([f, np.zeros(30)], 1)
+    Uz, _, _ = O.cg_multi(mv, inv, Fz, tol=1e-10, maxiter=500)
+    assert np.all(Uz[:, 1] == 0) and np.all(np.isfinite(Uz))
+    _, sm, hm = O.cg_multi(mv, inv, F, tol=1e-14, maxiter=3)
+    assert sm == 4 and len(hm) == 4
