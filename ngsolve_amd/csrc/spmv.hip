@@ -289,18 +289,19 @@ __global__ __launch_bounds__(kStreamBlock, NNZ <= 2048 ? 8 : 4) void CsrRunsKern
 
 // Y = A X for K vectors stored interleaved (entry j of dof i at i K + j, the
 // layout of a BaseVector with K components): each block's values and column
-// numbers are loaded once and used for all K vectors, G at a time (G = 2:
-// one 16-byte load of two x entries per value).  Per vector the products and
-// row sums follow CsrRunsKernel's order exactly, so each column of Y is
-// bit-identical to a single apply.
+// numbers are loaded once and used for all K vectors; the x entries of G
+// components of a value's column come in one pass (16-byte loads), then
+// each component goes through the single kernel's products (16 KB of LDS)
+// and row sums in the same order, so each column of Y is bit-identical to
+// a single apply.
 template <int NNZ, int G>
-__global__ __launch_bounds__(kStreamBlock, NNZ <= 2048 ? 8 : 4) void CsrRunsMultiKernel(
+__global__ __launch_bounds__(kStreamBlock, NNZ <= 2048 && G == 1 ? 8 : 4) void CsrRunsMultiKernel(
     const int* __restrict__ run_row, const int* __restrict__ run_ptr, const int* __restrict__ val_ptr,
     const int* __restrict__ rcols, const double* __restrict__ rvals, const double* __restrict__ x,
     double* __restrict__ y, int K, const int* __restrict__ blk_run, const int* __restrict__ blk_len, int nblocks,
     int blocks_per_wg) {
   constexpr int PER = NNZ / kStreamBlock;
-  __shared__ double prod[NNZ * G];
+  __shared__ double prod[NNZ];
   __shared__ double lds[kStreamBlock / 64];
   const int tid = threadIdx.x;
   const int lb = XcdContiguousBlock(blockIdx.x, gridDim.x);
@@ -346,49 +347,53 @@ __global__ __launch_bounds__(kStreamBlock, NNZ <= 2048 ? 8 : 4) void CsrRunsMult
     while (tpr > 1 && tpr * nr > kStreamBlock) tpr >>= 1;
     const int l = tid & (tpr - 1);
     for (int j0 = 0; j0 < K; j0 += G) {
+      // x of the G components of each value's column: loaded together
+      double xg[PER][G];
 #pragma unroll
       for (int i = 0; i < PER; i++) {
-        const int k = tid + i * kStreamBlock;
         const double* xp = x + (int64_t)c[i] * K + j0;
-        if constexpr (G == 2) {
-          const double2 xv = *reinterpret_cast<const double2*>(xp);
-          prod[2 * k] = k < nv ? v[i] * xv.x : 0.0;
-          prod[2 * k + 1] = k < nv ? v[i] * xv.y : 0.0;
+        if constexpr (G == 1) {
+          xg[i][0] = xp[0];
         } else {
-          prod[k] = k < nv ? v[i] * xp[0] : 0.0;
+#pragma unroll
+          for (int h = 0; h < G / 2; h++) {
+            const double2 t = reinterpret_cast<const double2*>(xp)[h];
+            xg[i][2 * h] = t.x;
+            xg[i][2 * h + 1] = t.y;
+          }
         }
       }
-      __syncthreads();
-      for (int jr = 0; jr < nr; jr += kStreamBlock / tpr) {
-        const int j = jr + tid / tpr;
-        double sum[G];
+      // then, component by component, the single kernel's products and sums
 #pragma unroll
-        for (int g = 0; g < G; g++) sum[g] = 0.0;
-        if (j < nr) {
-          const int run = RunDiv(j, inv), q = j - run * R;
-          const int s0 = run_ptr[u0 + run] - c0, s1 = run_ptr[u0 + run + 1] - c0;
+      for (int g = 0; g < G; g++) {
 #pragma unroll
-          for (int g = 0; g < G; g++) {
+        for (int i = 0; i < PER; i++) {
+          const int k = tid + i * kStreamBlock;
+          prod[k] = k < nv ? v[i] * xg[i][g] : 0.0;
+        }
+        __syncthreads();
+        for (int jr = 0; jr < nr; jr += kStreamBlock / tpr) {
+          const int j = jr + tid / tpr;
+          double sum = 0.0;
+          if (j < nr) {
+            const int run = RunDiv(j, inv), q = j - run * R;
+            const int s0 = run_ptr[u0 + run] - c0, s1 = run_ptr[u0 + run + 1] - c0;
             double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
             int k = s0 + l;
             for (; k + 3 * tpr < s1; k += 4 * tpr) {
-              a0 += prod[(k * R + q) * G + g];
-              a1 += prod[((k + tpr) * R + q) * G + g];
-              a2 += prod[((k + 2 * tpr) * R + q) * G + g];
-              a3 += prod[((k + 3 * tpr) * R + q) * G + g];
+              a0 += prod[k * R + q];
+              a1 += prod[(k + tpr) * R + q];
+              a2 += prod[(k + 2 * tpr) * R + q];
+              a3 += prod[(k + 3 * tpr) * R + q];
             }
-            for (; k < s1; k += tpr) a0 += prod[(k * R + q) * G + g];
-            sum[g] = (a0 + a1) + (a2 + a3);
+            for (; k < s1; k += tpr) a0 += prod[k * R + q];
+            sum = (a0 + a1) + (a2 + a3);
           }
+          for (int off = tpr / 2; off > 0; off >>= 1) sum += __shfl_xor(sum, off, 64);
+          if (j < nr && l == 0) y[(int64_t)(row0 + j) * K + j0 + g] = sum;
         }
-#pragma unroll
-        for (int g = 0; g < G; g++)
-          for (int off = tpr / 2; off > 0; off >>= 1) sum[g] += __shfl_xor(sum[g], off, 64);
-        if (j < nr && l == 0)
-#pragma unroll
-          for (int g = 0; g < G; g++) y[(int64_t)(row0 + j) * K + j0 + g] = sum[g];
+        __syncthreads();
       }
-      __syncthreads();
     }
     if (b + 1 < b1) fetch(b + 1);
   }
@@ -422,7 +427,14 @@ void CsrOp::MultMulti(int K, const double* x, double* y, hipStream_t st) {
              : block_nnz == 4096 ? CsrRunsMultiKernel<4096, Gc>
                                  : CsrRunsMultiKernel<2048, Gc>;
     };
-    auto kern = (K % 2 == 0) ? pick(std::integral_constant<int, 2>{}) : pick(std::integral_constant<int, 1>{});
+    // x of four components per pass when K allows (two 16-byte loads per
+    // value, more registers: 4 workgroups per CU); NGSAMD_MULTI_PASS=2 keeps
+    // two per pass (for comparison)
+    const char* pass_env = getenv("NGSAMD_MULTI_PASS");
+    const bool four = K % 4 == 0 && !(pass_env && std::string(pass_env) == "2");
+    auto kern = four ? pick(std::integral_constant<int, 4>{})
+                : (K % 2 == 0) ? pick(std::integral_constant<int, 2>{})
+                               : pick(std::integral_constant<int, 1>{});
     int wg = nblocks < max_grid ? nblocks : max_grid;
     if (wg > 8) wg &= ~7;
     if (wg < 1) wg = 1;
