@@ -147,7 +147,13 @@ BlockJacobiOp::BlockJacobiOp(int64_t n, const int64_t* rp, const int* cols, cons
   std::vector<char> ok(nthreads, 1);
   {
     std::vector<std::thread> pool;
-    for (int t = 1; t < nthreads; t++) pool.emplace_back([&, t] { ok[t] = invert_range(cut[t], cut[t + 1]); });
+    int t = 1;
+    try {
+      for (; t < nthreads; t++) pool.emplace_back([&, t] { ok[t] = invert_range(cut[t], cut[t + 1]); });
+    } catch (...) {
+      // no more threads: the remaining ranges run here
+      for (; t < nthreads; t++) ok[t] = invert_range(cut[t], cut[t + 1]);
+    }
     ok[0] = invert_range(cut[0], cut[1]);
     for (auto& th : pool) th.join();
   }
