@@ -115,3 +115,23 @@ def test_cg_multi_start_vector_and_zero_column(device):
     dU2 = N.DeviceVector(P["ndof"] * 3, 8)
     s3, h3 = N.CGSolveMulti(A, C, dF, dU2, 3, True, 1e-14, False, 3)
     assert s3 == 4 and len(h3) == 4
+
+
+def test_cg_multi_refuses_bad_input(device):
+    say("[cg multi] bad input")
+    P = problem(2, 4, 2)
+    A = N.CreateCSR(P["ndof"], P["ndof"], P["rowptr"], P["cols"], P["vals"])
+    free = P["fes"].freedofs.astype(np.uint8)
+    diag, _ = O.jacobi_invdiag(P["rowptr"], P["cols"], P["vals"], free)
+    C = N.CreateJacobi(diag, free)
+    for K in (0, 9):
+        f, u = N.DeviceVector(max(P["ndof"] * K, 1), 8), N.DeviceVector(max(P["ndof"] * K, 1), 8)
+        with pytest.raises(RuntimeError):
+            N.CGSolveMulti(A, C, f, u, K, True, 1e-10, False, 10)
+    f, u = N.DeviceVector(P["ndof"] * 2, 8), N.DeviceVector(P["ndof"] * 2, 8)
+    with pytest.raises(RuntimeError, match="CSR"):
+        N.CGSolveMulti(C, None, f, u, 2, True, 1e-10, False, 10)  # Jacobi is not a CSR matrix
+    with pytest.raises(RuntimeError, match="Jacobi"):
+        N.CGSolveMulti(A, A, f, u, 2, True, 1e-10, False, 10)
+    with pytest.raises(RuntimeError):
+        N.CSRMultMulti(A, f, u, 3)  # sizes do not match 3 components
