@@ -12,8 +12,9 @@
 //
 // The operator product reads the matrix once for all K components
 // (CsrOp::MultMulti); the vector kernels handle all components of a dof in
-// one thread; inner products are per-workgroup partials folded in a fixed
-// order (deterministic).  The host reads the K values of w.d and <s, A s>
+// one thread, the update kernel forming the partials of the new w.d as it
+// writes w and d; inner products are per-workgroup partials folded in a
+// fixed order (deterministic).  The host reads the K values of w.d and <s, A s>
 // each iteration for the stopping test.
 #include <hip/hip_runtime.h>
 
@@ -75,16 +76,23 @@ __global__ __launch_bounds__(kMcBlock) void MultiPrecondKernel(const double* __r
   }
 }
 
-// alpha_j = wd_j / kss_j;  u += alpha s;  d -= alpha w;  w = C d
+// alpha_j = wd_j / kss_j;  u += alpha s;  d -= alpha w;  w = C d, and the
+// workgroup partials of the new <w_j, d_j> (launched on the dot kernels'
+// grid, so the sums are those MultiDotPartialsKernel would form)
 __global__ __launch_bounds__(kMcBlock) void MultiCgUpdateKernel(double* __restrict__ u, double* __restrict__ d,
                                                                 double* __restrict__ w,
                                                                 const double* __restrict__ s,
                                                                 const double* __restrict__ inv, int64_t n, int K,
                                                                 const double* __restrict__ wd,
-                                                                const double* __restrict__ kss) {
-  double al[kMaxRhs];
+                                                                const double* __restrict__ kss,
+                                                                double* __restrict__ partials) {
+  __shared__ double lds[kMcBlock / 64];
+  double al[kMaxRhs], acc[kMaxRhs];
 #pragma unroll
-  for (int j = 0; j < kMaxRhs; j++) al[j] = j < K ? SafeDiv(wd[j], kss[j]) : 0.0;
+  for (int j = 0; j < kMaxRhs; j++) {
+    al[j] = j < K ? SafeDiv(wd[j], kss[j]) : 0.0;
+    acc[j] = 0.0;
+  }
   for (int64_t i = blockIdx.x * (int64_t)kMcBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kMcBlock) {
     const double c = inv ? inv[i] : 1.0;
 #pragma unroll
@@ -94,8 +102,14 @@ __global__ __launch_bounds__(kMcBlock) void MultiCgUpdateKernel(double* __restri
       u[k] += al[j] * s[k];
       const double dk = d[k] - al[j] * w[k];
       d[k] = dk;
-      w[k] = inv ? c * dk : dk;
+      const double wk = inv ? c * dk : dk;
+      w[k] = wk;
+      acc[j] += wk * dk;
     }
+  }
+  for (int j = 0; j < K; j++) {
+    const double v = BlockSum<kMcBlock>(acc[j], lds);
+    if (threadIdx.x == 0) partials[(int64_t)blockIdx.x * K + j] = v;
   }
 }
 
@@ -191,10 +205,10 @@ extern "C" int ngsamd_cg_solve_multi(ngsamd_op_t mat, ngsamd_op_t pre, int nrhs,
       A.MultMulti(K, s.p, w.p, st);  // w = A s
       std::swap(wd, wdn);             // wd = wdn
       dot(s.p, w.p, kss);
-      MultiCgUpdateKernel<<<GridFor(n, kMcBlock, 8192), kMcBlock, 0, st>>>(d_sol, d.p, w.p, s.p, inv, n, K, wd,
-                                                                          kss);
+      MultiCgUpdateKernel<<<grid, kMcBlock, 0, st>>>(d_sol, d.p, w.p, s.p, inv, n, K, wd, kss, partials.p);
       NGSAMD_CHECK_LAUNCH();
-      dot(w.p, d.p, wdn);
+      MultiDotFoldKernel<<<1, kMcBlock, 0, st>>>(partials.p, grid, K, wdn);
+      NGSAMD_CHECK_LAUNCH();
       MultiCgDirectionKernel<<<GridFor(n, kMcBlock, 8192), kMcBlock, 0, st>>>(s.p, w.p, n, K, wd, wdn);
       NGSAMD_CHECK_LAUNCH();
       NGSAMD_HIP(hipMemcpyAsync(h_kss.data(), kss, sizeof(double) * K, hipMemcpyDeviceToHost, st));
