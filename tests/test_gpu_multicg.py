@@ -34,7 +34,7 @@ def _multi_spmv(P, X):
 
 
 @pytest.mark.parametrize("dim,n,p", [(2, 10, 3), (3, 3, 4)])
-@pytest.mark.parametrize("K", [1, 2, 3, 4, 5, 8])
+@pytest.mark.parametrize("K", [1, 2, 3, 4, 5, 6, 8, 12])
 def test_mult_multi_matches_single(device, dim, n, p, K):
     say(f"[csr multi] dim={dim} n={n} p={p} K={K}")
     P = problem(dim, n, p)
