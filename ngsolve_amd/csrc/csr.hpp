@@ -45,7 +45,8 @@ struct CsrOp : ngsamd_op_s {
   int MultDot(const double* x, double* y, double* partials, const int* d_stop, hipStream_t st);
   // Y = A X for K vectors stored interleaved ([n][K]); the matrix is read
   // once for all K in the runs layout
-  void MultMulti(int K, const double* x, double* y, hipStream_t st);
+  // (all kernels return at once when *stop != 0, stop may be null)
+  void MultMulti(int K, const double* x, double* y, hipStream_t st, const int* stop = nullptr);
   DevBuf<double> xcomp, ycomp;  // one component, for the other layouts
   void Diagonal(double* d, hipStream_t st);
   // the values in CSR order (row by row, columns as given at creation)

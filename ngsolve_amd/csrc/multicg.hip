@@ -14,8 +14,9 @@
 // (CsrOp::MultMulti); the vector kernels handle all components of a dof in
 // one thread, the update kernel forming the partials of the new w.d as it
 // writes w and d; inner products are per-workgroup partials folded in a
-// fixed order (deterministic).  The host reads the K values of w.d and <s, A s>
-// each iteration for the stopping test.
+// fixed order (deterministic).  The stopping test runs on the device: every
+// kernel returns at once after it has stopped, and the host enqueues 16
+// iterations at a time, reading the control word once per chunk.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -36,7 +37,9 @@ constexpr int kMaxRhs = 8;
 // partials[w K + j] = sum over the dofs of workgroup w of a[i K + j] b[i K + j]
 __global__ __launch_bounds__(kMcBlock) void MultiDotPartialsKernel(const double* __restrict__ a,
                                                                    const double* __restrict__ b, int64_t n, int K,
-                                                                   double* __restrict__ partials) {
+                                                                   double* __restrict__ partials,
+                                                                   const int* __restrict__ stop) {
+  if (stop && *stop) return;
   __shared__ double lds[kMcBlock / 64];
   double acc[kMaxRhs];
 #pragma unroll
@@ -54,7 +57,8 @@ __global__ __launch_bounds__(kMcBlock) void MultiDotPartialsKernel(const double*
 
 // out[j] = sum over w < nwg of partials[w K + j], one workgroup, fixed order
 __global__ __launch_bounds__(kMcBlock) void MultiDotFoldKernel(const double* __restrict__ partials, int nwg, int K,
-                                                               double* __restrict__ out) {
+                                                               double* __restrict__ out, const int* __restrict__ stop) {
+  if (stop && *stop) return;
   __shared__ double lds[kMcBlock / 64];
   for (int j = 0; j < K; j++) {
     double v = 0.0;
@@ -85,7 +89,9 @@ __global__ __launch_bounds__(kMcBlock) void MultiCgUpdateKernel(double* __restri
                                                                 const double* __restrict__ inv, int64_t n, int K,
                                                                 const double* __restrict__ wd,
                                                                 const double* __restrict__ kss,
-                                                                double* __restrict__ partials) {
+                                                                double* __restrict__ partials,
+                                                                const int* __restrict__ stop) {
+  if (*stop) return;
   __shared__ double lds[kMcBlock / 64];
   double al[kMaxRhs], acc[kMaxRhs];
 #pragma unroll
@@ -117,7 +123,9 @@ __global__ __launch_bounds__(kMcBlock) void MultiCgUpdateKernel(double* __restri
 __global__ __launch_bounds__(kMcBlock) void MultiCgDirectionKernel(double* __restrict__ s,
                                                                    const double* __restrict__ w, int64_t n, int K,
                                                                    const double* __restrict__ wd,
-                                                                   const double* __restrict__ wdn) {
+                                                                   const double* __restrict__ wdn,
+                                                                   const int* __restrict__ stop) {
+  if (*stop) return;
   double be[kMaxRhs];
 #pragma unroll
   for (int j = 0; j < kMaxRhs; j++) be[j] = j < K ? SafeDiv(wdn[j], wd[j]) : 0.0;
@@ -129,6 +137,40 @@ __global__ __launch_bounds__(kMcBlock) void MultiCgDirectionKernel(double* __res
       s[k] = s[k] * be[j] + w[k];
     }
   }
+}
+
+__device__ double NormK(const double* v, int K) {
+  double s = 0.0;
+  for (int j = 0; j < K; j++) s += v[j] * v[j];
+  return sqrt(s);
+}
+
+// The loop test at the head of iteration it (1-based): go on while
+// it - 1 < maxiter and ||wdn|| > err; else stop with steps = it.
+// ctl[0] = stop flag, ctl[1] = steps.
+__global__ void MultiCgHeadKernel(const double* __restrict__ wdn, int K, double err, int it, int maxiter,
+                                  int* __restrict__ ctl) {
+  if (threadIdx.x != 0 || ctl[0]) return;
+  if (!(it - 1 < maxiter && NormK(wdn, K) > err)) {
+    ctl[0] = 1;
+    ctl[1] = it;
+  }
+}
+
+// <s, A s> = 0 for every component: stop before the update (steps = it)
+__global__ void MultiCgKssKernel(const double* __restrict__ kss, int K, int it, int* __restrict__ ctl) {
+  if (threadIdx.x != 0 || ctl[0]) return;
+  if (NormK(kss, K) == 0.0) {
+    ctl[0] = 1;
+    ctl[1] = it;
+  }
+}
+
+// end of iteration it: hist[it] = sqrt(||wdn||)
+__global__ void MultiCgTailKernel(const double* __restrict__ wdn, int K, int it, const int* __restrict__ ctl,
+                                  double* __restrict__ hist) {
+  if (threadIdx.x != 0 || ctl[0]) return;
+  hist[it] = sqrt(NormK(wdn, K));
 }
 
 // d = f - d (initial residual with d = A u on entry)
@@ -172,10 +214,13 @@ extern "C" int ngsamd_cg_solve_multi(ngsamd_op_t mat, ngsamd_op_t pre, int nrhs,
     double* wdn = dots.p + K;
     double* kss = dots.p + 2 * K;
     const int grid = GridFor(n, kMcBlock, kMcGrid);
-    auto dot = [&](const double* a, const double* b, double* out) {
-      MultiDotPartialsKernel<<<grid, kMcBlock, 0, st>>>(a, b, n, K, partials.p);
+    DevBuf<int> ctl(2);
+    DevBuf<double> hist((size_t)maxiter + 1);
+    NGSAMD_HIP(hipMemsetAsync(ctl.p, 0, sizeof(int) * 2, st));
+    auto dot = [&](const double* a, const double* b, double* out, const int* stop) {
+      MultiDotPartialsKernel<<<grid, kMcBlock, 0, st>>>(a, b, n, K, partials.p, stop);
       NGSAMD_CHECK_LAUNCH();
-      MultiDotFoldKernel<<<1, kMcBlock, 0, st>>>(partials.p, grid, K, out);
+      MultiDotFoldKernel<<<1, kMcBlock, 0, st>>>(partials.p, grid, K, out, stop);
       NGSAMD_CHECK_LAUNCH();
     };
     // d = f (u = 0) or f - A u
@@ -190,8 +235,8 @@ extern "C" int ngsamd_cg_solve_multi(ngsamd_op_t mat, ngsamd_op_t pre, int nrhs,
     MultiPrecondKernel<<<GridFor(n, kMcBlock, 8192), kMcBlock, 0, st>>>(d.p, inv, w.p, n, K);
     NGSAMD_CHECK_LAUNCH();
     NGSAMD_HIP(hipMemcpyAsync(s.p, w.p, sizeof(double) * m, hipMemcpyDeviceToDevice, st));
-    dot(w.p, d.p, wdn);
-    std::vector<double> h_wdn(K), h_kss(K);
+    dot(w.p, d.p, wdn, nullptr);
+    std::vector<double> h_wdn(K);
     NGSAMD_HIP(hipMemcpyAsync(h_wdn.data(), wdn, sizeof(double) * K, hipMemcpyDeviceToHost, st));
     NGSAMD_HIP(hipStreamSynchronize(st));
     if (Norm(h_wdn) == 0.0) {
@@ -199,27 +244,44 @@ extern "C" int ngsamd_cg_solve_multi(ngsamd_op_t mat, ngsamd_op_t pre, int nrhs,
       NGSAMD_HIP(hipMemcpyAsync(wdn, h_wdn.data(), sizeof(double) * K, hipMemcpyHostToDevice, st));
     }
     const double err = absolute ? tol * tol : tol * tol * Norm(h_wdn);
-    if (h_residuals) h_residuals[0] = std::sqrt(Norm(h_wdn));
-    int it = 0;
-    while (it++ < maxiter && Norm(h_wdn) > err) {
-      A.MultMulti(K, s.p, w.p, st);  // w = A s
-      std::swap(wd, wdn);             // wd = wdn
-      dot(s.p, w.p, kss);
-      MultiCgUpdateKernel<<<grid, kMcBlock, 0, st>>>(d_sol, d.p, w.p, s.p, inv, n, K, wd, kss, partials.p);
-      NGSAMD_CHECK_LAUNCH();
-      MultiDotFoldKernel<<<1, kMcBlock, 0, st>>>(partials.p, grid, K, wdn);
-      NGSAMD_CHECK_LAUNCH();
-      MultiCgDirectionKernel<<<GridFor(n, kMcBlock, 8192), kMcBlock, 0, st>>>(s.p, w.p, n, K, wd, wdn);
-      NGSAMD_CHECK_LAUNCH();
-      NGSAMD_HIP(hipMemcpyAsync(h_kss.data(), kss, sizeof(double) * K, hipMemcpyDeviceToHost, st));
-      NGSAMD_HIP(hipMemcpyAsync(h_wdn.data(), wdn, sizeof(double) * K, hipMemcpyDeviceToHost, st));
+    // The iterations are enqueued a chunk at a time; the stopping test runs
+    // on the device (every kernel of a stopped solve returns at once), so
+    // the host reads the control word once per chunk, not per iteration.
+    constexpr int kChunkIts = 16;
+    int it = 0, h_ctl[2] = {0, 0};
+    while (true) {
+      for (int c = 0; c < kChunkIts && it <= maxiter; c++) {
+        ++it;
+        MultiCgHeadKernel<<<1, 64, 0, st>>>(wdn, K, err, it, maxiter, ctl.p);
+        NGSAMD_CHECK_LAUNCH();
+        A.MultMulti(K, s.p, w.p, st, ctl.p);  // w = A s
+        std::swap(wd, wdn);                    // wd = wdn
+        dot(s.p, w.p, kss, ctl.p);
+        MultiCgKssKernel<<<1, 64, 0, st>>>(kss, K, it, ctl.p);
+        NGSAMD_CHECK_LAUNCH();
+        MultiCgUpdateKernel<<<grid, kMcBlock, 0, st>>>(d_sol, d.p, w.p, s.p, inv, n, K, wd, kss, partials.p,
+                                                       ctl.p);
+        NGSAMD_CHECK_LAUNCH();
+        MultiDotFoldKernel<<<1, kMcBlock, 0, st>>>(partials.p, grid, K, wdn, ctl.p);
+        NGSAMD_CHECK_LAUNCH();
+        MultiCgDirectionKernel<<<GridFor(n, kMcBlock, 8192), kMcBlock, 0, st>>>(s.p, w.p, n, K, wd, wdn, ctl.p);
+        NGSAMD_CHECK_LAUNCH();
+        MultiCgTailKernel<<<1, 64, 0, st>>>(wdn, K, it, ctl.p, hist.p);
+        NGSAMD_CHECK_LAUNCH();
+      }
+      NGSAMD_HIP(hipMemcpyAsync(h_ctl, ctl.p, sizeof(int) * 2, hipMemcpyDeviceToHost, st));
       NGSAMD_HIP(hipStreamSynchronize(st));
-      // the reference stops before the update when <s, A s> vanishes; here
-      // that update had alpha = 0 and left u unchanged
-      if (Norm(h_kss) == 0.0) break;
-      if (h_residuals && it <= maxiter) h_residuals[it] = std::sqrt(Norm(h_wdn));
+      // the head test of iteration maxiter + 1 always stops
+      if (h_ctl[0]) break;
+      NGSAMD_REQUIRE(it <= maxiter, "ngsamd_cg_solve_multi: internal error (no stop after maxiter)");
     }
-    *h_steps = it;
+    *h_steps = h_ctl[1];
+    if (h_residuals) {
+      h_residuals[0] = std::sqrt(Norm(h_wdn));
+      if (h_ctl[1] > 1)
+        NGSAMD_HIP(hipMemcpyAsync(h_residuals + 1, hist.p + 1, sizeof(double) * (h_ctl[1] - 1),
+                                  hipMemcpyDeviceToHost, st));
+    }
     NGSAMD_HIP(hipStreamSynchronize(st));
   });
 }

@@ -299,7 +299,8 @@ __global__ __launch_bounds__(kStreamBlock, NNZ <= 2048 && G == 1 ? 8 : 4) void C
     const int* __restrict__ run_row, const int* __restrict__ run_ptr, const int* __restrict__ val_ptr,
     const int* __restrict__ rcols, const double* __restrict__ rvals, const double* __restrict__ x,
     double* __restrict__ y, int K, const int* __restrict__ blk_run, const int* __restrict__ blk_len, int nblocks,
-    int blocks_per_wg) {
+    int blocks_per_wg, const int* __restrict__ stop) {
+  if (stop && *stop) return;
   constexpr int PER = NNZ / kStreamBlock;
   __shared__ double prod[NNZ];
   __shared__ double lds[kStreamBlock / 64];
@@ -402,17 +403,23 @@ __global__ __launch_bounds__(kStreamBlock, NNZ <= 2048 && G == 1 ? 8 : 4) void C
 // Component j of an interleaved K-vector to a plain vector and back (the
 // other layouts apply the single-vector kernel per component).
 __global__ void GatherComponentKernel(const double* __restrict__ x, int K, int j, double* __restrict__ out,
-                                      int64_t n) {
+                                      int64_t n, const int* __restrict__ stop) {
+  if (stop && *stop) return;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     out[i] = x[i * K + j];
 }
 __global__ void ScatterComponentKernel(const double* __restrict__ in, int K, int j, double* __restrict__ y,
-                                       int64_t n) {
+                                       int64_t n, const int* __restrict__ stop) {
+  if (stop && *stop) return;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     y[i * K + j] = in[i];
 }
 
-void CsrOp::MultMulti(int K, const double* x, double* y, hipStream_t st) {
+template <bool ACCUM, bool DOT>
+static int LaunchSpmv(const CsrOp& A, double s, const double* x, double* y, hipStream_t st, double* partials,
+                      const int* stop);
+
+void CsrOp::MultMulti(int K, const double* x, double* y, hipStream_t st, const int* stop) {
   NGSAMD_REQUIRE(K >= 1 && K <= 64, "multi-vector apply: 1 <= K <= 64 vectors");
   if (height == 0) return;
   if (width == 0) {
@@ -440,17 +447,17 @@ void CsrOp::MultMulti(int K, const double* x, double* y, hipStream_t st) {
     if (wg < 1) wg = 1;
     const int per = (nblocks + wg - 1) / wg;
     kern<<<wg, kStreamBlock, 0, st>>>(run_row.p, run_ptr.p, val_ptr.p, rcols.p, rvals.p, x, y, K, blk_run.p,
-                                      blk_len.p, nblocks, per);
+                                      blk_len.p, nblocks, per, stop);
     NGSAMD_CHECK_LAUNCH();
     return;
   }
   if (xcomp.n < (size_t)width) xcomp.alloc(width);
   if (ycomp.n < (size_t)height) ycomp.alloc(height);
   for (int j = 0; j < K; j++) {
-    GatherComponentKernel<<<GridFor(width, 256, 8192), 256, 0, st>>>(x, K, j, xcomp.p, width);
+    GatherComponentKernel<<<GridFor(width, 256, 8192), 256, 0, st>>>(x, K, j, xcomp.p, width, stop);
     NGSAMD_CHECK_LAUNCH();
-    Mult(xcomp.p, ycomp.p, st);
-    ScatterComponentKernel<<<GridFor(height, 256, 8192), 256, 0, st>>>(ycomp.p, K, j, y, height);
+    LaunchSpmv<false, false>(*this, 1.0, xcomp.p, ycomp.p, st, nullptr, stop);
+    ScatterComponentKernel<<<GridFor(height, 256, 8192), 256, 0, st>>>(ycomp.p, K, j, y, height, stop);
     NGSAMD_CHECK_LAUNCH();
   }
 }
