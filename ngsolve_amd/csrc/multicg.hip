@@ -239,6 +239,7 @@ extern "C" int ngsamd_cg_solve_multi(ngsamd_op_t mat, ngsamd_op_t pre, int nrhs,
     std::vector<double> h_wdn(K);
     NGSAMD_HIP(hipMemcpyAsync(h_wdn.data(), wdn, sizeof(double) * K, hipMemcpyDeviceToHost, st));
     NGSAMD_HIP(hipStreamSynchronize(st));
+    const double res0 = std::sqrt(Norm(h_wdn));  // recorded before the reset below, as the reference prints it
     if (Norm(h_wdn) == 0.0) {
       for (auto& v : h_wdn) v = 1.0;
       NGSAMD_HIP(hipMemcpyAsync(wdn, h_wdn.data(), sizeof(double) * K, hipMemcpyHostToDevice, st));
@@ -277,7 +278,7 @@ extern "C" int ngsamd_cg_solve_multi(ngsamd_op_t mat, ngsamd_op_t pre, int nrhs,
     }
     *h_steps = h_ctl[1];
     if (h_residuals) {
-      h_residuals[0] = std::sqrt(Norm(h_wdn));
+      h_residuals[0] = res0;
       if (h_ctl[1] > 1)
         NGSAMD_HIP(hipMemcpyAsync(h_residuals + 1, hist.p + 1, sizeof(double) * (h_ctl[1] - 1),
                                   hipMemcpyDeviceToHost, st));

@@ -673,10 +673,10 @@ def cg_multi(matvec, invdiag, F, tol=1e-8, maxiter=200, U0=None, absolute=False)
     W = pre(D)
     S = W.copy()
     wdn = dots(W, D)
+    hist = [math.sqrt(np.linalg.norm(wdn))]  # before the reset, as the reference prints it
     if np.linalg.norm(wdn) == 0.0:
         wdn = np.ones_like(wdn)
     err = tol * tol if absolute else tol * tol * np.linalg.norm(wdn)
-    hist = [math.sqrt(np.linalg.norm(wdn))]
     n = 0
     while True:
         go = n < maxiter

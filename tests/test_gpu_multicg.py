@@ -135,3 +135,20 @@ def test_cg_multi_refuses_bad_input(device):
         N.CGSolveMulti(A, A, f, u, 2, True, 1e-10, False, 10)
     with pytest.raises(RuntimeError):
         N.CSRMultMulti(A, f, u, 3)  # sizes do not match 3 components
+
+
+def test_cg_multi_zero_rhs(device):
+    """All right-hand sides 0: w.d is reset to 1, <s, A s> = 0 ends the loop
+    at once (steps 1), u stays 0, the recorded start residual is 0."""
+    say("[cg multi] zero right-hand sides")
+    P = _eliminated(problem(2, 4, 2))
+    free = P["fes"].freedofs.astype(np.uint8)
+    diag, inv = O.jacobi_invdiag(P["rowptr"], P["cols"], P["vals"], free)
+    A = N.CreateCSR(P["ndof"], P["ndof"], P["rowptr"], P["cols"], P["vals"])
+    C = N.CreateJacobi(diag, free)
+    F = np.zeros((P["ndof"], 3))
+    dF, dU = dvec(F.ravel()), N.DeviceVector(P["ndof"] * 3, 8)
+    steps, hist = N.CGSolveMulti(A, C, dF, dU, 3, True, 1e-10, False, 100)
+    Uo, so, ho = O.cg_multi(lambda V: _multi_spmv(P, V), inv, F, tol=1e-10, maxiter=100)
+    assert steps == so == 1 and hist.tolist() == ho.tolist() == [0.0]
+    assert np.all(dU.download() == 0.0)
