@@ -7,9 +7,13 @@ x ~ U[-1,1] from numpy default_rng(1).
 
 One step = one FP64 CSR operator application y = A x (the reference's
 SparseMatrix<double>::MultAdd path).  `value` is its throughput in GDoF/s
-(ndof / time per apply); the FP64/FP32 matrix-free applies, the FP64 CSR
-Jacobi-CG and the mixed-precision (FP32 matrix-free inner, FP64 refinement)
-time-to-1e-10 on the lognormal-kappa variant are reported beside it.
+(ndof / time per apply); reported beside it under "extra": the other CSR
+kernels, the matrix-free applies in every precision, the 4-vector apply and
+the lockstep 4-right-hand-side CG, the FP64 CSR Jacobi-CG and the
+mixed-precision (FP32 matrix-free inner, FP64 refinement) time-to-1e-10 on
+the lognormal-kappa variant, block-Jacobi CG, the device set-up (pattern,
+assembly, topology and dof tables) against the host, the 512 batched 2D
+problems and the 2D 64^2 p=3 case.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-cg]
 """
