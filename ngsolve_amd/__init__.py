@@ -9,7 +9,7 @@ without the built libraries raises ImportError: there is no CPU fallback.
 from ._native import N as _N  # noqa: F401  (fails loudly when unbuilt)
 from ._native import ensure_device
 from .comp import (H1, BilinearForm, CoefficientFunction, DevMatrixFree, ElementCoefficient, GridFunction,
-                   LinearForm, Mesh, MFOpts, dx, grad, x, y, z)
+                   Integrate, LinearForm, Mesh, MFOpts, dx, grad, x, y, z)
 from .krylovspace import CG, CGSolver, DevCGSolver, IRCGSolver, LinearSolver
 from .la import (BaseMatrix, BaseVector, BitArray, BlockJacobiPrecond, CreateVVector, DeviceMatrix, DevSparseMatrix,
                  InnerProduct, JacobiPrecond, Norm, Projector)
@@ -17,7 +17,7 @@ from .meshes import MakeStructured2DMesh, MakeStructured3DMesh
 
 __all__ = [
     "H1", "BilinearForm", "CoefficientFunction", "DevMatrixFree", "ElementCoefficient", "GridFunction",
-    "LinearForm", "Mesh", "MFOpts", "dx", "grad", "x", "y", "z", "CG", "CGSolver", "DevCGSolver", "IRCGSolver",
+    "Integrate", "LinearForm", "Mesh", "MFOpts", "dx", "grad", "x", "y", "z", "CG", "CGSolver", "DevCGSolver", "IRCGSolver",
     "LinearSolver", "BaseMatrix", "BaseVector", "BitArray", "CreateVVector", "DeviceMatrix", "DevSparseMatrix",
     "InnerProduct", "JacobiPrecond", "BlockJacobiPrecond", "Norm", "Projector", "MakeStructured2DMesh", "MakeStructured3DMesh",
     "ensure_device",

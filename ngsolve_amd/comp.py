@@ -81,6 +81,8 @@ class CoefficientFunction:
     def __mul__(self, o):
         if isinstance(o, (_Proxy, _FormTerm)):
             return _FormTerm(self, []) * o
+        if isinstance(o, _GfEnergy):
+            return o * self
         return self._bin(o, np.multiply)
 
     def __rmul__(self, o): return _cf(o)._bin(self, np.multiply)
@@ -122,9 +124,71 @@ class _Proxy:
 
 
 def grad(p):
+    if isinstance(p, GridFunction):
+        return _GfGrad(p)
     if not isinstance(p, _Proxy):
-        raise TypeError("grad() of a trial or test function")
+        raise TypeError("grad() of a trial or test function or of a GridFunction")
     return _Proxy(p.fes, p.trial, True)
+
+
+class _GfGrad:
+    """grad(gfu) inside an integrand of Integrate."""
+
+    def __init__(self, gf):
+        self.gf = gf
+
+    def __mul__(self, o):
+        if isinstance(o, _GfGrad):
+            return _GfEnergy(self.gf, o.gf, None)
+        return NotImplemented
+
+
+class _GfEnergy:
+    """c * grad(a) . grad(b) for GridFunctions a, b of one space and an
+    element-wise constant (or scalar) c."""
+
+    def __init__(self, a, b, coef):
+        if a.space is not b.space:
+            raise ValueError("grad(a)*grad(b): a and b must live in one space")
+        self.a, self.b, self.coef = a, b, coef
+
+    def __rmul__(self, c):
+        if isinstance(c, (int, float)):
+            val = float(c)
+        else:
+            c = _cf(c)
+            if c.elementwise is None:
+                raise ValueError("grad(a)*grad(b) takes element-wise constant coefficients only")
+            val = np.asarray(c.elementwise, dtype=np.float64)
+        return _GfEnergy(self.a, self.b, val if self.coef is None else self.coef * val)
+
+    __mul__ = __rmul__
+
+
+def Integrate(cf, mesh, element_wise=False, order=5):
+    """Integral of cf over the mesh, or per element (element_wise=True), as
+    Integrate (comp/python_comp.cpp:3697-3810).  c * grad(a) * grad(b) of
+    GridFunctions is integrated on the device, element by element with the
+    matrix-free operator's element matrices (ngsamd_mf_element_grad: exact
+    for the space's polynomials); any other coefficient function is
+    evaluated at the points of a rule of the given order and summed with the
+    weights |det F| w_q."""
+    if isinstance(cf, _GfGrad):
+        raise TypeError("Integrate: grad(gfu) alone is a vector; integrate grad(a)*grad(b)")
+    if isinstance(cf, _GfEnergy):
+        fes = cf.a.space
+        mf = DevMatrixFree(fes, max(2 * fes.order - 2, 1), None, "f64")
+        g = N.DeviceVector(mesh.ne, 8)
+        mf.op.element_grad(cf.a.vec._device(), cf.b.vec._device(), g)
+        vals = -g.download()  # element_grad gives -(a_e^T K_e b_e) / kappa_e with kappa = 1
+        if cf.coef is not None:
+            vals = vals * (np.asarray(cf.coef, dtype=np.float64) * np.ones(mesh.ne))
+    else:
+        ir = N.TrigRule(order) if mesh.dim == 2 else N.TetRule(order)
+        X, W = N.QuadraturePoints(mesh.ngmesh, ir)
+        fv = np.broadcast_to(np.asarray(_cf(cf).Evaluate(X), dtype=np.float64), W.shape)
+        vals = (np.abs(W) * fv).sum(axis=1)
+    return vals if element_wise else float(vals.sum())
 
 
 class _FormTerm:
