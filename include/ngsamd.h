@@ -70,6 +70,12 @@ int ngsamd_event_create(void** ev);
 int ngsamd_event_record(void* ev);
 int ngsamd_event_elapsed_ms(void* start, void* stop, float* ms); /* synchronises `stop` */
 int ngsamd_event_destroy(void* ev);
+/* Read-bandwidth calibration kernel (no reference counterpart): reads the n
+ * doubles of d_x (n even, 16-byte aligned) once with non-temporal 16-byte
+ * loads and writes their sum to *d_result (device), asynchronously; timed
+ * with the events above it gives the read ceiling that an HBM-bound,
+ * read-dominated kernel such as the CSR SpMV is compared with. */
+int ngsamd_stream_read(int64_t n, const double* d_x, double* d_result);
 
 /* ------------------------------------------------- vectors (BaseVector ops) */
 /* *h_result = sum_i x_i y_i, FP64 accumulation, fixed (deterministic) order */
@@ -143,14 +149,22 @@ int ngsamd_mf_create(ngsamd_op_t* op, int dim, int nd, int nq, int64_t ndof, int
                      const int64_t* h_class_offsets, const double* h_bmat, const double* h_weights,
                      const int* h_dofs, const double* h_metric, const double* h_kappa,
                      const int* h_elnr, int precision, int scatter);
-/* replace the element coefficient (device array indexed by global element number) */
-int ngsamd_mf_set_kappa(ngsamd_op_t op, const double* d_kappa);
+/* number of global elements of a matrix-free operator: the largest element
+ * number in h_elnr plus one (the row count of the tables if h_elnr was NULL);
+ * element coefficients and element gradients are indexed by it */
+int ngsamd_mf_num_elements(ngsamd_op_t op, int64_t* ne);
+/* replace the element coefficient: d_kappa is a device array of n values
+ * indexed by global element number, n must equal ngsamd_mf_num_elements; the
+ * operator keeps the pointer (not a copy) until the next call, so the array
+ * must stay allocated while the operator is applied */
+int ngsamd_mf_set_kappa(ngsamd_op_t op, const double* d_kappa, int64_t n);
 /* y = A x on FP32 vectors (inner solves of iterative refinement) */
 int ngsamd_mf_mult_f32(ngsamd_op_t op, const float* d_x, float* d_y);
 /* diagonal of the operator, FP64 */
 int ngsamd_mf_diagonal(ngsamd_op_t op, double* d_diag);
 /* g_e = -int_e grad(lam) . grad(u) = -(lam_e^T K_e u_e)/kappa_e, indexed by
- * global element number (adjoint gradient of an element coefficient). */
+ * global element number (adjoint gradient of an element coefficient); d_grad
+ * holds ngsamd_mf_num_elements values. */
 int ngsamd_mf_element_grad(ngsamd_op_t op, const double* d_lam, const double* d_u, double* d_grad);
 /* Assembles the stiffness matrix of a matrix-free operator's form into a new
  * CSR operator on the device: A[r][c] = sum over the elements e holding dofs

@@ -97,6 +97,8 @@ class PoissonProblem:
     def bind(self, kappa):
         """Point the operator at kappa (a tensor, one value per element) and
         rebuild the Jacobi preconditioner."""
+        if kappa.numel() != self.ne:
+            raise ValueError(f"kappa has {kappa.numel()} values, the mesh {self.ne} elements")
         self._kappa_ref = kappa  # keeps a CUDA tensor alive while bound
         self._kdev = _in(kappa)
         self.mf.op.set_kappa(self._kdev)

@@ -177,8 +177,10 @@ def Integrate(cf, mesh, element_wise=False, order=5):
         raise TypeError("Integrate: grad(gfu) alone is a vector; integrate grad(a)*grad(b)")
     if isinstance(cf, _GfEnergy):
         fes = cf.a.space
+        if cf.b.space.mesh.ngmesh is not fes.mesh.ngmesh or mesh.ngmesh is not fes.mesh.ngmesh:
+            raise ValueError("Integrate: the grid functions must live on the mesh that is integrated over")
         mf = DevMatrixFree(fes, max(2 * fes.order - 2, 1), None, "f64")
-        g = N.DeviceVector(mesh.ne, 8)
+        g = N.DeviceVector(mf.op.ne, 8)
         mf.op.element_grad(cf.a.vec._device(), cf.b.vec._device(), g)
         vals = -g.download()  # element_grad gives -(a_e^T K_e b_e) / kappa_e with kappa = 1
         if cf.coef is not None:

@@ -266,29 +266,39 @@ __global__ void AddScaledF32Kernel(double* __restrict__ u, const float* __restri
 constexpr int kCgMaxGrid = 1024;
 static int CgGrid(int64_t n) { return GridFor(n, kCgBlock, kCgMaxGrid); }
 
-// Partials of the operator kernel (<s, A s>) and of the update (<w, d>).
-static double* PartialsA() { return State().partials; }
-static double* PartialsW() { return State().partials + kMaxReduceBlocks; }
+// Partial sums of one CG instance: of the operator kernel (<s, A s>, a), of
+// the update (<w, d>, w), each room for kMaxReduceBlocks, and of the
+// operators' wide grids (kMaxWideBlocks, folded into a).
+struct CgPartials {
+  double* a;
+  double* w;
+  double* wide;
+};
+// the library's workspace (allocated by ngsamd_init)
+static CgPartials LibraryPartials() {
+  return {State().partials, State().partials + kMaxReduceBlocks, State().wide_partials};
+}
 
-// w = A s and the partials of <s, w> into PartialsA(); returns their number
+// w = A s and the partials of <s, w> into B.a; returns their number
 template <class V>
-static int EnqueueMatDot(ngsamd_op_t A, const V* s, V* w, int64_t n, CGScalars* dS, hipStream_t st) {
+static int EnqueueMatDot(ngsamd_op_t A, const V* s, V* w, int64_t n, CGScalars* dS, const CgPartials& B,
+                         hipStream_t st) {
   const int* done = &dS->done;
   if constexpr (sizeof(V) == 8) {
-    if (A->kind == OpKind::CSR) return static_cast<CsrOp*>(A)->MultDot(s, w, PartialsA(), done, st);
+    if (A->kind == OpKind::CSR) return static_cast<CsrOp*>(A)->MultDot(s, w, B.a, done, st, B.wide);
     if (A->kind == OpKind::MatrixFree) {
-      const int np = static_cast<MatFreeOp*>(A)->MultDot(s, w, PartialsA(), done, st);
+      const int np = static_cast<MatFreeOp*>(A)->MultDot(s, w, B.a, done, st, B.wide);
       if (np > 0) return np;
     } else {
       A->Mult(s, w, st);
     }
   } else {
     NGSAMD_REQUIRE(A->kind == OpKind::MatrixFree, "cg: FP32 vectors need a matrix-free operator");
-    const int np = static_cast<MatFreeOp*>(A)->MultDotF32(s, w, PartialsA(), done, st);
+    const int np = static_cast<MatFreeOp*>(A)->MultDotF32(s, w, B.a, done, st, B.wide);
     if (np > 0) return np;
   }
   const int g = CgGrid(n);
-  CgDotKernel<V><<<g, kCgBlock, 0, st>>>(s, w, n, done, PartialsA());
+  CgDotKernel<V><<<g, kCgBlock, 0, st>>>(s, w, n, done, B.a);
   NGSAMD_CHECK_LAUNCH();
   return g;
 }
@@ -309,17 +319,18 @@ static void EnqueuePrecond(ngsamd_op_t pre, const V* d, V* w, const int* stop, h
 // inv null: pre is applied between the update and the direction kernels.
 template <class V>
 static void EnqueueIteration(ngsamd_op_t A, const V* inv, ngsamd_op_t pre, V* u, V* d, V* w, V* s, int64_t n,
-                             CGScalars* dS, double* residuals, int P, hipStream_t st) {
-  const int na = EnqueueMatDot<V>(A, s, w, n, dS, st);
+                             CGScalars* dS, double* residuals, int P, hipStream_t st,
+                             const CgPartials& B = LibraryPartials()) {
+  const int na = EnqueueMatDot<V>(A, s, w, n, dS, B, st);
   const int g = CgGrid(n);
-  CgUpdateKernel<V><<<g, kCgBlock, 0, st>>>(inv, u, d, w, s, n, dS, P, PartialsA(), na, PartialsW());
+  CgUpdateKernel<V><<<g, kCgBlock, 0, st>>>(inv, u, d, w, s, n, dS, P, B.a, na, B.w);
   NGSAMD_CHECK_LAUNCH();
   if (!inv) {
     EnqueuePrecond<V>(pre, d, w, &dS->done, st);
-    CgDotKernel<V><<<g, kCgBlock, 0, st>>>(w, d, n, &dS->done, PartialsW());
+    CgDotKernel<V><<<g, kCgBlock, 0, st>>>(w, d, n, &dS->done, B.w);
     NGSAMD_CHECK_LAUNCH();
   }
-  CgDirectionKernel<V><<<g, kCgBlock, 0, st>>>(s, w, n, dS, P, PartialsW(), g, residuals);
+  CgDirectionKernel<V><<<g, kCgBlock, 0, st>>>(s, w, n, dS, P, B.w, g, residuals);
   NGSAMD_CHECK_LAUNCH();
 }
 
@@ -337,11 +348,28 @@ struct CgWork {
   CgWork(int64_t n, int maxiter) : d(n), w(n), s(n), residuals(maxiter), S(1) {}
 };
 
+// A captured graph of `chunk` CG iterations.  The kernels read the
+// tolerances and counters from the device scalars, so one instance serves
+// every solve on the same operators and vectors (the refinement's inner
+// solves capture it once per refinement, not once per step).
+struct CgGraph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  bool failed = false;  // capture not possible: launch the kernels directly
+  CgGraph() = default;
+  CgGraph(const CgGraph&) = delete;
+  ~CgGraph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+  }
+};
+
 // Iterates from an initialised state (d = f - A u already set, u set) until
-// the device flag is raised.  Uses a graph of `chunk` iterations if possible.
+// the device flag is raised.  Uses a graph of `chunk` iterations if possible,
+// kept in *cache (captured on first use) when cache is not null.
 template <class V>
 static CgResult IterateCg(ngsamd_op_t A, const V* inv, ngsamd_op_t pre, V* u, CgWork<V>& W, int64_t n, double tol,
-                          double atol, int maxiter, int chunk, hipStream_t st) {
+                          double atol, int maxiter, int chunk, hipStream_t st, CgGraph* cache = nullptr) {
   CGScalars hS{};
   hS.tol = tol;
   hS.atol = atol;
@@ -349,9 +377,9 @@ static CgResult IterateCg(ngsamd_op_t A, const V* inv, ngsamd_op_t pre, V* u, Cg
   NGSAMD_HIP(hipMemcpyAsync(W.S.p, &hS, sizeof(CGScalars), hipMemcpyHostToDevice, st));
   const int g = CgGrid(n);
   if (!inv) EnqueuePrecond<V>(pre, W.d.p, W.w.p, nullptr, st);
-  CgInitKernel<V><<<g, kCgBlock, 0, st>>>(inv, W.d.p, W.w.p, W.s.p, n, PartialsW());
+  CgInitKernel<V><<<g, kCgBlock, 0, st>>>(inv, W.d.p, W.w.p, W.s.p, n, LibraryPartials().w);
   NGSAMD_CHECK_LAUNCH();
-  CgInitCheckKernel<<<1, kCgBlock, 0, st>>>(PartialsW(), g, W.S.p, W.residuals.p);
+  CgInitCheckKernel<<<1, kCgBlock, 0, st>>>(LibraryPartials().w, g, W.S.p, W.residuals.p);
   NGSAMD_CHECK_LAUNCH();
 
   CGScalars* pin = nullptr;
@@ -361,13 +389,12 @@ static CgResult IterateCg(ngsamd_op_t A, const V* inv, ngsamd_op_t pre, V* u, Cg
     ~PinGuard() { (void)hipHostFree(p); }
   } pin_guard{pin};
 
-  hipGraphExec_t exec = nullptr;
-  hipGraph_t graph = nullptr;
   // graphs hold an even number of iterations so that every replay starts
   // at parity 0
   if (chunk > 0) chunk += chunk & 1;
-  bool use_graph = chunk > 0;
-  if (use_graph) {
+  CgGraph local;
+  CgGraph& G = cache ? *cache : local;
+  if (chunk > 0 && !G.exec && !G.failed) {
     hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
     if (e == hipSuccess) {
       bool ok = true;
@@ -377,26 +404,20 @@ static CgResult IterateCg(ngsamd_op_t A, const V* inv, ngsamd_op_t pre, V* u, Cg
       } catch (...) {
         ok = false;
       }
-      e = hipStreamEndCapture(st, &graph);
+      e = hipStreamEndCapture(st, &G.graph);
       if (!ok && e == hipSuccess) e = hipErrorUnknown;
-      if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      if (e == hipSuccess) e = hipGraphInstantiate(&G.exec, G.graph, nullptr, nullptr, 0);
     }
     if (e != hipSuccess) {
       (void)hipGetLastError();
-      if (graph) (void)hipGraphDestroy(graph);
-      graph = nullptr;
-      exec = nullptr;
-      use_graph = false;
+      if (G.graph) (void)hipGraphDestroy(G.graph);
+      G.graph = nullptr;
+      G.exec = nullptr;
+      G.failed = true;
     }
   }
-  struct GraphGuard {
-    hipGraph_t g;
-    hipGraphExec_t x;
-    ~GraphGuard() {
-      if (x) (void)hipGraphExecDestroy(x);
-      if (g) (void)hipGraphDestroy(g);
-    }
-  } graph_guard{graph, exec};
+  const bool use_graph = chunk > 0 && G.exec != nullptr;
+  hipGraphExec_t exec = G.exec;
 
   const int per_launch = use_graph ? chunk : 1;
   long long enqueued = 0;  // iterations enqueued without a graph (parity)
@@ -545,6 +566,8 @@ extern "C" int ngsamd_ir_cg_solve(ngsamd_op_t A, ngsamd_op_t inner, ngsamd_op_t 
     // kernel directly (no graph), e.g. for a kernel trace
     int chunk = 16;
     if (const char* ce = getenv("NGSAMD_CG_CHUNK")) chunk = std::max(0, atoi(ce));
+    // the inner solves of every step run on the same vectors: one graph
+    CgGraph inner_graph;
     while (true) {
       const double rn = n ? free_norm(r.p) : 0.0;
       relres = fn > 0 ? rn / fn : 0.0;
@@ -559,7 +582,7 @@ extern "C" int ngsamd_ir_cg_solve(ngsamd_op_t A, ngsamd_op_t inner, ngsamd_op_t 
       // the last refinement step only needs to gain the factor still missing
       double itol = inner_tol;
       if (fn > 0) itol = std::max(itol, std::min(0.5, 0.3 * tol * fn / rn));
-      CgResult ir = IterateCg<float>(Ai, inv32.p, pre, c32.p, W, n, itol, -1.0, max_inner, chunk, st);
+      CgResult ir = IterateCg<float>(Ai, inv32.p, pre, c32.p, W, n, itol, -1.0, max_inner, chunk, st, &inner_graph);
       inner_total += ir.iterations;
       // u += ||r|| c; r = f - A u (FP64)
       AddScaledF32Kernel<<<GridFor(n, 256, 8192), 256, 0, st>>>(u, c32.p, n, scal.p + 1);

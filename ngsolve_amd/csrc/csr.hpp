@@ -41,8 +41,11 @@ struct CsrOp : ngsamd_op_s {
   void MultAdd(double s, const double* x, double* y, hipStream_t st) override;
   void Mult(const double* x, double* y, hipStream_t st) override;
   // y = A x plus per-workgroup partial sums of <x, A x> in partials[];
-  // returns the number of partials; skipped entirely when *d_stop != 0
-  int MultDot(const double* x, double* y, double* partials, const int* d_stop, hipStream_t st);
+  // returns the number of partials; skipped entirely when *d_stop != 0.
+  // wide: room for kMaxWideBlocks partials of the wide grid (null: the
+  // library's)
+  int MultDot(const double* x, double* y, double* partials, const int* d_stop, hipStream_t st,
+              double* wide = nullptr);
   // Y = A X for K vectors stored interleaved ([n][K]); the matrix is read
   // once for all K in the runs layout
   // (all kernels return at once when *stop != 0, stop may be null)

@@ -1294,7 +1294,7 @@ static void LaunchElements(const MatFreeOp& A, const V* x, V* out, double s, con
 
 template <class V>
 int MatFreeOp::Apply(double s, const V* x, V* y, bool accumulate, const int* stop, hipStream_t st,
-                     double* partials) {
+                     double* partials, double* wide) {
   if (height == 0) return 0;
   if (scatter == NGSAMD_SCATTER_ATOMIC) {
     if (!accumulate) {
@@ -1312,7 +1312,7 @@ int MatFreeOp::Apply(double s, const V* x, V* y, bool accumulate, const int* sto
   LaunchElements<0, V>(*this, x, contrib, 1.0, stop, st, false);
   // with partials (CG): the gather on its wide grid writes one partial per
   // workgroup, folded to kFoldedPartials for CG's kernels
-  double* dotp = partials ? State().wide_partials : nullptr;
+  double* dotp = partials ? (wide ? wide : State().wide_partials) : nullptr;
   // one thread per dof: a wide grid hides each dof's load chain
   const int np = GridFor(height, kGatherBlock, kMaxWideBlocks);
   if (partials)
@@ -1335,11 +1335,13 @@ void MatFreeOp::MultStop(const double* x, double* y, const int* stop, hipStream_
 void MatFreeOp::MultF32(const float* x, float* y, const int* stop, hipStream_t st) {
   Apply<float>(1.0, x, y, false, stop, st);
 }
-int MatFreeOp::MultDot(const double* x, double* y, double* partials, const int* stop, hipStream_t st) {
-  return Apply<double>(1.0, x, y, false, stop, st, partials);
+int MatFreeOp::MultDot(const double* x, double* y, double* partials, const int* stop, hipStream_t st,
+                       double* wide) {
+  return Apply<double>(1.0, x, y, false, stop, st, partials, wide);
 }
-int MatFreeOp::MultDotF32(const float* x, float* y, double* partials, const int* stop, hipStream_t st) {
-  return Apply<float>(1.0, x, y, false, stop, st, partials);
+int MatFreeOp::MultDotF32(const float* x, float* y, double* partials, const int* stop, hipStream_t st,
+                          double* wide) {
+  return Apply<float>(1.0, x, y, false, stop, st, partials, wide);
 }
 
 void MatFreeOp::Diagonal(double* d, hipStream_t st) {
@@ -1714,10 +1716,21 @@ extern "C" int ngsamd_mf_create(ngsamd_op_t* op, int dim, int nd, int nq, int64_
   });
 }
 
-extern "C" int ngsamd_mf_set_kappa(ngsamd_op_t op, const double* d_kappa) {
+extern "C" int ngsamd_mf_num_elements(ngsamd_op_t op, int64_t* ne) {
+  return Guard([&] {
+    NGSAMD_REQUIRE(op && op->kind == OpKind::MatrixFree, "ngsamd_mf_num_elements: not a matrix-free operator");
+    NGSAMD_REQUIRE(ne, "ngsamd_mf_num_elements: null argument");
+    *ne = static_cast<MatFreeOp*>(op)->ne_global;
+  });
+}
+
+extern "C" int ngsamd_mf_set_kappa(ngsamd_op_t op, const double* d_kappa, int64_t n) {
   return Guard([&] {
     NGSAMD_REQUIRE(op && op->kind == OpKind::MatrixFree, "ngsamd_mf_set_kappa: not a matrix-free operator");
-    static_cast<MatFreeOp*>(op)->kappa_ext = d_kappa;
+    MatFreeOp* M = static_cast<MatFreeOp*>(op);
+    NGSAMD_REQUIRE(n == M->ne_global, "ngsamd_mf_set_kappa: coefficient count does not match the element count");
+    NGSAMD_REQUIRE(d_kappa || n == 0, "ngsamd_mf_set_kappa: null argument");
+    M->kappa_ext = d_kappa;
   });
 }
 

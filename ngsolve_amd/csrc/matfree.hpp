@@ -65,9 +65,12 @@ struct MatFreeOp : ngsamd_op_s {
   void MultF32(const float* x, float* y, const int* d_stop, hipStream_t st);
   // y = A x plus per-workgroup partial sums of <x, A x> in partials[];
   // returns their number, 0 when the scatter mode has no gather kernel
-  // (then the caller forms the inner product itself)
-  int MultDot(const double* x, double* y, double* partials, const int* d_stop, hipStream_t st);
-  int MultDotF32(const float* x, float* y, double* partials, const int* d_stop, hipStream_t st);
+  // (then the caller forms the inner product itself).  wide: room for
+  // kMaxWideBlocks partials of the gather's wide grid (null: the library's)
+  int MultDot(const double* x, double* y, double* partials, const int* d_stop, hipStream_t st,
+              double* wide = nullptr);
+  int MultDotF32(const float* x, float* y, double* partials, const int* d_stop, hipStream_t st,
+                 double* wide = nullptr);
   void Diagonal(double* d, hipStream_t st);
   void ElementGrad(const double* lam, const double* u, double* grad, hipStream_t st);
   // fills the values of A (same size, pattern holding every element pair)
@@ -88,7 +91,7 @@ struct MatFreeOp : ngsamd_op_s {
  private:
   template <class V>
   int Apply(double s, const V* x, V* y, bool accumulate, const int* d_stop, hipStream_t st,
-            double* partials = nullptr);
+            double* partials = nullptr, double* wide = nullptr);
 };
 
 }  // namespace ngsamd

@@ -127,6 +127,34 @@ __global__ __launch_bounds__(256) void SumPartialsKernel(const double* __restric
   if (threadIdx.x == 0) *result = v;
 }
 
+// Read-bandwidth calibration: every lane streams 16-byte non-temporal loads,
+// four in flight, over a grid-stride loop; each workgroup stores one partial
+// sum so that nothing is optimised away.
+typedef double dbl2v __attribute__((ext_vector_type(2)));
+
+__global__ __launch_bounds__(256) void StreamReadKernel(const dbl2v* __restrict__ x, int64_t n2,
+                                                        double* __restrict__ partials) {
+  __shared__ double lds[4];
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  const int64_t stride = (int64_t)gridDim.x * 256;
+  int64_t i = blockIdx.x * (int64_t)256 + threadIdx.x;
+  for (; i + 3 * stride < n2; i += 4 * stride) {
+    const dbl2v v0 = __builtin_nontemporal_load(x + i);
+    const dbl2v v1 = __builtin_nontemporal_load(x + i + stride);
+    const dbl2v v2 = __builtin_nontemporal_load(x + i + 2 * stride);
+    const dbl2v v3 = __builtin_nontemporal_load(x + i + 3 * stride);
+    a0 += v0.x + v0.y;
+    a1 += v1.x + v1.y;
+    a2 += v2.x + v2.y;
+    a3 += v3.x + v3.y;
+  }
+  for (; i < n2; i += stride) {
+    const dbl2v v = __builtin_nontemporal_load(x + i);
+    a0 += v.x + v.y;
+  }
+  StorePartial<256>((a0 + a1) + (a2 + a3), partials, lds);
+}
+
 template <class T>
 static double Dot(int64_t n, const T* x, const T* y) {
   RequireInit();
@@ -280,6 +308,22 @@ int ngsamd_vec_dot_f32(int64_t n, const float* x, const float* y, double* h_resu
   return Guard([&] {
     NGSAMD_REQUIRE(h_result != nullptr, "ngsamd_vec_dot_f32: null result");
     *h_result = Dot<float>(n, x, y);
+  });
+}
+
+int ngsamd_stream_read(int64_t n, const double* x, double* d_result) {
+  return Guard([&] {
+    RequireInit();
+    NGSAMD_REQUIRE(n >= 0 && n % 2 == 0, "ngsamd_stream_read: n must be even and >= 0");
+    NGSAMD_REQUIRE(d_result && (x || n == 0), "ngsamd_stream_read: null argument");
+    NGSAMD_REQUIRE(((uintptr_t)x & 15) == 0, "ngsamd_stream_read: x must be 16-byte aligned");
+    auto& S = State();
+    hipStream_t st = Stream();
+    const int grid = GridFor(n / 2, 256 * 16, kMaxReduceBlocks);
+    StreamReadKernel<<<grid, 256, 0, st>>>(reinterpret_cast<const dbl2v*>(x), n / 2, S.partials);
+    NGSAMD_CHECK_LAUNCH();
+    SumPartialsKernel<<<1, 256, 0, st>>>(S.partials, grid, d_result);
+    NGSAMD_CHECK_LAUNCH();
   });
 }
 

@@ -731,10 +731,11 @@ void CsrOp::Mult(const double* x, double* y, hipStream_t st) {
   LaunchSpmv<false, false>(*this, 1.0, x, y, st, nullptr, nullptr);
 }
 
-int CsrOp::MultDot(const double* x, double* y, double* partials, const int* d_stop, hipStream_t st) {
+int CsrOp::MultDot(const double* x, double* y, double* partials, const int* d_stop, hipStream_t st,
+                   double* wide) {
   // the operator on its wide grid writes one partial per workgroup; a
   // 64-workgroup kernel folds them so that CG's kernels read 64 values
-  double* wide = State().wide_partials;
+  if (!wide) wide = State().wide_partials;
   const int np = LaunchSpmv<false, true>(*this, 1.0, x, y, st, wide, d_stop);
   LaunchFoldPartials(wide, np, partials, d_stop, st);
   return kFoldedPartials;
@@ -770,6 +771,7 @@ extern "C" int ngsamd_csr_create(ngsamd_op_t* op, int64_t height, int64_t width,
                                  const int* cols, const double* vals) {
   return Guard([&] {
     RequireInit();
+    NGSAMD_REQUIRE(height >= 0 && width >= 0, "ngsamd_csr_create: negative size");
     NGSAMD_REQUIRE(op && rp && (rp[height] == 0 || (cols && vals)), "ngsamd_csr_create: null argument");
     *op = nullptr;
     *op = new CsrOp(height, width, rp, cols, vals);

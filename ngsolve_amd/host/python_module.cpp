@@ -55,9 +55,14 @@ struct Operator {
   std::string kind;
   int64_t height = 0, width = 0;
   int64_t nnz = 0;  // CSR operators: number of stored values
+  int64_t ne = 0;   // matrix-free operators: number of global elements
+  // matrix-free operators keep the pointer of an external coefficient, not a
+  // copy: the vector is held here for as long as the operator may read it
+  std::shared_ptr<DeviceVector> kappa_keep;
   Operator(ngsamd_op_t o, std::string k) : op(o), kind(std::move(k)) {
     Check(ngsamd_op_height(op, &height));
     Check(ngsamd_op_width(op, &width));
+    if (kind == "matfree") Check(ngsamd_mf_num_elements(op, &ne));
   }
   ~Operator() {
     if (op) ngsamd_op_destroy(op);
@@ -323,6 +328,11 @@ PYBIND11_MODULE(_ngsamd, m) {
     return ms;
   });
   m.def("event_destroy", [](uintptr_t e) { Check(ngsamd_event_destroy((void*)e)); });
+  m.def("stream_read", [](DeviceVector& x, DeviceVector& result) {
+    RequireF64(x, x.size, "stream_read");
+    RequireF64(result, 1, "stream_read (one result)");
+    Check(ngsamd_stream_read(x.size, x.d(), result.d()));
+  });
 
   py::class_<DeviceVector, std::shared_ptr<DeviceVector>>(m, "DeviceVector")
       .def(py::init<int64_t, int>(), py::arg("size"), py::arg("itemsize") = 8)
@@ -381,6 +391,8 @@ PYBIND11_MODULE(_ngsamd, m) {
 
   py::class_<Operator, std::shared_ptr<Operator>>(m, "Operator")
       .def_readonly("kind", &Operator::kind)
+      // the ngsamd_op_t handle, for calls through the C ABI (ctypes)
+      .def_property_readonly("handle", [](const Operator& A) { return (uintptr_t)A.op; })
       .def_readonly("height", &Operator::height)
       .def_readonly("width", &Operator::width)
       .def("mult",
@@ -408,15 +420,20 @@ PYBIND11_MODULE(_ngsamd, m) {
              else if (A.kind == "matfree") Check(ngsamd_mf_diagonal(A.op, d.d()));
              else throw std::runtime_error("diagonal: not available for this operator");
            })
+      .def_readonly("ne", &Operator::ne)
       .def("set_kappa",
            [](Operator& A, std::shared_ptr<DeviceVector> k) {
-             if (k->itemsize != 8) throw std::runtime_error("set_kappa: float64 vector expected");
-             Check(ngsamd_mf_set_kappa(A.op, k->d()));
+             if (A.kind != "matfree") throw std::runtime_error("set_kappa: not a matrix-free operator");
+             RequireF64(*k, A.ne, "set_kappa (one value per element)");
+             Check(ngsamd_mf_set_kappa(A.op, k->d(), k->size));
+             A.kappa_keep = std::move(k);
            })
       .def("element_grad",
            [](Operator& A, DeviceVector& lam, DeviceVector& u, DeviceVector& g) {
+             if (A.kind != "matfree") throw std::runtime_error("element_grad: not a matrix-free operator");
              RequireF64(lam, A.height, "element_grad");
              RequireF64(u, A.height, "element_grad");
+             RequireF64(g, A.ne, "element_grad (one value per element)");
              Check(ngsamd_mf_element_grad(A.op, lam.d(), u.d(), g.d()));
            })
       .def_readonly("nnz", &Operator::nnz)

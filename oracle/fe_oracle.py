@@ -568,14 +568,16 @@ class Problem:
 
 
 def jacobi_invdiag(rowptr, cols, vals, free):
+    """JacobiPrecond (linalg/jacobi.cpp:41-66): the diagonal entry of each row
+    (0 where the row stores none) and its inverse on the free dofs, 0 on the
+    others (1 where a free row's diagonal is 0)."""
+    rowptr = np.asarray(rowptr, dtype=np.int64)
     n = len(rowptr) - 1
+    rows = np.repeat(np.arange(n, dtype=np.int64), np.diff(rowptr))
+    on = np.asarray(cols, dtype=np.int64) == rows
     diag = np.zeros(n)
-    for r in range(n):
-        s, e = rowptr[r], rowptr[r + 1]
-        k = np.searchsorted(cols[s:e], r)
-        if k < e - s and cols[s + k] == r:
-            diag[r] = vals[s + k]
-    inv = np.where(free != 0, 1.0 / np.where(diag == 0, 1, diag), 0.0)
+    diag[rows[on]] = np.asarray(vals)[on]
+    inv = np.where(np.asarray(free) != 0, 1.0 / np.where(diag == 0, 1, diag), 0.0)
     return diag, inv
 
 

@@ -1,0 +1,169 @@
+"""The 3D configurations at their stated size against the oracle.
+
+32^3 cubes x 6 Kuhn tets, H1 p=4, integration order 8: 196,608 tets,
+2,146,689 dofs, 159,230,209 nonzeros.
+
+- kappa = 1, x ~ U[-1,1] (seed 1): the matrix-free apply (FP64 within 1e-13
+  of the oracle's per-point F^-T D F^-1 apply, comp/matrixfree.cpp:681-1087,
+  in the inf-norm; FP32 / FP16 / BF16 within their stated bounds) and
+  Jacobi-PCG to 1e-10 on the CSR and on the matrix-free operator (iteration
+  count within +-1 and solution within 1e-10 of the oracle's PCG with the
+  CGSolver recurrence, python/krylovspace.py:263-291);
+- kappa_e = exp(N(0,1)) (seed 2), f = 1: FP64 CG on both operators against
+  the oracle's PCG, and the mixed-precision refinement with FP32, BF16 and
+  FP16 inner solves at their default inner tolerances: true FP64 residual
+  <= 1e-10 (computed with the oracle's SpMV) and the solution against the
+  oracle's PCG run to 1e-12.
+
+The CSR values come from the host assembly, which the oracle's own assembly
+pins at the smaller sizes (tests/test_gpu_assembly.py); here, at full size,
+the oracle's matrix-free apply pins the same matrix (1e-13).
+"""
+import numpy as np
+import pytest
+
+import oracle
+from oracle import fe_oracle as O
+from helpers import N, dvec, mf_operator, problem, relerr_inf, say
+
+pytestmark = [pytest.mark.gpu, pytest.mark.slow]
+
+_cache = {}
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _release():
+    yield
+    _cache.clear()  # the full-size problems hold several GB of host and device memory
+
+
+NX, P_ORDER, ORDER = 32, 4, 8
+
+
+def _config(kappa_seed):
+    """Product host setup (pattern, values, dof tables) for kappa = 1
+    (kappa_seed None) or lognormal kappa; the oracle's mesh / dofs / geometry
+    and per-class reference gradients are shared."""
+    if kappa_seed in _cache:
+        return _cache[kappa_seed]
+    say(f"[full] host set-up 32^3 p=4, kappa seed {kappa_seed}")
+    P = problem(3, NX, P_ORDER, order=ORDER, seed_kappa=kappa_seed)
+    if "oracle" not in _cache:
+        pr = O.Problem(P["omesh"], P_ORDER, intorder=ORDER)
+        cls = O.class_number(3, pr.mesh["els"])
+        uniq = np.unique(cls)
+        reps = [int(np.where(cls == k)[0][0]) for k in uniq]
+        _, g = O.element_shapes(3, P_ORDER, pr.mesh["els"][reps], pr.pts)
+        _cache["oracle"] = (pr, g, np.searchsorted(uniq, cls).astype(np.int32))
+    pr, bref, elcls = _cache["oracle"]
+    P["oracle_problem"], P["bref"], P["elcls"] = pr, bref, elcls
+    free = P["fes"].freedofs
+    P["diag"], P["inv"] = O.jacobi_invdiag(P["rowptr"], P["cols"], P["vals"], free)
+    P["f"] = N.AssembleLinearForm(P["fes"], P["ir"], np.ones((P["mesh"].ne, len(P["ir"]))))
+    _cache[kappa_seed] = P
+    return P
+
+
+def _oracle_apply(P, x):
+    pr = P["oracle_problem"]
+    return oracle.mf_apply(3, P["bref"], P["elcls"], pr.wts, pr.dofs, pr.Finv, pr.detF, P["kappa"], x)
+
+
+def _oracle_pcg(P, tol):
+    key = ("pcg", tol)
+    if key not in P:
+        say(f"[full] oracle PCG tol {tol}")
+        P[key] = oracle.pcg_csr(P["rowptr"], P["cols"], P["vals"], P["inv"], P["f"], tol=tol, maxiter=100000)
+    return P[key]
+
+
+def _mf(P, precision):
+    key = ("mf", precision)
+    if key not in P:
+        P[key] = mf_operator(P, precision=precision)
+    return P[key]
+
+
+def test_config2_sizes_and_dofs_match_oracle(device):
+    P = _config(None)
+    pr = P["oracle_problem"]
+    assert P["ndof"] == pr.ndof == 2146689
+    assert int(P["rowptr"][-1]) == 159230209
+    assert P["mesh"].ne == 196608
+    assert np.array_equal(P["fes"].el_dofs, pr.dofs)
+    assert np.array_equal(P["fes"].freedofs.astype(bool), pr.free.astype(bool))
+    assert int((~pr.free.astype(bool)).sum()) == 98306
+
+
+def test_config2_mf_f64_apply(device):
+    P = _config(None)
+    x = np.random.default_rng(1).uniform(-1, 1, P["ndof"])
+    ref = _oracle_apply(P, x)
+    # the host-assembled matrix is the oracle's operator at full size
+    assert relerr_inf(oracle.spmv(P["rowptr"], P["cols"], P["vals"], x), ref) < 1e-13
+    A = _mf(P, 0)
+    dx, dy = dvec(x), N.DeviceVector(P["ndof"], 8)
+    A.mult(dx, dy)
+    err = relerr_inf(dy.download(), ref)
+    say(f"[full] FP64 matrix-free apply vs oracle: {err:.2e}")
+    assert err < 1e-13
+
+
+# bounds as at the smaller sizes: FP32 1e-5; FP16 (11-bit significand) 4e-3;
+# BF16 (8-bit significand) 3e-2, relative in the inf-norm, FP32 accumulation
+@pytest.mark.parametrize("precision,tol", [(1, 1e-5), (2, 3e-2), (3, 4e-3)])
+def test_config2_mf_low_precision_apply(device, precision, tol):
+    P = _config(None)
+    x = np.random.default_rng(1).uniform(-1, 1, P["ndof"])
+    ref = _oracle_apply(P, x)
+    A = _mf(P, precision)
+    dx, dy = dvec(x), N.DeviceVector(P["ndof"], 8)
+    A.mult(dx, dy)
+    err = relerr_inf(dy.download(), ref)
+    x32, y32 = dvec(x, 4), N.DeviceVector(P["ndof"], 4)
+    A.mult_f32(x32, y32)
+    err32 = relerr_inf(y32.download(), ref)
+    say(f"[full] precision {precision} apply vs oracle: {err:.2e} (FP32 vectors {err32:.2e})")
+    assert err < tol and err32 < tol
+
+
+@pytest.mark.parametrize("op", ["csr", "mf"])
+@pytest.mark.parametrize("kappa_seed", [None, 2])
+def test_fp64_cg_matches_oracle(device, op, kappa_seed):
+    P = _config(kappa_seed)
+    u_ref, its_ref, res_ref = _oracle_pcg(P, 1e-10)
+    A = N.CreateCSR(P["ndof"], P["ndof"], P["rowptr"], P["cols"], P["vals"]) if op == "csr" else _mf(P, 0)
+    C = N.CreateJacobi(P["diag"], P["fes"].freedofs)
+    df, du = dvec(P["f"]), N.DeviceVector(P["ndof"], 8)
+    its, res = N.CGSolve(A, C, df, du, True, 1e-10, -1.0, 100000, 16)
+    u = du.download()
+    err = np.linalg.norm(u - u_ref) / np.linalg.norm(u_ref)
+    say(f"[full] {op} CG kappa seed {kappa_seed}: its {its} oracle {its_ref}, solution {err:.2e}")
+    assert abs(its - its_ref) <= 1
+    assert err <= 1e-10
+    assert abs(res[0] - res_ref[0]) <= 1e-12 * res_ref[0]
+
+
+@pytest.mark.parametrize("precision", [1, 2, 3])
+def test_config3_refinement_default_inner_tolerance(device, precision):
+    """FP64 refinement around FP32 / BF16 / FP16 matrix-free inner solves, inner
+    tolerance left to its default (inner_tol <= 0)."""
+    P = _config(2)
+    A64 = _mf(P, 0)
+    A_in = _mf(P, precision)
+    C = N.CreateJacobi(P["diag"], P["fes"].freedofs)
+    f = P["f"]
+    df, du = dvec(f), N.DeviceVector(P["ndof"], 8)
+    outer, inner, rel = N.IRCGSolve(A64, A_in, C, df, du, 1e-10, -1.0, 60, 100000)
+    u = du.download()
+    fr = P["fes"].freedofs.astype(bool)
+    r = f - oracle.spmv(P["rowptr"], P["cols"], P["vals"], u)
+    true_rel = np.linalg.norm(r[fr]) / np.linalg.norm(f[fr])
+    u_ref, _, _ = _oracle_pcg(P, 1e-12)
+    err = np.linalg.norm(u - u_ref) / np.linalg.norm(u_ref)
+    say(f"[full] refinement precision {precision}: outer {outer} inner {inner}, residual {rel:.2e} "
+        f"(oracle SpMV {true_rel:.2e}), solution vs oracle PCG(1e-12) {err:.2e}")
+    assert rel <= 1e-10
+    assert true_rel <= 1.0001e-10
+    # a 1e-10 residual bounds the error by cond(A) * 1e-10; measured 2e-10 to 4e-10
+    assert err <= 1e-8

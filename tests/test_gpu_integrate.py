@@ -38,6 +38,15 @@ def test_element_energies_match_oracle(device, dim, n, p):
     assert abs(ngs.Integrate(ngs.grad(a) * ngs.grad(b), m) - total) <= 1e-12 * abs(total) + 1e-14
 
 
+def test_integrate_rejects_other_mesh(device):
+    """Element-wise energies of grid functions on one mesh over another mesh
+    (with fewer elements) are refused instead of written past the output."""
+    m, m2 = ngs.MakeStructured2DMesh(nx=4, ny=4), ngs.MakeStructured2DMesh(nx=2, ny=2)
+    a = ngs.GridFunction(ngs.H1(m, order=2))
+    with pytest.raises(ValueError):
+        ngs.Integrate(ngs.grad(a) * ngs.grad(a), m2, element_wise=True)
+
+
 def test_integrate_rejects_vector_integrand(device):
     m = ngs.MakeStructured2DMesh(nx=2, ny=2)
     gfu = ngs.GridFunction(ngs.H1(m, order=1))

@@ -122,6 +122,35 @@ def test_mf_set_kappa(device):
     assert relerr_inf(dy.download(), oracle.spmv(P["rowptr"], P["cols"], vals, x)) < 1e-13
 
 
+def test_mf_coefficient_and_gradient_sizes_checked(device):
+    """A coefficient or gradient vector whose length is not the element count
+    is refused before any kernel reads or writes it; a coefficient handed over
+    as a temporary stays alive while the operator holds it."""
+    import gc
+    say("[mf] coefficient / gradient sizes")
+    P = problem(2, 6, 3, seed_kappa=None)
+    A = mf_operator(P)
+    ne = P["mesh"].ne
+    assert A.ne == ne
+    with pytest.raises(RuntimeError):
+        A.set_kappa(N.DeviceVector(ne - 1, 8))
+    with pytest.raises(RuntimeError):
+        A.set_kappa(N.DeviceVector(ne, 4))
+    x = np.random.default_rng(1).uniform(-1, 1, P["ndof"])
+    dx = dvec(x)
+    with pytest.raises(RuntimeError):
+        A.element_grad(dx, dx, N.DeviceVector(ne - 3, 8))
+    kap = np.exp(np.random.default_rng(12).standard_normal(ne))
+    A.set_kappa(dvec(kap))           # the only reference is the operator's
+    gc.collect()
+    filler = [dvec(np.full(ne, 1e300)) for _ in range(4)]  # reuse freed memory, if any
+    dy = N.DeviceVector(P["ndof"], 8)
+    A.mult(dx, dy)
+    vals = N.AssembleCSR(P["fes"], P["rowptr"], P["cols"], P["ir"], True, kap, False, None)
+    assert relerr_inf(dy.download(), oracle.spmv(P["rowptr"], P["cols"], vals, x)) < 1e-13
+    del filler
+
+
 def test_mf_cg_matches_csr_cg(device):
     """Same operator, same recurrence: CG iteration counts agree within +-1."""
     say("[mf] CG vs CSR CG")

@@ -46,6 +46,41 @@ def test_spmv_block_and_grid_switches(device, monkeypatch, algo, nnz, grid):
     assert np.abs(du.download() - u_ref).max() <= 1e-10 * np.abs(u_ref).max()
 
 
+@pytest.mark.parametrize("case", ["runs_16cube_p4", "stream_64cube_p1"])
+def test_spmv_size_dependent_defaults(device, monkeypatch, case):
+    """The default block size and grid (no NGSAMD_SPMV* switch set) where they
+    differ from a one-block-per-workgroup launch: the runs kernel on 16^3 p=4
+    (about 10K blocks of 2048 values: a third as many workgroups as blocks)
+    and the stream kernel on P1 over 64^3 cubes (above 1024 blocks: half as
+    many workgroups as blocks), plain apply, accumulating apply and the CG
+    variant against the oracle."""
+    for k in ("NGSAMD_SPMV", "NGSAMD_SPMV_NNZ", "NGSAMD_SPMV_GRID"):
+        monkeypatch.delenv(k, raising=False)
+    n, p = (16, 4) if case.startswith("runs") else (64, 1)
+    say(f"[spmv defaults] {case}")
+    P = problem(3, n, p, seed_kappa=5)
+    A = N.CreateCSR(P["ndof"], P["ndof"], P["rowptr"], P["cols"], P["vals"])
+    x = np.random.default_rng(3).uniform(-1, 1, P["ndof"])
+    ref = oracle.spmv(P["rowptr"], P["cols"], P["vals"], x)
+    dx, dy = dvec(x), N.DeviceVector(P["ndof"], 8)
+    A.mult(dx, dy)
+    assert relerr_inf(dy.download(), ref) < 1e-13
+    y0 = np.random.default_rng(4).uniform(-1, 1, P["ndof"])
+    dy.upload(y0)
+    A.mult_add(0.5, dx, dy)
+    assert relerr_inf(dy.download(), y0 + 0.5 * ref) < 1e-13
+    free = P["fes"].freedofs.astype(np.uint8)
+    diag, inv = O.jacobi_invdiag(P["rowptr"], P["cols"], P["vals"], free)
+    C = N.CreateJacobi(diag, free)
+    f = np.where(free != 0, 1.0, 0.0)
+    df, du = dvec(f), N.DeviceVector(P["ndof"], 8)
+    its, _ = N.CGSolve(A, C, df, du, True, 1e-10, -1.0, 5000, 16)
+    u_ref, its_ref, _ = oracle.pcg_csr(P["rowptr"], P["cols"], P["vals"], inv, f, tol=1e-10, maxiter=5000)
+    say(f"[spmv defaults] {case}: its {its} oracle {its_ref}")
+    assert abs(its - its_ref) <= 1
+    assert np.linalg.norm(du.download() - u_ref) <= 1e-10 * np.linalg.norm(u_ref)
+
+
 def test_ir_without_graphs_matches_graphs(device, monkeypatch):
     say("[ir] NGSAMD_CG_CHUNK=0 vs graphs")
     P = problem(3, 4, 4, order=8, seed_kappa=3)
