@@ -82,9 +82,26 @@ def build_host(verbose=False):
     return out
 
 
+def build_adapter_test(verbose=False):
+    """The C++ operator layer's test program (tests/cpp), linked against
+    libngsamd.so; run by tests/test_adapter.py."""
+    src = os.path.join(ROOT, "tests", "cpp", "basematrix_adapter.cpp")
+    outdir = os.path.join(ROOT, "tests", "cpp", "build")
+    os.makedirs(outdir, exist_ok=True)
+    out = os.path.join(outdir, "basematrix_adapter")
+    tmp = out + ".tmp"
+    o = _run(["g++", "-O2", "-std=c++17", "-Wall", "-I" + os.path.join(ROOT, "include"), src, "-L" + HERE,
+              "-lngsamd", "-Wl,-rpath,$ORIGIN/../../../ngsolve_amd", "-o", tmp])
+    if verbose and o.strip():
+        print(o)
+    os.replace(tmp, out)
+    return out
+
+
 def build(verbose=False):
     lib = build_hip(verbose)
     mod = build_host(verbose)
+    build_adapter_test(verbose)
     return lib, mod
 
 
