@@ -10,12 +10,16 @@ SparseMatrix<double>::MultAdd path).  `value` is its throughput in GDoF/s
 (ndof / time per apply); reported beside it under "extra": the other CSR
 kernels, the matrix-free applies in every precision, the 4-vector apply and
 the lockstep 4-right-hand-side CG, the FP64 CSR Jacobi-CG and the
-mixed-precision (FP32 matrix-free inner, FP64 refinement) time-to-1e-10 on
-the lognormal-kappa variant, block-Jacobi CG, the device set-up (pattern,
-assembly, topology and dof tables) against the host, the 512 batched 2D
-problems and the 2D 64^2 p=3 case.
+mixed-precision (FP32 / BF16 / FP16 matrix-free inner, FP64 refinement)
+time-to-1e-10 on the lognormal-kappa variant, block-Jacobi CG, the device
+set-up (pattern, assembly, topology and dof tables) against the host, the
+adjoint gradient on 128^2 triangles, the 512 batched 2D problems and the 2D
+64^2 p=3 case.  Every result states its error against the oracle (the CPU
+restatement of the reference's algorithm, run on the host as the CPU
+baseline), unless --no-cpu is given.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-cg]
+                       [--no-batch] [--no-assembly] [--no-2d] [--no-adjoint]
 """
 import argparse
 import json
@@ -70,24 +74,104 @@ def build_mf(N, fes, mesh, ir, kappa, precision, scatter=0):
                               el, precision, scatter)
 
 
-def cpu_baseline(rowptr, cols, vals, x, ndof, budget_s=10.0):
-    """The oracle's row-parallel CSR SpMV (OpenMP, all host threads) on the
-    same matrix; a bounded sample of about budget_s seconds."""
-    import oracle
-    y = np.empty(ndof)
-    oracle.spmv(rowptr, cols, vals, x, y)  # warm-up
+def host_inventory():
+    """What the CPU baseline ran on: lscpu's model and core counts, nproc and
+    the threads this process may use."""
+    import subprocess
+    info = {"nproc": os.cpu_count(), "affinity_cpus": len(os.sched_getaffinity(0)),
+            "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS")}
+    try:
+        out = subprocess.run(["lscpu"], stdout=subprocess.PIPE, stderr=subprocess.DEVNULL, text=True,
+                             timeout=10).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in ("Model name", "CPU(s)", "Socket(s)", "Core(s) per socket", "Thread(s) per core",
+                             "NUMA node(s)"):
+                info["lscpu " + k.strip()] = v.strip()
+    except (OSError, subprocess.SubprocessError):
+        info["lscpu"] = "unavailable"
+    return info
+
+
+def _median_time(fn, budget_s, warmup=3, min_reps=3, max_reps=100000):
+    """Median seconds per call: `warmup` untimed calls, then calls timed one
+    by one (steady clock) until budget_s has passed and min_reps were made."""
+    for _ in range(warmup):
+        fn()
+    ts = []
     t0 = time.perf_counter()
-    reps = 0
-    while True:
+    while len(ts) < max_reps and (len(ts) < min_reps or time.perf_counter() - t0 < budget_s):
+        t = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t)
+    return float(np.median(ts)), len(ts)
+
+
+def cpu_baseline_apply(rowptr, cols, vals, x, ndof, n):
+    """CPU baseline of the apply: the oracle's row-parallel CSR SpMV and its
+    per-quadrature-point matrix-free apply (the reference's MatrixFreeBTDTB
+    formula, comp/matrixfree.cpp:681-1087) on the same 32^3 p=4 problem, on
+    all the host threads this process has and on one thread; bounded samples
+    (3 warm-ups, then the median of repeated calls).  Also returns the
+    oracle's outputs, against which the device results' errors are stated."""
+    import oracle
+    from oracle import fe_oracle as O
+    threads = oracle.num_threads()
+    pr = O.Problem(O.structured_3d(n), 4, intorder=8)
+    cls = O.class_number(3, pr.mesh["els"])
+    uniq = np.unique(cls)
+    reps = [int(np.where(cls == k)[0][0]) for k in uniq]
+    _, bref = O.element_shapes(3, 4, pr.mesh["els"][reps], pr.pts)
+    elcls = np.searchsorted(uniq, cls).astype(np.int32)
+    y = np.empty(ndof)
+
+    def spmv():
         oracle.spmv(rowptr, cols, vals, x, y)
-        reps += 1
-        el = time.perf_counter() - t0
-        if el >= budget_s:
-            break
-    return {"value": round(ndof * reps / el / 1e9, 4), "unit": "GDoF/s", "cores": oracle.num_threads(),
-            "kind": "port",
-            "sample": f"{reps} FP64 CSR applies y=Ax of the 32^3 p=4 matrix ({int(rowptr[-1])} nonzeros), "
-                      f"oracle C SpMV with OpenMP, {el:.1f} s"}
+
+    def mf():  # Mult = zero + MultAdd (the oracle's apply accumulates)
+        y[:] = 0.0
+        oracle.mf_apply(3, bref, elcls, pr.wts, pr.dofs, pr.Finv, pr.detF, None, x, y)
+
+    out = {"threads_all": threads}
+    t_all, r_all = _median_time(spmv, 8.0)
+    y_csr = y.copy()
+    tm_all, rm_all = _median_time(mf, 4.0, warmup=1)
+    y_mf = y.copy()
+    oracle.set_threads(1)
+    try:
+        t_one, r_one = _median_time(spmv, 3.0, warmup=1)
+        tm_one, rm_one = _median_time(mf, 0.0, warmup=0, min_reps=1)
+    finally:
+        oracle.set_threads(threads)
+    nnz = int(rowptr[-1])
+    out["csr_spmv"] = {"threads": threads, "seconds_median": t_all, "reps": r_all,
+                       "gdofs": round(ndof / t_all / 1e9, 4)}
+    out["csr_spmv_one_thread"] = {"threads": 1, "seconds_median": t_one, "reps": r_one,
+                                  "gdofs": round(ndof / t_one / 1e9, 4)}
+    out["matrix_free"] = {"threads": threads, "seconds_median": tm_all, "reps": rm_all,
+                          "gdofs": round(ndof / tm_all / 1e9, 4)}
+    out["matrix_free_one_thread"] = {"threads": 1, "seconds_median": tm_one, "reps": rm_one,
+                                     "gdofs": round(ndof / tm_one / 1e9, 4)}
+    out["sample"] = (f"FP64 y=Ax of the 32^3 p=4 problem ({nnz} nonzeros): oracle C CSR SpMV, {r_all} calls on "
+                     f"{threads} threads and {r_one} on 1; oracle C per-point matrix-free apply (125 points), "
+                     f"{rm_all} calls on {threads} threads and {rm_one} on 1; median per call")
+    return out, {"y_csr": y_csr, "y_mf": y_mf}
+
+
+def cpu_baseline_pcg(rowptr, cols, vals, invdiag, f, tol=1e-10):
+    """CPU baseline of a solve: the oracle's C Jacobi-PCG (CGSolver
+    recurrence, OpenMP on all host threads) on the same system; its
+    solution is the reference the device solutions are compared with."""
+    import oracle
+    t0 = time.perf_counter()
+    u, its, _ = oracle.pcg_csr(rowptr, cols, vals, invdiag, f, tol=tol, maxiter=200000)
+    return {"iterations": its, "seconds": round(time.perf_counter() - t0, 4), "threads": oracle.num_threads()}, u
+
+
+def _rel(a, b, norm=None):
+    if norm == "inf":
+        return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
 
 def bench_batch(N, args):
@@ -101,12 +185,14 @@ def bench_batch(N, args):
     bs = BatchSolver(probs)
     log(f"[bench] batch setup {time.perf_counter() - t:.1f}s ndof={bs.n} nnz={bs.nnz} block_nnz={bs.block_nnz}")
     out = {"problems": len(probs), "total_dofs": bs.n, "total_nnz": bs.nnz, "autotuned_block_nnz": bs.block_nnz}
+    sols = {}
     for name, fn in [("fp64_block_cg", lambda: bs.solve_fp64(tol=1e-10)),
                      ("mixed_block_ir", lambda: bs.solve_mixed(tol=1e-10))]:
         N.synchronize()
         t0 = time.perf_counter()
         r = fn()
         el = time.perf_counter() - t0
+        sols[name] = bs.solution()
         ms = r["finish_ms"]
         out[name] = {"seconds": round(el, 4), "problems_per_s": round(len(probs) / el, 1),
                      "p50_ms": round(float(np.percentile(ms, 50)), 3),
@@ -134,6 +220,27 @@ def bench_batch(N, args):
                                  "p50_ms": round(float(np.percentile(seq, 50) * 1e3), 3),
                                  "p99_ms": round(float(np.percentile(seq, 99) * 1e3), 3)}
     log(f"[bench] sequential: {seq.sum():.3f}s")
+    if not args.no_cpu:
+        # error against the oracle's PCG on a sample of the problems (every
+        # problem is checked in the tests): the 4 largest and 12 spread over
+        # the batch; the oracle's time for the whole sample is its baseline
+        order = np.argsort([pr.ndof for pr in probs])
+        sample = sorted(set(order[-4:].tolist()) | set(range(0, len(probs), max(1, len(probs) // 12))))
+        err = {k: 0.0 for k in sols}
+        t_cpu = 0.0
+        for k in sample:
+            pr = probs[k]
+            d = diagonal(pr.rowptr, pr.cols, pr.vals)
+            inv = np.where(pr.free != 0, 1.0 / np.where(d == 0, 1.0, d), 0.0)
+            res, u_ref = cpu_baseline_pcg(pr.rowptr, pr.cols, pr.vals, inv, pr.rhs)
+            t_cpu += res["seconds"]
+            for name in sols:
+                err[name] = max(err[name], _rel(sols[name][bs.offsets[k]:bs.offsets[k + 1]], u_ref))
+        for name in sols:
+            out[name]["max_rel_err_vs_oracle_sample"] = err[name]
+        out["oracle_sample"] = {"problems": len(sample), "cpu_oracle_pcg_seconds": round(t_cpu, 3),
+                                "threads": res["threads"]}
+        log(f"[bench] batch vs oracle on {len(sample)} problems: {err}")
     return out
 
 
@@ -146,7 +253,7 @@ def bench_2d(N, args):
     the host."""
     m = N.MakeStructured2DMesh(64, 64)
     fes = N.H1Space(m, 3, 0xFFFFFFFFFFFFFFFF)
-    ir = N.TrigRule(4)
+    ir = N.TrigRule(6)  # the default integration order 2p
     rp, cols = N.BuildPattern(fes)
     vals = N.AssembleCSR(fes, rp, cols, ir, True, None, False, None)
     n = fes.ndof
@@ -175,22 +282,105 @@ def bench_2d(N, args):
     log(f"[bench] 2D 64^2 p=3: apply {ms * 1e3:.1f} us, CG {its} its {t_cg * 1e3:.1f} ms")
     if not args.no_cpu:
         import oracle
+        dx.upload(x)
+        A.mult(dx, dy)
+        y_dev = dy.download()
         y = np.empty(n)
-        oracle.spmv(rp, cols, vals, x, y)
-        t0 = time.perf_counter()
-        reps = 0
-        while time.perf_counter() - t0 < 2.0:
-            oracle.spmv(rp, cols, vals, x, y)
-            reps += 1
-        out["cpu_oracle_apply_us"] = round(1e6 * (time.perf_counter() - t0) / reps, 2)
+        t_sp, reps = _median_time(lambda: oracle.spmv(rp, cols, vals, x, y), 2.0)
+        out["cpu_oracle_apply_us"] = round(1e6 * t_sp, 2)
+        out["apply_rel_err_vs_oracle"] = _rel(y_dev, y, "inf")
         free = fes.freedofs.astype(bool)
         inv = np.where(free, 1.0 / np.where(diag == 0, 1.0, diag), 0.0)
+        res, u_ref = cpu_baseline_pcg(rp, cols, vals, inv, f)
+        out.update({"cpu_oracle_pcg_iterations": res["iterations"], "cpu_oracle_pcg_seconds": res["seconds"],
+                    "cpu_threads": res["threads"], "cg_rel_err_vs_oracle": _rel(du.download(), u_ref)})
+        log(f"[bench] 2D 64^2 p=3 CPU oracle: apply {out['cpu_oracle_apply_us']} us, PCG {res['iterations']} its "
+            f"{res['seconds']} s; device errors {out['apply_rel_err_vs_oracle']:.1e} / "
+            f"{out['cg_rel_err_vs_oracle']:.1e}")
+    return out
+
+
+def bench_adjoint(N, args):
+    """Differentiation configuration: 2D unit square, 128 x 128 squares x 2
+    triangles, H1 p=3 (148,817 dofs), f = 1, kappa_true = exp(N(0,1)) (seed 3),
+    u_obs = u(kappa_true) + N(0, 1e-3^2) (same generator), J = 0.5 (u -
+    u_obs)^T M (u - u_obs) at kappa = exp(N(0, 0.5)) (seed 4).  Times the
+    forward solve and the backward pass (adjoint solve + element gradient
+    kernel) through torch.autograd; checks the gradient against central
+    differences (h = 1e-6) along 16 unit directions (seed 6, device solves)
+    and, with the CPU baseline, against the oracle's FP64 adjoint gradient
+    (sparse LU), whose time is reported beside it."""
+    import torch
+    import ngsolve_amd as ngs
+    from ngsolve_amd.autograd import ApplySymmetric, PoissonProblem, SolvePoisson
+    nx, p = 128, 3
+    m = ngs.MakeStructured2DMesh(nx=nx, ny=nx)
+    fes = ngs.H1(m, order=p, dirichlet=".*")
+    u_, v_ = fes.TnT()
+    f = ngs.LinearForm(1 * v_ * ngs.dx).Assemble().vec.NumPy().copy()
+    mass = ngs.BilinearForm(u_ * v_ * ngs.dx).Assemble()
+    ne = m.ne
+    rng = np.random.default_rng(3)
+    kappa_true = np.exp(rng.standard_normal(ne))
+    P = PoissonProblem(fes, f, tol=1e-13)
+    u_true = SolvePoisson.apply(torch.tensor(kappa_true, device="cuda"), P).cpu().numpy()
+    u_obs = u_true + rng.normal(0.0, 1e-3, fes.ndof)
+    kappa = np.exp(np.random.default_rng(4).normal(0.0, 0.5, ne))
+    uo = torch.tensor(u_obs, device="cuda")
+
+    def grad_once():
+        kt = torch.tensor(kappa, device="cuda", requires_grad=True)
+        torch.cuda.synchronize()
         t0 = time.perf_counter()
-        _, its_cpu, _ = oracle.pcg_csr(rp, cols, vals, inv, f, tol=1e-10, maxiter=100000)
-        out.update({"cpu_oracle_pcg_iterations": its_cpu, "cpu_oracle_pcg_seconds": round(time.perf_counter() - t0, 4),
-                    "cpu_threads": oracle.num_threads()})
-        log(f"[bench] 2D 64^2 p=3 CPU oracle: apply {out['cpu_oracle_apply_us']} us, PCG {its_cpu} its "
-            f"{out['cpu_oracle_pcg_seconds']} s")
+        u = SolvePoisson.apply(kt, P)
+        r = u - uo
+        J = 0.5 * torch.dot(r, ApplySymmetric.apply(r, mass.mat.op))
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        J.backward()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        return J.item(), kt.grad.cpu().numpy(), u.detach().cpu().numpy(), t1 - t0, t2 - t1
+
+    grad_once()  # warm-up
+    P.iterations.clear()
+    J, g, u, t_fwd, t_bwd = grad_once()
+    its = list(P.iterations)
+    out = {"ndof": fes.ndof, "elements": ne, "J": J, "forward_seconds": round(t_fwd, 4),
+           "backward_seconds": round(t_bwd, 4), "cg_iterations_forward_adjoint": its, "cg_tol": 1e-13}
+
+    def J_of(k):
+        uk = SolvePoisson.apply(torch.tensor(k, device="cuda"), P).cpu().numpy()
+        rk = uk - u_obs
+        return 0.5 * float(rk @ (Mh @ rk))
+
+    import scipy.sparse as sp
+    Mh = sp.csr_matrix((mass.mat.vals, mass.mat.cols, mass.mat.rowptr), shape=(fes.ndof, fes.ndof))
+    drng = np.random.default_rng(6)
+    worst = 0.0
+    for _ in range(16):
+        d = drng.standard_normal(ne)
+        d /= np.linalg.norm(d)
+        fd = (J_of(kappa + 1e-6 * d) - J_of(kappa - 1e-6 * d)) / 2e-6
+        worst = max(worst, abs(fd - g @ d) / max(abs(fd), 1e-30))
+    out["fd_worst_rel_err_16_directions"] = worst
+    log(f"[bench] adjoint 128^2 p=3: forward {t_fwd * 1e3:.1f} ms backward {t_bwd * 1e3:.1f} ms its {its}, "
+        f"FD worst {worst:.1e}")
+    if not args.no_cpu:
+        from oracle import fe_oracle as O
+        pr = O.Problem(O.structured_2d(nx), p, intorder=2 * p - 2)
+        prm = O.Problem(O.structured_2d(nx), p, intorder=2 * p)
+        rp, c, vals = prm.assemble(stiffness=False, mass=True)
+        M = sp.csr_matrix((vals, c, rp), shape=(pr.ndof, pr.ndof))
+        t0 = time.perf_counter()
+        J_ref, g_ref, u_ref = O.adjoint_gradient(pr, kappa, f, u_obs, M)
+        t_cpu = time.perf_counter() - t0
+        out.update({"gradient_rel_err_vs_oracle": _rel(g, g_ref), "u_rel_err_vs_oracle": _rel(u, u_ref),
+                    "J_rel_err_vs_oracle": abs(J - J_ref) / abs(J_ref),
+                    "cpu_oracle_adjoint_seconds": round(t_cpu, 3),
+                    "cpu_oracle_note": "numpy/scipy oracle: sparse LU (SuperLU) of the free-dof block, two solves, "
+                                       "element gradients"})
+        log(f"[bench] adjoint vs oracle: gradient {out['gradient_rel_err_vs_oracle']:.1e}, CPU oracle {t_cpu:.2f}s")
     return out
 
 
@@ -204,6 +394,8 @@ def main():
     ap.add_argument("--no-cg", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-assembly", action="store_true")
+    ap.add_argument("--no-2d", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--batch-problems", type=int, default=512)
     ap.add_argument("--profile-cg", action="store_true",
                     help="one FP64 CSR CG solve (kappa = 1, tol 1e-10) without graph capture, for a kernel trace")
@@ -211,8 +403,9 @@ def main():
                     help="one FP32-inner refinement solve (lognormal kappa, tol 1e-10) without graph capture, "
                          "for a kernel trace")
     ap.add_argument("--profile-mode", action="store_true",
-                    help="short run for rocprofv3 counter passes: a few CSR and matrix-free applies and one "
-                         "1 GiB device-to-device copy (the calibration kernel of known size); no JSON line")
+                    help="short run for rocprofv3 counter passes: a few CSR and matrix-free applies, one "
+                         "1 GiB device-to-device copy and one 1 GiB streaming read (the calibration kernels of "
+                         "known size); no JSON line")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the operator and solver have no multi-GPU path", file=sys.stderr)
@@ -236,6 +429,18 @@ def main():
     x = np.random.default_rng(1).uniform(-1, 1, ndof)
     dx, dy = N.DeviceVector(ndof, 8), N.DeviceVector(ndof, 8)
     dx.upload(x)
+    # the CPU baseline (oracle) runs first: its outputs are the references
+    # the device results' errors are stated against
+    cpu, ref = None, None
+    if not (args.no_cpu or args.profile_cg or args.profile_ir or args.profile_mode):
+        log("[bench] cpu baseline (oracle CSR SpMV and matrix-free apply, all threads and 1 thread) ...")
+        cpu, ref = cpu_baseline_apply(rowptr, cols, vals, x, ndof, n)
+        log(f"[bench] cpu SpMV {cpu['csr_spmv']['gdofs']} GDoF/s on {cpu['threads_all']} threads, "
+            f"{cpu['csr_spmv_one_thread']['gdofs']} on 1; matrix-free {cpu['matrix_free']['gdofs']} / "
+            f"{cpu['matrix_free_one_thread']['gdofs']}")
+
+    def err_apply(y):
+        return None if ref is None else _rel(y, ref["y_mf"], "inf")
 
     if args.profile_cg:
         dd = N.DeviceVector(ndof, 8)
@@ -285,7 +490,11 @@ def main():
                     M.mult_f32(x32, y32)
             del M
         big_a, big_b = N.DeviceVector(1 << 27, 8), N.DeviceVector(1 << 27, 8)  # 1 GiB each
+        big_a.fill(1.0)
         big_b.copy_from(big_a)
+        res = N.DeviceVector(1, 8)
+        for _ in range(3):
+            N.stream_read(big_a, res)
         N.synchronize()
         log("[bench] profile mode done")
         return
@@ -299,6 +508,18 @@ def main():
     log(f"[bench] csr spmv {ms_csr * 1e3:.1f} us  {gbs:.0f} GB/s  {gdofs:.2f} GDoF/s")
 
     extra = {}
+    extra["csr_apply_rel_err_vs_oracle"] = err_apply(y_csr)
+    # read ceiling: 1 GiB of non-temporal streaming loads (ngsamd_stream_read)
+    big = N.DeviceVector(1 << 27, 8)
+    big.fill(1.0)
+    rsum = N.DeviceVector(1, 8)
+    ms_read = timed(N, lambda: N.stream_read(big, rsum), 20, 3)
+    read_gbs = (1 << 30) / (ms_read * 1e-3) / 1e9
+    assert rsum.download()[0] == float(1 << 27)
+    del big
+    extra["read_ceiling"] = {"gbs": round(read_gbs, 1), "frac_of_8tbs": round(read_gbs / HBM_PEAK_GBS, 4),
+                             "kernel": "StreamReadKernel: 1 GiB FP64, 16-byte non-temporal loads"}
+    log(f"[bench] streaming read ceiling {read_gbs:.0f} GB/s; csr spmv at {gbs / read_gbs:.3f} of it")
     # K = 4 vectors at once: the multi-vector apply reads the matrix once
     K = 4
     dXk, dYk = N.DeviceVector(ndof * K, 8), N.DeviceVector(ndof * K, 8)
@@ -318,7 +539,7 @@ def main():
         ms_alt = timed(N, lambda: Aalt.mult(dx, dy), args.steps, args.warmup)
         extra[f"csr_{algo}_kernel"] = {
             "us_per_apply": round(ms_alt * 1e3, 1), "gbs": round(csr_bytes / (ms_alt * 1e-3) / 1e9, 1),
-            "rel_err_vs_runs_kernel": float(np.abs(dy.download() - y_csr).max() / np.abs(y_csr).max())}
+            "rel_err_vs_oracle": err_apply(dy.download())}
         del Aalt
         log(f"[bench] csr {algo} {ms_alt * 1e3:.1f} us")
 
@@ -328,9 +549,6 @@ def main():
     # symmetric nd x nd reference matrices per class (tet p=4: nd = 35)
     nd_el = fes.nd_el
     flops_mf = 2.0 * 6 * nd_el * nd_el * ne
-    # the reference's per-quadrature-point form (435 Q flop per element),
-    # reported as an equivalent rate only
-    flops_mf_quad = 435.0 * nq * ne
     x32, y32 = N.DeviceVector(ndof, 4), N.DeviceVector(ndof, 4)
     x32.upload(x.astype(np.float32))
     for name, prec, peak in [("mf_f64", 0, FP64_PEAK_TFLOPS), ("mf_f32", 1, FP32_PEAK_TFLOPS),
@@ -338,17 +556,17 @@ def main():
         for scatter in (0, 1):
             M = build_mf(N, fes, mesh, ir, None, prec, scatter)
             ms = timed(N, lambda: M.mult(dx, dy), args.steps, args.warmup)
-            err = float(np.abs(dy.download() - y_csr).max() / np.abs(y_csr).max())
+            err = err_apply(dy.download())
             tf = flops_mf / (ms * 1e-3) / 1e12
             key = name + ("" if scatter == 0 else "_atomic")
             extra[key] = {"us_per_apply": round(ms * 1e3, 1), "gdofs": round(ndof / (ms * 1e-3) / 1e9, 3),
                           "tflops": round(tf, 2), "frac_of_peak": round(tf / peak, 4),
-                          "per_point_equivalent_tflops": round(flops_mf_quad / (ms * 1e-3) / 1e12, 2),
-                          "rel_err_vs_fp64_csr": err}
+                          "rel_err_vs_oracle": err}
             if prec > 0:
                 ms32 = timed(N, lambda: M.mult_f32(x32, y32), args.steps, args.warmup)
                 extra[key]["us_per_apply_f32_vectors"] = round(ms32 * 1e3, 1)
-            log(f"[bench] {key} {ms * 1e3:.1f} us  {tf:.1f} TFLOP/s  err {err:.1e}")
+                extra[key]["rel_err_vs_oracle_f32_vectors"] = err_apply(y32.download().astype(np.float64))
+            log(f"[bench] {key} {ms * 1e3:.1f} us  {tf:.1f} TFLOP/s  err {err}")
             del M
 
     # ------------------------------------------- CSR assembly on the device
@@ -422,6 +640,11 @@ def main():
         extra["cg_fp64_csr_kappa1"] = {"iterations": its, "seconds": round(t_cg, 4),
                                        "ms_per_iteration": round(1e3 * t_cg / max(its - 1, 1), 4)}
         log(f"[bench] FP64 CSR CG kappa=1: {its} its {t_cg:.3f}s")
+        if cpu is not None:
+            inv1 = np.where(fes.freedofs.astype(bool), 1.0 / np.where(diag == 0, 1.0, diag), 0.0)
+            res1, u1_ref = cpu_baseline_pcg(rowptr, cols, vals, inv1, f)
+            extra["cg_fp64_csr_kappa1"].update({"rel_err_vs_oracle": _rel(du.download(), u1_ref),
+                                                "cpu_oracle_pcg": res1})
 
         # K right-hand sides of the same matrix: the lockstep CG
         # (CGSolver::MultiMult) against K single solves one after the other
@@ -462,6 +685,21 @@ def main():
             r = f - _spmv_host(rowptr, cols, vals_k, u)
             return float(np.linalg.norm(r[free]) / np.linalg.norm(f[free]))
 
+        u_oracle = None
+        if cpu is not None:
+            # the oracle's C Jacobi-PCG on the host: the CPU baseline of the
+            # solve, and the solution every device result is compared with
+            dk = np.empty(ndof)
+            Ak.diagonal(dd)
+            dk[:] = dd.download()
+            invk = np.where(free, 1.0 / np.where(dk == 0, 1.0, dk), 0.0)
+            res_cpu, u_oracle = cpu_baseline_pcg(rowptr, cols, vals_k, invk, f)
+            log(f"[bench] CPU oracle PCG lognormal: {res_cpu['iterations']} its {res_cpu['seconds']:.2f}s on "
+                f"{res_cpu['threads']} threads")
+
+        def err_sol(u):
+            return None if u_oracle is None else _rel(u, u_oracle)
+
         t0 = time.perf_counter()
         its_k, hist_k = N.CGSolve(Ak, Ck, df, du, True, 1e-10, -1.0, 200000, 16)
         t_k = time.perf_counter() - t0
@@ -470,7 +708,8 @@ def main():
         M64 = build_mf(N, fes, mesh, ir, kappa, 0)
         res = {"fp64_csr_cg": {"iterations": its_k, "seconds": round(t_k, 4), "true_rel_residual": true_res(u64),
                                # the reference's stopping quantity sqrt|w.d|, last over first
-                               "stopping_ratio": float(hist_k[-1] / hist_k[0]) if len(hist_k) > 1 else None}}
+                               "stopping_ratio": float(hist_k[-1] / hist_k[0]) if len(hist_k) > 1 else None,
+                               "rel_err_vs_oracle": err_sol(u64)}}
         # the same FP64 CG on the FP64 matrix-free operator
         du_mf = N.DeviceVector(ndof, 8)
         t0 = time.perf_counter()
@@ -479,8 +718,7 @@ def main():
         u_mf = du_mf.download()
         log(f"[bench] FP64 matrix-free CG lognormal: {its_mf} its {t_mf:.3f}s")
         res["fp64_mf_cg"] = {"iterations": its_mf, "seconds": round(t_mf, 4), "true_rel_residual": true_res(u_mf),
-                             "rel_diff_vs_csr_solution": float(np.linalg.norm(u_mf - u64) / np.linalg.norm(u64)),
-                             "speedup_vs_fp64_csr_cg": round(t_k / t_mf, 3)}
+                             "rel_err_vs_oracle": err_sol(u_mf), "speedup_vs_fp64_csr_cg": round(t_k / t_mf, 3)}
         del du_mf
         best = None
         for pname, prec in [("f32", 1), ("bf16", 2), ("f16", 3)]:
@@ -494,8 +732,7 @@ def main():
             log(f"[bench] mixed IR {pname}: outer {outer} inner {inner} rel {rel:.1e} {t_ir:.3f}s")
             res["mixed_ir_mf_" + pname] = {
                 "outer": outer, "inner_total": inner, "seconds": round(t_ir, 4),
-                "true_rel_residual": true_res(u_ir),
-                "rel_diff_vs_fp64_solution": float(np.linalg.norm(u_ir - u64) / np.linalg.norm(u64)),
+                "true_rel_residual": true_res(u_ir), "rel_err_vs_oracle": err_sol(u_ir),
                 "speedup_vs_fp64_csr_cg": round(t_k / t_ir, 3)}
             if rel <= 1e-10 and (best is None or t_ir < best[1]):
                 best = (pname, t_ir)
@@ -525,36 +762,28 @@ def main():
                 "iterations": its_b, "seconds": round(t_b, 4), "setup_seconds": round(t_setup, 3),
                 "blocks_seconds": round(t_blocks, 3),
                 "blocks": len(blocks), "max_block": int(np.diff(blocks.ptr).max()),
-                "true_rel_residual": true_res(u_b),
-                "rel_diff_vs_jacobi_solution": float(np.linalg.norm(u_b - u64) / np.linalg.norm(u64))}
+                "true_rel_residual": true_res(u_b), "rel_err_vs_oracle": err_sol(u_b)}
             del Cb
-        if not args.no_cpu:
-            # the same solve with the oracle's C Jacobi-PCG (OpenMP, all host threads)
-            import oracle
-            dk = np.empty(ndof)
-            Ak.diagonal(dd)
-            dk[:] = dd.download()
-            invk = np.where(free, 1.0 / np.where(dk == 0, 1.0, dk), 0.0)
-            t0 = time.perf_counter()
-            u_cpu, its_cpu, _ = oracle.pcg_csr(rowptr, cols, vals_k, invk, f, tol=1e-10, maxiter=200000)
-            t_cpu = time.perf_counter() - t0
-            log(f"[bench] CPU oracle PCG lognormal: {its_cpu} its {t_cpu:.2f}s on {oracle.num_threads()} threads")
-            res["cpu_oracle_pcg"] = {
-                "iterations": its_cpu, "seconds": round(t_cpu, 3), "threads": oracle.num_threads(),
-                "true_rel_residual": true_res(u_cpu),
-                "gpu_fp64_csr_cg_speedup": round(t_cpu / t_k, 1),
-                "gpu_best_mixed_speedup": round(t_cpu / best[1], 1) if best else None}
+        if u_oracle is not None:
+            res["cpu_oracle_pcg"] = dict(res_cpu, true_rel_residual=true_res(u_oracle),
+                                         gpu_fp64_csr_cg_speedup=round(res_cpu["seconds"] / t_k, 1),
+                                         gpu_best_mixed_speedup=round(res_cpu["seconds"] / best[1], 1) if best
+                                         else None)
         extra["cg_lognormal_kappa"] = res
 
+    if not args.no_adjoint:
+        extra["adjoint_128sq_p3"] = bench_adjoint(N, args)
     if not args.no_batch:
         extra["batch_512_problems"] = bench_batch(N, args)
-    extra["poisson_2d_64sq_p3"] = bench_2d(N, args)
+    if not args.no_2d:
+        extra["poisson_2d_64sq_p3"] = bench_2d(N, args)
 
-    cpu = None
-    if not args.no_cpu:
-        log("[bench] cpu baseline ...")
-        cpu = cpu_baseline(rowptr, cols, vals, x, ndof)
-        log(f"[bench] cpu {cpu['value']} GDoF/s on {cpu['cores']} threads")
+    cpu_line = None
+    if cpu is not None:
+        cpu_line = {"value": cpu["csr_spmv"]["gdofs"], "unit": "GDoF/s", "cores": cpu["threads_all"],
+                    "kind": "port", "sample": cpu["sample"],
+                    "one_core": cpu["csr_spmv_one_thread"], "matrix_free": cpu["matrix_free"],
+                    "matrix_free_one_core": cpu["matrix_free_one_thread"], "host": host_inventory()}
 
     # HBM bytes per launch of the default CSR kernel, from the rocprofv3 PMC
     # passes summarised under profiles/ (only if they measured this kernel)
@@ -576,8 +805,9 @@ def main():
                    "ndof": ndof, "nnz": nnz, "elements": ne},
         "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": "CsrRunsKernel", "algorithmic_bytes": csr_bytes},
-        "cpu_baseline": cpu,
+                     "kernel": "CsrRunsKernel", "algorithmic_bytes": csr_bytes,
+                     "read_ceiling_gbs": round(read_gbs, 1), "frac_of_read_ceiling": round(gbs / read_gbs, 4)},
+        "cpu_baseline": cpu_line,
         "extra": extra,
     }
     print(json.dumps(out), flush=True)
