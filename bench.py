@@ -302,7 +302,7 @@ def bench_2d(N, args):
 
 def bench_adjoint(N, args):
     """Differentiation configuration: 2D unit square, 128 x 128 squares x 2
-    triangles, H1 p=3 (148,817 dofs), f = 1, kappa_true = exp(N(0,1)) (seed 3),
+    triangles, H1 p=3 (148,225 dofs), f = 1, kappa_true = exp(N(0,1)) (seed 3),
     u_obs = u(kappa_true) + N(0, 1e-3^2) (same generator), J = 0.5 (u -
     u_obs)^T M (u - u_obs) at kappa = exp(N(0, 0.5)) (seed 4).  Times the
     forward solve and the backward pass (adjoint solve + element gradient

@@ -3,6 +3,15 @@ without the built HIP library fails with an ImportError naming the build
 step."""
 import os
 
+# torch, when installed, ships its own HIP runtime under the soname of the
+# system one: loaded first, it is the one libngsamd binds to, so the process
+# has a single runtime and torch tensors can be handed to the library
+# (ngsolve_amd.autograd); loaded after libngsamd, torch finds no device
+try:
+    import torch  # noqa: F401
+except ImportError:  # pragma: no cover
+    pass
+
 _HERE = os.path.dirname(os.path.abspath(__file__))
 
 try:

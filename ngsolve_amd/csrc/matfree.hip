@@ -690,6 +690,301 @@ __global__ __launch_bounds__(kMfBlock) void MfRef16Kernel(
   }
 }
 
+// ------------------------------------------- factored reference matrices
+// The weighted gradient components of a class's shape functions at the
+// points, rows Bw[(d,i)][q] = b_{q,i,d} sqrt(w_q), span a space of dimension
+// r (for H1 of degree p the gradients of degree-p polynomials, r = dim
+// P_{p-1}: tet p4 r = 20 against 3 x 35 rows).  With an orthonormal basis Phi
+// of that space Bw = D Phi (D = Bw Phi^T), hence
+//   S_(dd') = sum_q w_q b_d b_d'^T = D_d D_d'^T,
+//   y_e = sum_c g_{e,c} S_c x_e = sum_d D_d sum_d' G_{e,dd'} (D_d'^T x_e),
+// exactly, for any rule.  Per 16-element tile: U = D^T X (DIM x r rows,
+// MFMA), V = G U per element (VALU, lane-local), Y = D V (MFMA).  Tet p4,
+// FP64: 9 x 4 + 15 x 3 = 81 MFMAs against 162 for sum_c S_c (g_c X).
+//
+// Layout of U: the accumulator slots of lane group grp, sigma = 4 t + r
+// (tile t, accumulator element r; the row AccRow(grp, r) of tile t), hold
+// component d = sigma % DIM of basis function k = grp * KG + sigma / DIM, so
+// the DIM components of one k sit in one lane and the metric needs no lane
+// exchange.  The backward product uses slot sigma of lane group grp as its K
+// index 4 sigma + grp (FP64/FP32 16x16x4: lane l supplies K row l / 16 of
+// each step), so V goes from the accumulators straight into the B operand.
+// LDS, lane order so that each lane's A operands are 16-byte reads:
+//   A1[t][grp][col][KSP]  row col of tile t (slot of AccRow^-1(col)), K = dof
+//   A2[m][grp][col][JP]   row = dof 16 m + col, K = slot of lane group grp
+template <class T, int DIM, int NDP, int KG, int MODE, class V>
+__global__ __launch_bounds__(kMfBlock) void MfFacKernel(
+    const T* __restrict__ fmat, int64_t fstride, MfRounds R, const int* __restrict__ dofs,
+    const double* __restrict__ metric, const double* __restrict__ kappa, const int* __restrict__ elnr, int nd,
+    const V* __restrict__ x, V* __restrict__ out, double s, const int* __restrict__ stop) {
+  if (stop && *stop) return;
+  typedef typename Acc4<T>::type acc_t;
+  constexpr int NC = DIM * (DIM + 1) / 2;
+  constexpr int KS = NDP / 4;
+  constexpr int MT = (NDP + 15) / 16;
+  constexpr int NSL = DIM * KG;
+  constexpr int TT = (NSL + 3) / 4;
+  constexpr int VW = 16 / (int)sizeof(T);
+  constexpr int KSP = (KS + VW - 1) / VW * VW;
+  constexpr int JP = (NSL + VW - 1) / VW * VW;
+  typedef T vec_t __attribute__((ext_vector_type(VW)));
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  T* ls = reinterpret_cast<T*>(lds_raw);
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int col = lane & 15, grp = lane >> 4;
+  const int wg = XcdContiguousBlock(blockIdx.x, gridDim.x);
+  const int r0 = (int)((int64_t)wg * R.nrounds / gridDim.x);
+  const int r1 = (int)((int64_t)(wg + 1) * R.nrounds / gridDim.x);
+  __shared__ int s_round0[kMaxLdsClasses + 1];
+  __shared__ int64_t s_range[2 * kMaxLdsClasses];
+  for (int i = threadIdx.x; i <= R.nclass; i += kMfBlock) s_round0[i] = R.round0[i];
+  for (int i = threadIdx.x; i < 2 * R.nclass; i += kMfBlock) s_range[i] = R.range[i];
+  __syncthreads();
+  const T* a1 = ls + lane * KSP;
+  const T* a2 = ls + TT * 64 * KSP + lane * JP;
+  int cls = -1;
+  for (int rd = r0; rd < r1; rd++) {
+    int c = cls < 0 ? 0 : cls;
+    while (s_round0[c + 1] <= rd) c++;
+    if (c != cls) {
+      if (cls >= 0) __syncthreads();
+      CopyToLds(ls, fmat + (int64_t)c * fstride, (int)(fstride * sizeof(T) / 16));
+      __syncthreads();
+      cls = c;
+    }
+    const int64_t end = s_range[2 * c + 1];
+    const int64_t first = s_range[2 * c] + ((int64_t)(rd - s_round0[c]) * kMfWaves + wave) * 16;
+    if (first >= end) continue;
+    const int64_t e = first + col;
+    const bool valid = e < end;
+    // unconditional gathers from clamped indices, masked afterwards
+    const int64_t ev = valid ? e : first;
+    int dv[KS];
+#pragma unroll
+    for (int st = 0; st < KS; st++) dv[st] = dofs[ev * nd + min(4 * st + grp, nd - 1)];
+    const int en = elnr[ev];
+    double mv[NC];
+#pragma unroll
+    for (int cc = 0; cc < NC; cc++) mv[cc] = metric[ev * NC + cc];
+    V xv[KS];
+#pragma unroll
+    for (int st = 0; st < KS; st++) xv[st] = x[dv[st]];
+    const double kap = kappa[en];
+    T xk[KS];
+#pragma unroll
+    for (int st = 0; st < KS; st++) xk[st] = (valid && 4 * st + grp < nd) ? (T)xv[st] : (T)0;
+    // U = D^T X
+    acc_t u[TT];
+#pragma unroll
+    for (int t = 0; t < TT; t++) u[t] = acc_t{0, 0, 0, 0};
+#pragma unroll
+    for (int q = 0; q < KSP / VW; q++) {
+      vec_t a[TT];
+#pragma unroll
+      for (int t = 0; t < TT; t++) a[t] = *reinterpret_cast<const vec_t*>(a1 + t * 64 * KSP + VW * q);
+#pragma unroll
+      for (int j = 0; j < VW; j++)
+        if (VW * q + j < KS)
+#pragma unroll
+          for (int t = 0; t < TT; t++) u[t] = Mfma(a[t][j], xk[VW * q + j], u[t]);
+    }
+    // V = (kappa G) U, per basis function k in this lane
+    T g[NC];
+#pragma unroll
+    for (int cc = 0; cc < NC; cc++) g[cc] = valid ? (T)(kap * mv[cc]) : (T)0;
+    T v[JP];
+#pragma unroll
+    for (int j = 0; j < JP; j++) v[j] = (T)0;
+#pragma unroll
+    for (int kl = 0; kl < KG; kl++) {
+      T uu[DIM], vv[DIM];
+#pragma unroll
+      for (int d = 0; d < DIM; d++) uu[d] = u[(kl * DIM + d) / 4][(kl * DIM + d) % 4];
+      ApplyMetric<DIM, T>(g, uu, vv);
+#pragma unroll
+      for (int d = 0; d < DIM; d++) v[kl * DIM + d] = vv[d];
+    }
+    // Y = D V
+    acc_t yacc[MT];
+#pragma unroll
+    for (int m = 0; m < MT; m++) yacc[m] = acc_t{0, 0, 0, 0};
+#pragma unroll
+    for (int q = 0; q < JP / VW; q++) {
+      vec_t a[MT];
+#pragma unroll
+      for (int m = 0; m < MT; m++) a[m] = *reinterpret_cast<const vec_t*>(a2 + m * 64 * JP + VW * q);
+#pragma unroll
+      for (int j = 0; j < VW; j++)
+        if (VW * q + j < NSL)
+#pragma unroll
+          for (int m = 0; m < MT; m++) yacc[m] = Mfma(a[m][j], v[VW * q + j], yacc[m]);
+    }
+    if (valid) {
+#pragma unroll
+      for (int m = 0; m < MT; m++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const int dof = 16 * m + AccRow<T>(grp, r);
+          if (dof < nd) {
+            const V val = (V)yacc[m][r];
+            if (MODE == 0) out[e * nd + dof] = val;
+            else AtomicAddV(&out[dofs[e * nd + dof]], (V)(s * (double)val));
+          }
+        }
+    }
+  }
+}
+
+// 16-bit operands (BF16 / FP16), FP32 accumulation and FP32 metric, operand
+// layout of v_mfma_f32_16x16x32_{bf16,f16} as in MfBf16Kernel (result rows
+// 4 (l/16) + r).  Slots as in MfFacKernel with an even number TT of U tiles:
+// the backward K step ks2 takes lane group grp's slots sigma = 8 ks2 + j
+// (tiles 2 ks2 and 2 ks2 + 1) as its K rows 32 ks2 + 8 grp + j.  LDS:
+//   A1[t][ks][grp][col][8]   row col of tile t (slot 4 t + col % 4 of lane
+//                            group col / 4), K = dofs 32 ks + 8 grp + j
+//   A2[m][ks2][grp][col][8]  row = dof 16 m + col, K = slots 8 ks2 + j
+// FP16 scales each element's x and metric by powers of two as MfRef16Kernel.
+template <class H, int DIM, int KS1, int MT, int KG, int MODE, class V>
+__global__ __launch_bounds__(kMfBlock) void MfFac16Kernel(
+    const uint16_t* __restrict__ fmat, int64_t fstride, MfRounds R, const int* __restrict__ dofs,
+    const double* __restrict__ metric, const double* __restrict__ kappa, const int* __restrict__ elnr, int nd,
+    const V* __restrict__ x, V* __restrict__ out, double s, const int* __restrict__ stop) {
+  if (stop && *stop) return;
+  typedef typename Vec8<H>::type h8;
+  constexpr int NC = DIM * (DIM + 1) / 2;
+  constexpr int NSL = DIM * KG;
+  constexpr int TT = ((NSL + 3) / 4 + 1) / 2 * 2;
+  constexpr int KS2 = TT / 2;
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int col = lane & 15, grp = lane >> 4;
+  const int wg = XcdContiguousBlock(blockIdx.x, gridDim.x);
+  const int r0 = (int)((int64_t)wg * R.nrounds / gridDim.x);
+  const int r1 = (int)((int64_t)(wg + 1) * R.nrounds / gridDim.x);
+  __shared__ int s_round0[kMaxLdsClasses + 1];
+  __shared__ int64_t s_range[2 * kMaxLdsClasses];
+  for (int i = threadIdx.x; i <= R.nclass; i += kMfBlock) s_round0[i] = R.round0[i];
+  for (int i = threadIdx.x; i < 2 * R.nclass; i += kMfBlock) s_range[i] = R.range[i];
+  __syncthreads();
+  const h8* a1 = reinterpret_cast<const h8*>(lds_raw) + lane;
+  const h8* a2 = a1 + TT * KS1 * 64;
+  int cls = -1;
+  for (int rd = r0; rd < r1; rd++) {
+    int cl = cls < 0 ? 0 : cls;
+    while (s_round0[cl + 1] <= rd) cl++;
+    if (cl != cls) {
+      if (cls >= 0) __syncthreads();
+      // fstride 16-bit words, a multiple of 8
+      CopyToLds(lds_raw, fmat + (int64_t)cl * fstride, (int)(fstride / 8));
+      __syncthreads();
+      cls = cl;
+    }
+    const int64_t end = s_range[2 * cl + 1];
+    const int64_t first = s_range[2 * cl] + ((int64_t)(rd - s_round0[cl]) * kMfWaves + wave) * 16;
+    if (first >= end) continue;
+    const int64_t e = first + col;
+    const bool valid = e < end;
+    const int64_t ev = valid ? e : first;
+    int dv[KS1][8];
+#pragma unroll
+    for (int ks = 0; ks < KS1; ks++)
+#pragma unroll
+      for (int j = 0; j < 8; j++) dv[ks][j] = dofs[ev * nd + min(32 * ks + 8 * grp + j, nd - 1)];
+    const int en = elnr[ev];
+    double mv[NC];
+#pragma unroll
+    for (int c = 0; c < NC; c++) mv[c] = metric[ev * NC + c];
+    V xv[KS1][8];
+#pragma unroll
+    for (int ks = 0; ks < KS1; ks++)
+#pragma unroll
+      for (int j = 0; j < 8; j++) xv[ks][j] = x[dv[ks][j]];
+    const double kap = kappa[en];
+    float xf[KS1][8];
+#pragma unroll
+    for (int ks = 0; ks < KS1; ks++)
+#pragma unroll
+      for (int j = 0; j < 8; j++) xf[ks][j] = (valid && 32 * ks + 8 * grp + j < nd) ? (float)xv[ks][j] : 0.0f;
+    float g[NC];
+#pragma unroll
+    for (int c = 0; c < NC; c++) g[c] = valid ? (float)(kap * mv[c]) : 0.0f;
+    int escale = 0;
+    if constexpr (!std::is_same<H, __bf16>::value) {
+      float xm = 0.0f, gm = 0.0f;
+#pragma unroll
+      for (int ks = 0; ks < KS1; ks++)
+#pragma unroll
+        for (int j = 0; j < 8; j++) xm = fmaxf(xm, fabsf(xf[ks][j]));
+      xm = fmaxf(xm, __shfl_xor(xm, 16, 64));
+      xm = fmaxf(xm, __shfl_xor(xm, 32, 64));
+#pragma unroll
+      for (int c = 0; c < NC; c++) gm = fmaxf(gm, fabsf(g[c]));
+      int ex = 0, eg = 0;
+      (void)frexpf(xm, &ex);
+      (void)frexpf(gm, &eg);
+#pragma unroll
+      for (int ks = 0; ks < KS1; ks++)
+#pragma unroll
+        for (int j = 0; j < 8; j++) xf[ks][j] = ldexpf(xf[ks][j], -ex);
+#pragma unroll
+      for (int c = 0; c < NC; c++) g[c] = ldexpf(g[c], -eg);
+      escale = ex + eg;
+    }
+    h8 xg[KS1];
+#pragma unroll
+    for (int ks = 0; ks < KS1; ks++)
+#pragma unroll
+      for (int j = 0; j < 8; j++) xg[ks][j] = (H)xf[ks][j];
+    // U = D^T X
+    float4v u[TT];
+#pragma unroll
+    for (int t = 0; t < TT; t++) u[t] = float4v{0, 0, 0, 0};
+#pragma unroll
+    for (int ks = 0; ks < KS1; ks++)
+#pragma unroll
+      for (int t = 0; t < TT; t++) u[t] = Mfma16(a1[(t * KS1 + ks) * 64], xg[ks], u[t]);
+    // V = (kappa G) U in FP32, rounded once into the B operand
+    float v[4 * TT];
+#pragma unroll
+    for (int j = 0; j < 4 * TT; j++) v[j] = 0.0f;
+#pragma unroll
+    for (int kl = 0; kl < KG; kl++) {
+      float uu[DIM], vv[DIM];
+#pragma unroll
+      for (int d = 0; d < DIM; d++) uu[d] = u[(kl * DIM + d) / 4][(kl * DIM + d) % 4];
+      ApplyMetric<DIM, float>(g, uu, vv);
+#pragma unroll
+      for (int d = 0; d < DIM; d++) v[kl * DIM + d] = vv[d];
+    }
+    h8 vb[KS2];
+#pragma unroll
+    for (int k2 = 0; k2 < KS2; k2++)
+#pragma unroll
+      for (int j = 0; j < 8; j++) vb[k2][j] = (H)v[8 * k2 + j];
+    // Y = D V
+    float4v yacc[MT];
+#pragma unroll
+    for (int m = 0; m < MT; m++) yacc[m] = float4v{0, 0, 0, 0};
+#pragma unroll
+    for (int k2 = 0; k2 < KS2; k2++)
+#pragma unroll
+      for (int m = 0; m < MT; m++) yacc[m] = Mfma16(a2[(m * KS2 + k2) * 64], vb[k2], yacc[m]);
+    if (valid) {
+#pragma unroll
+      for (int m = 0; m < MT; m++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const int dof = 16 * m + 4 * grp + r;
+          if (dof < nd) {
+            const V val = (V)ldexpf(yacc[m][r], escale);
+            if (MODE == 0) out[e * nd + dof] = val;
+            else AtomicAddV(&out[dofs[e * nd + dof]], (V)(s * (double)val));
+          }
+        }
+    }
+  }
+}
+
 // Scalar fallback for shapes the MFMA path does not cover: one thread per
 // element, FP64, element vectors staged in LDS columns.
 constexpr int kScalarBlock = 64;
@@ -852,6 +1147,104 @@ static bool SupportedBf16(int ks1, int mt) {
   return (ks1 == 1 && (mt == 1 || mt == 2)) || (ks1 == 2 && mt == 3);
 }
 
+// Shapes with an instance of the factored kernels (H1 p = 1..5 in 2D, 1..4
+// in 3D): padded dofs NDP (16-bit: K steps KS1 and dof tiles MT) and basis
+// functions per lane group KG.
+static bool SupportedFac(int dim, int ndp, int kg) {
+  if (dim == 2)
+    return (ndp == 4 && kg == 1) || (ndp == 8 && kg == 1) || (ndp == 12 && kg == 2) || (ndp == 16 && kg == 3) ||
+           (ndp == 24 && kg == 4);
+  return (ndp == 4 && kg == 1) || (ndp == 12 && kg == 1) || (ndp == 20 && kg == 3) || (ndp == 36 && kg == 5);
+}
+static bool SupportedFac16(int dim, int ks1, int mt, int kg) {
+  if (ks1 != 1 && !(ks1 == 2 && mt == 3)) return false;
+  if (dim == 2) return (mt == 1 && kg >= 1 && kg <= 3) || (mt == 2 && kg == 4);
+  return (mt == 1 && kg == 1) || (mt == 2 && kg == 3) || (ks1 == 2 && mt == 3 && kg == 5);
+}
+
+// Factor of each class's reference matrices (see MfFacKernel): the rows
+// Bw[(d,i)][q] = b_{q,i,d} sqrt(w_q) are orthonormalised by Gram-Schmidt with
+// pivoting on the largest remaining row (each new vector orthogonalised twice
+// against the basis), stopping when every remaining row is below 1e-10 of
+// the largest row; D[(d,i)][k] = Bw[(d,i)] . phi_k.  Returns the largest rank
+// over the classes (D has that many columns, zero beyond a class's own rank),
+// or 0 if a weight is not positive or D Phi misses Bw by more than 1e-12
+// relative.  D is [class][d][i][rank].
+static int FactorGradients(int dim, int nd, int nq, int nclass, const double* bmat, const double* w,
+                           std::vector<double>& D) {
+  for (int q = 0; q < nq; q++)
+    if (!(w[q] > 0.0)) return 0;
+  const int nr = dim * nd;
+  const size_t bsz = (size_t)nq * nd * dim;
+  std::vector<std::vector<double>> phis(nclass);
+  std::vector<double> Bw((size_t)nr * nq);
+  int rank = 0;
+  auto dot = [&](const double* a, const double* b) {
+    double t = 0.0;
+    for (int q = 0; q < nq; q++) t += a[q] * b[q];
+    return t;
+  };
+  for (int c = 0; c < nclass; c++) {
+    for (int d = 0; d < dim; d++)
+      for (int i = 0; i < nd; i++)
+        for (int q = 0; q < nq; q++)
+          Bw[((size_t)d * nd + i) * nq + q] = bmat[c * bsz + ((size_t)q * nd + i) * dim + d] * std::sqrt(w[q]);
+    std::vector<double> res(Bw);
+    std::vector<double>& phi = phis[c];
+    double norm0 = 0.0;
+    for (int r = 0; r < nr; r++) norm0 = std::max(norm0, std::sqrt(dot(&Bw[(size_t)r * nq], &Bw[(size_t)r * nq])));
+    if (norm0 == 0.0) continue;
+    for (int k = 0; k < std::min(nr, nq); k++) {
+      int best = -1;
+      double bn = 0.0;
+      for (int r = 0; r < nr; r++) {
+        const double t = std::sqrt(dot(&res[(size_t)r * nq], &res[(size_t)r * nq]));
+        if (t > bn) bn = t, best = r;
+      }
+      if (best < 0 || bn <= 1e-10 * norm0) break;
+      std::vector<double> v(res.begin() + (size_t)best * nq, res.begin() + (size_t)(best + 1) * nq);
+      for (int pass = 0; pass < 2; pass++) {
+        for (size_t j = 0; j < phi.size() / nq; j++) {
+          const double t = dot(v.data(), &phi[j * nq]);
+          for (int q = 0; q < nq; q++) v[q] -= t * phi[j * nq + q];
+        }
+        const double vn = std::sqrt(dot(v.data(), v.data()));
+        for (int q = 0; q < nq; q++) v[q] /= vn;
+      }
+      phi.insert(phi.end(), v.begin(), v.end());
+      for (int r = 0; r < nr; r++) {
+        double* rr = &res[(size_t)r * nq];
+        const double t = dot(rr, v.data());
+        for (int q = 0; q < nq; q++) rr[q] -= t * v[q];
+      }
+    }
+    rank = std::max(rank, (int)(phi.size() / nq));
+  }
+  if (rank == 0) return 0;
+  D.assign((size_t)nclass * nr * rank, 0.0);
+  for (int c = 0; c < nclass; c++) {
+    const std::vector<double>& phi = phis[c];
+    const int rc = (int)(phi.size() / nq);
+    double norm0 = 0.0, err = 0.0;
+    for (int d = 0; d < dim; d++)
+      for (int i = 0; i < nd; i++) {
+        std::vector<double> row(nq), rec(nq, 0.0);
+        for (int q = 0; q < nq; q++) row[q] = bmat[c * bsz + ((size_t)q * nd + i) * dim + d] * std::sqrt(w[q]);
+        double* Dr = &D[((size_t)c * nr + (size_t)d * nd + i) * rank];
+        for (int k = 0; k < rc; k++) {
+          Dr[k] = dot(row.data(), &phi[(size_t)k * nq]);
+          for (int q = 0; q < nq; q++) rec[q] += Dr[k] * phi[(size_t)k * nq + q];
+        }
+        for (int q = 0; q < nq; q++) {
+          norm0 = std::max(norm0, std::fabs(row[q]));
+          err = std::max(err, std::fabs(row[q] - rec[q]));
+        }
+      }
+    if (err > 1e-12 * norm0) return 0;
+  }
+  return rank;
+}
+
 
 template <int MODE, class V>
 static void LaunchElements(const MatFreeOp& A, const V* x, V* out, double s, const int* stop, hipStream_t st,
@@ -911,12 +1304,13 @@ MatFreeOp::MatFreeOp(int dim_, int nd_, int nq_, int64_t ndof, int nclass, const
   }
 
   // Reference-matrix kernels by default (affine elements, one coefficient
-  // value per element: exact); NGSAMD_MF_KERNEL=quad selects the
+  // value per element: exact), in their factored form where an instance
+  // exists; NGSAMD_MF_KERNEL=ref keeps the sum over the S_c, =quad selects the
   // per-quadrature-point kernels.
   {
     const char* kenv = getenv("NGSAMD_MF_KERNEL");
-    NGSAMD_REQUIRE(!kenv || std::string(kenv) == "quad" || std::string(kenv) == "ref",
-                   "NGSAMD_MF_KERNEL must be quad or ref");
+    NGSAMD_REQUIRE(!kenv || std::string(kenv) == "quad" || std::string(kenv) == "ref" || std::string(kenv) == "fac",
+                   "NGSAMD_MF_KERNEL must be fac, ref or quad");
     ref_kernel = use_mfma && !(kenv && std::string(kenv) == "quad");
   }
   // S[c][k][i][j] in FP64, k over the packed metric components: the operand
@@ -995,6 +1389,92 @@ MatFreeOp::MatFreeOp(int dim_, int nd_, int nq_, int64_t ndof, int nclass, const
         NGSAMD_HIP(hipStreamSynchronize(Stream()));
       }
     }
+    NGSAMD_HIP(hipStreamSynchronize(Stream()));
+  }
+  // factored kernels (default where an instance exists; NGSAMD_MF_KERNEL=ref
+  // keeps the sum over S_c)
+  if (ref_kernel && nclass <= kMaxLdsClasses && !(getenv("NGSAMD_MF_KERNEL") &&
+                                                  std::string(getenv("NGSAMD_MF_KERNEL")) == "ref")) {
+    std::vector<double> D;
+    const int rank = FactorGradients(dim, nd, nq, nclass, bmat, w, D);
+    const int kg = (rank + 3) / 4;
+    const int nsl = dim * kg;
+    auto dval = [&](int c, int g, int sigma, int i) -> double {
+      if (sigma >= nsl || i >= nd) return 0.0;
+      const int k = g * kg + sigma / dim, d = sigma % dim;
+      if (k >= rank) return 0.0;
+      return D[(((size_t)c * dim + d) * nd + i) * rank + k];
+    };
+    if (rank > 0 && half && SupportedFac16(dim, ks1, mt16, kg)) {
+      const int tt = ((nsl + 3) / 4 + 1) / 2 * 2, ks2 = tt / 2;
+      fstride = (int64_t)(tt * ks1 + mt16 * ks2) * 64 * 8;
+      if ((size_t)fstride * 2 <= (size_t)DeviceMaxLds()) {
+        std::vector<uint16_t> lay((size_t)fstride * nclass, 0);
+        auto h = [&](double v) { return prec == NGSAMD_PREC_F16 ? ToF16(v) : ToBf16(v); };
+        for (int c = 0; c < nclass; c++) {
+          uint16_t* a1 = lay.data() + (size_t)c * fstride;
+          uint16_t* a2 = a1 + (size_t)tt * ks1 * 64 * 8;
+          for (int t = 0; t < tt; t++)
+            for (int k = 0; k < ks1; k++)
+              for (int l = 0; l < 64; l++)
+                for (int j = 0; j < 8; j++) {
+                  const int gl = l / 16, cl = l % 16;
+                  a1[(((size_t)t * ks1 + k) * 64 + l) * 8 + j] = h(dval(c, cl / 4, 4 * t + cl % 4, 32 * k + 8 * gl + j));
+                }
+          for (int m = 0; m < mt16; m++)
+            for (int k = 0; k < ks2; k++)
+              for (int l = 0; l < 64; l++)
+                for (int j = 0; j < 8; j++) {
+                  const int gl = l / 16, cl = l % 16;
+                  a2[(((size_t)m * ks2 + k) * 64 + l) * 8 + j] = h(dval(c, gl, 8 * k + j, 16 * m + cl));
+                }
+        }
+        fmat16.alloc(lay.size());
+        fmat16.upload(lay.data(), lay.size());
+        NGSAMD_HIP(hipStreamSynchronize(Stream()));
+        fac_kernel = true;
+      }
+    } else if (rank > 0 && !half && SupportedFac(dim, ndp, kg)) {
+      const bool f64 = prec == NGSAMD_PREC_F64;
+      const int vw = f64 ? 2 : 4;
+      const int ks = ndp / 4, mt = (ndp + 15) / 16, tt = (nsl + 3) / 4;
+      const int ksp = (ks + vw - 1) / vw * vw, jp = (nsl + vw - 1) / vw * vw;
+      fstride = (int64_t)tt * 64 * ksp + (int64_t)mt * 64 * jp;
+      if ((size_t)fstride * (f64 ? 8 : 4) <= (size_t)DeviceMaxLds()) {
+        std::vector<double> lay((size_t)fstride * nclass, 0.0);
+        for (int c = 0; c < nclass; c++) {
+          double* a1 = lay.data() + (size_t)c * fstride;
+          double* a2 = a1 + (size_t)tt * 64 * ksp;
+          for (int t = 0; t < tt; t++)
+            for (int l = 0; l < 64; l++)
+              for (int st = 0; st < ks; st++) {
+                const int gl = l / 16, cl = l % 16;
+                // the slot of accumulator row cl (AccRow: FP64 grp + 4 r, FP32 4 grp + r)
+                const int g = f64 ? cl % 4 : cl / 4, r = f64 ? cl / 4 : cl % 4;
+                a1[((size_t)t * 64 + l) * ksp + st] = dval(c, g, 4 * t + r, 4 * st + gl);
+              }
+          for (int m = 0; m < mt; m++)
+            for (int l = 0; l < 64; l++)
+              for (int j = 0; j < nsl; j++) {
+                const int gl = l / 16, cl = l % 16;
+                a2[((size_t)m * 64 + l) * jp + j] = dval(c, gl, j, 16 * m + cl);
+              }
+        }
+        if (f64) {
+          fmat64.alloc(lay.size());
+          fmat64.upload(lay.data(), lay.size());
+          NGSAMD_HIP(hipStreamSynchronize(Stream()));
+        } else {
+          std::vector<float> l32(lay.begin(), lay.end());
+          fmat32.alloc(l32.size());
+          fmat32.upload(l32.data(), l32.size());
+          NGSAMD_HIP(hipStreamSynchronize(Stream()));
+        }
+        fac_kernel = true;
+      }
+    }
+    fac_rank = fac_kernel ? rank : 0;
+    fac_kg = fac_kernel ? kg : 0;
     NGSAMD_HIP(hipStreamSynchronize(Stream()));
   }
 
@@ -1237,11 +1717,81 @@ static void DispatchRef16(const MatFreeOp& A, const V* x, V* out, double s, cons
   else throw Error(NGSAMD_E_STATE, "matrix-free: unsupported 16-bit shape");
 }
 
+template <class T, int DIM, int NDP, int KG, int MODE, class V>
+static void LaunchFac(const MatFreeOp& A, const T* fm, const V* x, V* out, double s, const int* stop,
+                      hipStream_t st, bool prepare) {
+  const size_t lds = (size_t)A.fstride * sizeof(T);
+  auto kern = MfFacKernel<T, DIM, NDP, KG, MODE, V>;
+  if (prepare) return PrepareKernel(A, kern, lds, MODE, sizeof(V) == 8);
+  const MfRounds R{A.round0.p, A.cls_range.p, A.nrounds, (int)A.classes.size()};
+  kern<<<MfGrid(A, MODE, sizeof(V) == 8), kMfBlock, lds, st>>>(fm, A.fstride, R, A.dofs.p, A.metric.p, A.Kappa(),
+                                                               A.elnr.p, A.nd, x, out, s, stop);
+  NGSAMD_CHECK_LAUNCH();
+}
+
+template <class T, int DIM, int MODE, class V>
+static void DispatchFac(const MatFreeOp& A, const T* fm, const V* x, V* out, double s, const int* stop,
+                        hipStream_t st, bool prepare) {
+#define NGSAMD_FAC(NDP, KG)                                                                   \
+  if (A.ndp == NDP && A.fac_kg == KG) return LaunchFac<T, DIM, NDP, KG, MODE, V>(A, fm, x, out, s, stop, st, prepare);
+  if constexpr (DIM == 2) {
+    NGSAMD_FAC(4, 1) NGSAMD_FAC(8, 1) NGSAMD_FAC(12, 2) NGSAMD_FAC(16, 3) NGSAMD_FAC(24, 4)
+  } else {
+    NGSAMD_FAC(4, 1) NGSAMD_FAC(12, 1) NGSAMD_FAC(20, 3) NGSAMD_FAC(36, 5)
+  }
+#undef NGSAMD_FAC
+  throw Error(NGSAMD_E_STATE, "matrix-free: unsupported factored shape");
+}
+
+template <class H, int DIM, int KS1, int MT, int KG, int MODE, class V>
+static void LaunchFac16(const MatFreeOp& A, const V* x, V* out, double s, const int* stop, hipStream_t st,
+                        bool prepare) {
+  const size_t lds = (size_t)A.fstride * 2;
+  auto kern = MfFac16Kernel<H, DIM, KS1, MT, KG, MODE, V>;
+  if (prepare) return PrepareKernel(A, kern, lds, MODE, sizeof(V) == 8);
+  const MfRounds R{A.round0.p, A.cls_range.p, A.nrounds, (int)A.classes.size()};
+  kern<<<MfGrid(A, MODE, sizeof(V) == 8), kMfBlock, lds, st>>>(A.fmat16.p, A.fstride, R, A.dofs.p, A.metric.p,
+                                                               A.Kappa(), A.elnr.p, A.nd, x, out, s, stop);
+  NGSAMD_CHECK_LAUNCH();
+}
+
+template <class H, int DIM, int MODE, class V>
+static void DispatchFac16(const MatFreeOp& A, const V* x, V* out, double s, const int* stop, hipStream_t st,
+                          bool prepare) {
+#define NGSAMD_FAC16(KS1, MT, KG)                                                 \
+  if (A.ks1 == KS1 && A.mt16 == MT && A.fac_kg == KG)                               \
+    return LaunchFac16<H, DIM, KS1, MT, KG, MODE, V>(A, x, out, s, stop, st, prepare);
+  if constexpr (DIM == 2) {
+    NGSAMD_FAC16(1, 1, 1) NGSAMD_FAC16(1, 1, 2) NGSAMD_FAC16(1, 1, 3) NGSAMD_FAC16(1, 2, 4)
+  } else {
+    NGSAMD_FAC16(1, 1, 1) NGSAMD_FAC16(1, 2, 3) NGSAMD_FAC16(2, 3, 5)
+  }
+#undef NGSAMD_FAC16
+  throw Error(NGSAMD_E_STATE, "matrix-free: unsupported factored 16-bit shape");
+}
+
 // All elements: one launch of the MFMA kernel (every class), or the scalar
 // kernel class by class.
 template <int MODE, class V>
 static void LaunchElements(const MatFreeOp& A, const V* x, V* out, double s, const int* stop, hipStream_t st,
                            bool prepare) {
+  if (A.use_mfma && A.fac_kernel) {
+    if (A.nrounds == 0 && !prepare) return;
+    if (A.precision == NGSAMD_PREC_BF16) {
+      if (A.dim == 2) DispatchFac16<__bf16, 2, MODE, V>(A, x, out, s, stop, st, prepare);
+      else DispatchFac16<__bf16, 3, MODE, V>(A, x, out, s, stop, st, prepare);
+    } else if (A.precision == NGSAMD_PREC_F16) {
+      if (A.dim == 2) DispatchFac16<_Float16, 2, MODE, V>(A, x, out, s, stop, st, prepare);
+      else DispatchFac16<_Float16, 3, MODE, V>(A, x, out, s, stop, st, prepare);
+    } else if (A.precision == NGSAMD_PREC_F64) {
+      if (A.dim == 2) DispatchFac<double, 2, MODE, V>(A, A.fmat64.p, x, out, s, stop, st, prepare);
+      else DispatchFac<double, 3, MODE, V>(A, A.fmat64.p, x, out, s, stop, st, prepare);
+    } else {
+      if (A.dim == 2) DispatchFac<float, 2, MODE, V>(A, A.fmat32.p, x, out, s, stop, st, prepare);
+      else DispatchFac<float, 3, MODE, V>(A, A.fmat32.p, x, out, s, stop, st, prepare);
+    }
+    return;
+  }
   if (A.use_mfma && A.ref_kernel) {
     if (A.nrounds == 0 && !prepare) return;
     if (A.precision == NGSAMD_PREC_BF16) {

@@ -40,6 +40,15 @@ struct MatFreeOp : ngsamd_op_s {
   DevBuf<float> smat32;
   DevBuf<uint16_t> smat16;
   int64_t sstride = 0;        // elements per class block in smat*
+  // factored reference matrices S_(dd') = D_d D_d'^T (the default kernels):
+  // rank r of the weighted gradient space, KG = ceil(r / 4) basis functions
+  // per lane group, D in the kernels' LDS layouts per class
+  bool fac_kernel = false;
+  int fac_rank = 0, fac_kg = 0;
+  DevBuf<double> fmat64;
+  DevBuf<float> fmat32;
+  DevBuf<uint16_t> fmat16;
+  int64_t fstride = 0;        // elements per class block in fmat*
   DevBuf<double> weights;     // [nqp], 0 beyond nq
   DevBuf<double> bref;        // [nclass][nq][nd][dim] FP64 (scalar path, diagonal, gradient)
   DevBuf<int> dofs;           // [nel][nd]

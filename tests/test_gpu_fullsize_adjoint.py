@@ -1,5 +1,6 @@
 """The differentiation configuration at its stated size: 2D unit square,
-128 x 128 squares x 2 triangles (32,768 elements), H1 p=3 (148,817 dofs),
+128 x 128 squares x 2 triangles (32,768 elements), H1 p=3 (148,225 dofs:
+129^2 vertices + 2 x 49,408 edge dofs + 32,768 cell dofs; 2,510,593 nonzeros),
 f = 1, homogeneous Dirichlet.  kappa_true = exp(N(0,1)) (seed 3), u_obs =
 u(kappa_true) + N(0, 1e-3^2) noise at the dofs (same generator), J(kappa) =
 0.5 (u - u_obs)^T M (u - u_obs) evaluated at kappa = exp(N(0, 0.5)) (seed 4).
@@ -28,12 +29,14 @@ def test_config4_adjoint_gradient_full_size(device):
     say("[full adjoint] 128x128 p=3 set-up")
     m = ngs.MakeStructured2DMesh(nx=NX, ny=NX)
     fes = ngs.H1(m, order=P_ORDER, dirichlet=".*")
-    assert fes.ndof == 148817 and m.ne == 32768
+    # (n+1)^2 vertices, p-1 dofs on each of the 3n^2+2n edges, (p-1)(p-2)/2 per cell
+    assert fes.ndof == 129 ** 2 + 2 * (3 * NX * NX + 2 * NX) + 2 * NX * NX == 148225 and m.ne == 32768
     u_, v_ = fes.TnT()
     f = ngs.LinearForm(1 * v_ * ngs.dx).Assemble().vec.NumPy().copy()
     mass = ngs.BilinearForm(u_ * v_ * ngs.dx).Assemble()
     pr = O.Problem(O.structured_2d(NX), P_ORDER, intorder=2 * P_ORDER - 2)
     prm = O.Problem(O.structured_2d(NX), P_ORDER, intorder=2 * P_ORDER)
+    assert pr.ndof == fes.ndof
     rp, c, vals = prm.assemble(stiffness=False, mass=True)
     M = sp.csr_matrix((vals, c, rp), shape=(pr.ndof, pr.ndof))
     ne = m.ne
