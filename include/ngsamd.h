@@ -254,6 +254,21 @@ int ngsamd_cg_solve(ngsamd_op_t mat, ngsamd_op_t pre, const double* d_rhs, doubl
                     int initialize, double tol, double atol, int maxiter, int chunk,
                     int* h_iterations, double* h_residuals);
 
+/* ngsamd_cg_solve without any allocation, for callers that cannot allocate
+ * device or pinned memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.  Its
+ * work vectors, device scalars and residual history live in a file-scope
+ * device array of the library (each GPU holds its own copy, about 97 MiB,
+ * until the process ends): at most 4,194,304 dofs and maxiter <= 65536, or
+ * the call fails with NGSAMD_E_ARG.  Kernels are launched directly (no
+ * graph); the host reads the device scalars every 16 iterations with a
+ * plain copy.  Same kernels in the same order as ngsamd_cg_solve, so the
+ * same iterates.  Every call on one GPU shares that array: calls that overlap
+ * on one GPU (several host threads or streams) race. */
+int ngsamd_cg_solve_noalloc(ngsamd_op_t mat, ngsamd_op_t pre, const double* d_rhs, double* d_sol,
+                            int initialize, double tol, double atol, int maxiter,
+                            int* h_iterations, double* h_residuals);
+
 /* Mixed-precision iterative refinement: outer FP64 residual r = f - A u with
  * `mat`, inner Jacobi-PCG on `inner` (FP32 vectors, FP64 dots and scalars)
  * to relative tolerance inner_tol, u += correction, until

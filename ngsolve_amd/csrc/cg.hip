@@ -462,6 +462,65 @@ static CgResult IterateCg(ngsamd_op_t A, const V* inv, ngsamd_op_t pre, V* u, Cg
   return res;
 }
 
+// ------------------------------------------------------ allocation-free CG
+// Work space of ngsamd_cg_solve_noalloc: the device scalars, the residual
+// history and the work vectors d, w, s, in one file-scope device array that
+// each GPU holds (its own copy, at its own address) until the process ends.
+// Host code reaches it only through the address hipGetSymbolAddress gives,
+// kept per device by InitCgScratch (called by ngsamd_init with the device
+// current); the array's name on the host is not a device address.
+constexpr int64_t kNoAllocMaxDofs = int64_t(1) << 22;  // 4,194,304 dofs
+constexpr int kNoAllocMaxIter = 1 << 16;
+constexpr size_t kNoAllocResOffset = 256;  // CGScalars first
+constexpr size_t kNoAllocVecOffset = kNoAllocResOffset + sizeof(double) * kNoAllocMaxIter;
+constexpr size_t kNoAllocScratchBytes = kNoAllocVecOffset + 3 * sizeof(double) * kNoAllocMaxDofs;
+static_assert(sizeof(CGScalars) <= kNoAllocResOffset, "CGScalars must fit the scratch header");
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+constexpr int kScratchDevices = 64;
+static void* g_scratch_ptr[kScratchDevices] = {};
+
+void InitCgScratch(int device) {
+  NGSAMD_REQUIRE(device >= 0 && device < kScratchDevices, "ngsamd_init: device number out of range");
+  void* p = nullptr;
+  NGSAMD_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  g_scratch_ptr[device] = p;
+}
+
+// CG from an initialised d (= f - A u) to the stopping test, every kernel
+// launched directly (no graph, no events, no pinned memory): `chunk`
+// iterations are enqueued, then the device scalars are copied to the host
+// and checked.  Same kernels in the same order as IterateCg: the same
+// iterates.
+static CgResult IterateCgDirect(ngsamd_op_t A, const double* inv, ngsamd_op_t pre, double* u, double* d, double* w,
+                                double* s, double* residuals, CGScalars* dS, int64_t n, double tol, double atol,
+                                int maxiter, hipStream_t st) {
+  CGScalars hS{};
+  hS.tol = tol;
+  hS.atol = atol;
+  hS.maxiter = maxiter;
+  NGSAMD_HIP(hipMemcpyAsync(dS, &hS, sizeof(CGScalars), hipMemcpyHostToDevice, st));
+  const int g = CgGrid(n);
+  if (!inv) EnqueuePrecond<double>(pre, d, w, nullptr, st);
+  CgInitKernel<double><<<g, kCgBlock, 0, st>>>(inv, d, w, s, n, LibraryPartials().w);
+  NGSAMD_CHECK_LAUNCH();
+  CgInitCheckKernel<<<1, kCgBlock, 0, st>>>(LibraryPartials().w, g, dS, residuals);
+  NGSAMD_CHECK_LAUNCH();
+  constexpr int kChunk = 16;  // even: every chunk starts at parity 0
+  long long launched = 0;
+  while (true) {
+    NGSAMD_HIP(hipMemcpyAsync(&hS, dS, sizeof(CGScalars), hipMemcpyDeviceToHost, st));
+    NGSAMD_HIP(hipStreamSynchronize(st));
+    if (hS.done || launched > (long long)maxiter + kChunk) break;
+    for (int k = 0; k < kChunk; k++) EnqueueIteration<double>(A, inv, pre, u, d, w, s, n, dS, residuals, k & 1, st);
+    launched += kChunk;
+  }
+  NGSAMD_REQUIRE(hS.done != 0, "cg: iteration did not terminate within maxiter (internal error)");
+  CgResult res;
+  res.iterations = hS.it;
+  res.done = hS.done;
+  return res;
+}
+
 static void CheckCgOperators(ngsamd_op_t A, ngsamd_op_t pre, bool jacobi_only) {
   NGSAMD_REQUIRE(A && pre, "cg: null operator");
   NGSAMD_REQUIRE(pre->kind == OpKind::Jacobi || (!jacobi_only && pre->kind == OpKind::BlockJacobi),
@@ -515,6 +574,40 @@ extern "C" int ngsamd_cg_solve(ngsamd_op_t A, ngsamd_op_t pre, const double* f, 
       NGSAMD_HIP(hipMemcpyAsync(h_residuals, W.residuals.p, cnt * sizeof(double), hipMemcpyDeviceToHost, st));
       NGSAMD_HIP(hipStreamSynchronize(st));
     }
+  });
+}
+
+extern "C" int ngsamd_cg_solve_noalloc(ngsamd_op_t A, ngsamd_op_t pre, const double* f, double* u, int initialize,
+                                       double tol, double atol, int maxiter, int* h_iterations,
+                                       double* h_residuals) {
+  return Guard([&] {
+    RequireInit();
+    CheckCgOperators(A, pre, false);
+    NGSAMD_REQUIRE(maxiter >= 1 && maxiter <= kNoAllocMaxIter, "cg_solve_noalloc: maxiter must be in 1..65536");
+    const int64_t n = A->height;
+    NGSAMD_REQUIRE(n <= kNoAllocMaxDofs, "cg_solve_noalloc: more than 4194304 dofs (the scratch array's limit)");
+    const int dev = State().device;
+    NGSAMD_REQUIRE(dev >= 0 && dev < kScratchDevices && g_scratch_ptr[dev],
+                   "cg_solve_noalloc: no scratch address for this device (ngsamd_init)");
+    unsigned char* base = static_cast<unsigned char*>(g_scratch_ptr[dev]);
+    CGScalars* dS = reinterpret_cast<CGScalars*>(base);
+    double* residuals = reinterpret_cast<double*>(base + kNoAllocResOffset);
+    double* d = reinterpret_cast<double*>(base + kNoAllocVecOffset);
+    double* w = d + kNoAllocMaxDofs;
+    double* s = w + kNoAllocMaxDofs;
+    hipStream_t st = Stream();
+    // d = f - A u
+    if (initialize) LaunchFill(u, n, 0.0, st);
+    if (n) NGSAMD_HIP(hipMemcpyAsync(d, f, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+    if (!initialize) A->MultAdd(-1.0, u, d, st);
+    const double* inv = pre->kind == OpKind::Jacobi ? static_cast<JacobiOp*>(pre)->invdiag.p : nullptr;
+    CgResult r = IterateCgDirect(A, inv, pre, u, d, w, s, residuals, dS, n, tol, atol, maxiter, st);
+    if (h_iterations) *h_iterations = r.iterations;
+    if (h_residuals && r.iterations > 0) {
+      const int cnt = r.iterations < maxiter ? r.iterations : maxiter;
+      NGSAMD_HIP(hipMemcpyAsync(h_residuals, residuals, cnt * sizeof(double), hipMemcpyDeviceToHost, st));
+    }
+    NGSAMD_HIP(hipStreamSynchronize(st));
   });
 }
 

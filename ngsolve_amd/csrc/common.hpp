@@ -120,6 +120,9 @@ void LaunchFoldPartials(const double* in, int n, double* out, const int* stop, h
 // device properties of the current device (queried once)
 int DeviceCUs();
 int DeviceMaxLds();
+// address of the allocation-free CG's scratch array on `device` (cg.hip),
+// kept for ngsamd_cg_solve_noalloc; called by ngsamd_init with the device current
+void InitCgScratch(int device);
 
 // Workgroups are dealt to the 8 XCDs round-robin (workgroup b runs on XCD
 // b mod 8).  This maps the hardware id to a logical id such that each XCD owns

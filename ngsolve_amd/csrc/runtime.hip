@@ -196,6 +196,7 @@ int ngsamd_init(int device) {
     NGSAMD_HIP(hipMalloc((void**)&S.wide_partials, sizeof(double) * kMaxWideBlocks));
     NGSAMD_HIP(hipMalloc((void**)&S.dev_scalars, sizeof(double) * kNumScalars));
     NGSAMD_HIP(hipHostMalloc((void**)&S.host_scalars, sizeof(double) * kNumScalars, hipHostMallocDefault));
+    InitCgScratch(device);
     NGSAMD_HIP(hipDeviceSynchronize());
     S.initialised = true;
   });
