@@ -184,7 +184,8 @@ def bench_batch(N, args):
     probs = make_problems(args.batch_problems, seed=5)
     bs = BatchSolver(probs)
     log(f"[bench] batch setup {time.perf_counter() - t:.1f}s ndof={bs.n} nnz={bs.nnz} block_nnz={bs.block_nnz}")
-    out = {"problems": len(probs), "total_dofs": bs.n, "total_nnz": bs.nnz, "autotuned_block_nnz": bs.block_nnz}
+    out = {"problems": len(probs), "total_dofs": bs.n, "total_nnz": bs.nnz, "autotuned_block_nnz": bs.block_nnz,
+           "autotune_ms_per_apply": {str(k): round(v, 4) for k, v in N.BatchTuning(bs.op).items()}}
     sols = {}
     for name, fn in [("fp64_block_cg", lambda: bs.solve_fp64(tol=1e-10)),
                      ("mixed_block_ir", lambda: bs.solve_mixed(tol=1e-10))]:
@@ -545,20 +546,27 @@ def main():
 
     # ------------------------------------------------- matrix-free applies
     nq = len(ir)
-    # flops of the form the kernels execute: y_e = sum_c S_c (g_ec x_e), six
-    # symmetric nd x nd reference matrices per class (tet p=4: nd = 35)
+    # flops the factored kernels execute per element: U = D^T x_e (nd x 3r),
+    # V = G U (3 x 3 per basis function), y_e = D V; r = dim P_(p-1) = 20 for
+    # tet p=4 (nd = 35): 2*35*60 + 18*20 + 2*35*60 = 8,760
     nd_el = fes.nd_el
-    flops_mf = 2.0 * 6 * nd_el * nd_el * ne
+    rank = 4 * 5 * 6 // 6
+    flops_mf = (4.0 * nd_el * 3 * rank + 18.0 * rank) * ne
+    classes_c = N.BuildClasses(fes, ir)
+    el_c = np.concatenate([c["elements"] for c in classes_c]).astype(np.int64)
+    colors = np.ascontiguousarray(N.ElementColoring(mesh)[el_c], dtype=np.int32)
     x32, y32 = N.DeviceVector(ndof, 4), N.DeviceVector(ndof, 4)
     x32.upload(x.astype(np.float32))
     for name, prec, peak in [("mf_f64", 0, FP64_PEAK_TFLOPS), ("mf_f32", 1, FP32_PEAK_TFLOPS),
                              ("mf_bf16", 2, BF16_PEAK_TFLOPS), ("mf_f16", 3, BF16_PEAK_TFLOPS)]:
-        for scatter in (0, 1):
-            M = build_mf(N, fes, mesh, ir, None, prec, scatter)
+        for scatter in (0, 1, 2):
+            M = build_mf(N, fes, mesh, ir, None, prec, min(scatter, 1))
+            if scatter == 2:  # colour-ordered direct scatter (33 colours at 32^3)
+                M.set_coloring(colors)
             ms = timed(N, lambda: M.mult(dx, dy), args.steps, args.warmup)
             err = err_apply(dy.download())
             tf = flops_mf / (ms * 1e-3) / 1e12
-            key = name + ("" if scatter == 0 else "_atomic")
+            key = name + ("", "_atomic", "_color")[scatter]
             extra[key] = {"us_per_apply": round(ms * 1e3, 1), "gdofs": round(ndof / (ms * 1e-3) / 1e9, 3),
                           "tflops": round(tf, 2), "frac_of_peak": round(tf / peak, 4),
                           "rel_err_vs_oracle": err}
@@ -764,6 +772,37 @@ def main():
                 "blocks": len(blocks), "max_block": int(np.diff(blocks.ptr).max()),
                 "true_rel_residual": true_res(u_b), "rel_err_vs_oracle": err_sol(u_b)}
             del Cb
+        # two-level: Jacobi + exact coarse solve on the P1 (vertex) block by
+        # the device sparse Cholesky, additively
+        nv = mesh.nv
+        keep = cols[:rowptr[nv]] < nv
+        vrows = np.repeat(np.arange(nv), np.diff(rowptr[:nv + 1]))[keep]
+        vrp = np.zeros(nv + 1, dtype=np.int64)
+        np.add.at(vrp, vrows + 1, 1)
+        vrp = np.cumsum(vrp)
+        t0 = time.perf_counter()
+        Cc = N.CreateSparseCholesky(nv, vrp, np.ascontiguousarray(cols[:rowptr[nv]][keep]),
+                                    np.ascontiguousarray(vals_k[:rowptr[nv]][keep]),
+                                    np.ascontiguousarray(fes.freedofs[:nv]))
+        t_fact = time.perf_counter() - t0
+        info = N.SparseCholeskyInfo(Cc)
+        Ctl = N.CreateTwoLevel(Ck, Cc)
+        dxv, dyv = N.DeviceVector(nv, 8), N.DeviceVector(nv, 8)
+        dxv.upload(np.random.default_rng(7).uniform(-1, 1, nv))
+        ms_coarse = timed(N, lambda: Cc.mult(dxv, dyv), 20, 3)
+        du4 = N.DeviceVector(ndof, 8)
+        t0 = time.perf_counter()
+        its_t, _ = N.CGSolve(Ak, Ctl, df, du4, True, 1e-10, -1.0, 200000, 16)
+        t_t = time.perf_counter() - t0
+        u_t = du4.download()
+        log(f"[bench] FP64 CSR CG two-level (Jacobi + P1 sparse Cholesky) lognormal: {its_t} its {t_t:.3f}s "
+            f"(factor {t_fact:.2f}s, coarse solve {ms_coarse * 1e3:.1f} us, {info})")
+        res["fp64_csr_cg_twolevel_p1"] = {
+            "iterations": its_t, "seconds": round(t_t, 4), "factor_seconds": round(t_fact, 3),
+            "coarse_solve_us": round(ms_coarse * 1e3, 1), "coarse": info,
+            "true_rel_residual": true_res(u_t), "rel_err_vs_oracle": err_sol(u_t),
+            "speedup_vs_fp64_csr_cg": round(t_k / t_t, 3)}
+        del Ctl, Cc, du4, dxv, dyv
         if u_oracle is not None:
             res["cpu_oracle_pcg"] = dict(res_cpu, true_rel_residual=true_res(u_oracle),
                                          gpu_fp64_csr_cg_speedup=round(res_cpu["seconds"] / t_k, 1),

@@ -49,6 +49,7 @@ extern "C" {
 /* scatter mode of a matrix-free operator */
 #define NGSAMD_SCATTER_GATHER 0   /* deterministic two-pass owner-computes gather */
 #define NGSAMD_SCATTER_ATOMIC 1   /* hardware FP64/FP32 atomics, as the reference kernel */
+#define NGSAMD_SCATTER_COLOR 2    /* colour-ordered direct scatter (ngsamd_mf_set_coloring) */
 
 typedef struct ngsamd_op_s* ngsamd_op_t;
 
@@ -127,6 +128,31 @@ int ngsamd_blockjacobi_create(ngsamd_op_t* op, int64_t n, const int64_t* h_rowpt
                               const double* h_vals, int64_t nblocks, const int64_t* h_block_ptr,
                               const int* h_block_dofs);
 
+/* Sparse Cholesky inverse of the free block of a symmetric positive definite
+ * matrix: y = A_ff^{-1} x on the free dofs (h_freedofs[i] != 0; NULL: all),
+ * 0 on the others; Mult overwrites, MultAdd accumulates.  SparseCholesky
+ * (linalg/sparsecholesky.cpp) behind a.mat.Inverse(freedofs,
+ * inverse="sparsecholesky"), device solve as DevSparseCholesky
+ * (ngscuda/dev_sparsecholesky.cpp:403-454).  Host: nested-dissection
+ * ordering and supernodal (multifrontal) factorisation in FP64; device:
+ * level-scheduled triangular solves (a few kernels per level of the
+ * dissection tree, no waiting between workgroups, deterministic).  Accepted
+ * by ngsamd_cg_solve as a preconditioner. */
+int ngsamd_sparsecholesky_create(ngsamd_op_t* op, int64_t n, const int64_t* h_rowptr, const int* h_cols,
+                                 const double* h_vals, const uint8_t* h_freedofs);
+/* h_info[0] = free dofs, [1] = supernodes, [2] = levels, [3] = nonzeros of L */
+int ngsamd_sparsecholesky_info(ngsamd_op_t op, int64_t* h_info);
+
+/* Additive two-level preconditioner y = S x + E C E^T x: S a Jacobi or
+ * block-Jacobi operator of the whole space, C a sparse Cholesky inverse
+ * acting on the first C->height dofs (E their embedding; for a hierarchical
+ * H1 space the vertex dofs, i.e. the P1 space).  The composite
+ * smoother + E @ a_lo.mat.Inverse(freedofs_lo) @ E.T of the reference's sum
+ * and product matrices (linalg/basematrix.hpp:413,592).  The operators are
+ * not owned: destroy the two-level operator first.  Accepted by
+ * ngsamd_cg_solve. */
+int ngsamd_twolevel_create(ngsamd_op_t* op, ngsamd_op_t smoother, ngsamd_op_t coarse);
+
 /* ------------------------------------------------- matrix-free H1 Laplace */
 /* Fused element-batched operator y += s * sum_e P_e^T B^T D_e B P_e x with
  * D_e = kappa_e w_q |det F_e| F_e^{-1} F_e^{-T} (comp/matrixfree.cpp:681-1087).
@@ -153,6 +179,14 @@ int ngsamd_mf_create(ngsamd_op_t* op, int dim, int nd, int nq, int64_t ndof, int
  * number in h_elnr plus one (the row count of the tables if h_elnr was NULL);
  * element coefficients and element gradients are indexed by it */
 int ngsamd_mf_num_elements(ngsamd_op_t op, int64_t* ne);
+/* Colour-ordered atomic-free scatter, as the reference's coloured element
+ * loop (IterateElements over FESpace::ElementColoring, comp/fespace.cpp:518-760,
+ * 2137-2200; comp/bilinearform.cpp:4669-4750): h_colors[nel] gives the colour
+ * of every table row (elements of one colour may share no dof: checked,
+ * NGSAMD_E_ARG otherwise).  From then on the apply runs one launch per colour
+ * that adds each element's result straight into y (no contribution buffer,
+ * no gather).  Needs the factored kernels (H1 p <= 5 in 2D, p <= 4 in 3D). */
+int ngsamd_mf_set_coloring(ngsamd_op_t op, const int* h_colors);
 /* replace the element coefficient: d_kappa is a device array of n values
  * indexed by global element number, n must equal ngsamd_mf_num_elements; the
  * operator keeps the pointer (not a copy) until the next call, so the array
@@ -306,6 +340,11 @@ int ngsamd_csr_mult_multi(ngsamd_op_t op, int nrhs, const double* d_x, double* d
 int ngsamd_batch_csr_create(ngsamd_op_t* op, int64_t n, const int64_t* h_rowptr, const int* h_cols,
                             const double* h_vals, int nseg, const int64_t* h_seg, int with_f32);
 int ngsamd_batch_info(ngsamd_op_t op, int* nseg, int* block_nnz, int* nblocks);
+/* The block-size autotune of the batched operator: *h_count candidates
+ * (0 when NGSAMD_SPMV_NNZ forced one), h_ms[k] (room for 3) the measured
+ * milliseconds per apply of 1024, 2048 and 4096 values (nonzeros) per block;
+ * ngsamd_batch_info reports the one kept. */
+int ngsamd_batch_tuning(ngsamd_op_t op, int* h_count, double* h_ms);
 /* Jacobi-PCG on every problem with the CGSolver recurrence and stopping rule
  * per problem (tol relative to each problem's initial residual).  Per problem:
  * h_iterations (checks incl. the initial one), h_final_res (last residual),

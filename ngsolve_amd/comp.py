@@ -313,6 +313,13 @@ def smoothing_blocks(m, sp, blocktype=4):
     return BlockTable(ptr, dof)
 
 
+class _FreeDofs(BitArray):
+    """A BitArray that can also be called (fes.FreeDofs(coupling=False))."""
+
+    def __call__(self, coupling=False):
+        return self
+
+
 class H1:
     def __init__(self, mesh, order=1, dirichlet=None, **kwargs):
         if not isinstance(mesh, Mesh):
@@ -336,7 +343,9 @@ class H1:
 
     @property
     def FreeDofs(self):
-        return BitArray(self.space.freedofs.astype(bool))
+        """The free dofs; fes.FreeDofs() as in NGSolve (FESpace::GetFreeDofs,
+        comp/fespace.cpp:765-810) and fes.FreeDofs alike."""
+        return _FreeDofs(self.space.freedofs.astype(bool))
 
     def GetDofNrs(self, el):
         return tuple(int(d) for d in self.space.el_dofs[el])
@@ -363,16 +372,21 @@ class H1:
 class MFOpts:
     """MatFreeOptions (comp/bilinearform.hpp:24-52)."""
 
-    def __init__(self, fused=True, atomic=False, fp32=False, precision=None, **kwargs):
+    def __init__(self, fused=True, atomic=False, fp32=False, precision=None, coloring=False, **kwargs):
+        """atomic=True: hardware atomics (the reference's default);
+        coloring=True: the colour-ordered plain += of the reference's
+        non-atomic path (one launch per element colour); neither: the
+        deterministic two-pass gather (the fastest here)."""
         self.fused = fused
         self.atomic = atomic
+        self.coloring = coloring
         self.precision = precision or ("f32" if fp32 else "f64")
 
 
 class DevMatrixFree(DeviceMatrix):
     """Fused element-batched operator on the device (MatrixFreeBTDTB)."""
 
-    def __init__(self, fes, order, kappa=None, precision="f64", atomic=False):
+    def __init__(self, fes, order, kappa=None, precision="f64", atomic=False, coloring=False):
         ensure_device()
         mesh = fes.mesh.ngmesh
         ir = fes.integration_rule(order)
@@ -393,6 +407,9 @@ class DevMatrixFree(DeviceMatrix):
                                 np.ascontiguousarray(ir.weights), np.ascontiguousarray(dofs, dtype=np.int32),
                                 np.ascontiguousarray(metric), kap, el, PREC[precision],
                                 SCATTER["atomic" if atomic else "gather"])
+        if coloring:
+            # FESpace element colouring (comp/fespace.cpp:518-760), in table order
+            op.set_coloring(np.ascontiguousarray(N.ElementColoring(mesh)[el], dtype=np.int32))
         super().__init__(op, True)
         self.fes, self.nq, self.nclass, self.precision = fes, len(ir), len(classes), precision
 
@@ -450,7 +467,7 @@ class BilinearForm:
                 raise ValueError("the matrix-free operator supports one grad(u)*grad(v) term")
             term, meas = self.integrals[0]
             self.mat = DevMatrixFree(fes, self._order(term, meas), self._coef(term), self.mf.precision,
-                                     self.mf.atomic)
+                                     self.mf.atomic, getattr(self.mf, "coloring", False))
             return self
         if self.device_assembly:
             if len(self.integrals) != 1:

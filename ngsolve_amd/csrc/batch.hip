@@ -88,6 +88,9 @@ struct BatchOp : ngsamd_op_s {
   DevBuf<int64_t> chunk_start;
   DevBuf<int> chunk_seg, seg_chunk;
   DevBuf<double> partials;
+  // autotune: ms per apply of each candidate block size (1024, 2048, 4096),
+  // empty when a single size was forced
+  std::vector<float> tuned_ms;
   BatchOp(int64_t n, const int64_t* rp, const int* c, const double* v, int nseg, const int64_t* seg, bool with_f32);
   void SetBlocks(const RowBlocks& b);
   void MultAdd(double s, const double* x, double* y, hipStream_t st) override;
@@ -289,9 +292,11 @@ static void LaunchBatchSpmv(const BatchOp& A, const M* vals, const V* x, V* y, d
     if (wg > 8) wg &= ~int64_t(7);
     if (wg < 1) wg = 1;
     const int per = (int)((A.nblocks + wg - 1) / wg);
-    BatchRunsKernel<2048, V, M, ACCUM, DOT><<<(int)wg, kBatchBlock, 0, st>>>(
-        A.run_row.p, A.run_ptr.p, A.val_ptr.p, A.rcols.p, rv, x, y, s, A.blk_run.p, A.blk_len.p, A.blk_seg.p,
-        A.nblocks, per, done, A.partials.p);
+    auto rk = block_nnz == 1024   ? BatchRunsKernel<1024, V, M, ACCUM, DOT>
+              : block_nnz == 4096 ? BatchRunsKernel<4096, V, M, ACCUM, DOT>
+                                  : BatchRunsKernel<2048, V, M, ACCUM, DOT>;
+    rk<<<(int)wg, kBatchBlock, 0, st>>>(A.run_row.p, A.run_ptr.p, A.val_ptr.p, A.rcols.p, rv, x, y, s, A.blk_run.p,
+                                       A.blk_len.p, A.blk_seg.p, A.nblocks, per, done, A.partials.p);
     NGSAMD_CHECK_LAUNCH();
     return;
   }
@@ -348,10 +353,12 @@ BatchOp::BatchOp(int64_t n, const int64_t* rp, const int* c, const double* v, in
     std::vector<double> rv;
     rc.reserve(nnz);
     rv.reserve(nnz);
-    const int64_t nsh = BuildRuns(n, rp, c, v, 2048, rr, rpp, vp, rc, rv, br, bl, seg, ns, &bs);
+    // runs of at most 1024 values (or one longer row), so that every block
+    // size tried below holds whole runs
+    const int64_t nsh = BuildRuns(n, rp, c, v, 1024, rr, rpp, vp, rc, rv, br, bl, seg, ns, &bs);
     if (4 * nsh <= 3 * nnz) {
       use_runs = true;
-      block_nnz = 2048;
+      block_nnz = 1024;
       nblocks = (int)br.size() - 1;
       run_row.alloc(rr.size());
       run_row.upload(rr.data(), rr.size());
@@ -372,22 +379,69 @@ BatchOp::BatchOp(int64_t n, const int64_t* rp, const int* c, const double* v, in
         rvals32.upload_at(zf, nnz, 4);
         NGSAMD_HIP(hipStreamSynchronize(Stream()));
       }
-      blk_run.alloc(br.size());
-      blk_run.upload(br.data(), br.size());
-      blk_len.alloc(std::max<size_t>(bl.size(), 1));
-      blk_len.upload(bl.data(), bl.size());
-      blk_seg.alloc(std::max<size_t>(bs.size(), 1));
-      blk_seg.upload(bs.data(), bs.size());
-      // first block of each problem (blocks are in problem order)
-      std::vector<int> sb(ns + 1);
-      int b = 0;
-      for (int q = 0; q <= ns; q++) {
-        while (b < nblocks && bs[b] < q) b++;
-        sb[q] = b;
+      // problem of each run
+      const int nruns = (int)rr.size() - 1;
+      std::vector<int> rseg(nruns);
+      {
+        int q = 0;
+        for (int u = 0; u < nruns; u++) {
+          while (q < ns && rr[u] >= seg[q + 1]) q++;
+          rseg[u] = q;
+        }
       }
-      seg_blk.alloc(sb.size());
-      seg_blk.upload(sb.data(), sb.size());
-      NGSAMD_HIP(hipStreamSynchronize(Stream()));
+      auto set_blocks = [&](int bn) {
+        GroupRunBlocks(rr, vp, rseg, bn, br, bl, &bs);
+        nblocks = (int)br.size() - 1;
+        blk_run.alloc(br.size());
+        blk_run.upload(br.data(), br.size());
+        blk_len.alloc(std::max<size_t>(bl.size(), 1));
+        blk_len.upload(bl.data(), bl.size());
+        blk_seg.alloc(std::max<size_t>(bs.size(), 1));
+        blk_seg.upload(bs.data(), bs.size());
+        // first block of each problem (blocks are in problem order)
+        std::vector<int> sb(ns + 1);
+        int b = 0;
+        for (int q = 0; q <= ns; q++) {
+          while (b < nblocks && bs[b] < q) b++;
+          sb[q] = b;
+        }
+        seg_blk.alloc(sb.size());
+        seg_blk.upload(sb.data(), sb.size());
+        partials.alloc(std::max<size_t>({(size_t)nblocks, (size_t)ns, (size_t)1}));
+        NGSAMD_HIP(hipStreamSynchronize(Stream()));
+      };
+      // autotune the values per block of the runs kernel: each candidate
+      // timed on the device (three applies), the fastest kept
+      const char* benv = getenv("NGSAMD_SPMV_NNZ");
+      std::vector<int> cand = benv ? std::vector<int>{atoi(benv)} : std::vector<int>{1024, 2048, 4096};
+      int best = cand[0];
+      float best_ms = 1e30f;
+      if (cand.size() > 1) {
+        hipStream_t st = Stream();
+        DevBuf<double> tx(n), ty(n);
+        LaunchFill(tx.p, n, 1.0, st);
+        hipEvent_t e0, e1;
+        NGSAMD_HIP(hipEventCreate(&e0));
+        NGSAMD_HIP(hipEventCreate(&e1));
+        for (int bn : cand) {
+          set_blocks(bn);
+          LaunchBatchSpmv<double, double, false, true>(*this, rvals.p, tx.p, ty.p, 1.0, nullptr, st, bn);
+          NGSAMD_HIP(hipEventRecord(e0, st));
+          for (int k = 0; k < 3; k++)
+            LaunchBatchSpmv<double, double, false, true>(*this, rvals.p, tx.p, ty.p, 1.0, nullptr, st, bn);
+          NGSAMD_HIP(hipEventRecord(e1, st));
+          NGSAMD_HIP(hipEventSynchronize(e1));
+          float ms = 0.f;
+          NGSAMD_HIP(hipEventElapsedTime(&ms, e0, e1));
+          tuned_ms.push_back(ms / 3);
+          if (ms < best_ms) best_ms = ms, best = bn;
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+      }
+      NGSAMD_REQUIRE(best == 1024 || best == 2048 || best == 4096, "NGSAMD_SPMV_NNZ must be 1024, 2048 or 4096");
+      block_nnz = best;
+      set_blocks(best);
     }
   }
   if (!use_runs) {
@@ -434,6 +488,7 @@ BatchOp::BatchOp(int64_t n, const int64_t* rp, const int* c, const double* v, in
     NGSAMD_HIP(hipStreamSynchronize(Stream()));
     return;
   }
+  tuned_ms.clear();
   // autotune the row-block size: time each candidate on the device
   hipStream_t st = Stream();
   const char* env = getenv("NGSAMD_SPMV_NNZ");
@@ -462,6 +517,7 @@ BatchOp::BatchOp(int64_t n, const int64_t* rp, const int* c, const double* v, in
       NGSAMD_HIP(hipEventSynchronize(e1));
       NGSAMD_HIP(hipEventElapsedTime(&ms, e0, e1));
     }
+    if (cand.size() > 1) tuned_ms.push_back(ms / 3);
     if (ms < best_ms) {
       best_ms = ms;
       best = bn;
@@ -854,6 +910,278 @@ __global__ __launch_bounds__(kBatchBlock) void RefineCheckKernel(
   }
 }
 
+// ------------------------------------------- refinement steps on the device
+// Each problem runs its own FP64 refinement: when its inner FP32 CG has
+// stopped (S.done[p] != 0) and it is not finished, the next outer block
+// takes its step: u += ||r|| c, r = f - A u (FP64 SpMV over its rows only),
+// the convergence test on ||r||_free, and either finishes it or restarts its
+// inner CG on r / ||r|| with the tolerance still needed.  The outer block
+// is part of the captured iteration graph, so problems leave the batch as
+// soon as they converge.
+
+// pending[p]: inner solve stopped, problem open; skip[p] = !pending[p] (the
+// FP64 SpMV's skip flags); the inner iterations of the step are counted
+__global__ void IrPendingKernel(const BatchState S, const int* __restrict__ frozen, int* __restrict__ pending,
+                                int* __restrict__ skip, int* __restrict__ inner_sum, int nseg) {
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < nseg; p += gridDim.x * blockDim.x) {
+    const int pd = (!frozen[p] && S.done[p]) ? 1 : 0;
+    pending[p] = pd;
+    skip[p] = 1 - pd;
+    if (pd) inner_sum[p] += S.it[p];
+  }
+}
+
+// chunks of pending problems: u += ||r|| c; r = f
+__global__ void IrCorrectKernel(double* __restrict__ u, const float* __restrict__ c32, double* __restrict__ r,
+                                const double* __restrict__ f, const int64_t* __restrict__ cstart,
+                                const int* __restrict__ cseg, const double* __restrict__ rnorm,
+                                const int* __restrict__ pending) {
+  const int c = blockIdx.x;
+  const int p = cseg[c];
+  if (!pending[p]) return;
+  const double sc = rnorm[p];
+  for (int64_t i = cstart[c] + threadIdx.x; i < cstart[c + 1]; i += blockDim.x) {
+    u[i] += sc * (double)c32[i];
+    r[i] = f[i];
+  }
+}
+
+// chunks of pending problems: partial sums over free dofs of r^2
+__global__ __launch_bounds__(kBatchBlock) void IrSqKernel(const double* __restrict__ r, const double* __restrict__ inv,
+                                                          const int64_t* __restrict__ cstart,
+                                                          const int* __restrict__ cseg,
+                                                          const int* __restrict__ pending,
+                                                          double* __restrict__ partials) {
+  __shared__ double lds[kBatchBlock / 64];
+  const int c = blockIdx.x;
+  if (!pending[cseg[c]]) {
+    if (threadIdx.x == 0) partials[c] = 0.0;
+    return;
+  }
+  double acc = 0.0;
+  for (int64_t i = cstart[c] + threadIdx.x; i < cstart[c + 1]; i += kBatchBlock)
+    if (inv[i] != 0.0) acc += r[i] * r[i];
+  StorePartial<kBatchBlock>(acc, partials, lds);
+}
+
+// per pending problem: the outer test (converged, or max_outer inner solves
+// done: finished), else the next inner tolerance and restart[p] = 1
+__global__ __launch_bounds__(kBatchBlock) void IrOuterCheckKernel(
+    const double* __restrict__ partials, const int* __restrict__ seg_chunk, const double* __restrict__ fnorm,
+    double tol, double inner_tol, int max_outer, const int* __restrict__ pending, int* __restrict__ frozen,
+    int* __restrict__ restart, double* __restrict__ rnorm, double* __restrict__ inv_rnorm, double* __restrict__ itol,
+    double* __restrict__ relres, long long* __restrict__ finish, int* __restrict__ outer) {
+  __shared__ double lds[kBatchBlock / 64];
+  const int p = blockIdx.x;
+  if (!pending[p]) {
+    if (threadIdx.x == 0) restart[p] = 0;
+    return;
+  }
+  const double rn = sqrt(SegmentSum(partials, seg_chunk, p, lds));
+  if (threadIdx.x == 0) {
+    const double fn = fnorm[p];
+    relres[p] = fn > 0 ? rn / fn : 0.0;
+    if (rn <= tol * fn || rn == 0.0 || outer[p] >= max_outer) {
+      frozen[p] = 1;
+      finish[p] = DeviceClock();
+      restart[p] = 0;
+      return;
+    }
+    outer[p] += 1;
+    rnorm[p] = rn;
+    inv_rnorm[p] = 1.0 / rn;
+    // the last step only needs to gain the factor still missing
+    double t = inner_tol;
+    const double need = 0.3 * tol * fn / rn;
+    if (need > t) t = need < 0.5 ? need : 0.5;
+    itol[p] = t;
+    restart[p] = 1;
+  }
+}
+
+// chunks of restarted problems: the inner CG from c = 0 on d = r / ||r||
+// (FP32): w = C d, s = w, partials of <w, d>
+__global__ __launch_bounds__(kBatchBlock) void IrRestartKernel(const double* __restrict__ r,
+                                                               const double* __restrict__ inv_rnorm,
+                                                               const float* __restrict__ inv32,
+                                                               float* __restrict__ c32, float* __restrict__ d,
+                                                               float* __restrict__ w, float* __restrict__ s,
+                                                               const int64_t* __restrict__ cstart,
+                                                               const int* __restrict__ cseg,
+                                                               const int* __restrict__ restart,
+                                                               double* __restrict__ partials) {
+  __shared__ double lds[kBatchBlock / 64];
+  const int c = blockIdx.x;
+  const int p = cseg[c];
+  if (!restart[p]) {
+    if (threadIdx.x == 0) partials[c] = 0.0;
+    return;
+  }
+  const double sc = inv_rnorm[p];
+  double acc = 0.0;
+  for (int64_t i = cstart[c] + threadIdx.x; i < cstart[c + 1]; i += kBatchBlock) {
+    const float di = (float)(r[i] * sc);
+    const float wi = inv32[i] * di;
+    c32[i] = 0.0f;
+    d[i] = di;
+    w[i] = wi;
+    s[i] = wi;
+    acc += (double)wi * (double)di;
+  }
+  StorePartial<kBatchBlock>(acc, partials, lds);
+}
+
+// per restarted problem: the inner CG's state and its first check
+__global__ __launch_bounds__(kBatchBlock) void IrRestartCheckKernel(const double* __restrict__ partials,
+                                                                    const int* __restrict__ seg_chunk,
+                                                                    const int* __restrict__ restart,
+                                                                    const double* __restrict__ itol, BatchState S) {
+  __shared__ double lds[kBatchBlock / 64];
+  const int p = blockIdx.x;
+  if (!restart[p]) return;
+  const double wdn = SegmentSum(partials, seg_chunk, p, lds);
+  if (threadIdx.x == 0) {
+    S.it[p] = 0;
+    S.done[p] = 0;
+    S.tol[p] = itol[p];
+    S.finish[p] = 0;
+    S.wdn[p] = wdn;
+    BatchCheck(S, p, sqrt(fabs(wdn)));
+  }
+}
+
+// all problems finished?  one workgroup
+__global__ __launch_bounds__(kBatchBlock) void IrAllFrozenKernel(const int* __restrict__ frozen, int nseg,
+                                                                 int* __restrict__ all) {
+  __shared__ double lds[kBatchBlock / 64];
+  double open = 0.0;
+  for (int p = threadIdx.x; p < nseg; p += kBatchBlock) open += frozen[p] ? 0.0 : 1.0;
+  open = BlockSum<kBatchBlock>(open, lds);
+  if (threadIdx.x == 0) *all = open == 0.0 ? 1 : 0;
+}
+
+struct IrDeviceWork {
+  DevBuf<double> r, fnorm, rnorm, inv_rnorm, itol, relres;
+  DevBuf<float> inv32, c32, d, w, s;
+  // all: every problem finished; inner_all: the inner iterations' own
+  // all-stopped flag (not used by the refinement)
+  DevBuf<int> frozen, pending, skip, restart, outer, inner_sum, all, inner_all;
+  DevBuf<long long> finish;
+  IrDeviceWork(int64_t n, int ns)
+      : r(n), fnorm(ns), rnorm(ns), inv_rnorm(ns), itol(ns), relres(ns), inv32(n), c32(n), d(n), w(n), s(n),
+        frozen(ns), pending(ns), skip(ns), restart(ns), outer(ns), inner_sum(ns), all(1), inner_all(1), finish(ns) {}
+};
+
+// one outer block: the steps of every problem whose inner solve stopped
+static void EnqueueOuterBlock(BatchOp& A, const double* f, double* u, const double* inv, IrDeviceWork& W,
+                              BatchState S, double tol, double inner_tol, int max_outer, hipStream_t st) {
+  const int ns = A.nseg;
+  IrPendingKernel<<<GridFor(ns, 256, 64), 256, 0, st>>>(S, W.frozen.p, W.pending.p, W.skip.p, W.inner_sum.p, ns);
+  NGSAMD_CHECK_LAUNCH();
+  IrCorrectKernel<<<A.nchunks, 256, 0, st>>>(u, W.c32.p, W.r.p, f, A.chunk_start.p, A.chunk_seg.p, W.rnorm.p,
+                                             W.pending.p);
+  NGSAMD_CHECK_LAUNCH();
+  LaunchBatchSpmv<double, double, true, false>(A, A.vals.p, u, W.r.p, -1.0, W.skip.p, st, A.block_nnz);
+  IrSqKernel<<<A.nchunks, kBatchBlock, 0, st>>>(W.r.p, inv, A.chunk_start.p, A.chunk_seg.p, W.pending.p,
+                                                A.partials.p);
+  NGSAMD_CHECK_LAUNCH();
+  IrOuterCheckKernel<<<ns, kBatchBlock, 0, st>>>(A.partials.p, A.seg_chunk.p, W.fnorm.p, tol, inner_tol, max_outer,
+                                                 W.pending.p, W.frozen.p, W.restart.p, W.rnorm.p, W.inv_rnorm.p,
+                                                 W.itol.p, W.relres.p, W.finish.p, W.outer.p);
+  NGSAMD_CHECK_LAUNCH();
+  IrRestartKernel<<<A.nchunks, kBatchBlock, 0, st>>>(W.r.p, W.inv_rnorm.p, W.inv32.p, W.c32.p, W.d.p, W.w.p, W.s.p,
+                                                     A.chunk_start.p, A.chunk_seg.p, W.restart.p, A.partials.p);
+  NGSAMD_CHECK_LAUNCH();
+  IrRestartCheckKernel<<<ns, kBatchBlock, 0, st>>>(A.partials.p, A.seg_chunk.p, W.restart.p, W.itol.p, S);
+  NGSAMD_CHECK_LAUNCH();
+  IrAllFrozenKernel<<<1, kBatchBlock, 0, st>>>(W.frozen.p, ns, W.all.p);
+  NGSAMD_CHECK_LAUNCH();
+}
+
+// Inner FP32 iterations and outer blocks of all problems, replayed from a
+// graph of kIrGraphIters inner iterations with an outer block after every
+// kIrOuterEvery; the host polls the all-finished flag once per replay.
+constexpr int kIrOuterEvery = 8;
+constexpr int kIrGraphIters = 16;
+
+static void RunBatchIrDevice(BatchOp& A, const double* inv, const double* f, double* u, double tol,
+                             double inner_tol, int max_outer, int max_inner, IrDeviceWork& W, BatchWork& B,
+                             hipStream_t st) {
+  const int64_t n = A.height;
+  const int ns = A.nseg;
+  BatchState S = B.state(max_inner);
+  NGSAMD_REQUIRE(ngsamd_vec_f64_to_f32(n, inv, W.inv32.p) == 0, "ir: conversion failed");
+  Fill<int>(W.frozen.p, ns, 0, st);
+  Fill<int>(W.outer.p, ns, 0, st);
+  Fill<int>(W.inner_sum.p, ns, 0, st);
+  Fill<long long>(W.finish.p, ns, 0LL, st);
+  Fill<double>(W.rnorm.p, ns, 0.0, st);
+  Fill<float>(W.c32.p, n, 0.0f, st);
+  Fill<double>(u, n, 0.0, st);
+  // every inner solve "stopped" with zero iterations: the first outer block
+  // forms r = f and starts the first inner solves
+  BatchResetKernel<<<GridFor(ns, 256, 64), 256, 0, st>>>(S, ns, nullptr, 0.0, nullptr);
+  NGSAMD_CHECK_LAUNCH();
+  Fill<int>(S.done, ns, 1, st);
+  // ||f||_free per problem
+  MaskedSqChunkKernel<<<A.nchunks, kBatchBlock, 0, st>>>(f, inv, A.chunk_start.p, A.partials.p);
+  NGSAMD_CHECK_LAUNCH();
+  SegNormKernel<<<ns, kBatchBlock, 0, st>>>(A.partials.p, A.seg_chunk.p, W.fnorm.p);
+  NGSAMD_CHECK_LAUNCH();
+  EnqueueOuterBlock(A, f, u, inv, W, S, tol, inner_tol, max_outer, st);
+
+  auto enqueue_iterations = [&](int count) {
+    for (int k = 0; k < count; k++) {
+      EnqueueBatchIteration<float, float>(A, A.vals32.p, W.inv32.p, W.c32.p, W.d.p, W.w.p, W.s.p, S,
+                                          W.inner_all.p, st);
+      if ((k + 1) % kIrOuterEvery == 0) EnqueueOuterBlock(A, f, u, inv, W, S, tol, inner_tol, max_outer, st);
+    }
+  };
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  {
+    hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+      bool ok = true;
+      try {
+        enqueue_iterations(kIrGraphIters);
+      } catch (...) {
+        ok = false;
+      }
+      e = hipStreamEndCapture(st, &graph);
+      if (!ok && e == hipSuccess) e = hipErrorUnknown;
+      if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    }
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      if (graph) (void)hipGraphDestroy(graph);
+      graph = nullptr;
+      exec = nullptr;
+    }
+  }
+  struct GraphGuard {
+    hipGraph_t g;
+    hipGraphExec_t x;
+    ~GraphGuard() {
+      if (x) (void)hipGraphExecDestroy(x);
+      if (g) (void)hipGraphDestroy(g);
+    }
+  } guard{graph, exec};
+  // every problem finishes after at most max_outer inner solves of at most
+  // max_inner iterations each; an outer block at least every kIrOuterEvery
+  const long long max_replays =
+      ((long long)(max_outer + 1) * (max_inner + kIrOuterEvery)) / kIrGraphIters + 4;
+  int done = 0;
+  for (long long k = 0; k < max_replays; k++) {
+    if (exec) NGSAMD_HIP(hipGraphLaunch(exec, st));
+    else enqueue_iterations(kIrGraphIters);
+    NGSAMD_HIP(hipMemcpyAsync(&done, W.all.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    NGSAMD_HIP(hipStreamSynchronize(st));
+    if (done) break;
+  }
+  NGSAMD_REQUIRE(done, "batch_ir_cg: refinement did not terminate (internal error)");
+}
+
 }  // namespace ngsamd
 
 using namespace ngsamd;
@@ -875,6 +1203,16 @@ extern "C" int ngsamd_batch_info(ngsamd_op_t op, int* nseg, int* block_nnz, int*
     if (nseg) *nseg = A->nseg;
     if (block_nnz) *block_nnz = A->block_nnz;
     if (nblocks) *nblocks = A->nblocks;
+  });
+}
+
+extern "C" int ngsamd_batch_tuning(ngsamd_op_t op, int* h_count, double* h_ms) {
+  return Guard([&] {
+    NGSAMD_REQUIRE(op && op->kind == OpKind::Batch && h_count, "ngsamd_batch_tuning: not a batched operator");
+    auto* A = static_cast<BatchOp*>(op);
+    *h_count = (int)A->tuned_ms.size();
+    if (h_ms)
+      for (size_t k = 0; k < A->tuned_ms.size(); k++) h_ms[k] = A->tuned_ms[k];
   });
 }
 
@@ -940,6 +1278,23 @@ extern "C" int ngsamd_batch_ir_cg_solve(ngsamd_op_t op, ngsamd_op_t pre, const d
     const int64_t n = A.height;
     const int ns = A.nseg;
     const double* inv = static_cast<JacobiOp*>(pre)->invdiag.p;
+    NGSAMD_REQUIRE(max_outer >= 1 && max_inner >= 1, "ngsamd_batch_ir_cg_solve: iteration limits must be >= 1");
+    const char* henv = getenv("NGSAMD_BATCH_IR_HOST");
+    if (!(henv && std::string(henv) == "1")) {
+      // every problem takes its refinement steps on the device
+      IrDeviceWork W(std::max<int64_t>(n, 1), ns);
+      BatchWork B(ns);
+      const long long t0 = StartClock(st);
+      RunBatchIrDevice(A, inv, f, u, tol, inner_tol, max_outer, max_inner, W, B, st);
+      if (h_outer) NGSAMD_HIP(hipMemcpyAsync(h_outer, W.outer.p, ns * sizeof(int), hipMemcpyDeviceToHost, st));
+      if (h_inner) NGSAMD_HIP(hipMemcpyAsync(h_inner, W.inner_sum.p, ns * sizeof(int), hipMemcpyDeviceToHost, st));
+      if (h_relres) NGSAMD_HIP(hipMemcpyAsync(h_relres, W.relres.p, ns * sizeof(double), hipMemcpyDeviceToHost, st));
+      NGSAMD_HIP(hipStreamSynchronize(st));
+      if (h_finish_ms) ReadClock(W.finish.p, t0, ns, h_finish_ms, st);
+      return;
+    }
+    // NGSAMD_BATCH_IR_HOST=1: the outer steps of all problems together, from
+    // the host (every problem finishes at the same outer check)
     DevBuf<float> inv32(std::max<int64_t>(n, 1)), r32(std::max<int64_t>(n, 1)), c32(std::max<int64_t>(n, 1));
     DevBuf<double> r(std::max<int64_t>(n, 1));
     NGSAMD_REQUIRE(ngsamd_vec_f64_to_f32(n, inv, inv32.p) == 0, "ir: conversion failed");

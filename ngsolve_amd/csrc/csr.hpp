@@ -86,4 +86,56 @@ struct BlockJacobiOp : ngsamd_op_s {
   void Apply(double s, const double* x, double* y, bool accumulate, const int* stop, hipStream_t st);
 };
 
+// Sparse Cholesky inverse of the free block (sparsechol.hip): y = A_ff^-1 x
+// on the free dofs, 0 elsewhere.  Factor data in dissection order: the
+// supernode table, the inverses of the diagonal blocks and the
+// off-diagonal blocks (both orientations), per forward level and per
+// backward level the rows it solves.
+// per supernode: first row, size m, boundary size b, offsets of L11^-1
+// (m x m, row-major), of L21 (b x m, row-major), of L21^T (m x b,
+// row-major) and of its boundary list
+struct ScNode {
+  int first, m, b, pad;
+  int64_t inv, l21, l21t, bnd;
+};
+
+// Host result of the factorisation (the device copy's contents, before upload)
+struct ScFactor {
+  int nf = 0, nsuper = 0, nlevels = 0;
+  int64_t nnz_factor = 0;
+  std::vector<ScNode> tab;
+  std::vector<double> lval;
+  std::vector<int64_t> invt;
+  std::vector<int> bnd, seg_ptr, seg_node, seg_row, rows_f, rows_b, fwd_level, bwd_level, sn_of_row, fdof, pos;
+};
+ScFactor FactorSparseCholesky(int64_t n, const int64_t* rp, const int* cols, const double* vals,
+                              const uint8_t* freedofs);
+
+struct SparseCholeskyOp : ngsamd_op_s {
+  int nf = 0, nsuper = 0, nlevels = 0;
+  int64_t nnz_factor = 0;
+  std::vector<int> fwd_level, bwd_level;  // row ranges of each level (host)
+  DevBuf<ScNode> d_nodes;
+  DevBuf<double> d_lval;
+  DevBuf<int64_t> d_invt;
+  DevBuf<int> d_bnd, d_seg_ptr, d_seg_node, d_seg_row, d_rows_f, d_rows_b, d_sn_of_row, d_fdof, d_pos;
+  DevBuf<double> d_b, d_t, d_y;  // work vectors (preallocated: the apply allocates nothing)
+  SparseCholeskyOp(int64_t n, const int64_t* rp, const int* cols, const double* vals, const uint8_t* freedofs);
+  void MultAdd(double s, const double* x, double* y, hipStream_t st) override;
+  void Mult(const double* x, double* y, hipStream_t st) override;
+  // y = (accumulate ? y : 0) + s A_ff^-1 x; every kernel returns at once when *stop != 0
+  void Apply(double s, const double* x, double* y, bool accumulate, const int* stop, hipStream_t st);
+};
+
+// Additive two-level preconditioner y = S x + E C E^T x (sparsechol.hip):
+// S Jacobi or block Jacobi, C a sparse Cholesky inverse of the first
+// C->height dofs.  The operators are not owned.
+struct TwoLevelOp : ngsamd_op_s {
+  ngsamd_op_t smoother, coarse;
+  TwoLevelOp(ngsamd_op_t smoother, ngsamd_op_t coarse);
+  void MultAdd(double s, const double* x, double* y, hipStream_t st) override;
+  void Mult(const double* x, double* y, hipStream_t st) override;
+  void MultStop(const double* x, double* y, const int* stop, hipStream_t st);
+};
+
 }  // namespace ngsamd

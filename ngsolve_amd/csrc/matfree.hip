@@ -133,6 +133,8 @@ __device__ __forceinline__ int RoundClass(const MfRounds& R, int r, int from) {
 
 // MODE 0: contributions (row-major [nel][nd], table row numbering)
 // MODE 1: atomic add s*y_e into y
+// MODE 2 (factored kernels): y += s*y_e with plain loads and stores, for a
+//        launch over one colour (no two of its elements share a dof)
 template <class T, int DIM, int NDP, int MODE, class V>
 __global__ __launch_bounds__(kMfBlock) void MfMfmaKernel(
     const T* __restrict__ bmat, int64_t bstride, MfRounds R, const double* __restrict__ weights, int nqb,
@@ -828,7 +830,11 @@ __global__ __launch_bounds__(kMfBlock) void MfFacKernel(
           if (dof < nd) {
             const V val = (V)yacc[m][r];
             if (MODE == 0) out[e * nd + dof] = val;
-            else AtomicAddV(&out[dofs[e * nd + dof]], (V)(s * (double)val));
+            else if (MODE == 1) AtomicAddV(&out[dofs[e * nd + dof]], (V)(s * (double)val));
+            else {  // colour-ordered: no other element of the launch touches this dof
+              V* yp = &out[dofs[e * nd + dof]];
+              *yp = (V)((double)*yp + s * (double)val);
+            }
           }
         }
     }
@@ -978,7 +984,11 @@ __global__ __launch_bounds__(kMfBlock) void MfFac16Kernel(
           if (dof < nd) {
             const V val = (V)ldexpf(yacc[m][r], escale);
             if (MODE == 0) out[e * nd + dof] = val;
-            else AtomicAddV(&out[dofs[e * nd + dof]], (V)(s * (double)val));
+            else if (MODE == 1) AtomicAddV(&out[dofs[e * nd + dof]], (V)(s * (double)val));
+            else {  // colour-ordered: no other element of the launch touches this dof
+              V* yp = &out[dofs[e * nd + dof]];
+              *yp = (V)((double)*yp + s * (double)val);
+            }
           }
         }
     }
@@ -1602,9 +1612,24 @@ MatFreeOp::MatFreeOp(int dim_, int nd_, int nq_, int64_t ndof, int nclass, const
 
 // Workgroups of one launch: every round's worth if it fits, else as many as
 // are resident at once (occupancy queried when the operator is built).
-static int MfGrid(const MatFreeOp& A, int mode, bool f64_vectors) {
+static int MfGrid(const MatFreeOp& A, int mode, bool f64_vectors, int nrounds = -1) {
   const int per_cu = std::max(1, A.occ[mode][f64_vectors ? 1 : 0]);
-  return (int)std::max<int64_t>(1, std::min<int64_t>(A.nrounds, (int64_t)DeviceCUs() * per_cu));
+  if (nrounds < 0) nrounds = A.nrounds;
+  return (int)std::max<int64_t>(1, std::min<int64_t>(nrounds, (int64_t)DeviceCUs() * per_cu));
+}
+
+// The element tables and round schedule one launch of the factored kernels
+// walks: the operator's own (class-sorted, all elements) or one colour's.
+struct MfSched {
+  const int* round0;
+  const int64_t* range;
+  int nrounds;
+  const int* dofs;
+  const double* metric;
+  const int* elnr;
+};
+static MfSched OwnSched(const MatFreeOp& A) {
+  return {A.round0.p, A.cls_range.p, A.nrounds, A.dofs.p, A.metric.p, A.elnr.p};
 }
 
 // prepare: enable the kernel's dynamic LDS size and record its occupancy
@@ -1719,21 +1744,21 @@ static void DispatchRef16(const MatFreeOp& A, const V* x, V* out, double s, cons
 
 template <class T, int DIM, int NDP, int KG, int MODE, class V>
 static void LaunchFac(const MatFreeOp& A, const T* fm, const V* x, V* out, double s, const int* stop,
-                      hipStream_t st, bool prepare) {
+                      hipStream_t st, bool prepare, const MfSched& S) {
   const size_t lds = (size_t)A.fstride * sizeof(T);
   auto kern = MfFacKernel<T, DIM, NDP, KG, MODE, V>;
   if (prepare) return PrepareKernel(A, kern, lds, MODE, sizeof(V) == 8);
-  const MfRounds R{A.round0.p, A.cls_range.p, A.nrounds, (int)A.classes.size()};
-  kern<<<MfGrid(A, MODE, sizeof(V) == 8), kMfBlock, lds, st>>>(fm, A.fstride, R, A.dofs.p, A.metric.p, A.Kappa(),
-                                                               A.elnr.p, A.nd, x, out, s, stop);
+  const MfRounds R{S.round0, S.range, S.nrounds, (int)A.classes.size()};
+  kern<<<MfGrid(A, MODE, sizeof(V) == 8, S.nrounds), kMfBlock, lds, st>>>(fm, A.fstride, R, S.dofs, S.metric,
+                                                                          A.Kappa(), S.elnr, A.nd, x, out, s, stop);
   NGSAMD_CHECK_LAUNCH();
 }
 
 template <class T, int DIM, int MODE, class V>
 static void DispatchFac(const MatFreeOp& A, const T* fm, const V* x, V* out, double s, const int* stop,
-                        hipStream_t st, bool prepare) {
+                        hipStream_t st, bool prepare, const MfSched& S) {
 #define NGSAMD_FAC(NDP, KG)                                                                   \
-  if (A.ndp == NDP && A.fac_kg == KG) return LaunchFac<T, DIM, NDP, KG, MODE, V>(A, fm, x, out, s, stop, st, prepare);
+  if (A.ndp == NDP && A.fac_kg == KG) return LaunchFac<T, DIM, NDP, KG, MODE, V>(A, fm, x, out, s, stop, st, prepare, S);
   if constexpr (DIM == 2) {
     NGSAMD_FAC(4, 1) NGSAMD_FAC(8, 1) NGSAMD_FAC(12, 2) NGSAMD_FAC(16, 3) NGSAMD_FAC(24, 4)
   } else {
@@ -1745,22 +1770,23 @@ static void DispatchFac(const MatFreeOp& A, const T* fm, const V* x, V* out, dou
 
 template <class H, int DIM, int KS1, int MT, int KG, int MODE, class V>
 static void LaunchFac16(const MatFreeOp& A, const V* x, V* out, double s, const int* stop, hipStream_t st,
-                        bool prepare) {
+                        bool prepare, const MfSched& S) {
   const size_t lds = (size_t)A.fstride * 2;
   auto kern = MfFac16Kernel<H, DIM, KS1, MT, KG, MODE, V>;
   if (prepare) return PrepareKernel(A, kern, lds, MODE, sizeof(V) == 8);
-  const MfRounds R{A.round0.p, A.cls_range.p, A.nrounds, (int)A.classes.size()};
-  kern<<<MfGrid(A, MODE, sizeof(V) == 8), kMfBlock, lds, st>>>(A.fmat16.p, A.fstride, R, A.dofs.p, A.metric.p,
-                                                               A.Kappa(), A.elnr.p, A.nd, x, out, s, stop);
+  const MfRounds R{S.round0, S.range, S.nrounds, (int)A.classes.size()};
+  kern<<<MfGrid(A, MODE, sizeof(V) == 8, S.nrounds), kMfBlock, lds, st>>>(A.fmat16.p, A.fstride, R, S.dofs,
+                                                                          S.metric, A.Kappa(), S.elnr, A.nd, x, out,
+                                                                          s, stop);
   NGSAMD_CHECK_LAUNCH();
 }
 
 template <class H, int DIM, int MODE, class V>
 static void DispatchFac16(const MatFreeOp& A, const V* x, V* out, double s, const int* stop, hipStream_t st,
-                          bool prepare) {
+                          bool prepare, const MfSched& S) {
 #define NGSAMD_FAC16(KS1, MT, KG)                                                 \
   if (A.ks1 == KS1 && A.mt16 == MT && A.fac_kg == KG)                               \
-    return LaunchFac16<H, DIM, KS1, MT, KG, MODE, V>(A, x, out, s, stop, st, prepare);
+    return LaunchFac16<H, DIM, KS1, MT, KG, MODE, V>(A, x, out, s, stop, st, prepare, S);
   if constexpr (DIM == 2) {
     NGSAMD_FAC16(1, 1, 1) NGSAMD_FAC16(1, 1, 2) NGSAMD_FAC16(1, 1, 3) NGSAMD_FAC16(1, 2, 4)
   } else {
@@ -1772,26 +1798,31 @@ static void DispatchFac16(const MatFreeOp& A, const V* x, V* out, double s, cons
 
 // All elements: one launch of the MFMA kernel (every class), or the scalar
 // kernel class by class.
+// The factored kernels over schedule S (MODE 2 only here: the colour
+// launches).
+template <int MODE, class V>
+static void LaunchFacElements(const MatFreeOp& A, const V* x, V* out, double s, const int* stop, hipStream_t st,
+                              bool prepare, const MfSched& S) {
+  if (S.nrounds == 0 && !prepare) return;
+  if (A.precision == NGSAMD_PREC_BF16) {
+    if (A.dim == 2) DispatchFac16<__bf16, 2, MODE, V>(A, x, out, s, stop, st, prepare, S);
+    else DispatchFac16<__bf16, 3, MODE, V>(A, x, out, s, stop, st, prepare, S);
+  } else if (A.precision == NGSAMD_PREC_F16) {
+    if (A.dim == 2) DispatchFac16<_Float16, 2, MODE, V>(A, x, out, s, stop, st, prepare, S);
+    else DispatchFac16<_Float16, 3, MODE, V>(A, x, out, s, stop, st, prepare, S);
+  } else if (A.precision == NGSAMD_PREC_F64) {
+    if (A.dim == 2) DispatchFac<double, 2, MODE, V>(A, A.fmat64.p, x, out, s, stop, st, prepare, S);
+    else DispatchFac<double, 3, MODE, V>(A, A.fmat64.p, x, out, s, stop, st, prepare, S);
+  } else {
+    if (A.dim == 2) DispatchFac<float, 2, MODE, V>(A, A.fmat32.p, x, out, s, stop, st, prepare, S);
+    else DispatchFac<float, 3, MODE, V>(A, A.fmat32.p, x, out, s, stop, st, prepare, S);
+  }
+}
+
 template <int MODE, class V>
 static void LaunchElements(const MatFreeOp& A, const V* x, V* out, double s, const int* stop, hipStream_t st,
                            bool prepare) {
-  if (A.use_mfma && A.fac_kernel) {
-    if (A.nrounds == 0 && !prepare) return;
-    if (A.precision == NGSAMD_PREC_BF16) {
-      if (A.dim == 2) DispatchFac16<__bf16, 2, MODE, V>(A, x, out, s, stop, st, prepare);
-      else DispatchFac16<__bf16, 3, MODE, V>(A, x, out, s, stop, st, prepare);
-    } else if (A.precision == NGSAMD_PREC_F16) {
-      if (A.dim == 2) DispatchFac16<_Float16, 2, MODE, V>(A, x, out, s, stop, st, prepare);
-      else DispatchFac16<_Float16, 3, MODE, V>(A, x, out, s, stop, st, prepare);
-    } else if (A.precision == NGSAMD_PREC_F64) {
-      if (A.dim == 2) DispatchFac<double, 2, MODE, V>(A, A.fmat64.p, x, out, s, stop, st, prepare);
-      else DispatchFac<double, 3, MODE, V>(A, A.fmat64.p, x, out, s, stop, st, prepare);
-    } else {
-      if (A.dim == 2) DispatchFac<float, 2, MODE, V>(A, A.fmat32.p, x, out, s, stop, st, prepare);
-      else DispatchFac<float, 3, MODE, V>(A, A.fmat32.p, x, out, s, stop, st, prepare);
-    }
-    return;
-  }
+  if (A.use_mfma && A.fac_kernel) return LaunchFacElements<MODE, V>(A, x, out, s, stop, st, prepare, OwnSched(A));
   if (A.use_mfma && A.ref_kernel) {
     if (A.nrounds == 0 && !prepare) return;
     if (A.precision == NGSAMD_PREC_BF16) {
@@ -1842,10 +1873,98 @@ static void LaunchElements(const MatFreeOp& A, const V* x, V* out, double s, con
   }
 }
 
+void MatFreeOp::SetColoring(const int* colors) {
+  NGSAMD_REQUIRE(use_mfma && fac_kernel, "ngsamd_mf_set_coloring: the colour scatter needs the factored kernels");
+  const int nclass = (int)classes.size();
+  int nc = 0;
+  for (int64_t e = 0; e < nel; e++) {
+    NGSAMD_REQUIRE(colors[e] >= 0 && colors[e] < (1 << 20), "ngsamd_mf_set_coloring: colour out of range");
+    nc = std::max(nc, colors[e] + 1);
+  }
+  std::vector<int> hd((size_t)nel * nd), he(nel);
+  std::vector<double> hm((size_t)nel * ncomp);
+  NGSAMD_HIP(hipMemcpy(hd.data(), dofs.p, sizeof(int) * hd.size(), hipMemcpyDeviceToHost));
+  NGSAMD_HIP(hipMemcpy(he.data(), elnr.p, sizeof(int) * he.size(), hipMemcpyDeviceToHost));
+  NGSAMD_HIP(hipMemcpy(hm.data(), metric.p, sizeof(double) * hm.size(), hipMemcpyDeviceToHost));
+  // rows by colour, row order kept (so each colour's rows stay class-sorted)
+  std::vector<int64_t> cptr(nc + 1, 0);
+  for (int64_t e = 0; e < nel; e++) cptr[colors[e] + 1]++;
+  for (int c = 0; c < nc; c++) cptr[c + 1] += cptr[c];
+  std::vector<int64_t> order(nel);
+  {
+    std::vector<int64_t> fill(cptr.begin(), cptr.end() - 1);
+    for (int64_t e = 0; e < nel; e++) order[fill[colors[e]]++] = e;
+  }
+  // no two elements of one colour share a dof
+  std::vector<int> stamp(height, -1);
+  for (int c = 0; c < nc; c++)
+    for (int64_t k = cptr[c]; k < cptr[c + 1]; k++)
+      for (int i = 0; i < nd; i++) {
+        const int d = hd[(size_t)order[k] * nd + i];
+        NGSAMD_REQUIRE(stamp[d] != c, "ngsamd_mf_set_coloring: two elements of one colour share a dof");
+        stamp[d] = c;
+      }
+  std::vector<int> cd((size_t)nel * nd), ce(nel);
+  std::vector<double> cm((size_t)nel * ncomp);
+  for (int64_t k = 0; k < nel; k++) {
+    const int64_t e = order[k];
+    std::copy(&hd[(size_t)e * nd], &hd[(size_t)e * nd] + nd, &cd[(size_t)k * nd]);
+    std::copy(&hm[(size_t)e * ncomp], &hm[(size_t)e * ncomp] + ncomp, &cm[(size_t)k * ncomp]);
+    ce[k] = he[e];
+  }
+  // per colour: class segments and rounds (a round never straddles a class)
+  std::vector<int> r0((size_t)nc * (nclass + 1), 0);
+  std::vector<int64_t> rng((size_t)nc * 2 * nclass, 0);
+  c_nrounds.assign(nc, 0);
+  for (int c = 0; c < nc; c++) {
+    int64_t k = cptr[c];
+    int* rc = &r0[(size_t)c * (nclass + 1)];
+    for (int cl = 0; cl < nclass; cl++) {
+      const int64_t b = k;
+      while (k < cptr[c + 1] && order[k] < classes[cl].end) k++;
+      rng[(size_t)c * 2 * nclass + 2 * cl] = b;
+      rng[(size_t)c * 2 * nclass + 2 * cl + 1] = k;
+      const int64_t tiles = (k - b + 15) / 16;
+      rc[cl + 1] = rc[cl] + (int)((tiles + kMfWaves - 1) / kMfWaves);
+    }
+    c_nrounds[c] = rc[nclass];
+  }
+  c_round0.alloc(r0.size());
+  c_round0.upload(r0.data(), r0.size());
+  c_range.alloc(std::max<size_t>(rng.size(), 1));
+  if (!rng.empty()) c_range.upload(rng.data(), rng.size());
+  c_dofs.alloc(std::max<size_t>(cd.size(), 1));
+  if (!cd.empty()) c_dofs.upload(cd.data(), cd.size());
+  c_metric.alloc(std::max<size_t>(cm.size(), 1));
+  if (!cm.empty()) c_metric.upload(cm.data(), cm.size());
+  c_elnr.alloc(std::max<size_t>(ce.size(), 1));
+  if (!ce.empty()) c_elnr.upload(ce.data(), ce.size());
+  NGSAMD_HIP(hipStreamSynchronize(Stream()));
+  ncolors = nc;
+  scatter = NGSAMD_SCATTER_COLOR;
+  const MfSched S0 = OwnSched(*this);
+  LaunchFacElements<2, double>(*this, nullptr, nullptr, 1.0, nullptr, nullptr, true, S0);
+  LaunchFacElements<2, float>(*this, nullptr, nullptr, 1.0, nullptr, nullptr, true, S0);
+}
+
 template <class V>
 int MatFreeOp::Apply(double s, const V* x, V* y, bool accumulate, const int* stop, hipStream_t st,
                      double* partials, double* wide) {
   if (height == 0) return 0;
+  if (scatter == NGSAMD_SCATTER_COLOR) {
+    // colour after colour, each element's result added straight into y
+    if (!accumulate) {
+      ScaleFillKernel<V><<<GridFor(height, 256, 8192), 256, 0, st>>>(y, height, stop);
+      NGSAMD_CHECK_LAUNCH();
+    }
+    const int nclass = (int)classes.size();
+    for (int c = 0; c < ncolors; c++) {
+      const MfSched S{c_round0.p + (size_t)c * (nclass + 1), c_range.p + (size_t)c * 2 * nclass, c_nrounds[c],
+                      c_dofs.p, c_metric.p, c_elnr.p};
+      LaunchFacElements<2, V>(*this, x, y, s, stop, st, false, S);
+    }
+    return 0;  // no gather kernel: the caller forms <x, A x> itself
+  }
   if (scatter == NGSAMD_SCATTER_ATOMIC) {
     if (!accumulate) {
       ScaleFillKernel<V><<<GridFor(height, 256, 8192), 256, 0, st>>>(y, height, stop);
@@ -2271,6 +2390,16 @@ extern "C" int ngsamd_mf_num_elements(ngsamd_op_t op, int64_t* ne) {
     NGSAMD_REQUIRE(op && op->kind == OpKind::MatrixFree, "ngsamd_mf_num_elements: not a matrix-free operator");
     NGSAMD_REQUIRE(ne, "ngsamd_mf_num_elements: null argument");
     *ne = static_cast<MatFreeOp*>(op)->ne_global;
+  });
+}
+
+extern "C" int ngsamd_mf_set_coloring(ngsamd_op_t op, const int* h_colors) {
+  return Guard([&] {
+    RequireInit();
+    NGSAMD_REQUIRE(op && op->kind == OpKind::MatrixFree, "ngsamd_mf_set_coloring: not a matrix-free operator");
+    MatFreeOp* M = static_cast<MatFreeOp*>(op);
+    NGSAMD_REQUIRE(h_colors || M->nel == 0, "ngsamd_mf_set_coloring: null argument");
+    M->SetColoring(h_colors);
   });
 }
 

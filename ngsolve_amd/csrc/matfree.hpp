@@ -25,7 +25,7 @@ struct MatFreeOp : ngsamd_op_s {
   DevBuf<int> round0;         // first round of each class, nclass + 1
   DevBuf<int64_t> cls_range;  // [begin, end) of each class
   int nrounds = 0;
-  mutable int occ[2][2] = {{1, 1}, {1, 1}};  // resident workgroups per CU [mode][FP64 vectors]
+  mutable int occ[3][2] = {{1, 1}, {1, 1}, {1, 1}};  // resident workgroups per CU [mode][FP64 vectors]
   // per class reference gradients, MFMA layout [nqp][dim][ndp] (+1 pad per q),
   // in the contraction type (double / float / bf16 bits)
   DevBuf<double> bmat64;
@@ -62,6 +62,19 @@ struct MatFreeOp : ngsamd_op_s {
   DevBuf<int> g_rowptr, g_slots, g_order;
   DevBuf<double> contrib64;
   DevBuf<float> contrib32;
+  // colour-ordered direct scatter (NGSAMD_SCATTER_COLOR, set by
+  // SetColoring): copies of the element tables sorted by colour (class order
+  // kept inside a colour) and per colour its round schedule; one launch per
+  // colour adds the element results straight into y
+  int ncolors = 0;
+  std::vector<int> c_nrounds;  // per colour
+  DevBuf<int> c_round0;        // [ncolors][nclass + 1]
+  DevBuf<int64_t> c_range;     // [ncolors][2 nclass]
+  DevBuf<int> c_dofs, c_elnr;
+  DevBuf<double> c_metric;
+  // colour of every table row; refuses a colouring whose elements of one
+  // colour share a dof
+  void SetColoring(const int* colors);
 
   MatFreeOp(int dim, int nd, int nq, int64_t ndof, int nclass, const int64_t* class_offsets, const double* bmat,
             const double* w, const int* dofs, const double* metric, const double* kappa, const int* elnr,

@@ -85,4 +85,40 @@ inline int64_t BuildRuns(int64_t h, const int64_t* rp, const int* c, const doubl
 }
 
 
+// Regroups existing runs into blocks of equal run length with at most
+// max_vals values and max_vals rows (a run longer than max_vals alone), and
+// never across a problem boundary (run_seg: problem of each run, or empty).
+// The grouping rule of BuildRuns, for another block size on the same runs.
+inline void GroupRunBlocks(const std::vector<int>& run_row, const std::vector<int>& val_ptr,
+                           const std::vector<int>& run_seg, int max_vals, std::vector<int>& blk_run,
+                           std::vector<int>& blk_len, std::vector<int>* blk_seg) {
+  blk_run.assign(1, 0);
+  blk_len.clear();
+  if (blk_seg) blk_seg->clear();
+  const int nruns = (int)run_row.size() - 1;
+  int64_t acc = 0, rows = 0;
+  int cur_len = 0, cur_seg = 0;
+  for (int u = 0; u < nruns; u++) {
+    const int R = run_row[u + 1] - run_row[u];
+    const int64_t nv = val_ptr[u + 1] - val_ptr[u];
+    const int p = run_seg.empty() ? 0 : run_seg[u];
+    if (u > blk_run.back() && (R != cur_len || acc + nv > max_vals || rows + R > max_vals || p != cur_seg)) {
+      blk_len.push_back(cur_len);
+      blk_run.push_back(u);
+      if (blk_seg) blk_seg->push_back(cur_seg);
+      acc = 0;
+      rows = 0;
+    }
+    cur_seg = p;
+    cur_len = R;
+    acc += nv;
+    rows += R;
+  }
+  if (nruns > blk_run.back()) {
+    blk_len.push_back(cur_len);
+    blk_run.push_back(nruns);
+    if (blk_seg) blk_seg->push_back(cur_seg);
+  }
+}
+
 }  // namespace ngsamd

@@ -572,4 +572,28 @@ void ExtractDiagonal(const CSRPattern& pat, const double* values, double* diag) 
   }
 }
 
+std::vector<int> ElementColoring(const Mesh& m) {
+  const int ne = m.NE(), nve = m.NVE();
+  std::vector<int> col(ne, -1);
+  std::vector<uint32_t> mask(m.nv);
+  int base = 0, found = 0;
+  while (found < ne) {
+    std::fill(mask.begin(), mask.end(), 0u);
+    for (int e = 0; e < ne; e++) {
+      if (col[e] >= 0) continue;
+      const int* v = &m.els[(size_t)e * nve];
+      uint32_t check = 0;
+      for (int k = 0; k < nve; k++) check |= mask[v[k]];
+      if (check == 0xFFFFFFFFu) continue;
+      int c = 0;
+      while (check & (1u << c)) c++;
+      col[e] = base + c;
+      found++;
+      for (int k = 0; k < nve; k++) mask[v[k]] |= 1u << c;
+    }
+    base += 32;
+  }
+  return col;
+}
+
 }  // namespace ngsamd

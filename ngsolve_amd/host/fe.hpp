@@ -76,6 +76,13 @@ struct H1Space {
   H1Space(const Mesh& m, int order, uint64_t dirichlet_mask);
 };
 
+// Element colouring of FESpace::FinalizeUpdate (comp/fespace.cpp:518-760)
+// for an H1 space (the colouring of its P1 space: elements of one colour
+// share no vertex, hence no dof): rounds of 32 colours, each uncoloured
+// element takes the lowest colour of the round free at all its vertices
+// (32-bit masks), elements in order.  Returns the colour of every element.
+std::vector<int> ElementColoring(const Mesh& m);
+
 // Per-class reference data of the fused operator (comp/matrixfree.cpp:1159-1211).
 struct ElementClass {
   int classnr = 0;

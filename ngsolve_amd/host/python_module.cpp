@@ -59,6 +59,9 @@ struct Operator {
   // matrix-free operators keep the pointer of an external coefficient, not a
   // copy: the vector is held here for as long as the operator may read it
   std::shared_ptr<DeviceVector> kappa_keep;
+  // operators a composite uses (not owned by its handle): released after
+  // the composite's handle is destroyed in ~Operator
+  std::vector<std::shared_ptr<Operator>> parts;
   Operator(ngsamd_op_t o, std::string k) : op(o), kind(std::move(k)) {
     Check(ngsamd_op_height(op, &height));
     Check(ngsamd_op_width(op, &width));
@@ -211,6 +214,7 @@ PYBIND11_MODULE(_ngsamd, m) {
     return out;
   });
 
+  m.def("ElementColoring", [](std::shared_ptr<Mesh> mesh) { return ToNumpy(ElementColoring(*mesh)); });
   m.def("BuildPattern", [](std::shared_ptr<H1Space> fes) {
     CSRPattern p;
     {
@@ -421,6 +425,11 @@ PYBIND11_MODULE(_ngsamd, m) {
              else throw std::runtime_error("diagonal: not available for this operator");
            })
       .def_readonly("ne", &Operator::ne)
+      .def("set_coloring",
+           [](Operator& A, Arr<int> colors) {
+             if (A.kind != "matfree") throw std::runtime_error("set_coloring: not a matrix-free operator");
+             Check(ngsamd_mf_set_coloring(A.op, colors.data()));
+           })
       .def("set_kappa",
            [](Operator& A, std::shared_ptr<DeviceVector> k) {
              if (A.kind != "matfree") throw std::runtime_error("set_kappa: not a matrix-free operator");
@@ -570,6 +579,41 @@ PYBIND11_MODULE(_ngsamd, m) {
                                     block_dofs.data()));
     return std::make_shared<Operator>(op, "blockjacobi");
   });
+  m.def("CreateSparseCholesky", [](int64_t n, Arr<int64_t> rowptr, Arr<int> cols, Arr<double> vals,
+                                   py::object freedofs) {
+    if (rowptr.size() != n + 1 || cols.size() != vals.size() || rowptr.data()[n] != (int64_t)cols.size())
+      throw std::runtime_error("CreateSparseCholesky: inconsistent CSR arrays");
+    std::vector<uint8_t> fr;
+    if (!freedofs.is_none()) {
+      auto a = freedofs.cast<Arr<uint8_t>>();
+      if ((int64_t)a.size() != n) throw std::runtime_error("CreateSparseCholesky: freedofs size");
+      fr.assign(a.data(), a.data() + a.size());
+    }
+    ngsamd_op_t op = nullptr;
+    {
+      py::gil_scoped_release rel;
+      Check(ngsamd_sparsecholesky_create(&op, n, rowptr.data(), cols.data(), vals.data(),
+                                         fr.empty() ? nullptr : fr.data()));
+    }
+    return std::make_shared<Operator>(op, "sparsecholesky");
+  });
+  m.def("SparseCholeskyInfo", [](Operator& A) {
+    int64_t info[4] = {0, 0, 0, 0};
+    Check(ngsamd_sparsecholesky_info(A.op, info));
+    py::dict d;
+    d["nfree"] = info[0];
+    d["supernodes"] = info[1];
+    d["levels"] = info[2];
+    d["nnz_factor"] = info[3];
+    return d;
+  });
+  m.def("CreateTwoLevel", [](std::shared_ptr<Operator> smoother, std::shared_ptr<Operator> coarse) {
+    ngsamd_op_t op = nullptr;
+    Check(ngsamd_twolevel_create(&op, smoother->op, coarse->op));
+    auto r = std::make_shared<Operator>(op, "twolevel");
+    r->parts = {smoother, coarse};
+    return r;
+  });
   m.def("CreateJacobiDevice", [](DeviceVector& diag, py::object freedofs) {
     ngsamd_op_t op = nullptr;
     std::vector<uint8_t> fr;
@@ -667,6 +711,15 @@ PYBIND11_MODULE(_ngsamd, m) {
     int ns = 0, bn = 0, nb = 0;
     Check(ngsamd_batch_info(A.op, &ns, &bn, &nb));
     return py::make_tuple(ns, bn, nb);
+  });
+  m.def("BatchTuning", [](Operator& A) {
+    int cnt = 0;
+    double ms[3] = {0, 0, 0};
+    Check(ngsamd_batch_tuning(A.op, &cnt, ms));
+    py::dict d;
+    const int cand[3] = {1024, 2048, 4096};
+    for (int k = 0; k < cnt && k < 3; k++) d[py::int_(cand[k])] = ms[k];
+    return d;
   });
   m.def("BatchCGSolve",
         [](Operator& A, Operator& pre, DeviceVector& f, DeviceVector& u, double tol, int maxiter, int chunk) {

@@ -14,7 +14,7 @@ IRCGSolver    mixed-precision iterative refinement: FP64 residual f - A u
 """
 from math import log, sqrt
 
-from .la import BaseMatrix, BlockJacobiPrecond, DeviceMatrix, JacobiPrecond, Projector
+from .la import BaseMatrix, BlockJacobiPrecond, DeviceMatrix, JacobiPrecond, Projector, SparseCholesky, TwoLevelPrecond
 
 
 class LinearSolver(BaseMatrix):
@@ -148,8 +148,10 @@ class DevCGSolver(LinearSolver):
         if maxsteps is not None:
             maxiter = maxsteps
         super().__init__(mat, pre=pre, tol=tol, maxiter=maxiter, atol=atol, printrates=printrates)
-        if not isinstance(mat, DeviceMatrix) or not isinstance(pre, (JacobiPrecond, BlockJacobiPrecond)):
-            raise TypeError("DevCGSolver needs a device matrix and a JacobiPrecond or BlockJacobiPrecond")
+        if not isinstance(mat, DeviceMatrix) or not isinstance(pre, (JacobiPrecond, BlockJacobiPrecond,
+                                                                     TwoLevelPrecond, SparseCholesky)):
+            raise TypeError("DevCGSolver needs a device matrix and a JacobiPrecond, BlockJacobiPrecond, "
+                            "TwoLevelPrecond or SparseCholesky preconditioner")
         self.chunk = chunk
 
     def _SolveImpl(self, rhs, sol):

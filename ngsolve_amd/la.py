@@ -602,6 +602,14 @@ class DevSparseMatrix(DeviceMatrix):
         """BaseSparseMatrix::CreateBlockJacobiPrecond (Python CreateBlockSmoother)."""
         return BlockJacobiPrecond(self, blocks)
 
+    def Inverse(self, freedofs=None, inverse="sparsecholesky"):
+        """The inverse of the free block (a.mat.Inverse(freedofs,
+        inverse="sparsecholesky"), linalg/sparsecholesky.cpp): factorised on
+        the host, solved on the device."""
+        if inverse not in ("sparsecholesky", None, ""):
+            raise ValueError(f"inverse type {inverse!r}: only 'sparsecholesky' is available on the device")
+        return SparseCholesky(self, freedofs)
+
 
 class JacobiPrecond(DeviceMatrix):
     """JacobiPrecond (linalg/jacobi.cpp:41-90): y += s * diag^{-1} x on free dofs."""
@@ -662,6 +670,50 @@ class BlockJacobiPrecond(DeviceMatrix):
                                  np.ascontiguousarray(mat.vals, dtype=np.float64), bptr, bd)
         super().__init__(op, True)
         self.blocks = blocks
+
+
+class SparseCholesky(DeviceMatrix):
+    """Sparse Cholesky inverse of the free block of a symmetric positive
+    definite device matrix (SparseCholesky, linalg/sparsecholesky.cpp; device
+    solve DevSparseCholesky, ngscuda/dev_sparsecholesky.cpp:403-454):
+    y = A_ff^-1 x on the free dofs, 0 elsewhere."""
+
+    def __init__(self, mat, freedofs=None):
+        N = ensure_device()
+        fr = None if freedofs is None else np.ascontiguousarray(_bits(freedofs), dtype=np.uint8)
+        op = N.CreateSparseCholesky(int(mat.Height()), np.ascontiguousarray(mat.rowptr, dtype=np.int64),
+                                    np.ascontiguousarray(mat.cols, dtype=np.int32),
+                                    np.ascontiguousarray(mat.vals, dtype=np.float64), fr)
+        super().__init__(op, True)
+        self.info = N.SparseCholeskyInfo(op)
+
+
+class TwoLevelPrecond(DeviceMatrix):
+    """Additive two-level preconditioner y = S x + E C E^T x: S a Jacobi or
+    block-Jacobi preconditioner, C a SparseCholesky acting on the first
+    C.Height() dofs (E their embedding: the vertex dofs of a hierarchical H1
+    space, i.e. its P1 part).  The device form of the composite
+    smoother + E @ a_lo.mat.Inverse(freedofs_lo) @ E.T (SumMatrix /
+    ProductMatrix, linalg/basematrix.hpp:413,592); DevCGSolver accepts it."""
+
+    def __init__(self, smoother, coarse):
+        N = ensure_device()
+        if not isinstance(smoother, (JacobiPrecond, BlockJacobiPrecond)) or not isinstance(coarse, SparseCholesky):
+            raise TypeError("TwoLevelPrecond needs a Jacobi or block-Jacobi smoother and a SparseCholesky")
+        super().__init__(N.CreateTwoLevel(smoother.op, coarse.op), True)
+        self.smoother, self.coarse = smoother, coarse
+
+
+def LowOrderBlock(mat, nlow):
+    """The leading nlow x nlow block of a device CSR matrix (for a
+    hierarchical H1 space with nlow = number of vertices: the P1 matrix of the
+    same form), as a DevSparseMatrix."""
+    rp, cols, vals = np.asarray(mat.rowptr), np.asarray(mat.cols), np.asarray(mat.vals)
+    keep = cols[:rp[nlow]] < nlow
+    rows = np.repeat(np.arange(nlow), np.diff(rp[:nlow + 1]))[keep]
+    brp = np.zeros(nlow + 1, dtype=np.int64)
+    np.add.at(brp, rows + 1, 1)
+    return DevSparseMatrix(nlow, nlow, np.cumsum(brp), cols[:rp[nlow]][keep], vals[:rp[nlow]][keep])
 
 
 class Projector(BaseMatrix):

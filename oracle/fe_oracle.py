@@ -5,6 +5,7 @@ This module restates, in numpy, the reference's algorithms for:
   affine element map       comp/meshaccess.cpp:541-562
   H1 dof numbering         comp/h1hofespace.cpp:823-950, :1389-1413
   free dofs                comp/fespace.cpp:765-810
+  element colouring        comp/fespace.cpp:518-760 (sequential order)
   shape functions          fem/h1hofe_impl.hpp:154-185, :510-570 with the
                            polynomials of fem/recursive_pol.hpp:1344-1373,
                            1641-1676, 1857-1940, 2820-2861, 2974-3005
@@ -18,6 +19,8 @@ This module restates, in numpy, the reference's algorithms for:
   Jacobi-PCG               python/krylovspace.py:263-291 (Jacobi or any
                            preconditioner)
   block Jacobi             linalg/blockjacobi.cpp:390, 404, 498, 596-632
+  sparse direct inverse    linalg/sparsecholesky.cpp (a.mat.Inverse on free dofs)
+  two-level preconditioner smoother + E C E^T (linalg/basematrix.hpp:413,592)
   lockstep multi-RHS CG    linalg/cg.cpp:132-250 (CGSolver::MultiMult)
 
 It is deliberately written differently from the product code: shapes are
@@ -606,6 +609,74 @@ def block_jacobi(rowptr, cols, vals, blocks):
         return y
 
     return invs, apply
+
+
+def element_coloring(els, nv):
+    """Element colouring of FESpace::FinalizeUpdate (comp/fespace.cpp:518-760)
+    for an H1 space, which takes its low-order (P1, vertex-dof) space's
+    colouring: in rounds of 32 colours, each element not yet coloured gets
+    the lowest colour of the round that no element of the round sharing one
+    of its vertices has (a 32-bit mask per vertex), elements in order.  This
+    is the sequential order of that loop (the reference runs it over threads,
+    so its colours vary with the schedule; any such colouring is valid).
+    Returns the colour of every element."""
+    els = np.asarray(els, dtype=np.int64)
+    col = np.full(len(els), -1, dtype=np.int64)
+    mask = np.zeros(nv, dtype=np.int64)
+    base, found = 0, 0
+    while found < len(els):
+        mask[:] = 0
+        for e in range(len(els)):
+            if col[e] >= 0:
+                continue
+            check = 0
+            for v in els[e]:
+                check |= int(mask[v])
+            if check != 0xFFFFFFFF:
+                c = 0
+                while (check >> c) & 1:
+                    c += 1
+                col[e] = base + c
+                found += 1
+                for v in els[e]:
+                    mask[v] |= 1 << c
+        base += 32
+    return col
+
+
+def sparse_inverse(rowptr, cols, vals, free):
+    """SparseCholesky behind a.mat.Inverse(freedofs, "sparsecholesky")
+    (linalg/sparsecholesky.cpp): apply(x) = A_ff^-1 x on the free dofs, 0 on
+    the others.  The oracle factorises A_ff with scipy's SuperLU (any exact
+    factorisation gives the same inverse up to rounding); returns apply."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    rowptr = np.asarray(rowptr, dtype=np.int64)
+    n = len(rowptr) - 1
+    A = sp.csr_matrix((np.asarray(vals), np.asarray(cols), rowptr), shape=(n, n))
+    f = np.flatnonzero(np.asarray(free) != 0)
+    lu = spla.splu(A[f][:, f].tocsc())
+
+    def apply(x):
+        y = np.zeros(n)
+        if len(f):
+            y[f] = lu.solve(np.asarray(x, dtype=float)[f])
+        return y
+
+    return apply
+
+
+def two_level(smoother, coarse, nc):
+    """Additive two-level preconditioner y = S x + E C E^T x with E the
+    embedding of the first nc dofs: the composite smoother + E @ C @ E.T of
+    the reference's SumMatrix / ProductMatrix (linalg/basematrix.hpp:413,592).
+    smoother, coarse: callables (full space, first nc dofs)."""
+    def apply(x):
+        y = np.array(smoother(x), dtype=float)
+        y[:nc] += coarse(np.asarray(x)[:nc])
+        return y
+
+    return apply
 
 
 def pcg(matvec, invdiag, f, tol=1e-12, maxiter=100, atol=None):

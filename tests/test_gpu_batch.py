@@ -57,3 +57,53 @@ def test_batch_rejects_coupled_problems(device):
     cols = np.array([0, 1, 1], dtype=np.int32)   # row 0 couples to problem 1
     with pytest.raises(RuntimeError):
         N.CreateBatchCSR(rp, cols, np.ones(3), np.array([0, 1, 2], dtype=np.int64), False)
+
+
+def test_batch_mixed_device_outer_steps(device, monkeypatch):
+    """Each problem takes its refinement steps on the device as soon as its
+    inner solve stops: the same residual bound as the host-driven outer loop
+    (NGSAMD_BATCH_IR_HOST=1, every problem finishing at the last outer
+    check), and the problems finish at different times."""
+    from ngsolve_amd.batch import BatchSolver
+    probs = _problems()
+    out = {}
+    for mode in ("device", "host"):
+        if mode == "host":
+            monkeypatch.setenv("NGSAMD_BATCH_IR_HOST", "1")
+        else:
+            monkeypatch.delenv("NGSAMD_BATCH_IR_HOST", raising=False)
+        bs = BatchSolver(probs)
+        r = bs.solve_mixed(tol=1e-10)
+        out[mode] = r
+        for k, pr in enumerate(probs):
+            u = bs.solution(k)
+            res = pr.rhs - oracle.spmv(pr.rowptr, pr.cols, pr.vals, u)
+            fr = pr.free.astype(bool)
+            assert np.linalg.norm(res[fr]) <= 1.0001e-10 * np.linalg.norm(pr.rhs[fr]), (mode, k)
+        say(f"[batch] {mode} outer steps: outer={r['outer'].tolist()} finish ms "
+            f"{np.min(r['finish_ms']):.2f}..{np.max(r['finish_ms']):.2f}")
+    ms = np.asarray(out["device"]["finish_ms"])
+    assert (ms >= 0).all() and ms.min() < ms.max()
+    assert (out["device"]["outer"] >= 1).all() and (out["device"]["inner"] >= 1).all()
+
+
+def test_batch_autotune_runs_layout(device, monkeypatch):
+    """The runs layout's block size is autotuned over 1024 / 2048 / 4096
+    values per block (all three timed, the fastest kept); NGSAMD_SPMV_NNZ
+    forces one.  The CG result is the same for every block size."""
+    from ngsolve_amd import _ngsamd as N
+    from ngsolve_amd.batch import BatchSolver
+    probs = _problems()
+    monkeypatch.delenv("NGSAMD_SPMV_NNZ", raising=False)
+    bs = BatchSolver(probs)
+    tune = N.BatchTuning(bs.op)
+    say(f"[batch] autotune {tune} -> {bs.block_nnz}")
+    assert sorted(tune) == [1024, 2048, 4096] and bs.block_nnz in tune
+    assert bs.block_nnz == min(tune, key=tune.get)
+    its = {}
+    for bn in ("1024", "4096"):
+        monkeypatch.setenv("NGSAMD_SPMV_NNZ", bn)
+        b2 = BatchSolver(probs)
+        assert b2.block_nnz == int(bn) and N.BatchTuning(b2.op) == {}
+        its[bn] = b2.solve_fp64(tol=1e-10)["iterations"]
+    assert np.array_equal(its["1024"], its["4096"])

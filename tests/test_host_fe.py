@@ -112,3 +112,18 @@ def test_vol_reader_matches_oracle(name):
     fes = N.H1Space(m, 3, ALL)
     pr = O.Problem(om, 3)
     assert np.array_equal(fes.el_dofs, pr.dofs) and np.array_equal(fes.freedofs, pr.free)
+
+
+@pytest.mark.parametrize("dim,n", [(2, 1), (2, 7), (3, 1), (3, 5)])
+def test_element_coloring_matches_oracle(dim, n):
+    """FESpace element colouring (comp/fespace.cpp:518-760): the host C++
+    colouring equals the oracle's, bit for bit, and is valid (elements of one
+    colour share no vertex, hence no H1 dof)."""
+    m, om = _meshes(dim, n)
+    col = np.asarray(N.ElementColoring(m))
+    ocol = O.element_coloring(om["els"], int(om["els"].max()) + 1)
+    assert np.array_equal(col, ocol)
+    els = np.asarray(om["els"])
+    for c in np.unique(col):
+        v = els[col == c].ravel()
+        assert len(np.unique(v)) == len(v)

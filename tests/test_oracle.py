@@ -284,6 +284,73 @@ def test_block_jacobi_special_cases():
     assert its == 2 and np.allclose((A @ u)[fd], f[fd], atol=1e-12)
 
 
+def test_sparse_inverse_and_two_level():
+    """The oracle's sparse inverse solves A_ff y = x and is 0 off the free
+    dofs, equal to the dense inverse of the free block (block Jacobi with one
+    block); the leading vertex block of a hierarchical p=3 matrix is the P1
+    matrix of the same form (vertex shape functions are the barycentric
+    coordinates), so the two-level preconditioner's coarse space is P1; with
+    a zero smoother and the whole space as coarse space the two-level
+    preconditioner is the exact inverse (PCG stops after one step)."""
+    rng = np.random.default_rng(3)
+    pr = O.Problem(O.structured_2d(6), 3)
+    kappa = np.exp(rng.standard_normal(pr.mesh["els"].shape[0]))
+    rp, c, v = pr.assemble(kappa=kappa)
+    n = pr.ndof
+    free = np.array(pr.free, dtype=np.uint8)
+    fd = np.flatnonzero(free)
+    A = sp.csr_matrix((v, c, rp), shape=(n, n))
+    inv = O.sparse_inverse(rp, c, v, free)
+    x = rng.uniform(-1, 1, n)
+    y = inv(x)
+    assert np.all(y[free == 0] == 0)
+    assert np.abs((A @ y)[fd] - x[fd]).max() <= 1e-12 * np.abs(x).max()
+    _, dense = O.block_jacobi(rp, c, v, [fd])
+    assert np.allclose(y, dense(x), rtol=0, atol=1e-12 * np.abs(y).max())
+    # leading vertex block = P1 matrix
+    p1 = O.Problem(O.structured_2d(6), 1)
+    rp1, c1, v1 = p1.assemble(kappa=kappa)
+    nv = p1.ndof
+    A1 = sp.csr_matrix((v1, c1, rp1), shape=(nv, nv))
+    assert abs(A[:nv][:, :nv] - A1).max() <= 1e-13 * abs(A1).max()
+    # two-level with the whole space as coarse space and no smoother: exact
+    tl = O.two_level(lambda t: np.zeros_like(t), inv, n)
+    f = np.where(free != 0, 1.0, 0.0)
+    u, its, _ = O.pcg(lambda t: A @ t, tl, f, tol=1e-12, maxiter=10)
+    assert its == 2 and np.allclose((A @ u)[fd], f[fd], atol=1e-12)
+    # Jacobi + P1 coarse: the coarse part acts on the first nv entries only
+    _, jinv = O.jacobi_invdiag(rp, c, v, free)
+    cinv = O.sparse_inverse(rp1, c1, v1, free[:nv])
+    tl2 = O.two_level(lambda t: jinv * t, cinv, nv)
+    z = tl2(x)
+    assert np.allclose(z[nv:], jinv[nv:] * x[nv:], rtol=0, atol=0)
+    assert np.allclose(z[:nv], jinv[:nv] * x[:nv] + cinv(x[:nv]), rtol=1e-14, atol=0)
+
+
+def test_element_coloring_known_answers_and_first_fit():
+    """Element colouring (comp/fespace.cpp:518-760): the 2 triangles of one
+    square share an edge (colours 0, 1); the 6 Kuhn tetrahedra of one cube
+    share the main diagonal (colours 0..5).  On larger meshes (fewer than 32
+    colours) every element has the smallest colour that no earlier element
+    sharing a vertex has (sequential first fit, checked against vertex
+    neighbour sets), and no colour class shares a vertex."""
+    assert O.element_coloring(O.structured_2d(1)["els"], 4).tolist() == [0, 1]
+    assert O.element_coloring(O.structured_3d(1)["els"], 8).tolist() == [0, 1, 2, 3, 4, 5]
+    for mesh in (O.structured_2d(6), O.structured_3d(3)):
+        els = np.asarray(mesh["els"])
+        col = O.element_coloring(els, int(els.max()) + 1)
+        assert col.max() < 32
+        by_vertex = {}
+        for e, vs in enumerate(els):
+            for v in vs:
+                by_vertex.setdefault(int(v), []).append(e)
+        for e, vs in enumerate(els):
+            earlier = {int(col[f]) for v in vs for f in by_vertex[int(v)] if f < e}
+            assert int(col[e]) == min(set(range(33)) - earlier)
+            later = {int(col[f]) for v in vs for f in by_vertex[int(v)] if f != e}
+            assert int(col[e]) not in later
+
+
 def _spd(n, seed):
     rng = np.random.default_rng(seed)
     Q = rng.standard_normal((n, n))
