@@ -1565,6 +1565,21 @@ MatFreeOp::MatFreeOp(int dim_, int nd_, int nq_, int64_t ndof, int nclass, const
     for (int64_t k = 0; k <= ne_global; k++) kc[k + 1] += kc[k];
     std::vector<int> order(ndof);
     for (int64_t d = 0; d < ndof; d++) order[kc[key[d]]++] = (int)d;
+    // Inside each workgroup's range of the gather kernel (the launch grid
+    // depends on ndof only), dofs by decreasing contribution count (stable):
+    // the wavefronts of a range then hold dofs of like count, so a vertex dof
+    // (up to 24 contributions on tets) no longer holds up a wavefront of
+    // edge, face and cell dofs (2 to 6).  Which dofs a workgroup gathers and
+    // each dof's summation order are unchanged.
+    if (ndof > 0) {
+      const int64_t grid = GridFor(ndof, kGatherBlock, kMaxWideBlocks);
+      const int64_t per = (ndof + grid - 1) / grid;
+      for (int64_t a = 0; a < ndof; a += per) {
+        const int64_t b = std::min<int64_t>(ndof, a + per);
+        std::stable_sort(order.begin() + a, order.begin() + b,
+                         [&](int u, int v) { return cnt[u + 1] - cnt[u] > cnt[v + 1] - cnt[v]; });
+      }
+    }
     std::vector<int> prp(ndof + 1, 0), psl(sl.size());
     for (int64_t r = 0; r < ndof; r++) {
       const int d = order[r];
