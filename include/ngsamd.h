@@ -348,7 +348,15 @@ int ngsamd_batch_tuning(ngsamd_op_t op, int* h_count, double* h_ms);
 /* Jacobi-PCG on every problem with the CGSolver recurrence and stopping rule
  * per problem (tol relative to each problem's initial residual).  Per problem:
  * h_iterations (checks incl. the initial one), h_final_res (last residual),
- * h_finish_ms (device time from the start of the call to its convergence). */
+ * h_finish_ms (device time from the start of the call to its convergence).
+ * chunk > 0: the iteration replayed from a HIP graph of chunk iterations;
+ * 0: launched kernel by kernel; NGSAMD_BATCH_PERSISTENT (-1): all
+ * iterations in one cooperative launch that stays resident (grid-wide
+ * barriers between the phases of an iteration) -- the same iterates bit for
+ * bit.  No counterpart in the reference's interface (its CG is one solve per
+ * call); the persistent form is its task-queue idea (ngscuda/cuda_tasks.cpp:17-140)
+ * applied to the lockstep MultiMult (linalg/cg.cpp:132-250). */
+#define NGSAMD_BATCH_PERSISTENT (-1)
 int ngsamd_batch_cg_solve(ngsamd_op_t op, ngsamd_op_t pre, const double* d_rhs, double* d_sol, double tol,
                           int maxiter, int chunk, int* h_iterations, double* h_final_res, double* h_finish_ms);
 /* FP64 iterative refinement per problem around FP32 batched inner solves

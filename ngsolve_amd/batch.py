@@ -17,6 +17,7 @@ import numpy as np
 from ._native import N, ensure_device
 
 ALL = 0xFFFFFFFFFFFFFFFF
+PERSISTENT = -1  # NGSAMD_BATCH_PERSISTENT of ngsamd.h
 
 
 class Problem:
@@ -91,7 +92,12 @@ class BatchSolver:
         self.sol = N.DeviceVector(self.n, 8)
         self.nseg, self.block_nnz, self.nblocks = N.BatchInfo(self.op)
 
-    def solve_fp64(self, tol=1e-10, maxiter=100000, chunk=16):
+    def solve_fp64(self, tol=1e-10, maxiter=100000, chunk=16, persistent=False):
+        """FP64 block CG.  chunk: iterations per replayed HIP graph (0: kernel
+        by kernel); persistent: all iterations in one cooperative launch
+        instead (NGSAMD_BATCH_PERSISTENT), with the same iterates."""
+        if persistent:
+            chunk = PERSISTENT
         its, res, ms = N.BatchCGSolve(self.op, self.pre, self.rhs, self.sol, tol, maxiter, chunk)
         return dict(iterations=its, final_residual=res, finish_ms=ms)
 

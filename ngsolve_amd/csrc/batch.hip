@@ -107,22 +107,19 @@ __device__ __forceinline__ int XcdBlock(int b, int grid) {
 // y = A x (ACCUM: y += s A x) over row blocks; with DOT the partial
 // sum_r x[r] (A x)[r] of every row block goes to partials[block]; blocks of
 // problems with done[seg] != 0 are skipped.
+// Row block b (prod: NNZ doubles of LDS, lds: kBatchBlock / 64 doubles).
 template <int NNZ, class V, class M, bool ACCUM, bool DOT>
-__global__ __launch_bounds__(kBatchBlock) void BatchSpmvKernel(
-    const int* __restrict__ rowptr, const int* __restrict__ cols, const M* __restrict__ vals, const V* __restrict__ x,
-    V* __restrict__ y, double s, const int* __restrict__ blk_row, const int* __restrict__ blk_seg, int nblocks,
-    int blocks_per_wg, const int* __restrict__ done, double* __restrict__ partials) {
+__device__ __forceinline__ void BatchSpmvBlock(const int* __restrict__ rowptr, const int* __restrict__ cols,
+                                               const M* __restrict__ vals, const V* __restrict__ x,
+                                               V* __restrict__ y, double s, const int* __restrict__ blk_row,
+                                               const int* __restrict__ blk_seg, const int* __restrict__ done,
+                                               double* __restrict__ partials, int b, double* prod, double* lds) {
   constexpr int PER = NNZ / kBatchBlock;
-  __shared__ double prod[NNZ];
-  __shared__ double lds[kBatchBlock / 64];
   const int tid = threadIdx.x;
-  const int lb = XcdBlock(blockIdx.x, gridDim.x);
-  const int b0 = lb * blocks_per_wg;
-  const int b1 = min(b0 + blocks_per_wg, nblocks);
-  for (int b = b0; b < b1; b++) {
+  {
     if (done && done[blk_seg[b]]) {
       if (DOT && tid == 0) partials[b] = 0.0;
-      continue;
+      return;
     }
     const int r0 = blk_row[b], r1 = blk_row[b + 1];
     const int k0 = rowptr[r0], k1 = rowptr[r1];
@@ -138,7 +135,7 @@ __global__ __launch_bounds__(kBatchBlock) void BatchSpmvKernel(
         dotacc = (double)x[r0] * a;
         if (DOT) partials[b] = dotacc;
       }
-      continue;
+      return;
     }
     // unconditional loads (lanes past the block read entry k0, in bounds
     // thanks to the padding) so that all of them are in flight at once
@@ -184,26 +181,39 @@ __global__ __launch_bounds__(kBatchBlock) void BatchSpmvKernel(
   }
 }
 
-// Runs-layout form of BatchSpmvKernel (the runs kernel of spmv.hip with the
-// frozen-problem skip and one partial per block).
 template <int NNZ, class V, class M, bool ACCUM, bool DOT>
-__global__ __launch_bounds__(kBatchBlock) void BatchRunsKernel(
-    const int* __restrict__ run_row, const int* __restrict__ run_ptr, const int* __restrict__ val_ptr,
-    const int* __restrict__ rcols, const M* __restrict__ rvals, const V* __restrict__ x, V* __restrict__ y, double s,
-    const int* __restrict__ blk_run, const int* __restrict__ blk_len, const int* __restrict__ blk_seg, int nblocks,
+__global__ __launch_bounds__(kBatchBlock) void BatchSpmvKernel(
+    const int* __restrict__ rowptr, const int* __restrict__ cols, const M* __restrict__ vals, const V* __restrict__ x,
+    V* __restrict__ y, double s, const int* __restrict__ blk_row, const int* __restrict__ blk_seg, int nblocks,
     int blocks_per_wg, const int* __restrict__ done, double* __restrict__ partials) {
-  static_assert(NNZ <= 4096, "RunDiv is exact below 4096");
-  constexpr int PER = NNZ / kBatchBlock;
   __shared__ double prod[NNZ];
   __shared__ double lds[kBatchBlock / 64];
-  const int tid = threadIdx.x;
   const int lb = XcdBlock(blockIdx.x, gridDim.x);
   const int b0 = lb * blocks_per_wg;
   const int b1 = min(b0 + blocks_per_wg, nblocks);
-  for (int b = b0; b < b1; b++) {
+  for (int b = b0; b < b1; b++)
+    BatchSpmvBlock<NNZ, V, M, ACCUM, DOT>(rowptr, cols, vals, x, y, s, blk_row, blk_seg, done, partials, b, prod,
+                                          lds);
+}
+
+// Runs-layout form of BatchSpmvKernel (the runs kernel of spmv.hip with the
+// frozen-problem skip and one partial per block).
+// Block b of the runs layout (prod / lds as in BatchSpmvBlock).
+template <int NNZ, class V, class M, bool ACCUM, bool DOT>
+__device__ __forceinline__ void BatchRunsBlock(const int* __restrict__ run_row, const int* __restrict__ run_ptr,
+                                               const int* __restrict__ val_ptr, const int* __restrict__ rcols,
+                                               const M* __restrict__ rvals, const V* __restrict__ x,
+                                               V* __restrict__ y, double s, const int* __restrict__ blk_run,
+                                               const int* __restrict__ blk_len, const int* __restrict__ blk_seg,
+                                               const int* __restrict__ done, double* __restrict__ partials, int b,
+                                               double* prod, double* lds) {
+  static_assert(NNZ <= 4096, "RunDiv is exact below 4096");
+  constexpr int PER = NNZ / kBatchBlock;
+  const int tid = threadIdx.x;
+  {
     if (done && done[blk_seg[b]]) {
       if (DOT && tid == 0) partials[b] = 0.0;
-      continue;
+      return;
     }
     const int u0 = blk_run[b], u1 = blk_run[b + 1];
     const int R = blk_len[b];
@@ -223,7 +233,7 @@ __global__ __launch_bounds__(kBatchBlock) void BatchRunsKernel(
         if (ACCUM) y[r] = (V)((double)y[r] + s * a); else y[r] = (V)(s * a);
         if (DOT) partials[b] = (double)x[r] * a;
       }
-      continue;
+      return;
     }
     int c[PER];
     double v[PER], xv[PER];
@@ -276,6 +286,22 @@ __global__ __launch_bounds__(kBatchBlock) void BatchRunsKernel(
     }
     __syncthreads();
   }
+}
+
+template <int NNZ, class V, class M, bool ACCUM, bool DOT>
+__global__ __launch_bounds__(kBatchBlock) void BatchRunsKernel(
+    const int* __restrict__ run_row, const int* __restrict__ run_ptr, const int* __restrict__ val_ptr,
+    const int* __restrict__ rcols, const M* __restrict__ rvals, const V* __restrict__ x, V* __restrict__ y, double s,
+    const int* __restrict__ blk_run, const int* __restrict__ blk_len, const int* __restrict__ blk_seg, int nblocks,
+    int blocks_per_wg, const int* __restrict__ done, double* __restrict__ partials) {
+  __shared__ double prod[NNZ];
+  __shared__ double lds[kBatchBlock / 64];
+  const int lb = XcdBlock(blockIdx.x, gridDim.x);
+  const int b0 = lb * blocks_per_wg;
+  const int b1 = min(b0 + blocks_per_wg, nblocks);
+  for (int b = b0; b < b1; b++)
+    BatchRunsBlock<NNZ, V, M, ACCUM, DOT>(run_row, run_ptr, val_ptr, rcols, rvals, x, y, s, blk_run, blk_len,
+                                          blk_seg, done, partials, b, prod, lds);
 }
 
 template <class V, class M, bool ACCUM, bool DOT>
@@ -665,6 +691,32 @@ __global__ __launch_bounds__(kBatchBlock) void BatchAlphaKernel(const double* __
   }
 }
 
+// u += al s, d -= al w, w = C d over dofs [i0, i1); returns this thread's
+// part of <w, d>
+template <class V>
+__device__ __forceinline__ double BatchUpdateRange(const V* __restrict__ inv, V* __restrict__ u, V* __restrict__ d,
+                                                   V* __restrict__ w, const V* __restrict__ s, int64_t i0,
+                                                   int64_t i1, V al) {
+  double acc = 0.0;
+  for (int64_t i = i0 + threadIdx.x; i < i1; i += kBatchBlock) {
+    const V si = s[i], wi = w[i];
+    u[i] += al * si;
+    const V di = d[i] - al * wi;
+    d[i] = di;
+    const V zi = inv[i] * di;
+    w[i] = zi;
+    acc += (double)zi * (double)di;
+  }
+  return acc;
+}
+
+// s = beta s + w over dofs [i0, i1)
+template <class V>
+__device__ __forceinline__ void BatchDirectionRange(V* __restrict__ s, const V* __restrict__ w, int64_t i0,
+                                                    int64_t i1, V beta) {
+  for (int64_t i = i0 + threadIdx.x; i < i1; i += kBatchBlock) s[i] = beta * s[i] + w[i];
+}
+
 template <class V>
 __global__ __launch_bounds__(kBatchBlock) void BatchUpdateKernel(const V* __restrict__ inv, V* __restrict__ u,
                                                                  V* __restrict__ d, V* __restrict__ w,
@@ -679,17 +731,7 @@ __global__ __launch_bounds__(kBatchBlock) void BatchUpdateKernel(const V* __rest
     if (threadIdx.x == 0) partials[c] = 0.0;
     return;
   }
-  const V al = (V)S.alpha[p];
-  double acc = 0.0;
-  for (int64_t i = cstart[c] + threadIdx.x; i < cstart[c + 1]; i += kBatchBlock) {
-    const V si = s[i], wi = w[i];
-    u[i] += al * si;
-    const V di = d[i] - al * wi;
-    d[i] = di;
-    const V zi = inv[i] * di;
-    w[i] = zi;
-    acc += (double)zi * (double)di;
-  }
+  const double acc = BatchUpdateRange<V>(inv, u, d, w, s, cstart[c], cstart[c + 1], (V)S.alpha[p]);
   StorePartial<kBatchBlock>(acc, partials, lds);
 }
 
@@ -714,8 +756,7 @@ __global__ __launch_bounds__(kBatchBlock) void BatchDirectionKernel(V* __restric
   const int c = blockIdx.x;
   const int p = cseg[c];
   if (S.done[p]) return;
-  const V beta = (V)S.beta[p];
-  for (int64_t i = cstart[c] + threadIdx.x; i < cstart[c + 1]; i += kBatchBlock) s[i] = beta * s[i] + w[i];
+  BatchDirectionRange<V>(s, w, cstart[c], cstart[c + 1], (V)S.beta[p]);
 }
 
 // all problems done?  one workgroup
@@ -726,6 +767,200 @@ __global__ __launch_bounds__(kBatchBlock) void BatchAllDoneKernel(const int* __r
   for (int p = threadIdx.x; p < nseg; p += kBatchBlock) open += done[p] ? 0.0 : 1.0;
   open = BlockSum<kBatchBlock>(open, lds);
   if (threadIdx.x == 0) *all = open == 0.0 ? 1 : 0;
+}
+
+// ------------------------------------------------------ persistent block CG
+//
+// The whole FP64 block CG in one cooperative launch: every workgroup stays
+// resident and walks the iterations, three phases each, separated by
+// grid-wide barriers: (S) w = A s and <s, A s> per row block; (U) alpha per
+// problem, u += alpha s, d -= alpha w, w = C d and <w, d> per chunk; (C)
+// beta, the stopping test and s = beta s + w.  Workgroup lb owns the same
+// contiguous ranges of row blocks and chunks in every iteration.  A
+// problem's alpha and beta are formed redundantly by every workgroup that
+// holds one of its chunks, with the same sums in the same order as the
+// one-workgroup-per-problem kernels of the graph form, so both forms give
+// the same iterates bit for bit.  Only phase C writes done[], so every
+// workgroup reads the same done[] at the top of an iteration and all leave
+// the loop together.
+
+constexpr long kSpinLimit = 1L << 22;  // polls of a barrier wait (about a second)
+
+// Grid-wide barrier: a monotone arrival counter; this is arrival round
+// `round` of nwg workgroups.  Thread 0 publishes the workgroup's writes
+// (agent-scope release: the XCD's L2 is written back), arrives, waits for
+// the others and acquires (L2 invalidated).  The wait is bounded: after
+// kSpinLimit polls, or when another workgroup has given up, *err is set and
+// false returned, so a launch that lost co-residency ends instead of
+// hanging.
+__device__ __forceinline__ bool GridBarrier(unsigned long long* arrive, unsigned long long round, int nwg, int* err,
+                                            int* lds_flag) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned long long target = round * (unsigned long long)nwg;
+    __threadfence();
+    __hip_atomic_fetch_add(arrive, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    int bad = 0;
+    long spins = 0;
+    while (__hip_atomic_load(arrive, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+      if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 || ++spins > kSpinLimit) {
+        __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        bad = 1;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(2);
+    }
+    __threadfence();
+    *lds_flag = bad;
+  }
+  __syncthreads();
+  return *lds_flag == 0;
+}
+
+// SegmentSum returned to every thread (lds: kBatchBlock / 64 + 1 doubles)
+__device__ __forceinline__ double SegmentSumAll(const double* partials, const int* first, int p, double* lds) {
+  const double v = SegmentSum(partials, first, p, lds);
+  if (threadIdx.x == 0) lds[kBatchBlock / 64] = v;
+  __syncthreads();
+  return lds[kBatchBlock / 64];
+}
+
+struct PersistentArgs {
+  // plain CSR layout
+  const int* rowptr;
+  const int* cols;
+  const double* vals;
+  const int* blk_row;
+  // runs layout
+  const int* run_row;
+  const int* run_ptr;
+  const int* val_ptr;
+  const int* rcols;
+  const double* rvals;
+  const int* blk_run;
+  const int* blk_len;
+  // blocks and chunks
+  const int* blk_seg;
+  const int* seg_blk;
+  int nblocks;
+  const int64_t* cstart;
+  const int* cseg;
+  const int* seg_chunk;
+  int nchunks;
+  int nseg;
+  // vectors
+  const double* inv;
+  double* u;
+  double* d;
+  double* w;
+  double* s;
+  // partial sums and per-problem scalars
+  double* bpart;  // nblocks: <s, A s> per row block
+  double* cpart;  // nchunks: <w, d> per chunk
+  double* wdn2;   // 2 nseg: <w, d> of the even / odd iterations
+  int* brk;       // nseg: breakdown seen in phase U
+  BatchState S;
+  unsigned long long* arrive;
+  int* err;
+};
+
+template <int NNZ, bool RUNS>
+__global__ __launch_bounds__(kBatchBlock) void BatchCgPersistentKernel(PersistentArgs P) {
+  __shared__ double prod[NNZ];
+  __shared__ double lds[kBatchBlock / 64 + 1];
+  __shared__ int flag;
+  const int G = gridDim.x;
+  const int lb = XcdBlock(blockIdx.x, G);
+  const int bper = (P.nblocks + G - 1) / G;
+  const int b0 = min(lb * bper, P.nblocks), b1 = min(b0 + bper, P.nblocks);
+  const int cper = (P.nchunks + G - 1) / G;
+  const int c0 = min(lb * cper, P.nchunks), c1 = min(c0 + cper, P.nchunks);
+  BatchState& S = P.S;
+  unsigned long long round = 0;
+  for (int k = 0; k < S.maxiter; k++) {
+    // every problem done?  (done[] is only written in phase C)
+    double open = 0.0;
+    for (int p = threadIdx.x; p < P.nseg; p += kBatchBlock) open += S.done[p] ? 0.0 : 1.0;
+    open = BlockSum<kBatchBlock>(open, lds);
+    if (threadIdx.x == 0) lds[kBatchBlock / 64] = open;
+    __syncthreads();
+    if (lds[kBatchBlock / 64] == 0.0) break;
+    const int par = k & 1;
+    // (S) w = A s, <s, A s> per row block
+    for (int b = b0; b < b1; b++) {
+      if constexpr (RUNS)
+        BatchRunsBlock<NNZ, double, double, false, true>(P.run_row, P.run_ptr, P.val_ptr, P.rcols, P.rvals, P.s, P.w,
+                                                         1.0, P.blk_run, P.blk_len, P.blk_seg, S.done, P.bpart, b,
+                                                         prod, lds);
+      else
+        BatchSpmvBlock<NNZ, double, double, false, true>(P.rowptr, P.cols, P.vals, P.s, P.w, 1.0, P.blk_row,
+                                                         P.blk_seg, S.done, P.bpart, b, prod, lds);
+    }
+    if (!GridBarrier(P.arrive, ++round, G, P.err, &flag)) return;
+    // (U) alpha, updates, <w, d> per chunk
+    int lastp = -1;
+    double al = 0.0;
+    bool brk = false;
+    for (int c = c0; c < c1; c++) {
+      const int p = P.cseg[c];
+      if (S.done[p]) continue;
+      if (p != lastp) {
+        const double as_s = SegmentSumAll(P.bpart, P.seg_blk, p, lds);
+        const double wd = P.wdn2[par * P.nseg + p];
+        brk = as_s == 0.0 || wd == 0.0;
+        al = brk ? 0.0 : wd / as_s;
+        lastp = p;
+        if (threadIdx.x == 0 && c == P.seg_chunk[p]) {
+          if (brk) P.brk[p] = 1;
+          else S.alpha[p] = al;
+        }
+      }
+      if (brk) continue;
+      const double acc = BlockSum<kBatchBlock>(
+          BatchUpdateRange<double>(P.inv, P.u, P.d, P.w, P.s, P.cstart[c], P.cstart[c + 1], al), lds);
+      if (threadIdx.x == 0) P.cpart[c] = acc;
+    }
+    if (!GridBarrier(P.arrive, ++round, G, P.err, &flag)) return;
+    // (C) beta, stopping test, s = beta s + w
+    lastp = -1;
+    bool stop = false;
+    double beta = 0.0;
+    for (int c = c0; c < c1; c++) {
+      const int p = P.cseg[c];
+      if (S.done[p]) continue;
+      if (p != lastp) {
+        lastp = p;
+        const bool first = threadIdx.x == 0 && c == P.seg_chunk[p];
+        if (P.brk[p]) {
+          stop = true;
+          if (first) {
+            S.done[p] = 2;
+            S.finish[p] = DeviceClock();
+          }
+        } else {
+          const double wdn = SegmentSumAll(P.cpart, P.seg_chunk, p, lds);
+          const double wd = P.wdn2[par * P.nseg + p];
+          const double res = sqrt(fabs(wdn));
+          const int it = k + 2;  // the start check counted 1
+          beta = wdn / wd;
+          stop = it >= S.maxiter || res <= S.final_res[p];
+          if (first) {
+            P.wdn2[(par ^ 1) * P.nseg + p] = wdn;
+            S.wdn[p] = wdn;
+            S.beta[p] = beta;
+            S.it[p] = it;
+            S.last_res[p] = res;
+            if (stop) {
+              S.done[p] = 1;
+              S.finish[p] = DeviceClock();
+            }
+          }
+        }
+      }
+      if (!stop) BatchDirectionRange<double>(P.s, P.w, P.cstart[c], P.cstart[c + 1], beta);
+    }
+    if (!GridBarrier(P.arrive, ++round, G, P.err, &flag)) return;
+  }
 }
 
 template <class V, class M>
@@ -743,6 +978,82 @@ static void EnqueueBatchIteration(BatchOp& A, const M* vals, const V* inv, V* u,
   NGSAMD_CHECK_LAUNCH();
   BatchAllDoneKernel<<<1, kBatchBlock, 0, st>>>(S.done, A.nseg, all_done);
   NGSAMD_CHECK_LAUNCH();
+}
+
+// The iterations after the start check in one cooperative launch of
+// BatchCgPersistentKernel (grid: as many workgroups as are resident at once,
+// at most one per row block or chunk).
+template <int NNZ, bool RUNS>
+static void LaunchPersistent(const PersistentArgs& P, hipStream_t st) {
+  auto kern = BatchCgPersistentKernel<NNZ, RUNS>;
+  int occ = 0;
+  NGSAMD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, kBatchBlock, 0));
+  NGSAMD_REQUIRE(occ >= 1, "batch cg: persistent kernel does not fit a compute unit");
+  int64_t g = std::min<int64_t>((int64_t)occ * DeviceCUs(), std::max(P.nblocks, P.nchunks));
+  if (g > 8) g &= ~int64_t(7);
+  if (g < 1) g = 1;
+  PersistentArgs args = P;
+  void* kargs[] = {&args};
+  NGSAMD_HIP(hipLaunchCooperativeKernel((const void*)kern, dim3((unsigned)g), dim3(kBatchBlock), kargs, 0, st));
+}
+
+static void RunPersistent(BatchOp& A, const double* inv, double* u, double* d, double* w, double* s, BatchState S,
+                          hipStream_t st) {
+  const int ns = A.nseg;
+  DevBuf<double> cpart(std::max(A.nchunks, 1)), wdn2(2 * (size_t)ns);
+  DevBuf<int> brk(ns), err(1);
+  DevBuf<unsigned long long> arrive(1);
+  NGSAMD_HIP(hipMemcpyAsync(wdn2.p, S.wdn, ns * sizeof(double), hipMemcpyDeviceToDevice, st));
+  NGSAMD_HIP(hipMemsetAsync(brk.p, 0, ns * sizeof(int), st));
+  NGSAMD_HIP(hipMemsetAsync(err.p, 0, sizeof(int), st));
+  NGSAMD_HIP(hipMemsetAsync(arrive.p, 0, sizeof(unsigned long long), st));
+  PersistentArgs P{};
+  P.rowptr = A.rowptr.p;
+  P.cols = A.cols.p;
+  P.vals = A.vals.p;
+  P.blk_row = A.blk_row.p;
+  P.run_row = A.run_row.p;
+  P.run_ptr = A.run_ptr.p;
+  P.val_ptr = A.val_ptr.p;
+  P.rcols = A.rcols.p;
+  P.rvals = A.rvals.p;
+  P.blk_run = A.blk_run.p;
+  P.blk_len = A.blk_len.p;
+  P.blk_seg = A.blk_seg.p;
+  P.seg_blk = A.seg_blk.p;
+  P.nblocks = A.nblocks;
+  P.cstart = A.chunk_start.p;
+  P.cseg = A.chunk_seg.p;
+  P.seg_chunk = A.seg_chunk.p;
+  P.nchunks = A.nchunks;
+  P.nseg = ns;
+  P.inv = inv;
+  P.u = u;
+  P.d = d;
+  P.w = w;
+  P.s = s;
+  P.bpart = A.partials.p;
+  P.cpart = cpart.p;
+  P.wdn2 = wdn2.p;
+  P.brk = brk.p;
+  P.S = S;
+  P.arrive = arrive.p;
+  P.err = err.p;
+  if (A.nblocks > 0 || A.nchunks > 0) {
+    if (A.use_runs) {
+      if (A.block_nnz == 1024) LaunchPersistent<1024, true>(P, st);
+      else if (A.block_nnz == 4096) LaunchPersistent<4096, true>(P, st);
+      else LaunchPersistent<2048, true>(P, st);
+    } else {
+      if (A.block_nnz == 1024) LaunchPersistent<1024, false>(P, st);
+      else if (A.block_nnz == 4096) LaunchPersistent<4096, false>(P, st);
+      else LaunchPersistent<2048, false>(P, st);
+    }
+  }
+  int h_err = 0;
+  NGSAMD_HIP(hipMemcpyAsync(&h_err, err.p, sizeof(int), hipMemcpyDeviceToHost, st));
+  NGSAMD_HIP(hipStreamSynchronize(st));
+  if (h_err) throw Error(NGSAMD_E_HIP, "batch cg: a grid barrier of the persistent launch timed out");
 }
 
 // Runs the batched CG from d = rhs (u = 0) until every active problem is
@@ -766,6 +1077,14 @@ static long long RunBatchCg(BatchOp& A, const M* vals, const V* inv, const V* rh
   NGSAMD_CHECK_LAUNCH();
   BatchInitCheckKernel<<<A.nseg, kBatchBlock, 0, st>>>(A.partials.p, A.seg_chunk.p, S);
   NGSAMD_CHECK_LAUNCH();
+  if constexpr (sizeof(V) == 8 && sizeof(M) == 8) {
+    if (chunk < 0) {
+      RunPersistent(A, inv, u, d.p, w.p, s.p, S, st);
+      return 0;
+    }
+  } else {
+    NGSAMD_REQUIRE(chunk >= 0, "batch cg: the persistent form is FP64 only");
+  }
   BatchAllDoneKernel<<<1, kBatchBlock, 0, st>>>(S.done, A.nseg, all.p);
   NGSAMD_CHECK_LAUNCH();
 
@@ -1247,6 +1566,8 @@ extern "C" int ngsamd_batch_cg_solve(ngsamd_op_t op, ngsamd_op_t pre, const doub
     NGSAMD_REQUIRE(pre && pre->kind == OpKind::Jacobi && pre->height == op->height,
                    "ngsamd_batch_cg_solve: needs a Jacobi preconditioner of the same size");
     NGSAMD_REQUIRE(maxiter >= 1, "ngsamd_batch_cg_solve: maxiter must be >= 1");
+    NGSAMD_REQUIRE(chunk >= 0 || chunk == NGSAMD_BATCH_PERSISTENT,
+                   "ngsamd_batch_cg_solve: chunk must be >= 0 or NGSAMD_BATCH_PERSISTENT");
     auto& A = *static_cast<BatchOp*>(op);
     hipStream_t st = Stream();
     const double* inv = static_cast<JacobiOp*>(pre)->invdiag.p;

@@ -51,6 +51,41 @@ def test_batch_mixed_reaches_1e10(device, monkeypatch, runs):
         assert np.linalg.norm(res[fr]) <= 1.0001e-10 * np.linalg.norm(pr.rhs[fr])
 
 
+@pytest.mark.parametrize("runs,nnz", [("1", None), ("0", None), ("1", "1024"), ("0", "4096")])
+def test_batch_persistent_matches_graph(device, monkeypatch, runs, nnz):
+    """All iterations in one cooperative launch (grid-wide barriers between
+    the phases of an iteration) give the graph form's iterates bit for bit:
+    iteration counts, last residuals and solutions, for both layouts and
+    other block sizes, to convergence and with a maxiter stop."""
+    from ngsolve_amd.batch import BatchSolver
+    monkeypatch.setenv("NGSAMD_BATCH_RUNS", runs)
+    if nnz:
+        monkeypatch.setenv("NGSAMD_SPMV_NNZ", nnz)
+    else:
+        monkeypatch.delenv("NGSAMD_SPMV_NNZ", raising=False)
+    say(f"[batch] persistent vs graph, runs layout {runs}, block_nnz {nnz}")
+    bs = BatchSolver(_problems())
+    for maxiter in (100000, 7):
+        g = bs.solve_fp64(tol=1e-10, maxiter=maxiter)
+        ug = bs.solution()
+        q = bs.solve_fp64(tol=1e-10, maxiter=maxiter, persistent=True)
+        uq = bs.solution()
+        say(f"[batch] maxiter {maxiter}: its {q['iterations'].tolist()}")
+        assert np.array_equal(g["iterations"], q["iterations"])
+        assert np.array_equal(g["final_residual"], q["final_residual"])
+        assert np.array_equal(ug, uq)
+        assert (q["finish_ms"] >= 0).all()
+        if maxiter == 7:
+            assert (q["iterations"] <= 7).all()
+
+
+def test_batch_rejects_bad_chunk(device):
+    from ngsolve_amd.batch import BatchSolver
+    bs = BatchSolver(_problems()[:3])
+    with pytest.raises(RuntimeError, match="chunk"):
+        bs.solve_fp64(chunk=-2)
+
+
 def test_batch_rejects_coupled_problems(device):
     from ngsolve_amd import _ngsamd as N
     rp = np.array([0, 2, 3], dtype=np.int64)

@@ -1,7 +1,8 @@
 """The batching configuration at its stated size: 512 independent 2D Poisson
 problems from one generator (seed 5: nx ~ U{8..96}, p ~ U{1..5}, kappa_e =
 exp(N(0,1)), rhs ~ U[-1,1]; 20,045,740 dofs in all), solved together on the
-device to 1e-10 -- FP64 block CG and FP64 refinement over FP32 inner solves.
+device to 1e-10 -- FP64 block CG (graph replays and one persistent launch)
+and FP64 refinement over FP32 inner solves.
 Every problem is checked against the oracle's Jacobi-PCG on that problem
 alone (CGSolver recurrence, python/krylovspace.py:263-291): FP64 iteration
 count within +-1 and solution within 1e-9; refinement: true FP64 residual
@@ -36,12 +37,24 @@ def _solved():
     say(f"[full batch] FP64 block CG, block_nnz={bs.block_nnz}")
     r64 = bs.solve_fp64(tol=1e-10)
     u64 = bs.solution()
+    say("[full batch] FP64 block CG in one persistent launch")
+    rps = bs.solve_fp64(tol=1e-10, persistent=True)
+    ups = bs.solution()
     say("[full batch] mixed refinement")
     rmx = bs.solve_mixed(tol=1e-10)
     umx = bs.solution()
-    _state.update(probs=probs, offsets=bs.offsets, r64=r64, u64=u64, rmx=rmx, umx=umx)
+    _state.update(probs=probs, offsets=bs.offsets, r64=r64, u64=u64, rps=rps, ups=ups, rmx=rmx, umx=umx)
     del bs
     return _state
+
+
+def test_config5_persistent_launch_matches_graph(device):
+    """The persistent form (one cooperative launch) on all 512 problems: the
+    graph form's iteration counts, residuals and solutions bit for bit."""
+    S = _solved()
+    assert np.array_equal(S["rps"]["iterations"], S["r64"]["iterations"])
+    assert np.array_equal(S["rps"]["final_residual"], S["r64"]["final_residual"])
+    assert np.array_equal(S["ups"], S["u64"])
 
 
 @pytest.mark.parametrize("part", [0, 1, 2, 3])
