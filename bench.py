@@ -503,16 +503,11 @@ def main():
         log("[bench] profile mode done")
         return
 
-    # read ceiling: 1 GiB of non-temporal streaming loads (ngsamd_stream_read),
-    # measured first over ~0.1 s of streaming, which also brings the memory
-    # system to its sustained state before the headline timing: measured on
-    # this part, the first ~50 ms of HBM-bound work after an idle spell or
-    # after host-to-device uploads run 3-5% slower (the same CSR kernel: 400
-    # us in its first timing, 383-389 us after ~50 ms of streaming)
+    # read ceiling: 1 GiB of non-temporal streaming loads (ngsamd_stream_read)
     big = N.DeviceVector(1 << 27, 8)
     big.fill(1.0)
     rsum = N.DeviceVector(1, 8)
-    ms_read = timed(N, lambda: N.stream_read(big, rsum), 500, 10)
+    ms_read = timed(N, lambda: N.stream_read(big, rsum), 100, 10)
     read_gbs = (1 << 30) / (ms_read * 1e-3) / 1e9
     assert rsum.download()[0] == float(1 << 27)
     del big

@@ -674,10 +674,10 @@ __global__ __launch_bounds__(kBatchBlock) void BatchInitCheckKernel(const double
   }
 }
 
-__global__ __launch_bounds__(kBatchBlock) void BatchAlphaKernel(const double* __restrict__ partials,
-                                                                const int* __restrict__ seg_blk, BatchState S) {
-  __shared__ double lds[kBatchBlock / 64];
-  const int p = blockIdx.x;
+// alpha of problem p from its row blocks' <s, A s> partials (breakdown:
+// done = 2); one workgroup
+__device__ __forceinline__ void BatchAlphaProblem(const double* __restrict__ partials, const int* __restrict__ seg_blk,
+                                                  BatchState& S, int p, double* lds) {
   if (S.done[p]) return;
   const double as_s = SegmentSum(partials, seg_blk, p, lds);
   if (threadIdx.x == 0) {
@@ -689,6 +689,12 @@ __global__ __launch_bounds__(kBatchBlock) void BatchAlphaKernel(const double* __
       S.alpha[p] = wd / as_s;
     }
   }
+}
+
+__global__ __launch_bounds__(kBatchBlock) void BatchAlphaKernel(const double* __restrict__ partials,
+                                                                const int* __restrict__ seg_blk, BatchState S) {
+  __shared__ double lds[kBatchBlock / 64];
+  BatchAlphaProblem(partials, seg_blk, S, blockIdx.x, lds);
 }
 
 // u += al s, d -= al w, w = C d over dofs [i0, i1); returns this thread's
@@ -735,10 +741,11 @@ __global__ __launch_bounds__(kBatchBlock) void BatchUpdateKernel(const V* __rest
   StorePartial<kBatchBlock>(acc, partials, lds);
 }
 
-__global__ __launch_bounds__(kBatchBlock) void BatchCheckKernel(const double* __restrict__ partials,
-                                                                const int* __restrict__ seg_chunk, BatchState S) {
-  __shared__ double lds[kBatchBlock / 64];
-  const int p = blockIdx.x;
+// <w, d> of problem p from its chunks' partials, the stopping test and
+// beta; one workgroup
+__device__ __forceinline__ void BatchCheckProblem(const double* __restrict__ partials,
+                                                  const int* __restrict__ seg_chunk, BatchState& S, int p,
+                                                  double* lds) {
   if (S.done[p]) return;
   const double wdn = SegmentSum(partials, seg_chunk, p, lds);
   if (threadIdx.x == 0) {
@@ -747,6 +754,12 @@ __global__ __launch_bounds__(kBatchBlock) void BatchCheckKernel(const double* __
     BatchCheck(S, p, sqrt(fabs(wdn)));
     S.beta[p] = wdn / wd;
   }
+}
+
+__global__ __launch_bounds__(kBatchBlock) void BatchCheckKernel(const double* __restrict__ partials,
+                                                                const int* __restrict__ seg_chunk, BatchState S) {
+  __shared__ double lds[kBatchBlock / 64];
+  BatchCheckProblem(partials, seg_chunk, S, blockIdx.x, lds);
 }
 
 template <class V>
@@ -772,17 +785,18 @@ __global__ __launch_bounds__(kBatchBlock) void BatchAllDoneKernel(const int* __r
 // ------------------------------------------------------ persistent block CG
 //
 // The whole FP64 block CG in one cooperative launch: every workgroup stays
-// resident and walks the iterations, three phases each, separated by
-// grid-wide barriers: (S) w = A s and <s, A s> per row block; (U) alpha per
-// problem, u += alpha s, d -= alpha w, w = C d and <w, d> per chunk; (C)
-// beta, the stopping test and s = beta s + w.  Workgroup lb owns the same
-// contiguous ranges of row blocks and chunks in every iteration.  A
-// problem's alpha and beta are formed redundantly by every workgroup that
-// holds one of its chunks, with the same sums in the same order as the
-// one-workgroup-per-problem kernels of the graph form, so both forms give
-// the same iterates bit for bit.  Only phase C writes done[], so every
-// workgroup reads the same done[] at the top of an iteration and all leave
-// the loop together.
+// resident and walks the iterations, five phases each, separated by
+// grid-wide barriers: (S) w = A s and <s, A s> per row block; (A) alpha per
+// problem; (U) u += alpha s, d -= alpha w, w = C d and <w, d> per chunk; (B)
+// the stopping test and beta per problem; (C) s = beta s + w per chunk.
+// Row blocks and chunks are dealt to the workgroups round-robin (block b to
+// workgroup b mod G), so every workgroup keeps a share of every problem's
+// work while problems converge; problems go to workgroups p, p + G, ...
+// Phases A and B run the graph form's per-problem bodies and S, U and C its
+// per-block and per-chunk bodies, so both forms give the same iterates bit
+// for bit.  done[] is written only in phases A and B, by the workgroup that
+// owns the problem, so every workgroup reads the same done[] at the top of
+// an iteration and all leave the loop together.
 
 constexpr long kSpinLimit = 1L << 22;  // polls of a barrier wait (about a second)
 
@@ -817,14 +831,6 @@ __device__ __forceinline__ bool GridBarrier(unsigned long long* arrive, unsigned
   return *lds_flag == 0;
 }
 
-// SegmentSum returned to every thread (lds: kBatchBlock / 64 + 1 doubles)
-__device__ __forceinline__ double SegmentSumAll(const double* partials, const int* first, int p, double* lds) {
-  const double v = SegmentSum(partials, first, p, lds);
-  if (threadIdx.x == 0) lds[kBatchBlock / 64] = v;
-  __syncthreads();
-  return lds[kBatchBlock / 64];
-}
-
 struct PersistentArgs {
   // plain CSR layout
   const int* rowptr;
@@ -854,11 +860,9 @@ struct PersistentArgs {
   double* d;
   double* w;
   double* s;
-  // partial sums and per-problem scalars
+  // partial sums
   double* bpart;  // nblocks: <s, A s> per row block
   double* cpart;  // nchunks: <w, d> per chunk
-  double* wdn2;   // 2 nseg: <w, d> of the even / odd iterations
-  int* brk;       // nseg: breakdown seen in phase U
   BatchState S;
   unsigned long long* arrive;
   int* err;
@@ -871,23 +875,18 @@ __global__ __launch_bounds__(kBatchBlock) void BatchCgPersistentKernel(Persisten
   __shared__ int flag;
   const int G = gridDim.x;
   const int lb = XcdBlock(blockIdx.x, G);
-  const int bper = (P.nblocks + G - 1) / G;
-  const int b0 = min(lb * bper, P.nblocks), b1 = min(b0 + bper, P.nblocks);
-  const int cper = (P.nchunks + G - 1) / G;
-  const int c0 = min(lb * cper, P.nchunks), c1 = min(c0 + cper, P.nchunks);
   BatchState& S = P.S;
   unsigned long long round = 0;
   for (int k = 0; k < S.maxiter; k++) {
-    // every problem done?  (done[] is only written in phase C)
+    // every problem done?  (done[] is written only in phases A and B)
     double open = 0.0;
     for (int p = threadIdx.x; p < P.nseg; p += kBatchBlock) open += S.done[p] ? 0.0 : 1.0;
     open = BlockSum<kBatchBlock>(open, lds);
     if (threadIdx.x == 0) lds[kBatchBlock / 64] = open;
     __syncthreads();
     if (lds[kBatchBlock / 64] == 0.0) break;
-    const int par = k & 1;
     // (S) w = A s, <s, A s> per row block
-    for (int b = b0; b < b1; b++) {
+    for (int b = lb; b < P.nblocks; b += G) {
       if constexpr (RUNS)
         BatchRunsBlock<NNZ, double, double, false, true>(P.run_row, P.run_ptr, P.val_ptr, P.rcols, P.rvals, P.s, P.w,
                                                          1.0, P.blk_run, P.blk_len, P.blk_seg, S.done, P.bpart, b,
@@ -897,67 +896,25 @@ __global__ __launch_bounds__(kBatchBlock) void BatchCgPersistentKernel(Persisten
                                                          P.blk_seg, S.done, P.bpart, b, prod, lds);
     }
     if (!GridBarrier(P.arrive, ++round, G, P.err, &flag)) return;
-    // (U) alpha, updates, <w, d> per chunk
-    int lastp = -1;
-    double al = 0.0;
-    bool brk = false;
-    for (int c = c0; c < c1; c++) {
+    // (A) alpha per problem
+    for (int p = lb; p < P.nseg; p += G) BatchAlphaProblem(P.bpart, P.seg_blk, S, p, lds);
+    if (!GridBarrier(P.arrive, ++round, G, P.err, &flag)) return;
+    // (U) updates and <w, d> per chunk
+    for (int c = lb; c < P.nchunks; c += G) {
       const int p = P.cseg[c];
       if (S.done[p]) continue;
-      if (p != lastp) {
-        const double as_s = SegmentSumAll(P.bpart, P.seg_blk, p, lds);
-        const double wd = P.wdn2[par * P.nseg + p];
-        brk = as_s == 0.0 || wd == 0.0;
-        al = brk ? 0.0 : wd / as_s;
-        lastp = p;
-        if (threadIdx.x == 0 && c == P.seg_chunk[p]) {
-          if (brk) P.brk[p] = 1;
-          else S.alpha[p] = al;
-        }
-      }
-      if (brk) continue;
       const double acc = BlockSum<kBatchBlock>(
-          BatchUpdateRange<double>(P.inv, P.u, P.d, P.w, P.s, P.cstart[c], P.cstart[c + 1], al), lds);
+          BatchUpdateRange<double>(P.inv, P.u, P.d, P.w, P.s, P.cstart[c], P.cstart[c + 1], S.alpha[p]), lds);
       if (threadIdx.x == 0) P.cpart[c] = acc;
     }
     if (!GridBarrier(P.arrive, ++round, G, P.err, &flag)) return;
-    // (C) beta, stopping test, s = beta s + w
-    lastp = -1;
-    bool stop = false;
-    double beta = 0.0;
-    for (int c = c0; c < c1; c++) {
+    // (B) stopping test and beta per problem
+    for (int p = lb; p < P.nseg; p += G) BatchCheckProblem(P.cpart, P.seg_chunk, S, p, lds);
+    if (!GridBarrier(P.arrive, ++round, G, P.err, &flag)) return;
+    // (C) s = beta s + w per chunk
+    for (int c = lb; c < P.nchunks; c += G) {
       const int p = P.cseg[c];
-      if (S.done[p]) continue;
-      if (p != lastp) {
-        lastp = p;
-        const bool first = threadIdx.x == 0 && c == P.seg_chunk[p];
-        if (P.brk[p]) {
-          stop = true;
-          if (first) {
-            S.done[p] = 2;
-            S.finish[p] = DeviceClock();
-          }
-        } else {
-          const double wdn = SegmentSumAll(P.cpart, P.seg_chunk, p, lds);
-          const double wd = P.wdn2[par * P.nseg + p];
-          const double res = sqrt(fabs(wdn));
-          const int it = k + 2;  // the start check counted 1
-          beta = wdn / wd;
-          stop = it >= S.maxiter || res <= S.final_res[p];
-          if (first) {
-            P.wdn2[(par ^ 1) * P.nseg + p] = wdn;
-            S.wdn[p] = wdn;
-            S.beta[p] = beta;
-            S.it[p] = it;
-            S.last_res[p] = res;
-            if (stop) {
-              S.done[p] = 1;
-              S.finish[p] = DeviceClock();
-            }
-          }
-        }
-      }
-      if (!stop) BatchDirectionRange<double>(P.s, P.w, P.cstart[c], P.cstart[c + 1], beta);
+      if (!S.done[p]) BatchDirectionRange<double>(P.s, P.w, P.cstart[c], P.cstart[c + 1], S.beta[p]);
     }
     if (!GridBarrier(P.arrive, ++round, G, P.err, &flag)) return;
   }
@@ -982,14 +939,18 @@ static void EnqueueBatchIteration(BatchOp& A, const M* vals, const V* inv, V* u,
 
 // The iterations after the start check in one cooperative launch of
 // BatchCgPersistentKernel (grid: as many workgroups as are resident at once,
-// at most one per row block or chunk).
+// at most one per row block, chunk or problem).
 template <int NNZ, bool RUNS>
 static void LaunchPersistent(const PersistentArgs& P, hipStream_t st) {
   auto kern = BatchCgPersistentKernel<NNZ, RUNS>;
   int occ = 0;
   NGSAMD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, kBatchBlock, 0));
   NGSAMD_REQUIRE(occ >= 1, "batch cg: persistent kernel does not fit a compute unit");
-  int64_t g = std::min<int64_t>((int64_t)occ * DeviceCUs(), std::max(P.nblocks, P.nchunks));
+  // NGSAMD_BATCH_PERSISTENT_WG: workgroups per compute unit (at most the
+  // occupancy)
+  const char* wenv = getenv("NGSAMD_BATCH_PERSISTENT_WG");
+  if (wenv) occ = std::max(1, std::min(occ, atoi(wenv)));
+  int64_t g = std::min<int64_t>((int64_t)occ * DeviceCUs(), std::max({P.nblocks, P.nchunks, P.nseg}));
   if (g > 8) g &= ~int64_t(7);
   if (g < 1) g = 1;
   PersistentArgs args = P;
@@ -999,12 +960,9 @@ static void LaunchPersistent(const PersistentArgs& P, hipStream_t st) {
 
 static void RunPersistent(BatchOp& A, const double* inv, double* u, double* d, double* w, double* s, BatchState S,
                           hipStream_t st) {
-  const int ns = A.nseg;
-  DevBuf<double> cpart(std::max(A.nchunks, 1)), wdn2(2 * (size_t)ns);
-  DevBuf<int> brk(ns), err(1);
+  DevBuf<double> cpart(std::max(A.nchunks, 1));
+  DevBuf<int> err(1);
   DevBuf<unsigned long long> arrive(1);
-  NGSAMD_HIP(hipMemcpyAsync(wdn2.p, S.wdn, ns * sizeof(double), hipMemcpyDeviceToDevice, st));
-  NGSAMD_HIP(hipMemsetAsync(brk.p, 0, ns * sizeof(int), st));
   NGSAMD_HIP(hipMemsetAsync(err.p, 0, sizeof(int), st));
   NGSAMD_HIP(hipMemsetAsync(arrive.p, 0, sizeof(unsigned long long), st));
   PersistentArgs P{};
@@ -1026,7 +984,7 @@ static void RunPersistent(BatchOp& A, const double* inv, double* u, double* d, d
   P.cseg = A.chunk_seg.p;
   P.seg_chunk = A.seg_chunk.p;
   P.nchunks = A.nchunks;
-  P.nseg = ns;
+  P.nseg = A.nseg;
   P.inv = inv;
   P.u = u;
   P.d = d;
@@ -1034,8 +992,6 @@ static void RunPersistent(BatchOp& A, const double* inv, double* u, double* d, d
   P.s = s;
   P.bpart = A.partials.p;
   P.cpart = cpart.p;
-  P.wdn2 = wdn2.p;
-  P.brk = brk.p;
   P.S = S;
   P.arrive = arrive.p;
   P.err = err.p;
