@@ -143,12 +143,33 @@ int ngsamd_sparsecholesky_create(ngsamd_op_t* op, int64_t n, const int64_t* h_ro
 /* h_info[0] = free dofs, [1] = supernodes, [2] = levels, [3] = nonzeros of L */
 int ngsamd_sparsecholesky_info(ngsamd_op_t op, int64_t* h_info);
 
+/* Multigrid V-cycle for the P1 matrix of a structured mesh (the vertex
+ * block of an H1 problem on MakeStructured2DMesh(dims[0], dims[1]) or
+ * MakeStructured3DMesh(dims[0], dims[1], dims[2]); n = number of vertices,
+ * vertices numbered as those builders number them): y ~= A_ff^-1 x on the
+ * free vertices, 0 elsewhere.  Replaces MultigridPreconditioner::Mult / MGM
+ * (multigrid/mgpre.cpp:205-318) with LinearProlongation
+ * (multigrid/prolongation.cpp:77-144) over the hierarchy n, n/2, n/4, ...;
+ * coarse matrices are the Galerkin products P^T A P, smoothing is
+ * smoothing_steps damped-Jacobi sweeps (weight omega in (0, 2)) before and
+ * after each coarse correction, the coarsest level (at most 1024 free
+ * vertices, reached within max_levels levels) is solved with its dense
+ * inverse.  Accepted by ngsamd_cg_solve and as the coarse operator of
+ * ngsamd_twolevel_create. */
+int ngsamd_multigrid_create(ngsamd_op_t* op, int dim, const int* dims, int64_t n, const int64_t* h_rowptr,
+                            const int* h_cols, const double* h_vals, const uint8_t* h_freedofs, int smoothing_steps,
+                            double omega, int max_levels);
+/* *h_nlevels = number of levels; h_level_dofs (may be null, else room for
+ * *h_nlevels entries, at most 32) = free dofs per level, finest first */
+int ngsamd_multigrid_info(ngsamd_op_t op, int* h_nlevels, int64_t* h_level_dofs);
+
 /* Additive two-level preconditioner y = S x + E C E^T x: S a Jacobi or
- * block-Jacobi operator of the whole space, C a sparse Cholesky inverse
- * acting on the first C->height dofs (E their embedding; for a hierarchical
- * H1 space the vertex dofs, i.e. the P1 space).  The composite
- * smoother + E @ a_lo.mat.Inverse(freedofs_lo) @ E.T of the reference's sum
- * and product matrices (linalg/basematrix.hpp:413,592).  The operators are
+ * block-Jacobi operator of the whole space, C a sparse Cholesky inverse or a
+ * multigrid operator acting on the first C->height dofs (E their embedding;
+ * for a hierarchical H1 space the vertex dofs, i.e. the P1 space).  The
+ * composite smoother + E @ a_lo.mat.Inverse(freedofs_lo) @ E.T of the
+ * reference's sum and product matrices (linalg/basematrix.hpp:413,592), or
+ * with the multigrid operator in place of the inverse.  The operators are
  * not owned: destroy the two-level operator first.  Accepted by
  * ngsamd_cg_solve. */
 int ngsamd_twolevel_create(ngsamd_op_t* op, ngsamd_op_t smoother, ngsamd_op_t coarse);

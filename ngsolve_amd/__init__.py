@@ -12,14 +12,16 @@ from .comp import (H1, BilinearForm, CoefficientFunction, DevMatrixFree, Element
                    Integrate, LinearForm, Mesh, MFOpts, dx, grad, x, y, z)
 from .krylovspace import CG, CGSolver, DevCGSolver, IRCGSolver, LinearSolver
 from .la import (BaseMatrix, BaseVector, BitArray, BlockJacobiPrecond, CreateVVector, DeviceMatrix, DevSparseMatrix,
-                 InnerProduct, JacobiPrecond, LowOrderBlock, Norm, Projector, SparseCholesky, TwoLevelPrecond)
+                 InnerProduct, JacobiPrecond, LowOrderBlock, MultiGrid, Norm, Projector, SparseCholesky,
+                 TwoLevelPrecond)
 from .meshes import MakeStructured2DMesh, MakeStructured3DMesh
 
 __all__ = [
     "H1", "BilinearForm", "CoefficientFunction", "DevMatrixFree", "ElementCoefficient", "GridFunction",
     "Integrate", "LinearForm", "Mesh", "MFOpts", "dx", "grad", "x", "y", "z", "CG", "CGSolver", "DevCGSolver", "IRCGSolver",
     "LinearSolver", "BaseMatrix", "BaseVector", "BitArray", "CreateVVector", "DeviceMatrix", "DevSparseMatrix",
-    "InnerProduct", "JacobiPrecond", "BlockJacobiPrecond", "LowOrderBlock", "Norm", "Projector", "SparseCholesky",
+    "InnerProduct", "JacobiPrecond", "BlockJacobiPrecond", "LowOrderBlock", "MultiGrid", "Norm", "Projector",
+    "SparseCholesky",
     "TwoLevelPrecond", "MakeStructured2DMesh", "MakeStructured3DMesh",
     "ensure_device",
 ]

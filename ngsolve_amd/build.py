@@ -20,7 +20,7 @@ BUILD = os.path.join(ROOT, "_build")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("NGSAMD_ARCH", "gfx950")
 
-HIP_SOURCES = ["runtime.hip", "spmv.hip", "cg.hip", "matfree.hip", "batch.hip", "blockjacobi.hip", "topology.hip", "multicg.hip", "sparsechol.hip"]
+HIP_SOURCES = ["runtime.hip", "spmv.hip", "cg.hip", "matfree.hip", "batch.hip", "blockjacobi.hip", "topology.hip", "multicg.hip", "sparsechol.hip", "mg.hip"]
 HOST_SOURCES = ["fe.cpp", "intrule.cpp", "python_module.cpp"]
 
 HIP_FLAGS = [

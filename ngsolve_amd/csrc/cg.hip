@@ -303,14 +303,15 @@ static int EnqueueMatDot(ngsamd_op_t A, const V* s, V* w, int64_t n, CGScalars* 
   return g;
 }
 
-// w = C d for a preconditioner other than Jacobi (block Jacobi); its kernels
-// return at once when *stop != 0
+// w = C d for a preconditioner other than Jacobi (block Jacobi, two-level,
+// sparse Cholesky, multigrid); its kernels return at once when *stop != 0
 template <class V>
 static void EnqueuePrecond(ngsamd_op_t pre, const V* d, V* w, const int* stop, hipStream_t st) {
   if constexpr (sizeof(V) == 8) {
     if (pre->kind == OpKind::BlockJacobi) static_cast<BlockJacobiOp*>(pre)->MultStop(d, w, stop, st);
     else if (pre->kind == OpKind::TwoLevel) static_cast<TwoLevelOp*>(pre)->MultStop(d, w, stop, st);
     else if (pre->kind == OpKind::SparseCholesky) static_cast<SparseCholeskyOp*>(pre)->Apply(1.0, d, w, false, stop, st);
+    else if (pre->kind == OpKind::MultiGrid) MultiGridApply(pre, 1.0, d, w, false, stop, st);
     else throw Error(NGSAMD_E_ARG, "cg: unsupported preconditioner");
   } else {
     throw Error(NGSAMD_E_ARG, "cg: FP32 vectors need a Jacobi preconditioner");
@@ -526,11 +527,11 @@ static CgResult IterateCgDirect(ngsamd_op_t A, const double* inv, ngsamd_op_t pr
 static void CheckCgOperators(ngsamd_op_t A, ngsamd_op_t pre, bool jacobi_only) {
   NGSAMD_REQUIRE(A && pre, "cg: null operator");
   const bool generic = pre->kind == OpKind::BlockJacobi || pre->kind == OpKind::TwoLevel ||
-                       pre->kind == OpKind::SparseCholesky;
+                       pre->kind == OpKind::SparseCholesky || pre->kind == OpKind::MultiGrid;
   NGSAMD_REQUIRE(pre->kind == OpKind::Jacobi || (!jacobi_only && generic),
                  jacobi_only ? "cg: this solver needs a Jacobi preconditioner"
-                             : "cg: the device CG needs a Jacobi, block-Jacobi, two-level or sparse Cholesky "
-                               "preconditioner");
+                             : "cg: the device CG needs a Jacobi, block-Jacobi, two-level, sparse Cholesky or "
+                               "multigrid preconditioner");
   NGSAMD_REQUIRE(A->height == A->width && pre->height == A->height, "cg: operator shapes do not match");
 }
 

@@ -93,7 +93,7 @@ struct DevBuf {
   }
 };
 
-enum class OpKind { CSR, Jacobi, MatrixFree, Batch, BlockJacobi, SparseCholesky, TwoLevel };
+enum class OpKind { CSR, Jacobi, MatrixFree, Batch, BlockJacobi, SparseCholesky, TwoLevel, MultiGrid };
 
 }  // namespace ngsamd
 

@@ -44,6 +44,8 @@ struct CsrOp : ngsamd_op_s {
   // returns the number of partials; skipped entirely when *d_stop != 0.
   // wide: room for kMaxWideBlocks partials of the wide grid (null: the
   // library's)
+  // y = (accumulate ? y : 0) + s A x; returns at once when *stop != 0
+  void Apply(double s, const double* x, double* y, bool accumulate, const int* stop, hipStream_t st);
   int MultDot(const double* x, double* y, double* partials, const int* d_stop, hipStream_t st,
               double* wide = nullptr);
   // Y = A X for K vectors stored interleaved ([n][K]); the matrix is read
@@ -127,15 +129,23 @@ struct SparseCholeskyOp : ngsamd_op_s {
   void Apply(double s, const double* x, double* y, bool accumulate, const int* stop, hipStream_t st);
 };
 
+// Multigrid V-cycle for the P1 block of a structured mesh (mg.hip):
+// y = (accumulate ? y : 0) + s M x on the first op->height dofs; every kernel
+// returns at once when *stop != 0
+void MultiGridApply(ngsamd_op_t op, double s, const double* x, double* y, bool accumulate, const int* stop,
+                    hipStream_t st);
+
 // Additive two-level preconditioner y = S x + E C E^T x (sparsechol.hip):
-// S Jacobi or block Jacobi, C a sparse Cholesky inverse of the first
-// C->height dofs.  The operators are not owned.
+// S Jacobi or block Jacobi, C a sparse Cholesky inverse or a multigrid
+// V-cycle of the first C->height dofs.  The operators are not owned.
 struct TwoLevelOp : ngsamd_op_s {
   ngsamd_op_t smoother, coarse;
   TwoLevelOp(ngsamd_op_t smoother, ngsamd_op_t coarse);
   void MultAdd(double s, const double* x, double* y, hipStream_t st) override;
   void Mult(const double* x, double* y, hipStream_t st) override;
   void MultStop(const double* x, double* y, const int* stop, hipStream_t st);
+  // y += s E C E^T x
+  void CoarseApply(double s, const double* x, double* y, const int* stop, hipStream_t st);
 };
 
 }  // namespace ngsamd

@@ -598,7 +598,8 @@ TwoLevelOp::TwoLevelOp(ngsamd_op_t sm, ngsamd_op_t co) : ngsamd_op_s(OpKind::Two
   NGSAMD_REQUIRE(sm && co, "ngsamd_twolevel_create: null operator");
   NGSAMD_REQUIRE(sm->kind == OpKind::Jacobi || sm->kind == OpKind::BlockJacobi,
                  "ngsamd_twolevel_create: the smoother must be a Jacobi or block-Jacobi operator");
-  NGSAMD_REQUIRE(co->kind == OpKind::SparseCholesky, "ngsamd_twolevel_create: the coarse operator must be a sparse Cholesky inverse");
+  NGSAMD_REQUIRE(co->kind == OpKind::SparseCholesky || co->kind == OpKind::MultiGrid,
+                 "ngsamd_twolevel_create: the coarse operator must be a sparse Cholesky inverse or a multigrid operator");
   NGSAMD_REQUIRE(sm->height == sm->width && co->height == co->width && co->height <= sm->height,
                  "ngsamd_twolevel_create: the coarse space must be the first dofs of the smoother's");
   height = width = sm->height;
@@ -613,17 +614,22 @@ void TwoLevelOp::MultStop(const double* x, double* y, const int* stop, hipStream
   } else {
     static_cast<BlockJacobiOp*>(smoother)->MultStop(x, y, stop, st);
   }
-  static_cast<SparseCholeskyOp*>(coarse)->Apply(1.0, x, y, true, stop, st);
+  CoarseApply(1.0, x, y, stop, st);
+}
+
+void TwoLevelOp::CoarseApply(double s, const double* x, double* y, const int* stop, hipStream_t st) {
+  if (coarse->kind == OpKind::MultiGrid) MultiGridApply(coarse, s, x, y, true, stop, st);
+  else static_cast<SparseCholeskyOp*>(coarse)->Apply(s, x, y, true, stop, st);
 }
 
 void TwoLevelOp::MultAdd(double s, const double* x, double* y, hipStream_t st) {
   smoother->MultAdd(s, x, y, st);
-  static_cast<SparseCholeskyOp*>(coarse)->Apply(s, x, y, true, nullptr, st);
+  CoarseApply(s, x, y, nullptr, st);
 }
 
 void TwoLevelOp::Mult(const double* x, double* y, hipStream_t st) {
   smoother->Mult(x, y, st);
-  static_cast<SparseCholeskyOp*>(coarse)->Apply(1.0, x, y, true, nullptr, st);
+  CoarseApply(1.0, x, y, nullptr, st);
 }
 
 }  // namespace ngsamd

@@ -731,6 +731,12 @@ void CsrOp::Mult(const double* x, double* y, hipStream_t st) {
   LaunchSpmv<false, false>(*this, 1.0, x, y, st, nullptr, nullptr);
 }
 
+void CsrOp::Apply(double s, const double* x, double* y, bool accumulate, const int* stop, hipStream_t st) {
+  if (height == 0) return;
+  if (accumulate) LaunchSpmv<true, false>(*this, s, x, y, st, nullptr, stop);
+  else LaunchSpmv<false, false>(*this, s, x, y, st, nullptr, stop);
+}
+
 int CsrOp::MultDot(const double* x, double* y, double* partials, const int* d_stop, hipStream_t st,
                    double* wide) {
   // the operator on its wide grid writes one partial per workgroup; a

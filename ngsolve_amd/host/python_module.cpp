@@ -607,6 +607,33 @@ PYBIND11_MODULE(_ngsamd, m) {
     d["nnz_factor"] = info[3];
     return d;
   });
+  m.def("CreateMultiGrid", [](int dim, std::vector<int> dims, int64_t n, Arr<int64_t> rowptr, Arr<int> cols,
+                              Arr<double> vals, py::object freedofs, int smoothing_steps, double omega,
+                              int max_levels) {
+    if (rowptr.size() != n + 1 || cols.size() != vals.size() || rowptr.data()[n] != (int64_t)cols.size())
+      throw std::runtime_error("CreateMultiGrid: inconsistent CSR arrays");
+    if ((int)dims.size() != dim) throw std::runtime_error("CreateMultiGrid: dims must have dim entries");
+    std::vector<uint8_t> fr;
+    if (!freedofs.is_none()) {
+      auto a = freedofs.cast<Arr<uint8_t>>();
+      if ((int64_t)a.size() != n) throw std::runtime_error("CreateMultiGrid: freedofs size");
+      fr.assign(a.data(), a.data() + a.size());
+    }
+    ngsamd_op_t op = nullptr;
+    {
+      py::gil_scoped_release rel;
+      Check(ngsamd_multigrid_create(&op, dim, dims.data(), n, rowptr.data(), cols.data(), vals.data(),
+                                    fr.empty() ? nullptr : fr.data(), smoothing_steps, omega, max_levels));
+    }
+    return std::make_shared<Operator>(op, "multigrid");
+  });
+  m.def("MultiGridInfo", [](Operator& A) {
+    int nl = 0;
+    std::vector<int64_t> dofs(32, 0);
+    Check(ngsamd_multigrid_info(A.op, &nl, dofs.data()));
+    dofs.resize(nl);
+    return dofs;
+  });
   m.def("CreateTwoLevel", [](std::shared_ptr<Operator> smoother, std::shared_ptr<Operator> coarse) {
     ngsamd_op_t op = nullptr;
     Check(ngsamd_twolevel_create(&op, smoother->op, coarse->op));

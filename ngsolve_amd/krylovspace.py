@@ -14,7 +14,8 @@ IRCGSolver    mixed-precision iterative refinement: FP64 residual f - A u
 """
 from math import log, sqrt
 
-from .la import BaseMatrix, BlockJacobiPrecond, DeviceMatrix, JacobiPrecond, Projector, SparseCholesky, TwoLevelPrecond
+from .la import (BaseMatrix, BlockJacobiPrecond, DeviceMatrix, JacobiPrecond, MultiGrid, Projector, SparseCholesky,
+                 TwoLevelPrecond)
 
 
 class LinearSolver(BaseMatrix):
@@ -149,9 +150,9 @@ class DevCGSolver(LinearSolver):
             maxiter = maxsteps
         super().__init__(mat, pre=pre, tol=tol, maxiter=maxiter, atol=atol, printrates=printrates)
         if not isinstance(mat, DeviceMatrix) or not isinstance(pre, (JacobiPrecond, BlockJacobiPrecond,
-                                                                     TwoLevelPrecond, SparseCholesky)):
+                                                                     TwoLevelPrecond, SparseCholesky, MultiGrid)):
             raise TypeError("DevCGSolver needs a device matrix and a JacobiPrecond, BlockJacobiPrecond, "
-                            "TwoLevelPrecond or SparseCholesky preconditioner")
+                            "TwoLevelPrecond, SparseCholesky or MultiGrid preconditioner")
         self.chunk = chunk
 
     def _SolveImpl(self, rhs, sol):

@@ -688,18 +688,46 @@ class SparseCholesky(DeviceMatrix):
         self.info = N.SparseCholeskyInfo(op)
 
 
+class MultiGrid(DeviceMatrix):
+    """Multigrid V-cycle preconditioner for the P1 matrix of a structured
+    mesh (MultigridPreconditioner, multigrid/mgpre.cpp:205-318, with
+    LinearProlongation, multigrid/prolongation.cpp:77-144): y ~= A_ff^-1 x.
+    mat: the P1 matrix over the mesh's vertices (for a hierarchical H1 space
+    LowOrderBlock(a.mat, mesh.nv)); dims: the cells per direction the mesh
+    was built with (MakeStructured2DMesh(nx, ny) -> (nx, ny),
+    MakeStructured3DMesh(nx, ny, nz) -> (nx, ny, nz)).  The hierarchy halves
+    the grid while every size is even and the level has more than 1024 free
+    vertices; coarse levels are Galerkin products; smoothing_steps damped
+    Jacobi sweeps (weight omega) before and after each coarse correction.
+    DevCGSolver accepts it, and TwoLevelPrecond as its coarse part."""
+
+    def __init__(self, mat, dims, freedofs=None, smoothing_steps=2, omega=0.85, max_levels=32):
+        N = ensure_device()
+        fr = None if freedofs is None else np.ascontiguousarray(_bits(freedofs), dtype=np.uint8)
+        dims = [int(d) for d in dims]
+        op = N.CreateMultiGrid(len(dims), dims, int(mat.Height()), np.ascontiguousarray(mat.rowptr, dtype=np.int64),
+                               np.ascontiguousarray(mat.cols, dtype=np.int32),
+                               np.ascontiguousarray(mat.vals, dtype=np.float64), fr, int(smoothing_steps),
+                               float(omega), int(max_levels))
+        super().__init__(op, True)
+        self.levels = N.MultiGridInfo(op)
+
+
 class TwoLevelPrecond(DeviceMatrix):
     """Additive two-level preconditioner y = S x + E C E^T x: S a Jacobi or
-    block-Jacobi preconditioner, C a SparseCholesky acting on the first
-    C.Height() dofs (E their embedding: the vertex dofs of a hierarchical H1
-    space, i.e. its P1 part).  The device form of the composite
-    smoother + E @ a_lo.mat.Inverse(freedofs_lo) @ E.T (SumMatrix /
-    ProductMatrix, linalg/basematrix.hpp:413,592); DevCGSolver accepts it."""
+    block-Jacobi preconditioner, C a SparseCholesky or a MultiGrid acting on
+    the first C.Height() dofs (E their embedding: the vertex dofs of a
+    hierarchical H1 space, i.e. its P1 part).  The device form of the
+    composite smoother + E @ a_lo.mat.Inverse(freedofs_lo) @ E.T (SumMatrix /
+    ProductMatrix, linalg/basematrix.hpp:413,592), or with a multigrid cycle
+    for the inverse; DevCGSolver accepts it."""
 
     def __init__(self, smoother, coarse):
         N = ensure_device()
-        if not isinstance(smoother, (JacobiPrecond, BlockJacobiPrecond)) or not isinstance(coarse, SparseCholesky):
-            raise TypeError("TwoLevelPrecond needs a Jacobi or block-Jacobi smoother and a SparseCholesky")
+        if not isinstance(smoother, (JacobiPrecond, BlockJacobiPrecond)) or \
+                not isinstance(coarse, (SparseCholesky, MultiGrid)):
+            raise TypeError("TwoLevelPrecond needs a Jacobi or block-Jacobi smoother and a SparseCholesky or "
+                            "MultiGrid")
         super().__init__(N.CreateTwoLevel(smoother.op, coarse.op), True)
         self.smoother, self.coarse = smoother, coarse
 

@@ -21,6 +21,9 @@ This module restates, in numpy, the reference's algorithms for:
   block Jacobi             linalg/blockjacobi.cpp:390, 404, 498, 596-632
   sparse direct inverse    linalg/sparsecholesky.cpp (a.mat.Inverse on free dofs)
   two-level preconditioner smoother + E C E^T (linalg/basematrix.hpp:413,592)
+  multigrid V-cycle        multigrid/mgpre.cpp:205-318 with LinearProlongation
+                           (multigrid/prolongation.cpp:77-144) on the
+                           structured vertex hierarchy, Galerkin coarse levels
   lockstep multi-RHS CG    linalg/cg.cpp:132-250 (CGSolver::MultiMult)
 
 It is deliberately written differently from the product code: shapes are
@@ -677,6 +680,124 @@ def two_level(smoother, coarse, nc):
         return y
 
     return apply
+
+
+def _grid_coords(dim, dims):
+    """Grid coordinates of the vertices of the structured meshes above, in
+    vertex order: 3D (i, j, k) with v = (i (ny+1) + j) (nz+1) + k; 2D
+    (row, column) with v = row (nx+1) + column."""
+    if dim == 3:
+        nx, ny, nz = dims
+        i, j, k = np.meshgrid(np.arange(nx + 1), np.arange(ny + 1), np.arange(nz + 1), indexing="ij")
+        return np.stack([i.ravel(), j.ravel(), k.ravel()], 1)
+    nx, ny = dims[0], dims[1]
+    r, c = np.meshgrid(np.arange(ny + 1), np.arange(nx + 1), indexing="ij")
+    return np.stack([r.ravel(), c.ravel()], 1)
+
+
+def _grid_number(dim, dims, X):
+    if dim == 3:
+        nx, ny, nz = dims
+        return (X[:, 0] * (ny + 1) + X[:, 1]) * (nz + 1) + X[:, 2]
+    return X[:, 0] * (dims[0] + 1) + X[:, 1]
+
+
+def mg_prolongation(dim, dims):
+    """Linear prolongation between the structured vertex grids dims/2 (coarse)
+    and dims (fine) over all vertices: LinearProlongation
+    (multigrid/prolongation.cpp:77-144), each vertex created by refinement
+    the mean of the two end points of the coarse edge it halves.  The edge of
+    a vertex with odd coordinates: in 3D from X//2 along the 0/1 vector X%2
+    (the Kuhn cubes' edges); in 2D with both coordinates odd the square's
+    diagonal from (row R, column C+1) to (row R+1, column C).  scipy CSR,
+    shape (fine vertices, coarse vertices)."""
+    import scipy.sparse as sp
+    X = _grid_coords(dim, dims)
+    cdims = [d // 2 for d in dims]
+    lo, odd = X // 2, X % 2
+    nf = len(X)
+    rows, cols, vals = [], [], []
+    plain = odd.sum(1) == 0
+    rows.append(np.flatnonzero(plain))
+    cols.append(_grid_number(dim, cdims, lo[plain]))
+    vals.append(np.ones(plain.sum()))
+    mid = ~plain
+    if dim == 2:
+        diag = odd.sum(1) == 2
+        edge = mid & ~diag
+        a, b = lo[edge], lo[edge] + odd[edge]
+        da = lo[diag] + np.array([0, 1])
+        db = lo[diag] + np.array([1, 0])
+        for sel, P, Q in ((edge, a, b), (diag, da, db)):
+            idx = np.flatnonzero(sel)
+            rows += [idx, idx]
+            cols += [_grid_number(dim, cdims, P), _grid_number(dim, cdims, Q)]
+            vals += [np.full(len(idx), 0.5)] * 2
+    else:
+        idx = np.flatnonzero(mid)
+        rows += [idx, idx]
+        cols += [_grid_number(dim, cdims, lo[mid]), _grid_number(dim, cdims, lo[mid] + odd[mid])]
+        vals += [np.full(len(idx), 0.5)] * 2
+    ncv = int(np.prod([d + 1 for d in cdims]))
+    return sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(nf, ncv))
+
+
+def mg_hierarchy(dim, dims, rowptr, cols, vals, free, max_levels=32, dense_max=1024):
+    """Level hierarchy of the structured-mesh multigrid: the free block of the
+    finest P1 matrix, then coarse levels while every grid size is even, the
+    level holds more than dense_max free vertices and max_levels allows; a
+    coarse vertex is free iff the fine vertex at the same place is; coarse
+    matrices are the Galerkin products P^T A P over the free vertices (the
+    reference assembles each level of its mesh hierarchy, mgpre.cpp; for a
+    coefficient constant on coarse elements the same matrices).  Returns
+    dict(free=[free vertex numbers of the finest level], A=[...], P=[...])."""
+    import scipy.sparse as sp
+    n = len(rowptr) - 1
+    A = sp.csr_matrix((np.asarray(vals, dtype=float), np.asarray(cols), np.asarray(rowptr)), shape=(n, n))
+    fr = np.flatnonzero(np.asarray(free) != 0)
+    As, Ps = [A[fr][:, fr].tocsr()], []
+    fmask = np.asarray(free) != 0
+    d = list(dims)
+    while len(As) < max_levels and As[-1].shape[0] > dense_max and all(x % 2 == 0 for x in d):
+        cd = [x // 2 for x in d]
+        Pall = mg_prolongation(dim, d)
+        Xc = _grid_coords(dim, cd)
+        cmask = fmask[_grid_number(dim, d, 2 * Xc)]
+        if not cmask.any():
+            break
+        P = Pall[np.flatnonzero(fmask)][:, np.flatnonzero(cmask)].tocsr()
+        Ps.append(P)
+        As.append((P.T @ As[-1] @ P).tocsr())
+        fmask, d = cmask, cd
+    return dict(free=fr, A=As, P=Ps, n=n)
+
+
+def mg_vcycle(H, r, nu=2, omega=2.0 / 3.0, level=0):
+    """One V-cycle MGM (multigrid/mgpre.cpp:229-318) on level `level` for the
+    right-hand side r: nu damped-Jacobi sweeps from 0, residual, restriction
+    P^T, the cycle on the next level, prolongation, nu sweeps; the coarsest
+    level solved exactly (dense solve)."""
+    A = H["A"][level]
+    if level == len(H["A"]) - 1:
+        return np.linalg.solve(A.toarray(), r) if A.shape[0] else np.zeros(0)
+    dinv = 1.0 / A.diagonal()
+    x = np.zeros_like(r)
+    for _ in range(nu):
+        x = x + omega * dinv * (r - A @ x)
+    P = H["P"][level]
+    x = x + P @ mg_vcycle(H, P.T @ (r - A @ x), nu, omega, level + 1)
+    for _ in range(nu):
+        x = x + omega * dinv * (r - A @ x)
+    return x
+
+
+def mg_apply(H, x, nu=2, omega=2.0 / 3.0):
+    """The multigrid preconditioner on a full vertex vector: the V-cycle on the
+    free entries, 0 on the others."""
+    y = np.zeros(H["n"])
+    f = H["free"]
+    y[f] = mg_vcycle(H, np.asarray(x, dtype=float)[f], nu, omega)
+    return y
 
 
 def pcg(matvec, invdiag, f, tol=1e-12, maxiter=100, atol=None):
