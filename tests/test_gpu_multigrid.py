@@ -66,6 +66,23 @@ def test_multigrid_apply_matches_oracle(device, dim, n, p):
     assert np.array_equal(dz.download(), y)
 
 
+@pytest.mark.parametrize("nu", [1, 3])
+def test_multigrid_smoothing_steps(device, nu):
+    """Other numbers of smoothing sweeps (the fused first and last passes
+    take other branches): the apply against the oracle's V-cycle."""
+    P = problem(3, 16, 1, seed_kappa=8)
+    A, free = _vertex_block(P)
+    M = N.CreateMultiGrid(3, [16] * 3, A.shape[0], A.indptr.astype(np.int64), A.indices.astype(np.int32),
+                          A.data.astype(np.float64), free, nu, 0.7, 32)
+    H = O.mg_hierarchy(3, [16] * 3, A.indptr, A.indices, A.data, free)
+    x = np.random.default_rng(9).uniform(-1, 1, A.shape[0])
+    dx, dy = dvec(x), N.DeviceVector(A.shape[0], 8)
+    M.mult(dx, dy)
+    err = relerr_inf(dy.download(), O.mg_apply(H, x, nu, 0.7))
+    say(f"[mg] nu={nu}: apply vs oracle {err:.2e}")
+    assert err < 1e-12
+
+
 def test_multigrid_rejects_bad_input(device):
     P = problem(3, 6, 1)
     A, free = _vertex_block(P)
