@@ -714,6 +714,11 @@ __global__ __launch_bounds__(kMfBlock) void MfRef16Kernel(
 // LDS, lane order so that each lane's A operands are 16-byte reads:
 //   A1[t][grp][col][KSP]  row col of tile t (slot of AccRow^-1(col)), K = dof
 //   A2[m][grp][col][JP]   row = dof 16 m + col, K = slot of lane group grp
+// Register budget: unbounded (FP64 106 VGPRs, FP32 81, 16-bit 104: two
+// 8-wave workgroups per CU).  Holding the FP32 and 16-bit tiles to 80 VGPRs
+// (three workgroups per CU, no spills) measured slower at 32^3 p=4: FP32
+// 38.6 -> 42.9 us, 16-bit 39.9 -> 44 us, 16-bit with FP64 vectors 52 -> 73
+// us -- more tiles in flight per CU only add pressure on L1 and L2.
 template <class T, int DIM, int NDP, int KG, int MODE, class V>
 __global__ __launch_bounds__(kMfBlock) void MfFacKernel(
     const T* __restrict__ fmat, int64_t fstride, MfRounds R, const int* __restrict__ dofs,
