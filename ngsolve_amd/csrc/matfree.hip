@@ -718,7 +718,9 @@ __global__ __launch_bounds__(kMfBlock) void MfRef16Kernel(
 // 8-wave workgroups per CU).  Holding the FP32 and 16-bit tiles to 80 VGPRs
 // (three workgroups per CU, no spills) measured slower at 32^3 p=4: FP32
 // 38.6 -> 42.9 us, 16-bit 39.9 -> 44 us, 16-bit with FP64 vectors 52 -> 73
-// us -- more tiles in flight per CU only add pressure on L1 and L2.
+// us -- more tiles in flight per CU only add pressure on L1 and L2; one
+// workgroup per CU is slower too (FP32 vectors 64.6 against 62.6 us per
+// apply).
 template <class T, int DIM, int NDP, int KG, int MODE, class V>
 __global__ __launch_bounds__(kMfBlock) void MfFacKernel(
     const T* __restrict__ fmat, int64_t fstride, MfRounds R, const int* __restrict__ dofs,
