@@ -36,6 +36,7 @@ struct CsrOp : ngsamd_op_s {
   int nblocks = 0;
   int block_nnz = 2048;
   int max_grid = 16384;  // workgroups of the stream/runs kernels
+  int flip = 0;          // direction of the next Mult / MultAdd block walk (alternates)
   // v null: the structure only, values zero until filled on the device
   CsrOp(int64_t h, int64_t w, const int64_t* rp, const int* c, const double* v);
   void MultAdd(double s, const double* x, double* y, hipStream_t st) override;
@@ -44,8 +45,6 @@ struct CsrOp : ngsamd_op_s {
   // returns the number of partials; skipped entirely when *d_stop != 0.
   // wide: room for kMaxWideBlocks partials of the wide grid (null: the
   // library's)
-  // y = (accumulate ? y : 0) + s A x; returns at once when *stop != 0
-  void Apply(double s, const double* x, double* y, bool accumulate, const int* stop, hipStream_t st);
   int MultDot(const double* x, double* y, double* partials, const int* d_stop, hipStream_t st,
               double* wide = nullptr);
   // Y = A X for K vectors stored interleaved ([n][K]); the matrix is read

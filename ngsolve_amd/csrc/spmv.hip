@@ -182,22 +182,37 @@ __global__ __launch_bounds__(kStreamBlock) void CsrStreamKernel(
 // + q then sums the slots of its run at stride R.  A block holding more
 // than NNZ values is a single row (R = 1) that the whole workgroup strides.
 // NNZ <= 2048: at most 64 VGPRs, so 8 workgroups fit a CU (72 registers
-// allowed 7; 394 vs 398 us at 32^3 p=4)
+// allowed 7; 394 vs 398 us at 32^3 p=4).
+// Consecutive plain applies (Mult / MultAdd) alternate the direction of the
+// block walk (reverse), so that each apply starts on the blocks the previous
+// one read last, which the 256 MB Infinity Cache still holds; their values
+// are read with plain loads for that (non-temporal ones are not kept):
+// 374-375 against 383-384 us per apply at 32^3 p=4 (forward only with plain
+// loads: 395).  Every row is summed by the same code in the same order
+// either way.  Inside CG (DOT) the vector kernels between two applies take
+// that cache, so the CG variant keeps the forward walk and non-temporal
+// value loads (with the alternating walk and plain loads Jacobi-CG at 32^3
+// p=4 took 0.142 against 0.140 s, and with the sparse Cholesky coarse solve
+// 0.184 against 0.173 s).
 template <int NNZ, bool ACCUM, bool DOT>
 __global__ __launch_bounds__(kStreamBlock, NNZ <= 2048 ? 8 : 4) void CsrRunsKernel(
     const int* __restrict__ run_row, const int* __restrict__ run_ptr, const int* __restrict__ val_ptr,
     const int* __restrict__ rcols, const double* __restrict__ rvals, const double* __restrict__ x,
     double* __restrict__ y, const int* __restrict__ blk_run, const int* __restrict__ blk_len, int nblocks,
-    int blocks_per_wg, double s, double* __restrict__ partials, const int* __restrict__ stop_flag) {
+    int blocks_per_wg, double s, double* __restrict__ partials, const int* __restrict__ stop_flag, int reverse) {
   static_assert(NNZ <= 4096, "RunDiv is exact below 4096");
   if (stop_flag && *stop_flag) return;
   constexpr int PER = NNZ / kStreamBlock;
   __shared__ double prod[NNZ];
   __shared__ double lds[kStreamBlock / 64];
   const int tid = threadIdx.x;
-  const int lb = XcdContiguousBlock(blockIdx.x, gridDim.x);
+  // reverse: the first workgroups dispatched take the last blocks, each
+  // workgroup walks its range backwards
+  const int lx = XcdContiguousBlock(blockIdx.x, gridDim.x);
+  const int lb = reverse ? (int)gridDim.x - 1 - lx : lx;
   const int b0 = lb * blocks_per_wg;
   const int b1 = min(b0 + blocks_per_wg, nblocks);
+  auto blk = [&](int t) { return reverse ? b0 + b1 - 1 - t : t; };
   double dotacc = 0.0;
 
   // header and register-held loads of the block in flight
@@ -220,10 +235,10 @@ __global__ __launch_bounds__(kStreamBlock, NNZ <= 2048 ? 8 : 4) void CsrRunsKern
       const int k = tid + i * kStreamBlock;
       const int kk = k < nv ? k : 0;
       c[i] = __builtin_nontemporal_load(&rcols[c0 + RunDiv(kk, inv)]);
-      v[i] = __builtin_nontemporal_load(&rvals[vb + kk]);
+      v[i] = DOT ? __builtin_nontemporal_load(&rvals[vb + kk]) : rvals[vb + kk];
     }
   };
-  if (b0 < b1) fetch(b0);
+  if (b0 < b1) fetch(blk(b0));
   for (int b = b0; b < b1; b++) {
     if (nv > NNZ) {
       // one long row: the whole workgroup strides over it
@@ -236,7 +251,7 @@ __global__ __launch_bounds__(kStreamBlock, NNZ <= 2048 ? 8 : 4) void CsrRunsKern
         if (ACCUM) y[r] += s * a; else y[r] = s * a;
         if (DOT) dotacc += x[r] * a;
       }
-      if (b + 1 < b1) fetch(b + 1);
+      if (b + 1 < b1) fetch(blk(b + 1));
       continue;
     }
     {
@@ -282,7 +297,7 @@ __global__ __launch_bounds__(kStreamBlock, NNZ <= 2048 ? 8 : 4) void CsrRunsKern
       }
     }
     __syncthreads();
-    if (b + 1 < b1) fetch(b + 1);
+    if (b + 1 < b1) fetch(blk(b + 1));
   }
   if (DOT) StorePartial<kStreamBlock>(dotacc, partials, lds);
 }
@@ -417,7 +432,7 @@ __global__ void ScatterComponentKernel(const double* __restrict__ in, int K, int
 
 template <bool ACCUM, bool DOT>
 static int LaunchSpmv(const CsrOp& A, double s, const double* x, double* y, hipStream_t st, double* partials,
-                      const int* stop);
+                      const int* stop, int reverse = 0);
 
 void CsrOp::MultMulti(int K, const double* x, double* y, hipStream_t st, const int* stop) {
   NGSAMD_REQUIRE(K >= 1 && K <= 64, "multi-vector apply: 1 <= K <= 64 vectors");
@@ -688,7 +703,7 @@ static int LaunchStream(const CsrOp& A, double s, const double* x, double* y, hi
 
 template <bool ACCUM, bool DOT>
 static int LaunchRuns(const CsrOp& A, double s, const double* x, double* y, hipStream_t st, double* partials,
-                      const int* stop) {
+                      const int* stop, int reverse) {
   auto kern = A.block_nnz == 1024   ? CsrRunsKernel<1024, ACCUM, DOT>
               : A.block_nnz == 4096 ? CsrRunsKernel<4096, ACCUM, DOT>
                                     : CsrRunsKernel<2048, ACCUM, DOT>;
@@ -702,16 +717,16 @@ static int LaunchRuns(const CsrOp& A, double s, const double* x, double* y, hipS
   if (wg < 1) wg = 1;
   const int per = (A.nblocks + wg - 1) / wg;
   kern<<<(int)wg, kStreamBlock, 0, st>>>(A.run_row.p, A.run_ptr.p, A.val_ptr.p, A.rcols.p, A.rvals.p, x, y,
-                                         A.blk_run.p, A.blk_len.p, A.nblocks, per, s, partials, stop);
+                                         A.blk_run.p, A.blk_len.p, A.nblocks, per, s, partials, stop, reverse);
   NGSAMD_CHECK_LAUNCH();
   return (int)wg;
 }
 
 template <bool ACCUM, bool DOT>
 static int LaunchSpmv(const CsrOp& A, double s, const double* x, double* y, hipStream_t st, double* partials,
-                      const int* stop) {
+                      const int* stop, int reverse) {
   if (A.width == 0) x = A.xpad.p;
-  if (A.algo == 2) return LaunchRuns<ACCUM, DOT>(A, s, x, y, st, partials, stop);
+  if (A.algo == 2) return LaunchRuns<ACCUM, DOT>(A, s, x, y, st, partials, stop, reverse);
   if (A.algo == 1) return LaunchStream<ACCUM, DOT>(A, s, x, y, st, partials, stop);
   switch (A.group) {
     case 4: return LaunchSpmvG<4, ACCUM, DOT>(A, s, x, y, st, partials, stop);
@@ -723,18 +738,14 @@ static int LaunchSpmv(const CsrOp& A, double s, const double* x, double* y, hipS
 
 void CsrOp::MultAdd(double s, const double* x, double* y, hipStream_t st) {
   if (height == 0) return;
-  LaunchSpmv<true, false>(*this, s, x, y, st, nullptr, nullptr);
+  LaunchSpmv<true, false>(*this, s, x, y, st, nullptr, nullptr, flip);
+  flip ^= 1;
 }
 
 void CsrOp::Mult(const double* x, double* y, hipStream_t st) {
   if (height == 0) return;
-  LaunchSpmv<false, false>(*this, 1.0, x, y, st, nullptr, nullptr);
-}
-
-void CsrOp::Apply(double s, const double* x, double* y, bool accumulate, const int* stop, hipStream_t st) {
-  if (height == 0) return;
-  if (accumulate) LaunchSpmv<true, false>(*this, s, x, y, st, nullptr, stop);
-  else LaunchSpmv<false, false>(*this, s, x, y, st, nullptr, stop);
+  LaunchSpmv<false, false>(*this, 1.0, x, y, st, nullptr, nullptr, flip);
+  flip ^= 1;
 }
 
 int CsrOp::MultDot(const double* x, double* y, double* partials, const int* d_stop, hipStream_t st,

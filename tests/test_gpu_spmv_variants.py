@@ -81,6 +81,29 @@ def test_spmv_size_dependent_defaults(device, monkeypatch, case):
     assert np.linalg.norm(du.download() - u_ref) <= 1e-10 * np.linalg.norm(u_ref)
 
 
+def test_spmv_alternating_block_walk_is_bit_identical(device):
+    """Consecutive applies of the runs kernel walk the blocks in alternating
+    directions: every apply gives the same bits (each row is summed by the
+    same code in the same order), for Mult and MultAdd."""
+    P = problem(3, 8, 4, seed_kappa=5)
+    A = N.CreateCSR(P["ndof"], P["ndof"], P["rowptr"], P["cols"], P["vals"])
+    x = np.random.default_rng(3).uniform(-1, 1, P["ndof"])
+    dx, dy = dvec(x), N.DeviceVector(P["ndof"], 8)
+    outs = []
+    for _ in range(3):
+        A.mult(dx, dy)
+        outs.append(dy.download())
+    assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[1], outs[2])
+    assert relerr_inf(outs[0], oracle.spmv(P["rowptr"], P["cols"], P["vals"], x)) < 1e-13
+    y0 = np.random.default_rng(4).uniform(-1, 1, P["ndof"])
+    adds = []
+    for _ in range(2):
+        dy.upload(y0)
+        A.mult_add(0.25, dx, dy)
+        adds.append(dy.download())
+    assert np.array_equal(adds[0], adds[1])
+
+
 def test_ir_without_graphs_matches_graphs(device, monkeypatch):
     say("[ir] NGSAMD_CG_CHUNK=0 vs graphs")
     P = problem(3, 4, 4, order=8, seed_kappa=3)
