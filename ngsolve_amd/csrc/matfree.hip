@@ -99,6 +99,7 @@ struct MfRounds {
   const int64_t* range;  // element range [begin, end) of each class
   int nrounds;
   int nclass;
+  const int* tbase = nullptr;  // first tile of each class (tile-major tables), nclass + 1
 };
 
 // classes whose schedule the reference-matrix kernels keep in LDS
@@ -744,9 +745,15 @@ __global__ __launch_bounds__(kMfBlock) void MfFacKernel(
   const int wg = XcdContiguousBlock(blockIdx.x, gridDim.x);
   const int r0 = (int)((int64_t)wg * R.nrounds / gridDim.x);
   const int r1 = (int)((int64_t)(wg + 1) * R.nrounds / gridDim.x);
+  // MODE 0 / 1 read the tile-major tables, MODE 2 (a colour's) element-major ones
+  constexpr bool TILED = MODE != 2;
   __shared__ int s_round0[kMaxLdsClasses + 1];
+  __shared__ int s_tbase[kMaxLdsClasses + 1];
   __shared__ int64_t s_range[2 * kMaxLdsClasses];
-  for (int i = threadIdx.x; i <= R.nclass; i += kMfBlock) s_round0[i] = R.round0[i];
+  for (int i = threadIdx.x; i <= R.nclass; i += kMfBlock) {
+    s_round0[i] = R.round0[i];
+    if (TILED) s_tbase[i] = R.tbase[i];
+  }
   for (int i = threadIdx.x; i < 2 * R.nclass; i += kMfBlock) s_range[i] = R.range[i];
   __syncthreads();
   const T* a1 = ls + lane * KSP;
@@ -768,13 +775,17 @@ __global__ __launch_bounds__(kMfBlock) void MfFacKernel(
     const bool valid = e < end;
     // unconditional gathers from clamped indices, masked afterwards
     const int64_t ev = valid ? e : first;
+    const int64_t tile = TILED ? s_tbase[c] + (first - s_range[2 * c]) / 16 : 0;
     int dv[KS];
 #pragma unroll
-    for (int st = 0; st < KS; st++) dv[st] = dofs[ev * nd + min(4 * st + grp, nd - 1)];
-    const int en = elnr[ev];
+    for (int st = 0; st < KS; st++) {
+      const int k = min(4 * st + grp, nd - 1);
+      dv[st] = TILED ? dofs[(tile * nd + k) * 16 + col] : dofs[ev * nd + k];
+    }
+    const int en = TILED ? elnr[tile * 16 + col] : elnr[ev];
     double mv[NC];
 #pragma unroll
-    for (int cc = 0; cc < NC; cc++) mv[cc] = metric[ev * NC + cc];
+    for (int cc = 0; cc < NC; cc++) mv[cc] = TILED ? metric[(tile * NC + cc) * 16 + col] : metric[ev * NC + cc];
     V xv[KS];
 #pragma unroll
     for (int st = 0; st < KS; st++) xv[st] = x[dv[st]];
@@ -837,7 +848,7 @@ __global__ __launch_bounds__(kMfBlock) void MfFacKernel(
           if (dof < nd) {
             const V val = (V)yacc[m][r];
             if (MODE == 0) out[e * nd + dof] = val;
-            else if (MODE == 1) AtomicAddV(&out[dofs[e * nd + dof]], (V)(s * (double)val));
+            else if (MODE == 1) AtomicAddV(&out[dofs[(tile * nd + dof) * 16 + col]], (V)(s * (double)val));
             else {  // colour-ordered: no other element of the launch touches this dof
               V* yp = &out[dofs[e * nd + dof]];
               *yp = (V)((double)*yp + s * (double)val);
@@ -874,9 +885,14 @@ __global__ __launch_bounds__(kMfBlock) void MfFac16Kernel(
   const int wg = XcdContiguousBlock(blockIdx.x, gridDim.x);
   const int r0 = (int)((int64_t)wg * R.nrounds / gridDim.x);
   const int r1 = (int)((int64_t)(wg + 1) * R.nrounds / gridDim.x);
+  constexpr bool TILED = MODE != 2;  // as in MfFacKernel
   __shared__ int s_round0[kMaxLdsClasses + 1];
+  __shared__ int s_tbase[kMaxLdsClasses + 1];
   __shared__ int64_t s_range[2 * kMaxLdsClasses];
-  for (int i = threadIdx.x; i <= R.nclass; i += kMfBlock) s_round0[i] = R.round0[i];
+  for (int i = threadIdx.x; i <= R.nclass; i += kMfBlock) {
+    s_round0[i] = R.round0[i];
+    if (TILED) s_tbase[i] = R.tbase[i];
+  }
   for (int i = threadIdx.x; i < 2 * R.nclass; i += kMfBlock) s_range[i] = R.range[i];
   __syncthreads();
   const h8* a1 = reinterpret_cast<const h8*>(lds_raw) + lane;
@@ -898,15 +914,19 @@ __global__ __launch_bounds__(kMfBlock) void MfFac16Kernel(
     const int64_t e = first + col;
     const bool valid = e < end;
     const int64_t ev = valid ? e : first;
+    const int64_t tile = TILED ? s_tbase[cl] + (first - s_range[2 * cl]) / 16 : 0;
     int dv[KS1][8];
 #pragma unroll
     for (int ks = 0; ks < KS1; ks++)
 #pragma unroll
-      for (int j = 0; j < 8; j++) dv[ks][j] = dofs[ev * nd + min(32 * ks + 8 * grp + j, nd - 1)];
-    const int en = elnr[ev];
+      for (int j = 0; j < 8; j++) {
+        const int k = min(32 * ks + 8 * grp + j, nd - 1);
+        dv[ks][j] = TILED ? dofs[(tile * nd + k) * 16 + col] : dofs[ev * nd + k];
+      }
+    const int en = TILED ? elnr[tile * 16 + col] : elnr[ev];
     double mv[NC];
 #pragma unroll
-    for (int c = 0; c < NC; c++) mv[c] = metric[ev * NC + c];
+    for (int c = 0; c < NC; c++) mv[c] = TILED ? metric[(tile * NC + c) * 16 + col] : metric[ev * NC + c];
     V xv[KS1][8];
 #pragma unroll
     for (int ks = 0; ks < KS1; ks++)
@@ -991,7 +1011,7 @@ __global__ __launch_bounds__(kMfBlock) void MfFac16Kernel(
           if (dof < nd) {
             const V val = (V)ldexpf(yacc[m][r], escale);
             if (MODE == 0) out[e * nd + dof] = val;
-            else if (MODE == 1) AtomicAddV(&out[dofs[e * nd + dof]], (V)(s * (double)val));
+            else if (MODE == 1) AtomicAddV(&out[dofs[(tile * nd + dof) * 16 + col]], (V)(s * (double)val));
             else {  // colour-ordered: no other element of the launch touches this dof
               V* yp = &out[dofs[e * nd + dof]];
               *yp = (V)((double)*yp + s * (double)val);
@@ -1623,6 +1643,34 @@ MatFreeOp::MatFreeOp(int dim_, int nd_, int nq_, int64_t ndof, int nclass, const
     cls_range.alloc(2 * (size_t)nclass);
     cls_range.upload(rng.data(), 2 * (size_t)nclass);
   }
+  // tile-major tables of the factored kernels: a wavefront's load of one dof
+  // row (or metric component) of its 16 elements covers one or two lines
+  // instead of 16
+  if (fac_kernel) {
+    std::vector<int> tb(nclass + 1, 0);
+    for (int c = 0; c < nclass; c++) tb[c + 1] = tb[c] + (int)((off[c + 1] - off[c] + 15) / 16);
+    const int64_t nt = tb[nclass];
+    std::vector<int> td((size_t)nt * nd * 16), te((size_t)nt * 16);
+    std::vector<double> tm((size_t)nt * ncomp * 16);
+    for (int c = 0; c < nclass; c++)
+      for (int64_t t = 0; t < tb[c + 1] - tb[c]; t++) {
+        const int64_t T = tb[c] + t, first = off[c] + 16 * t;
+        for (int col = 0; col < 16; col++) {
+          const int64_t pe = first + col < off[c + 1] ? first + col : first;
+          for (int k = 0; k < nd; k++) td[((size_t)T * nd + k) * 16 + col] = hdofs[pe * nd + k];
+          for (int q = 0; q < ncomp; q++) tm[((size_t)T * ncomp + q) * 16 + col] = hmetric[pe * ncomp + q];
+          te[(size_t)T * 16 + col] = el[pe];
+        }
+      }
+    t_dofs.alloc(std::max<size_t>(td.size(), 1));
+    if (!td.empty()) t_dofs.upload(td.data(), td.size());
+    t_metric.alloc(std::max<size_t>(tm.size(), 1));
+    if (!tm.empty()) t_metric.upload(tm.data(), tm.size());
+    t_elnr.alloc(std::max<size_t>(te.size(), 1));
+    if (!te.empty()) t_elnr.upload(te.data(), te.size());
+    tile_base.alloc(tb.size());
+    tile_base.upload(tb.data(), tb.size());
+  }
   // enable the dynamic LDS size of every kernel variant this operator uses
   // and record its occupancy
   LaunchElements<0, double>(*this, nullptr, nullptr, 1.0, nullptr, nullptr, true);
@@ -1642,6 +1690,8 @@ static int MfGrid(const MatFreeOp& A, int mode, bool f64_vectors, int nrounds = 
 
 // The element tables and round schedule one launch of the factored kernels
 // walks: the operator's own (class-sorted, all elements) or one colour's.
+// The operator's own schedule comes with the tile-major tables (MODE 0 and
+// 1 of the factored kernels), a colour's with element-major ones (MODE 2).
 struct MfSched {
   const int* round0;
   const int64_t* range;
@@ -1649,9 +1699,10 @@ struct MfSched {
   const int* dofs;
   const double* metric;
   const int* elnr;
+  const int* tbase = nullptr;
 };
 static MfSched OwnSched(const MatFreeOp& A) {
-  return {A.round0.p, A.cls_range.p, A.nrounds, A.dofs.p, A.metric.p, A.elnr.p};
+  return {A.round0.p, A.cls_range.p, A.nrounds, A.t_dofs.p, A.t_metric.p, A.t_elnr.p, A.tile_base.p};
 }
 
 // prepare: enable the kernel's dynamic LDS size and record its occupancy
@@ -1770,7 +1821,7 @@ static void LaunchFac(const MatFreeOp& A, const T* fm, const V* x, V* out, doubl
   const size_t lds = (size_t)A.fstride * sizeof(T);
   auto kern = MfFacKernel<T, DIM, NDP, KG, MODE, V>;
   if (prepare) return PrepareKernel(A, kern, lds, MODE, sizeof(V) == 8);
-  const MfRounds R{S.round0, S.range, S.nrounds, (int)A.classes.size()};
+  const MfRounds R{S.round0, S.range, S.nrounds, (int)A.classes.size(), S.tbase};
   kern<<<MfGrid(A, MODE, sizeof(V) == 8, S.nrounds), kMfBlock, lds, st>>>(fm, A.fstride, R, S.dofs, S.metric,
                                                                           A.Kappa(), S.elnr, A.nd, x, out, s, stop);
   NGSAMD_CHECK_LAUNCH();
@@ -1796,7 +1847,7 @@ static void LaunchFac16(const MatFreeOp& A, const V* x, V* out, double s, const 
   const size_t lds = (size_t)A.fstride * 2;
   auto kern = MfFac16Kernel<H, DIM, KS1, MT, KG, MODE, V>;
   if (prepare) return PrepareKernel(A, kern, lds, MODE, sizeof(V) == 8);
-  const MfRounds R{S.round0, S.range, S.nrounds, (int)A.classes.size()};
+  const MfRounds R{S.round0, S.range, S.nrounds, (int)A.classes.size(), S.tbase};
   kern<<<MfGrid(A, MODE, sizeof(V) == 8, S.nrounds), kMfBlock, lds, st>>>(A.fmat16.p, A.fstride, R, S.dofs,
                                                                           S.metric, A.Kappa(), S.elnr, A.nd, x, out,
                                                                           s, stop);

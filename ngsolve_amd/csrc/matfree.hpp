@@ -56,6 +56,12 @@ struct MatFreeOp : ngsamd_op_s {
   DevBuf<double> kappa;       // [ne_global] (owned copy)
   const double* kappa_ext = nullptr;  // set by ngsamd_mf_set_kappa
   DevBuf<int> elnr;           // [nel] global element numbers
+  // tile-major copies for the factored kernels: tile T = 16 consecutive
+  // elements of one class, dofs [T][nd][16], metric [T][ncomp][16], element
+  // numbers [T][16] (lanes past the class end: the tile's first element);
+  // tile_base[c] = first tile of class c
+  DevBuf<int> t_dofs, t_elnr, tile_base;
+  DevBuf<double> t_metric;
   int64_t ne_global = 0;
   // gather scatter: dof -> contribution slots (element row * nd + local),
   // dofs in the gather order g_order (by lowest element number)
