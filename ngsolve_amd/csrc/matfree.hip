@@ -1010,7 +1010,7 @@ __global__ __launch_bounds__(kMfBlock) void MfFac16Kernel(
           const int dof = 16 * m + 4 * grp + r;
           if (dof < nd) {
             const V val = (V)ldexpf(yacc[m][r], escale);
-            if (MODE == 0) out[e * nd + dof] = val;
+            if (MODE == 0) out[(tile * nd + dof) * 16 + col] = val;
             else if (MODE == 1) AtomicAddV(&out[dofs[(tile * nd + dof) * 16 + col]], (V)(s * (double)val));
             else {  // colour-ordered: no other element of the launch touches this dof
               V* yp = &out[dofs[e * nd + dof]];
@@ -1572,6 +1572,7 @@ MatFreeOp::MatFreeOp(int dim_, int nd_, int nq_, int64_t ndof, int nclass, const
   kappa.alloc(ne_global);
   kappa.upload(kap.data(), ne_global);
 
+  std::vector<int> gather_slots;  // element-major slots in gather order (kept for the tile-major copy)
   // dof -> contribution slots, ascending slot order (the order of each sum)
   std::vector<int> cnt(ndof + 1, 0);
   for (int64_t i = 0; i < nel * nd; i++) cnt[hdofs[i] + 1]++;
@@ -1622,6 +1623,7 @@ MatFreeOp::MatFreeOp(int dim_, int nd_, int nq_, int64_t ndof, int nclass, const
     g_order.alloc(std::max<int64_t>(ndof, 1));
     if (ndof) g_order.upload(order.data(), ndof);
     NGSAMD_HIP(hipStreamSynchronize(Stream()));
+    gather_slots = std::move(psl);
   }
   // at least one entry (operators without elements)
   contrib64.alloc(std::max<size_t>((size_t)nel * nd, 1));
@@ -1670,6 +1672,30 @@ MatFreeOp::MatFreeOp(int dim_, int nd_, int nq_, int64_t ndof, int nclass, const
     if (!te.empty()) t_elnr.upload(te.data(), te.size());
     tile_base.alloc(tb.size());
     tile_base.upload(tb.data(), tb.size());
+    // the factored 16-bit kernels store their element rows tile-major too
+    // ([tile][dof][16]: one store instruction covers whole lines); the gather
+    // reads them through the same slots in the same order, renumbered.  At
+    // 32^3 p=4 that took the 16-bit contraction from 35 to 27 us and the
+    // FP32 gather from 27 to 30 us (a net 4 us per apply with FP32 vectors);
+    // for the FP32 / FP64 kernels the gather lost what the contraction
+    // gained (FP32 33.7 + 30.1 against 37.3 + 27.2 us, FP64 gather 36 -> 44
+    // us), so they keep element-major rows.
+    std::vector<int> tslot((size_t)nel * nd);
+    for (int c = 0; c < nclass; c++)
+      for (int64_t pe = off[c]; pe < off[c + 1]; pe++) {
+        const int64_t T = tb[c] + (pe - off[c]) / 16, col = (pe - off[c]) % 16;
+        for (int k = 0; k < nd; k++) tslot[(size_t)pe * nd + k] = (int)((T * nd + k) * 16 + col);
+      }
+    std::vector<int> gts(gather_slots.size());
+    for (size_t i = 0; i < gts.size(); i++) gts[i] = tslot[gather_slots[i]];
+    g_tslots.alloc(std::max<size_t>(gts.size(), 1));
+    if (!gts.empty()) g_tslots.upload(gts.data(), gts.size());
+    const size_t need = std::max<size_t>((size_t)nt * nd * 16, (size_t)nel * nd);
+    if (need > contrib64.n) {
+      contrib64.alloc(need);
+      contrib32.alloc(need);
+    }
+    NGSAMD_HIP(hipStreamSynchronize(Stream()));
   }
   // enable the dynamic LDS size of every kernel variant this operator uses
   // and record its occupancy
@@ -2057,12 +2083,15 @@ int MatFreeOp::Apply(double s, const V* x, V* y, bool accumulate, const int* sto
   double* dotp = partials ? (wide ? wide : State().wide_partials) : nullptr;
   // one thread per dof: a wide grid hides each dof's load chain
   const int np = GridFor(height, kGatherBlock, kMaxWideBlocks);
+  // the factored 16-bit kernels wrote tile-major rows
+  const bool half = precision == NGSAMD_PREC_BF16 || precision == NGSAMD_PREC_F16;
+  const int* slots = use_mfma && fac_kernel && half ? g_tslots.p : g_slots.p;
   if (partials)
-    MfGatherKernel<V, V, true><<<np, kGatherBlock, 0, st>>>(g_rowptr.p, g_slots.p, g_order.p, contrib, y, height, s,
+    MfGatherKernel<V, V, true><<<np, kGatherBlock, 0, st>>>(g_rowptr.p, slots, g_order.p, contrib, y, height, s,
                                                             accumulate ? 1 : 0, stop, x, dotp);
   else
-    MfGatherKernel<V, V, false><<<np, kGatherBlock, 0, st>>>(g_rowptr.p, g_slots.p, g_order.p, contrib, y, height,
-                                                             s, accumulate ? 1 : 0, stop, nullptr, nullptr);
+    MfGatherKernel<V, V, false><<<np, kGatherBlock, 0, st>>>(g_rowptr.p, slots, g_order.p, contrib, y, height, s,
+                                                             accumulate ? 1 : 0, stop, nullptr, nullptr);
   NGSAMD_CHECK_LAUNCH();
   if (!partials) return 0;
   LaunchFoldPartials(dotp, np, partials, stop, st);

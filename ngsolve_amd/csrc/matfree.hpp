@@ -59,13 +59,15 @@ struct MatFreeOp : ngsamd_op_s {
   // tile-major copies for the factored kernels: tile T = 16 consecutive
   // elements of one class, dofs [T][nd][16], metric [T][ncomp][16], element
   // numbers [T][16] (lanes past the class end: the tile's first element);
-  // tile_base[c] = first tile of class c
+  // tile_base[c] = first tile of class c; the 16-bit kernels' contribution
+  // rows are stored [T][nd][16] as well
   DevBuf<int> t_dofs, t_elnr, tile_base;
   DevBuf<double> t_metric;
   int64_t ne_global = 0;
   // gather scatter: dof -> contribution slots (element row * nd + local),
   // dofs in the gather order g_order (by lowest element number)
   DevBuf<int> g_rowptr, g_slots, g_order;
+  DevBuf<int> g_tslots;  // g_slots renumbered for the tile-major rows of the factored 16-bit kernels
   DevBuf<double> contrib64;
   DevBuf<float> contrib32;
   // colour-ordered direct scatter (NGSAMD_SCATTER_COLOR, set by
