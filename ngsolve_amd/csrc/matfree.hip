@@ -715,8 +715,11 @@ __global__ __launch_bounds__(kMfBlock) void MfRef16Kernel(
 // LDS, lane order so that each lane's A operands are 16-byte reads:
 //   A1[t][grp][col][KSP]  row col of tile t (slot of AccRow^-1(col)), K = dof
 //   A2[m][grp][col][JP]   row = dof 16 m + col, K = slot of lane group grp
-// Register budget: unbounded (FP64 106 VGPRs, FP32 81, 16-bit 104: two
-// 8-wave workgroups per CU).  Holding the FP32 and 16-bit tiles to 80 VGPRs
+// With partials (MODE 0, CG's <x, A x>): U . V summed over a lane's slots is
+// its share of x_e^T K_e x_e (sum_d (D_d^T x_e) . V_d); each workgroup
+// stores the sum over its tiles to partials[blockIdx.x].
+// Register budget: unbounded (tet p4: FP64 124 VGPRs, FP32 93, 16-bit
+// 120-128: two 8-wave workgroups per CU).  Holding the FP32 and 16-bit tiles to 80 VGPRs
 // (three workgroups per CU, no spills) measured slower at 32^3 p=4: FP32
 // 38.6 -> 42.9 us, 16-bit 39.9 -> 44 us, 16-bit with FP64 vectors 52 -> 73
 // us -- more tiles in flight per CU only add pressure on L1 and L2; one
@@ -726,7 +729,8 @@ template <class T, int DIM, int NDP, int KG, int MODE, class V>
 __global__ __launch_bounds__(kMfBlock) void MfFacKernel(
     const T* __restrict__ fmat, int64_t fstride, MfRounds R, const int* __restrict__ dofs,
     const double* __restrict__ metric, const double* __restrict__ kappa, const int* __restrict__ elnr, int nd,
-    const V* __restrict__ x, V* __restrict__ out, double s, const int* __restrict__ stop) {
+    const V* __restrict__ x, V* __restrict__ out, double s, const int* __restrict__ stop,
+    double* __restrict__ partials) {
   if (stop && *stop) return;
   typedef typename Acc4<T>::type acc_t;
   constexpr int NC = DIM * (DIM + 1) / 2;
@@ -758,6 +762,7 @@ __global__ __launch_bounds__(kMfBlock) void MfFacKernel(
   __syncthreads();
   const T* a1 = ls + lane * KSP;
   const T* a2 = ls + TT * 64 * KSP + lane * JP;
+  double dacc = 0.0;  // with partials: sum of U . V = x_e^T K_e x_e over this lane's tiles
   int cls = -1;
   for (int rd = r0; rd < r1; rd++) {
     int c = cls < 0 ? 0 : cls;
@@ -824,6 +829,12 @@ __global__ __launch_bounds__(kMfBlock) void MfFacKernel(
 #pragma unroll
       for (int d = 0; d < DIM; d++) v[kl * DIM + d] = vv[d];
     }
+    if (MODE == 0 && partials) {
+      T ld = (T)0;
+#pragma unroll
+      for (int j = 0; j < NSL; j++) ld += u[j / 4][j % 4] * v[j];
+      dacc += (double)ld;
+    }
     // Y = D V
     acc_t yacc[MT];
 #pragma unroll
@@ -857,6 +868,10 @@ __global__ __launch_bounds__(kMfBlock) void MfFacKernel(
         }
     }
   }
+  if (MODE == 0 && partials) {
+    __shared__ double s_dot[kMfWaves];
+    StorePartial<kMfBlock>(dacc, partials, s_dot);
+  }
 }
 
 // 16-bit operands (BF16 / FP16), FP32 accumulation and FP32 metric, operand
@@ -868,11 +883,14 @@ __global__ __launch_bounds__(kMfBlock) void MfFacKernel(
 //                            group col / 4), K = dofs 32 ks + 8 grp + j
 //   A2[m][ks2][grp][col][8]  row = dof 16 m + col, K = slots 8 ks2 + j
 // FP16 scales each element's x and metric by powers of two as MfRef16Kernel.
-template <class H, int DIM, int KS1, int MT, int KG, int MODE, class V>
+// ND (dofs per element) is a template parameter: K steps and lane groups
+// beyond it issue no loads (3D p=4: 10 of 40 load instructions per tile).
+template <class H, int DIM, int KS1, int MT, int KG, int ND, int MODE, class V>
 __global__ __launch_bounds__(kMfBlock) void MfFac16Kernel(
     const uint16_t* __restrict__ fmat, int64_t fstride, MfRounds R, const int* __restrict__ dofs,
     const double* __restrict__ metric, const double* __restrict__ kappa, const int* __restrict__ elnr, int nd,
-    const V* __restrict__ x, V* __restrict__ out, double s, const int* __restrict__ stop) {
+    const V* __restrict__ x, V* __restrict__ out, double s, const int* __restrict__ stop,
+    double* __restrict__ partials) {
   if (stop && *stop) return;
   typedef typename Vec8<H>::type h8;
   constexpr int NC = DIM * (DIM + 1) / 2;
@@ -897,6 +915,9 @@ __global__ __launch_bounds__(kMfBlock) void MfFac16Kernel(
   __syncthreads();
   const h8* a1 = reinterpret_cast<const h8*>(lds_raw) + lane;
   const h8* a2 = a1 + TT * KS1 * 64;
+  // with partials: sum of U . V (V as rounded for the backward product, the
+  // FP16 scaling undone) = x_e^T K_e x_e over this lane's tiles
+  double dacc = 0.0;
   int cls = -1;
   for (int rd = r0; rd < r1; rd++) {
     int cl = cls < 0 ? 0 : cls;
@@ -915,13 +936,16 @@ __global__ __launch_bounds__(kMfBlock) void MfFac16Kernel(
     const bool valid = e < end;
     const int64_t ev = valid ? e : first;
     const int64_t tile = TILED ? s_tbase[cl] + (first - s_range[2 * cl]) / 16 : 0;
+    // K row 32 ks + 8 grp + j; (ks, j) with 32 ks + j >= ND is padding in
+    // every lane group and loads nothing
     int dv[KS1][8];
 #pragma unroll
     for (int ks = 0; ks < KS1; ks++)
 #pragma unroll
       for (int j = 0; j < 8; j++) {
-        const int k = min(32 * ks + 8 * grp + j, nd - 1);
-        dv[ks][j] = TILED ? dofs[(tile * nd + k) * 16 + col] : dofs[ev * nd + k];
+        if (32 * ks + j >= ND) continue;
+        const int k = min(32 * ks + 8 * grp + j, ND - 1);
+        dv[ks][j] = TILED ? dofs[(tile * ND + k) * 16 + col] : dofs[ev * ND + k];
       }
     const int en = TILED ? elnr[tile * 16 + col] : elnr[ev];
     double mv[NC];
@@ -931,17 +955,17 @@ __global__ __launch_bounds__(kMfBlock) void MfFac16Kernel(
 #pragma unroll
     for (int ks = 0; ks < KS1; ks++)
 #pragma unroll
-      for (int j = 0; j < 8; j++) xv[ks][j] = x[dv[ks][j]];
+      for (int j = 0; j < 8; j++) xv[ks][j] = 32 * ks + j < ND ? x[dv[ks][j]] : (V)0;
     const double kap = kappa[en];
     float xf[KS1][8];
 #pragma unroll
     for (int ks = 0; ks < KS1; ks++)
 #pragma unroll
-      for (int j = 0; j < 8; j++) xf[ks][j] = (valid && 32 * ks + 8 * grp + j < nd) ? (float)xv[ks][j] : 0.0f;
+      for (int j = 0; j < 8; j++) xf[ks][j] = (valid && 32 * ks + 8 * grp + j < ND) ? (float)xv[ks][j] : 0.0f;
     float g[NC];
 #pragma unroll
     for (int c = 0; c < NC; c++) g[c] = valid ? (float)(kap * mv[c]) : 0.0f;
-    int escale = 0;
+    int escale = 0, edot = 0;
     if constexpr (!std::is_same<H, __bf16>::value) {
       float xm = 0.0f, gm = 0.0f;
 #pragma unroll
@@ -962,6 +986,7 @@ __global__ __launch_bounds__(kMfBlock) void MfFac16Kernel(
 #pragma unroll
       for (int c = 0; c < NC; c++) g[c] = ldexpf(g[c], -eg);
       escale = ex + eg;
+      edot = 2 * ex + eg;
     }
     h8 xg[KS1];
 #pragma unroll
@@ -994,6 +1019,12 @@ __global__ __launch_bounds__(kMfBlock) void MfFac16Kernel(
     for (int k2 = 0; k2 < KS2; k2++)
 #pragma unroll
       for (int j = 0; j < 8; j++) vb[k2][j] = (H)v[8 * k2 + j];
+    if (MODE == 0 && partials) {
+      float ld = 0.0f;
+#pragma unroll
+      for (int j = 0; j < NSL; j++) ld += u[j / 4][j % 4] * (float)vb[j / 8][j % 8];
+      dacc += ldexp((double)ld, edot);
+    }
     // Y = D V
     float4v yacc[MT];
 #pragma unroll
@@ -1008,17 +1039,21 @@ __global__ __launch_bounds__(kMfBlock) void MfFac16Kernel(
 #pragma unroll
         for (int r = 0; r < 4; r++) {
           const int dof = 16 * m + 4 * grp + r;
-          if (dof < nd) {
+          if (dof < ND) {
             const V val = (V)ldexpf(yacc[m][r], escale);
-            if (MODE == 0) out[(tile * nd + dof) * 16 + col] = val;
-            else if (MODE == 1) AtomicAddV(&out[dofs[(tile * nd + dof) * 16 + col]], (V)(s * (double)val));
+            if (MODE == 0) out[(tile * ND + dof) * 16 + col] = val;
+            else if (MODE == 1) AtomicAddV(&out[dofs[(tile * ND + dof) * 16 + col]], (V)(s * (double)val));
             else {  // colour-ordered: no other element of the launch touches this dof
-              V* yp = &out[dofs[e * nd + dof]];
+              V* yp = &out[dofs[e * ND + dof]];
               *yp = (V)((double)*yp + s * (double)val);
             }
           }
         }
     }
+  }
+  if (MODE == 0 && partials) {
+    __shared__ double s_dot[kMfWaves];
+    StorePartial<kMfBlock>(dacc, partials, s_dot);
   }
 }
 
@@ -1185,18 +1220,19 @@ static bool SupportedBf16(int ks1, int mt) {
 }
 
 // Shapes with an instance of the factored kernels (H1 p = 1..5 in 2D, 1..4
-// in 3D): padded dofs NDP (16-bit: K steps KS1 and dof tiles MT) and basis
-// functions per lane group KG.
+// in 3D): padded dofs NDP (16-bit: K steps KS1, dof tiles MT and the dofs per
+// element ND) and basis functions per lane group KG.
 static bool SupportedFac(int dim, int ndp, int kg) {
   if (dim == 2)
     return (ndp == 4 && kg == 1) || (ndp == 8 && kg == 1) || (ndp == 12 && kg == 2) || (ndp == 16 && kg == 3) ||
            (ndp == 24 && kg == 4);
   return (ndp == 4 && kg == 1) || (ndp == 12 && kg == 1) || (ndp == 20 && kg == 3) || (ndp == 36 && kg == 5);
 }
-static bool SupportedFac16(int dim, int ks1, int mt, int kg) {
-  if (ks1 != 1 && !(ks1 == 2 && mt == 3)) return false;
-  if (dim == 2) return (mt == 1 && kg >= 1 && kg <= 3) || (mt == 2 && kg == 4);
-  return (mt == 1 && kg == 1) || (mt == 2 && kg == 3) || (ks1 == 2 && mt == 3 && kg == 5);
+static bool SupportedFac16(int dim, int ks1, int mt, int kg, int nd) {
+  auto is = [&](int k, int m, int g, int n) { return ks1 == k && mt == m && kg == g && nd == n; };
+  if (dim == 2)
+    return is(1, 1, 1, 3) || is(1, 1, 1, 6) || is(1, 1, 2, 10) || is(1, 1, 3, 15) || is(1, 2, 4, 21);
+  return is(1, 1, 1, 4) || is(1, 1, 1, 10) || is(1, 2, 3, 20) || is(2, 3, 5, 35);
 }
 
 // Factor of each class's reference matrices (see MfFacKernel): the rows
@@ -1442,7 +1478,7 @@ MatFreeOp::MatFreeOp(int dim_, int nd_, int nq_, int64_t ndof, int nclass, const
       if (k >= rank) return 0.0;
       return D[(((size_t)c * dim + d) * nd + i) * rank + k];
     };
-    if (rank > 0 && half && SupportedFac16(dim, ks1, mt16, kg)) {
+    if (rank > 0 && half && SupportedFac16(dim, ks1, mt16, kg, nd)) {
       const int tt = ((nsl + 3) / 4 + 1) / 2 * 2, ks2 = tt / 2;
       fstride = (int64_t)(tt * ks1 + mt16 * ks2) * 64 * 8;
       if ((size_t)fstride * 2 <= (size_t)DeviceMaxLds()) {
@@ -1726,6 +1762,8 @@ struct MfSched {
   const double* metric;
   const int* elnr;
   const int* tbase = nullptr;
+  // MODE 0 only: one partial of <x, A x> per workgroup (the factored kernels)
+  double* partials = nullptr;
 };
 static MfSched OwnSched(const MatFreeOp& A) {
   return {A.round0.p, A.cls_range.p, A.nrounds, A.t_dofs.p, A.t_metric.p, A.t_elnr.p, A.tile_base.p};
@@ -1848,8 +1886,8 @@ static void LaunchFac(const MatFreeOp& A, const T* fm, const V* x, V* out, doubl
   auto kern = MfFacKernel<T, DIM, NDP, KG, MODE, V>;
   if (prepare) return PrepareKernel(A, kern, lds, MODE, sizeof(V) == 8);
   const MfRounds R{S.round0, S.range, S.nrounds, (int)A.classes.size(), S.tbase};
-  kern<<<MfGrid(A, MODE, sizeof(V) == 8, S.nrounds), kMfBlock, lds, st>>>(fm, A.fstride, R, S.dofs, S.metric,
-                                                                          A.Kappa(), S.elnr, A.nd, x, out, s, stop);
+  kern<<<MfGrid(A, MODE, sizeof(V) == 8, S.nrounds), kMfBlock, lds, st>>>(
+      fm, A.fstride, R, S.dofs, S.metric, A.Kappa(), S.elnr, A.nd, x, out, s, stop, S.partials);
   NGSAMD_CHECK_LAUNCH();
 }
 
@@ -1867,29 +1905,30 @@ static void DispatchFac(const MatFreeOp& A, const T* fm, const V* x, V* out, dou
   throw Error(NGSAMD_E_STATE, "matrix-free: unsupported factored shape");
 }
 
-template <class H, int DIM, int KS1, int MT, int KG, int MODE, class V>
+template <class H, int DIM, int KS1, int MT, int KG, int ND, int MODE, class V>
 static void LaunchFac16(const MatFreeOp& A, const V* x, V* out, double s, const int* stop, hipStream_t st,
                         bool prepare, const MfSched& S) {
   const size_t lds = (size_t)A.fstride * 2;
-  auto kern = MfFac16Kernel<H, DIM, KS1, MT, KG, MODE, V>;
+  auto kern = MfFac16Kernel<H, DIM, KS1, MT, KG, ND, MODE, V>;
   if (prepare) return PrepareKernel(A, kern, lds, MODE, sizeof(V) == 8);
   const MfRounds R{S.round0, S.range, S.nrounds, (int)A.classes.size(), S.tbase};
-  kern<<<MfGrid(A, MODE, sizeof(V) == 8, S.nrounds), kMfBlock, lds, st>>>(A.fmat16.p, A.fstride, R, S.dofs,
-                                                                          S.metric, A.Kappa(), S.elnr, A.nd, x, out,
-                                                                          s, stop);
+  kern<<<MfGrid(A, MODE, sizeof(V) == 8, S.nrounds), kMfBlock, lds, st>>>(
+      A.fmat16.p, A.fstride, R, S.dofs, S.metric, A.Kappa(), S.elnr, A.nd, x, out, s, stop, S.partials);
   NGSAMD_CHECK_LAUNCH();
 }
 
 template <class H, int DIM, int MODE, class V>
 static void DispatchFac16(const MatFreeOp& A, const V* x, V* out, double s, const int* stop, hipStream_t st,
                           bool prepare, const MfSched& S) {
-#define NGSAMD_FAC16(KS1, MT, KG)                                                 \
-  if (A.ks1 == KS1 && A.mt16 == MT && A.fac_kg == KG)                               \
-    return LaunchFac16<H, DIM, KS1, MT, KG, MODE, V>(A, x, out, s, stop, st, prepare, S);
+#define NGSAMD_FAC16(KS1, MT, KG, ND)                                                  \
+  if (A.ks1 == KS1 && A.mt16 == MT && A.fac_kg == KG && A.nd == ND)                      \
+    return LaunchFac16<H, DIM, KS1, MT, KG, ND, MODE, V>(A, x, out, s, stop, st, prepare, S);
+  // H1 triangles p = 1..5, tetrahedra p = 1..4
   if constexpr (DIM == 2) {
-    NGSAMD_FAC16(1, 1, 1) NGSAMD_FAC16(1, 1, 2) NGSAMD_FAC16(1, 1, 3) NGSAMD_FAC16(1, 2, 4)
+    NGSAMD_FAC16(1, 1, 1, 3) NGSAMD_FAC16(1, 1, 1, 6) NGSAMD_FAC16(1, 1, 2, 10) NGSAMD_FAC16(1, 1, 3, 15)
+    NGSAMD_FAC16(1, 2, 4, 21)
   } else {
-    NGSAMD_FAC16(1, 1, 1) NGSAMD_FAC16(1, 2, 3) NGSAMD_FAC16(2, 3, 5)
+    NGSAMD_FAC16(1, 1, 1, 4) NGSAMD_FAC16(1, 1, 1, 10) NGSAMD_FAC16(1, 2, 3, 20) NGSAMD_FAC16(2, 3, 5, 35)
   }
 #undef NGSAMD_FAC16
   throw Error(NGSAMD_E_STATE, "matrix-free: unsupported factored 16-bit shape");
@@ -2077,15 +2116,37 @@ int MatFreeOp::Apply(double s, const V* x, V* y, bool accumulate, const int* sto
   V* contrib;
   if constexpr (sizeof(V) == 8) contrib = reinterpret_cast<V*>(contrib64.p);
   else contrib = reinterpret_cast<V*>(contrib32.p);
-  LaunchElements<0, V>(*this, x, contrib, 1.0, stop, st, false);
-  // with partials (CG): the gather on its wide grid writes one partial per
-  // workgroup, folded to kFoldedPartials for CG's kernels
+  // with partials (CG): one partial of <x, y> per workgroup, folded to
+  // kFoldedPartials for CG's kernels.  For y = A x the factored kernels form
+  // it themselves as sum_e U_e . V_e = sum_e x_e^T K_e x_e (no second read
+  // of x by dof); otherwise the gather forms sum_r x[r] y[r].
   double* dotp = partials ? (wide ? wide : State().wide_partials) : nullptr;
+  const bool fac_dot = partials && use_mfma && fac_kernel && !accumulate && s == 1.0;
+  int ndot = 0;
+  if (fac_dot) {
+    MfSched S = OwnSched(*this);
+    S.partials = dotp;
+    LaunchFacElements<0, V>(*this, x, contrib, 1.0, stop, st, false, S);
+    ndot = nrounds ? MfGrid(*this, 0, sizeof(V) == 8, nrounds) : 0;
+    if (ndot == 0) {  // no elements: <x, A x> = 0
+      LaunchFill(dotp, 1, 0.0, st);
+      ndot = 1;
+    }
+  } else {
+    LaunchElements<0, V>(*this, x, contrib, 1.0, stop, st, false);
+  }
   // one thread per dof: a wide grid hides each dof's load chain
   const int np = GridFor(height, kGatherBlock, kMaxWideBlocks);
   // the factored 16-bit kernels wrote tile-major rows
   const bool half = precision == NGSAMD_PREC_BF16 || precision == NGSAMD_PREC_F16;
   const int* slots = use_mfma && fac_kernel && half ? g_tslots.p : g_slots.p;
+  if (fac_dot) {
+    MfGatherKernel<V, V, false><<<np, kGatherBlock, 0, st>>>(g_rowptr.p, slots, g_order.p, contrib, y, height, s,
+                                                             0, stop, nullptr, nullptr);
+    NGSAMD_CHECK_LAUNCH();
+    LaunchFoldPartials(dotp, ndot, partials, stop, st);
+    return kFoldedPartials;
+  }
   if (partials)
     MfGatherKernel<V, V, true><<<np, kGatherBlock, 0, st>>>(g_rowptr.p, slots, g_order.p, contrib, y, height, s,
                                                             accumulate ? 1 : 0, stop, x, dotp);
