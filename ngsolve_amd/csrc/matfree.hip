@@ -2116,12 +2116,15 @@ int MatFreeOp::Apply(double s, const V* x, V* y, bool accumulate, const int* sto
   V* contrib;
   if constexpr (sizeof(V) == 8) contrib = reinterpret_cast<V*>(contrib64.p);
   else contrib = reinterpret_cast<V*>(contrib32.p);
-  // with partials (CG): one partial of <x, y> per workgroup, folded to
-  // kFoldedPartials for CG's kernels.  For y = A x the factored kernels form
-  // it themselves as sum_e U_e . V_e = sum_e x_e^T K_e x_e (no second read
-  // of x by dof); otherwise the gather forms sum_r x[r] y[r].
-  double* dotp = partials ? (wide ? wide : State().wide_partials) : nullptr;
-  const bool fac_dot = partials && use_mfma && fac_kernel && !accumulate && s == 1.0;
+  // with partials (CG): one partial of <x, y> per workgroup.  For y = A x
+  // the factored kernels form it themselves as sum_e U_e . V_e =
+  // sum_e x_e^T K_e x_e (no second read of x by dof), straight into
+  // partials[] (at most kMaxReduceBlocks, one per workgroup of the
+  // contraction's resident grid); otherwise the gather forms
+  // sum_r x[r] y[r] on its wide grid, folded to kFoldedPartials.
+  const bool fac_dot = partials && use_mfma && fac_kernel && !accumulate && s == 1.0 &&
+                       MfGrid(*this, 0, sizeof(V) == 8, nrounds) <= kMaxReduceBlocks;
+  double* dotp = partials ? (fac_dot ? partials : wide ? wide : State().wide_partials) : nullptr;
   int ndot = 0;
   if (fac_dot) {
     MfSched S = OwnSched(*this);
@@ -2144,8 +2147,7 @@ int MatFreeOp::Apply(double s, const V* x, V* y, bool accumulate, const int* sto
     MfGatherKernel<V, V, false><<<np, kGatherBlock, 0, st>>>(g_rowptr.p, slots, g_order.p, contrib, y, height, s,
                                                              0, stop, nullptr, nullptr);
     NGSAMD_CHECK_LAUNCH();
-    LaunchFoldPartials(dotp, ndot, partials, stop, st);
-    return kFoldedPartials;
+    return ndot;
   }
   if (partials)
     MfGatherKernel<V, V, true><<<np, kGatherBlock, 0, st>>>(g_rowptr.p, slots, g_order.p, contrib, y, height, s,

@@ -93,10 +93,11 @@ struct MatFreeOp : ngsamd_op_s {
   // y = A x, all kernels return immediately when *d_stop != 0
   void MultStop(const double* x, double* y, const int* d_stop, hipStream_t st);
   void MultF32(const float* x, float* y, const int* d_stop, hipStream_t st);
-  // y = A x plus per-workgroup partial sums of <x, A x> in partials[];
-  // returns their number, 0 when the scatter mode has no gather kernel
-  // (then the caller forms the inner product itself).  wide: room for
-  // kMaxWideBlocks partials of the gather's wide grid (null: the library's)
+  // y = A x plus per-workgroup partial sums of <x, A x> in partials[] (room
+  // for kMaxReduceBlocks); returns their number, 0 when the scatter mode has
+  // no gather kernel (then the caller forms the inner product itself).
+  // wide: room for kMaxWideBlocks partials of the gather's wide grid (null:
+  // the library's)
   int MultDot(const double* x, double* y, double* partials, const int* d_stop, hipStream_t st,
               double* wide = nullptr);
   int MultDotF32(const float* x, float* y, double* partials, const int* d_stop, hipStream_t st,
