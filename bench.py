@@ -486,7 +486,7 @@ def main():
         del As
         x32, y32 = N.DeviceVector(ndof, 4), N.DeviceVector(ndof, 4)
         x32.upload(x.astype(np.float32))
-        for prec in (0, 1, 2, 3):
+        for prec in (0, 1, 2, 3, 4):
             M = build_mf(N, fes, mesh, ir, None, prec)
             for _ in range(3):
                 M.mult(dx, dy)
@@ -562,7 +562,8 @@ def main():
     x32, y32 = N.DeviceVector(ndof, 4), N.DeviceVector(ndof, 4)
     x32.upload(x.astype(np.float32))
     for name, prec, peak in [("mf_f64", 0, FP64_PEAK_TFLOPS), ("mf_f32", 1, FP32_PEAK_TFLOPS),
-                             ("mf_bf16", 2, BF16_PEAK_TFLOPS), ("mf_f16", 3, BF16_PEAK_TFLOPS)]:
+                             ("mf_bf16", 2, BF16_PEAK_TFLOPS), ("mf_f16", 3, BF16_PEAK_TFLOPS),
+                             ("mf_f16x3", 4, BF16_PEAK_TFLOPS)]:
         for scatter in (0, 1, 2):
             M = build_mf(N, fes, mesh, ir, None, prec, min(scatter, 1))
             if scatter == 2:  # colour-ordered direct scatter (33 colours at 32^3)
@@ -733,7 +734,7 @@ def main():
                              "rel_err_vs_oracle": err_sol(u_mf), "speedup_vs_fp64_csr_cg": round(t_k / t_mf, 3)}
         del du_mf
         best = None
-        for pname, prec in [("f32", 1), ("bf16", 2), ("f16", 3)]:
+        for pname, prec in [("f32", 1), ("bf16", 2), ("f16", 3), ("f16x3", 4)]:
             Mlow = build_mf(N, fes, mesh, ir, kappa, prec)
             du2 = N.DeviceVector(ndof, 8)
             t0 = time.perf_counter()

@@ -45,6 +45,11 @@ extern "C" {
 #define NGSAMD_PREC_F32 1
 #define NGSAMD_PREC_BF16 2
 #define NGSAMD_PREC_F16 3
+/* split FP16: each operand as the sum of two FP16 values, three FP16 MFMA
+ * products per contraction (about FP32 accuracy; matrix-free factored
+ * kernels only, i.e. H1 triangles p <= 5 and tetrahedra p <= 4).  No
+ * counterpart in the reference. */
+#define NGSAMD_PREC_F16X3 4
 
 /* scatter mode of a matrix-free operator */
 #define NGSAMD_SCATTER_GATHER 0   /* deterministic two-pass owner-computes gather */
@@ -190,7 +195,8 @@ int ngsamd_twolevel_create(ngsamd_op_t* op, ngsamd_op_t smoother, ngsamd_op_t co
  *   h_kappa    [nel] element coefficient or NULL (=1)
  *   h_elnr     [nel] global element number of each table row or NULL (=row)
  *   precision  NGSAMD_PREC_*  (B and the contraction in that type, FP32
- *              accumulation for BF16 / FP16; input/output vectors stay FP64)
+ *              accumulation for BF16 / FP16 / split FP16; input/output
+ *              vectors stay FP64)
  *   scatter    NGSAMD_SCATTER_* */
 int ngsamd_mf_create(ngsamd_op_t* op, int dim, int nd, int nq, int64_t ndof, int nclass,
                      const int64_t* h_class_offsets, const double* h_bmat, const double* h_weights,

@@ -36,5 +36,5 @@ def ensure_device(device=0):
     return N
 
 
-PREC = {"f64": 0, "fp64": 0, "float64": 0, "f32": 1, "fp32": 1, "float32": 1, "bf16": 2, "bfloat16": 2, "f16": 3, "fp16": 3, "float16": 3}
+PREC = {"f64": 0, "fp64": 0, "float64": 0, "f32": 1, "fp32": 1, "float32": 1, "bf16": 2, "bfloat16": 2, "f16": 3, "fp16": 3, "float16": 3, "f16x3": 4}
 SCATTER = {"gather": 0, "atomic": 1}

@@ -10,7 +10,8 @@ BilinearForm  grad(u)*grad(v)*dx with an element-wise coefficient and/or
               whose products run on the GPU.
 LinearForm    f*v*dx (fem/symbolicintegrator.cpp:1036-1110)
 MFOpts        MatFreeOptions (comp/bilinearform.hpp:24-52): fused, atomic, fp32,
-              plus `precision` ("f64"/"f32"/"bf16"/"f16"), which the reference lacks
+              plus `precision` ("f64"/"f32"/"bf16"/"f16"/"f16x3" = split FP16), which
+              the reference lacks
 """
 import gzip
 import re

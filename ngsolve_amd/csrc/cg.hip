@@ -659,7 +659,9 @@ extern "C" int ngsamd_ir_cg_solve(ngsamd_op_t A, ngsamd_op_t inner, ngsamd_op_t 
     // default inner tolerance by the inner operator's precision (the
     // fastest setting measured on the 32^3 p=4 lognormal-kappa problem)
     if (inner_tol <= 0.0)
-      inner_tol = Ai->precision == NGSAMD_PREC_F32 ? 3e-4 : Ai->precision == NGSAMD_PREC_F16 ? 1e-3 : 3e-2;
+      inner_tol = Ai->precision == NGSAMD_PREC_F32 || Ai->precision == NGSAMD_PREC_F16X3 ? 3e-4
+                  : Ai->precision == NGSAMD_PREC_F16                                      ? 1e-3
+                                                                                          : 3e-2;
     double hsc[2] = {1.0, 1.0};  // host source of the async copy below: lives until the next sync
     // inner iterations per captured graph; NGSAMD_CG_CHUNK=0 launches every
     // kernel directly (no graph), e.g. for a kernel trace

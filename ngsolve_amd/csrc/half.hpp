@@ -46,4 +46,23 @@ inline uint16_t ToF16(double v) {
   return (uint16_t)(sign | h);
 }
 
+// value of an IEEE binary16 bit pattern (exact)
+inline double F16ToDouble(uint16_t h) {
+  const int e = (h >> 10) & 0x1F, m = h & 0x3FF;
+  const double sign = (h & 0x8000u) ? -1.0 : 1.0;
+  if (e == 31) return m ? sign * (0.0 / 0.0) : sign * (1.0 / 0.0);
+  if (e == 0) return sign * m * 0x1p-24;  // subnormal: m 2^-24
+  double v = (1024 + m) * 0x1p-10;
+  for (int i = 15; i < e; i++) v *= 2.0;
+  for (int i = e; i < 15; i++) v *= 0.5;
+  return sign * v;
+}
+
+// Two-term FP16 split v ~ hi + lo: hi = fp16(v), lo = fp16(v - hi) (the
+// operands of the split-FP16 matrix-free kernels)
+inline void SplitF16(double v, uint16_t* hi, uint16_t* lo) {
+  *hi = ToF16(v);
+  *lo = ToF16(v - F16ToDouble(*hi));
+}
+
 }  // namespace ngsamd

@@ -885,7 +885,13 @@ __global__ __launch_bounds__(kMfBlock) void MfFacKernel(
 // FP16 scales each element's x and metric by powers of two as MfRef16Kernel.
 // ND (dofs per element) is a template parameter: K steps and lane groups
 // beyond it issue no loads (3D p=4: 10 of 40 load instructions per tile).
-template <class H, int DIM, int KS1, int MT, int KG, int ND, int MODE, class V>
+// SPLIT (FP16 only): three-term split products (Ootomo and Yokota 2022):
+// D = D_hi + D_lo (both FP16, the A blocks of D_lo after those of D_hi in
+// LDS, fstride counts both), x and V split the same way in registers, and
+// each product is D_hi X_hi + D_hi X_lo + D_lo X_hi in the FP32
+// accumulators: about 22 significant bits, the FP32 kernel's accuracy, at
+// three times the 16-bit MFMA work (still a few percent of the MFMA pipe).
+template <class H, int DIM, int KS1, int MT, int KG, int ND, int MODE, class V, bool SPLIT>
 __global__ __launch_bounds__(kMfBlock) void MfFac16Kernel(
     const uint16_t* __restrict__ fmat, int64_t fstride, MfRounds R, const int* __restrict__ dofs,
     const double* __restrict__ metric, const double* __restrict__ kappa, const int* __restrict__ elnr, int nd,
@@ -915,6 +921,9 @@ __global__ __launch_bounds__(kMfBlock) void MfFac16Kernel(
   __syncthreads();
   const h8* a1 = reinterpret_cast<const h8*>(lds_raw) + lane;
   const h8* a2 = a1 + TT * KS1 * 64;
+  // D_lo: the second half of the class's block
+  const h8* a1l = a1 + (TT * KS1 + MT * KS2) * 64;
+  const h8* a2l = a1l + TT * KS1 * 64;
   // with partials: sum of U . V (V as rounded for the backward product, the
   // FP16 scaling undone) = x_e^T K_e x_e over this lane's tiles
   double dacc = 0.0;
@@ -988,11 +997,14 @@ __global__ __launch_bounds__(kMfBlock) void MfFac16Kernel(
       escale = ex + eg;
       edot = 2 * ex + eg;
     }
-    h8 xg[KS1];
+    h8 xg[KS1], xl[KS1];
 #pragma unroll
     for (int ks = 0; ks < KS1; ks++)
 #pragma unroll
-      for (int j = 0; j < 8; j++) xg[ks][j] = (H)xf[ks][j];
+      for (int j = 0; j < 8; j++) {
+        xg[ks][j] = (H)xf[ks][j];
+        if (SPLIT) xl[ks][j] = (H)(xf[ks][j] - (float)xg[ks][j]);
+      }
     // U = D^T X
     float4v u[TT];
 #pragma unroll
@@ -1000,7 +1012,13 @@ __global__ __launch_bounds__(kMfBlock) void MfFac16Kernel(
 #pragma unroll
     for (int ks = 0; ks < KS1; ks++)
 #pragma unroll
-      for (int t = 0; t < TT; t++) u[t] = Mfma16(a1[(t * KS1 + ks) * 64], xg[ks], u[t]);
+      for (int t = 0; t < TT; t++) {
+        u[t] = Mfma16(a1[(t * KS1 + ks) * 64], xg[ks], u[t]);
+        if (SPLIT) {
+          u[t] = Mfma16(a1[(t * KS1 + ks) * 64], xl[ks], u[t]);
+          u[t] = Mfma16(a1l[(t * KS1 + ks) * 64], xg[ks], u[t]);
+        }
+      }
     // V = (kappa G) U in FP32, rounded once into the B operand
     float v[4 * TT];
 #pragma unroll
@@ -1014,15 +1032,19 @@ __global__ __launch_bounds__(kMfBlock) void MfFac16Kernel(
 #pragma unroll
       for (int d = 0; d < DIM; d++) v[kl * DIM + d] = vv[d];
     }
-    h8 vb[KS2];
+    h8 vb[KS2], vl[KS2];
 #pragma unroll
     for (int k2 = 0; k2 < KS2; k2++)
 #pragma unroll
-      for (int j = 0; j < 8; j++) vb[k2][j] = (H)v[8 * k2 + j];
+      for (int j = 0; j < 8; j++) {
+        vb[k2][j] = (H)v[8 * k2 + j];
+        if (SPLIT) vl[k2][j] = (H)(v[8 * k2 + j] - (float)vb[k2][j]);
+      }
     if (MODE == 0 && partials) {
       float ld = 0.0f;
 #pragma unroll
-      for (int j = 0; j < NSL; j++) ld += u[j / 4][j % 4] * (float)vb[j / 8][j % 8];
+      for (int j = 0; j < NSL; j++)
+        ld += u[j / 4][j % 4] * (SPLIT ? (float)vb[j / 8][j % 8] + (float)vl[j / 8][j % 8] : (float)vb[j / 8][j % 8]);
       dacc += ldexp((double)ld, edot);
     }
     // Y = D V
@@ -1032,7 +1054,13 @@ __global__ __launch_bounds__(kMfBlock) void MfFac16Kernel(
 #pragma unroll
     for (int k2 = 0; k2 < KS2; k2++)
 #pragma unroll
-      for (int m = 0; m < MT; m++) yacc[m] = Mfma16(a2[(m * KS2 + k2) * 64], vb[k2], yacc[m]);
+      for (int m = 0; m < MT; m++) {
+        yacc[m] = Mfma16(a2[(m * KS2 + k2) * 64], vb[k2], yacc[m]);
+        if (SPLIT) {
+          yacc[m] = Mfma16(a2[(m * KS2 + k2) * 64], vl[k2], yacc[m]);
+          yacc[m] = Mfma16(a2l[(m * KS2 + k2) * 64], vb[k2], yacc[m]);
+        }
+      }
     if (valid) {
 #pragma unroll
       for (int m = 0; m < MT; m++)
@@ -1331,7 +1359,7 @@ MatFreeOp::MatFreeOp(int dim_, int nd_, int nq_, int64_t ndof, int nclass, const
   NGSAMD_REQUIRE(nd_ >= 1 && nd_ <= kScalarMaxNd, "ngsamd_mf_create: nd out of range (1..64)");
   NGSAMD_REQUIRE(nq_ >= 1 && nclass >= 1, "ngsamd_mf_create: empty rule or no classes");
   NGSAMD_REQUIRE(prec == NGSAMD_PREC_F64 || prec == NGSAMD_PREC_F32 || prec == NGSAMD_PREC_BF16 ||
-                     prec == NGSAMD_PREC_F16,
+                     prec == NGSAMD_PREC_F16 || prec == NGSAMD_PREC_F16X3,
                  "ngsamd_mf_create: unknown precision");
   NGSAMD_REQUIRE(scat == NGSAMD_SCATTER_GATHER || scat == NGSAMD_SCATTER_ATOMIC, "ngsamd_mf_create: unknown scatter");
   NGSAMD_REQUIRE(ndof >= 0 && ndof < (int64_t(1) << 31), "ngsamd_mf_create: ndof out of range");
@@ -1363,7 +1391,8 @@ MatFreeOp::MatFreeOp(int dim_, int nd_, int nq_, int64_t ndof, int nclass, const
   const int qs = dim * ndp + 1;
   bstride = (int64_t)nqp * qs;
   // MFMA path if the padded shape is instantiated and B fits in LDS
-  const bool half = prec == NGSAMD_PREC_BF16 || prec == NGSAMD_PREC_F16;
+  // the 16-bit operand formats (split FP16 rounds its FP16 part like FP16)
+  const bool half = prec == NGSAMD_PREC_BF16 || prec == NGSAMD_PREC_F16 || prec == NGSAMD_PREC_F16X3;
   if (half) {
     nqc = (nq + 31) / 32;
     k1p = (nd + 7) / 8 * 8;
@@ -1424,7 +1453,7 @@ MatFreeOp::MatFreeOp(int dim_, int nd_, int nq_, int64_t ndof, int nclass, const
             for (int j = 0; j < nd; j++) {
               const double v = S[(((size_t)c * nc + k) * nd + i) * nd + j];
               lay[(size_t)c * sstride + ((size_t)k * nd + i) * rs + j] =
-                  prec == NGSAMD_PREC_F16 ? ToF16(v) : ToBf16(v);
+                  prec != NGSAMD_PREC_BF16 ? ToF16(v) : ToBf16(v);
             }
       smat16.alloc(lay.size());
       smat16.upload(lay.data(), lay.size());
@@ -1480,28 +1509,39 @@ MatFreeOp::MatFreeOp(int dim_, int nd_, int nq_, int64_t ndof, int nclass, const
     };
     if (rank > 0 && half && SupportedFac16(dim, ks1, mt16, kg, nd)) {
       const int tt = ((nsl + 3) / 4 + 1) / 2 * 2, ks2 = tt / 2;
-      fstride = (int64_t)(tt * ks1 + mt16 * ks2) * 64 * 8;
+      // split FP16: the block of D_hi, then the block of D_lo
+      const bool split = prec == NGSAMD_PREC_F16X3;
+      const int64_t blk = (int64_t)(tt * ks1 + mt16 * ks2) * 64 * 8;
+      fstride = split ? 2 * blk : blk;
       if ((size_t)fstride * 2 <= (size_t)DeviceMaxLds()) {
         std::vector<uint16_t> lay((size_t)fstride * nclass, 0);
-        auto h = [&](double v) { return prec == NGSAMD_PREC_F16 ? ToF16(v) : ToBf16(v); };
-        for (int c = 0; c < nclass; c++) {
-          uint16_t* a1 = lay.data() + (size_t)c * fstride;
-          uint16_t* a2 = a1 + (size_t)tt * ks1 * 64 * 8;
-          for (int t = 0; t < tt; t++)
-            for (int k = 0; k < ks1; k++)
-              for (int l = 0; l < 64; l++)
-                for (int j = 0; j < 8; j++) {
-                  const int gl = l / 16, cl = l % 16;
-                  a1[(((size_t)t * ks1 + k) * 64 + l) * 8 + j] = h(dval(c, cl / 4, 4 * t + cl % 4, 32 * k + 8 * gl + j));
-                }
-          for (int m = 0; m < mt16; m++)
-            for (int k = 0; k < ks2; k++)
-              for (int l = 0; l < 64; l++)
-                for (int j = 0; j < 8; j++) {
-                  const int gl = l / 16, cl = l % 16;
-                  a2[(((size_t)m * ks2 + k) * 64 + l) * 8 + j] = h(dval(c, gl, 8 * k + j, 16 * m + cl));
-                }
-        }
+        // part 0: the 16-bit value (D_hi for split FP16), part 1: D_lo
+        auto h = [&](double v, int part) -> uint16_t {
+          if (!split) return prec == NGSAMD_PREC_F16 ? ToF16(v) : ToBf16(v);
+          uint16_t hi, lo;
+          SplitF16(v, &hi, &lo);
+          return part ? lo : hi;
+        };
+        for (int c = 0; c < nclass; c++)
+          for (int part = 0; part < (split ? 2 : 1); part++) {
+            uint16_t* a1 = lay.data() + (size_t)c * fstride + (size_t)part * blk;
+            uint16_t* a2 = a1 + (size_t)tt * ks1 * 64 * 8;
+            for (int t = 0; t < tt; t++)
+              for (int k = 0; k < ks1; k++)
+                for (int l = 0; l < 64; l++)
+                  for (int j = 0; j < 8; j++) {
+                    const int gl = l / 16, cl = l % 16;
+                    a1[(((size_t)t * ks1 + k) * 64 + l) * 8 + j] =
+                        h(dval(c, cl / 4, 4 * t + cl % 4, 32 * k + 8 * gl + j), part);
+                  }
+            for (int m = 0; m < mt16; m++)
+              for (int k = 0; k < ks2; k++)
+                for (int l = 0; l < 64; l++)
+                  for (int j = 0; j < 8; j++) {
+                    const int gl = l / 16, cl = l % 16;
+                    a2[(((size_t)m * ks2 + k) * 64 + l) * 8 + j] = h(dval(c, gl, 8 * k + j, 16 * m + cl), part);
+                  }
+          }
         fmat16.alloc(lay.size());
         fmat16.upload(lay.data(), lay.size());
         NGSAMD_HIP(hipStreamSynchronize(Stream()));
@@ -1550,6 +1590,9 @@ MatFreeOp::MatFreeOp(int dim_, int nd_, int nq_, int64_t ndof, int nclass, const
     fac_kg = fac_kernel ? kg : 0;
     NGSAMD_HIP(hipStreamSynchronize(Stream()));
   }
+  NGSAMD_REQUIRE(prec != NGSAMD_PREC_F16X3 || (use_mfma && fac_kernel),
+                 "ngsamd_mf_create: NGSAMD_PREC_F16X3 needs the factored kernels (H1 triangles p <= 5, "
+                 "tetrahedra p <= 4)");
 
   // FP64 reference gradients [class][nq][nd][dim] for the scalar kernels
   const size_t bsz = (size_t)nq * nd * dim;
@@ -1570,7 +1613,7 @@ MatFreeOp::MatFreeOp(int dim_, int nd_, int nq_, int64_t ndof, int nclass, const
         for (int i = 0; i < nd; i++)
           for (int d = 0; d < dim; d++) {
             const double bv = bmat[c * bsz + ((size_t)q * nd + i) * dim + d];
-            const uint16_t v = prec == NGSAMD_PREC_F16 ? ToF16(bv) : ToBf16(bv);
+            const uint16_t v = prec != NGSAMD_PREC_BF16 ? ToF16(bv) : ToBf16(bv);
             b1[((size_t)q * dim + d) * k1p + i] = v;
             // position of point q inside its chunk in the accumulator order
             const int ch = q / 32, ql = q % 32, h = ql / 16, grp = (ql % 16) / 4, r = ql % 4;
@@ -1905,11 +1948,11 @@ static void DispatchFac(const MatFreeOp& A, const T* fm, const V* x, V* out, dou
   throw Error(NGSAMD_E_STATE, "matrix-free: unsupported factored shape");
 }
 
-template <class H, int DIM, int KS1, int MT, int KG, int ND, int MODE, class V>
+template <class H, int DIM, int KS1, int MT, int KG, int ND, int MODE, class V, bool SPLIT>
 static void LaunchFac16(const MatFreeOp& A, const V* x, V* out, double s, const int* stop, hipStream_t st,
                         bool prepare, const MfSched& S) {
   const size_t lds = (size_t)A.fstride * 2;
-  auto kern = MfFac16Kernel<H, DIM, KS1, MT, KG, ND, MODE, V>;
+  auto kern = MfFac16Kernel<H, DIM, KS1, MT, KG, ND, MODE, V, SPLIT>;
   if (prepare) return PrepareKernel(A, kern, lds, MODE, sizeof(V) == 8);
   const MfRounds R{S.round0, S.range, S.nrounds, (int)A.classes.size(), S.tbase};
   kern<<<MfGrid(A, MODE, sizeof(V) == 8, S.nrounds), kMfBlock, lds, st>>>(
@@ -1917,12 +1960,12 @@ static void LaunchFac16(const MatFreeOp& A, const V* x, V* out, double s, const 
   NGSAMD_CHECK_LAUNCH();
 }
 
-template <class H, int DIM, int MODE, class V>
+template <class H, int DIM, int MODE, class V, bool SPLIT = false>
 static void DispatchFac16(const MatFreeOp& A, const V* x, V* out, double s, const int* stop, hipStream_t st,
                           bool prepare, const MfSched& S) {
 #define NGSAMD_FAC16(KS1, MT, KG, ND)                                                  \
   if (A.ks1 == KS1 && A.mt16 == MT && A.fac_kg == KG && A.nd == ND)                      \
-    return LaunchFac16<H, DIM, KS1, MT, KG, ND, MODE, V>(A, x, out, s, stop, st, prepare, S);
+    return LaunchFac16<H, DIM, KS1, MT, KG, ND, MODE, V, SPLIT>(A, x, out, s, stop, st, prepare, S);
   // H1 triangles p = 1..5, tetrahedra p = 1..4
   if constexpr (DIM == 2) {
     NGSAMD_FAC16(1, 1, 1, 3) NGSAMD_FAC16(1, 1, 1, 6) NGSAMD_FAC16(1, 1, 2, 10) NGSAMD_FAC16(1, 1, 3, 15)
@@ -1948,6 +1991,9 @@ static void LaunchFacElements(const MatFreeOp& A, const V* x, V* out, double s, 
   } else if (A.precision == NGSAMD_PREC_F16) {
     if (A.dim == 2) DispatchFac16<_Float16, 2, MODE, V>(A, x, out, s, stop, st, prepare, S);
     else DispatchFac16<_Float16, 3, MODE, V>(A, x, out, s, stop, st, prepare, S);
+  } else if (A.precision == NGSAMD_PREC_F16X3) {
+    if (A.dim == 2) DispatchFac16<_Float16, 2, MODE, V, true>(A, x, out, s, stop, st, prepare, S);
+    else DispatchFac16<_Float16, 3, MODE, V, true>(A, x, out, s, stop, st, prepare, S);
   } else if (A.precision == NGSAMD_PREC_F64) {
     if (A.dim == 2) DispatchFac<double, 2, MODE, V>(A, A.fmat64.p, x, out, s, stop, st, prepare, S);
     else DispatchFac<double, 3, MODE, V>(A, A.fmat64.p, x, out, s, stop, st, prepare, S);
@@ -2141,7 +2187,7 @@ int MatFreeOp::Apply(double s, const V* x, V* y, bool accumulate, const int* sto
   // one thread per dof: a wide grid hides each dof's load chain
   const int np = GridFor(height, kGatherBlock, kMaxWideBlocks);
   // the factored 16-bit kernels wrote tile-major rows
-  const bool half = precision == NGSAMD_PREC_BF16 || precision == NGSAMD_PREC_F16;
+  const bool half = precision == NGSAMD_PREC_BF16 || precision == NGSAMD_PREC_F16 || precision == NGSAMD_PREC_F16X3;
   const int* slots = use_mfma && fac_kernel && half ? g_tslots.p : g_slots.p;
   if (fac_dot) {
     MfGatherKernel<V, V, false><<<np, kGatherBlock, 0, st>>>(g_rowptr.p, slots, g_order.p, contrib, y, height, s,

@@ -24,7 +24,7 @@ def _colors_table_order(P):
 
 
 @pytest.mark.parametrize("dim,n,p", [(2, 9, 1), (2, 8, 3), (2, 6, 5), (3, 4, 2), (3, 4, 4)])
-@pytest.mark.parametrize("precision,tol", [(0, 1e-13), (1, 1e-5), (2, 3e-2), (3, 4e-3)])
+@pytest.mark.parametrize("precision,tol", [(0, 1e-13), (1, 1e-5), (2, 3e-2), (3, 4e-3), (4, 1e-5)])
 def test_color_scatter_apply(device, dim, n, p, precision, tol):
     say(f"[coloring] dim {dim} n {n} p {p} precision {precision}")
     P = problem(dim, n, p, seed_kappa=2)

@@ -110,8 +110,9 @@ def test_config2_mf_f64_apply(device):
 
 
 # bounds as at the smaller sizes: FP32 1e-5; FP16 (11-bit significand) 4e-3;
-# BF16 (8-bit significand) 3e-2, relative in the inf-norm, FP32 accumulation
-@pytest.mark.parametrize("precision,tol", [(1, 1e-5), (2, 3e-2), (3, 4e-3)])
+# BF16 (8-bit significand) 3e-2; split FP16 1e-5, relative in the inf-norm,
+# FP32 accumulation
+@pytest.mark.parametrize("precision,tol", [(1, 1e-5), (2, 3e-2), (3, 4e-3), (4, 1e-5)])
 def test_config2_mf_low_precision_apply(device, precision, tol):
     P = _config(None)
     x = np.random.default_rng(1).uniform(-1, 1, P["ndof"])
@@ -144,10 +145,10 @@ def test_fp64_cg_matches_oracle(device, op, kappa_seed):
     assert abs(res[0] - res_ref[0]) <= 1e-12 * res_ref[0]
 
 
-@pytest.mark.parametrize("precision", [1, 2, 3])
+@pytest.mark.parametrize("precision", [1, 2, 3, 4])
 def test_config3_refinement_default_inner_tolerance(device, precision):
-    """FP64 refinement around FP32 / BF16 / FP16 matrix-free inner solves, inner
-    tolerance left to its default (inner_tol <= 0)."""
+    """FP64 refinement around FP32 / BF16 / FP16 / split-FP16 matrix-free inner
+    solves, inner tolerance left to its default (inner_tol <= 0)."""
     P = _config(2)
     A64 = _mf(P, 0)
     A_in = _mf(P, precision)

@@ -1,6 +1,6 @@
-"""Mixed-precision iterative refinement: FP64 residual around an FP32, BF16
-or FP16 matrix-free inner CG reaches 1e-10 relative residual and agrees with
-the FP64 solution."""
+"""Mixed-precision iterative refinement: FP64 residual around an FP32, BF16,
+FP16 or split-FP16 matrix-free inner CG reaches 1e-10 relative residual and
+agrees with the FP64 solution."""
 import numpy as np
 import pytest
 
@@ -11,7 +11,7 @@ from helpers import N, dvec, mf_operator, problem, say
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("precision", [1, 2, 3])
+@pytest.mark.parametrize("precision", [1, 2, 3, 4])
 @pytest.mark.parametrize("dim,n,p,order", [(3, 4, 4, 8), (2, 32, 3, 6)])
 def test_ir_reaches_1e10(device, precision, dim, n, p, order):
     say(f"[ir prec={precision}] dim={dim} n={n} p={p}")

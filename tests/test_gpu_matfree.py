@@ -1,7 +1,8 @@
 """Fused matrix-free operator on the GPU against the oracle: FP64 within
 1e-13 relative in the inf-norm of the assembled product, FP32 within 1e-5,
-both scatter modes, both MFMA kernel forms (reference matrices, the default,
-and per-quadrature-point), the diagonal and the element gradient kernel."""
+both scatter modes, the three MFMA kernel forms (factored reference
+matrices, the default; the sum over the reference matrices; per quadrature
+point), split FP16, the diagonal and the element gradient kernel."""
 import numpy as np
 import pytest
 
@@ -15,7 +16,7 @@ CASES = [(2, 6, 1, None), (2, 6, 2, None), (2, 8, 3, 6), (2, 5, 4, None), (2, 5,
          (3, 3, 1, None), (3, 3, 2, None), (3, 3, 3, None), (3, 3, 4, 8), (3, 2, 4, None), (3, 2, 5, None)]
 
 
-KERNELS = ["ref", "quad"]
+KERNELS = ["fac", "ref", "quad"]
 
 
 def _select(monkeypatch, kernel):
@@ -79,6 +80,38 @@ def test_mf_low_precision(device, monkeypatch, precision, tol, dim, n, p, order,
     x32, y32 = dvec(x, 4), N.DeviceVector(P["ndof"], 4)
     A.mult_f32(x32, y32)
     assert relerr_inf(y32.download().astype(np.float64), ref) < tol
+
+
+# Split FP16 (three FP16 products per contraction): FP32's tolerance, in the
+# factored kernels (the only ones that take it).
+@pytest.mark.parametrize("dim,n,p,order", [(2, 8, 3, 6), (3, 3, 4, 8), (3, 2, 2, None), (2, 5, 5, None),
+                                           (2, 6, 1, None), (3, 3, 1, None), (3, 2, 3, None)])
+@pytest.mark.parametrize("scatter", [0, 1])
+def test_mf_split_f16(device, dim, n, p, order, scatter):
+    say(f"[mf f16x3] dim={dim} n={n} p={p} scatter={scatter}")
+    P = problem(dim, n, p, order=order, seed_kappa=3)
+    A = mf_operator(P, precision=4, scatter=scatter)
+    x = np.random.default_rng(1).uniform(-1, 1, P["ndof"])
+    ref = oracle.spmv(P["rowptr"], P["cols"], P["vals"], x)
+    dx, dy = dvec(x), N.DeviceVector(P["ndof"], 8)
+    A.mult(dx, dy)
+    err = relerr_inf(dy.download(), ref)
+    x32, y32 = dvec(x, 4), N.DeviceVector(P["ndof"], 4)
+    A.mult_f32(x32, y32)
+    err32 = relerr_inf(y32.download().astype(np.float64), ref)
+    say(f"  rel err {err:.2e} (FP32 vectors {err32:.2e})")
+    assert err < 1e-5 and err32 < 1e-5
+    # and far below plain FP16's error on the same operator
+    A16 = mf_operator(P, precision=3, scatter=scatter)
+    A16.mult(dx, dy)
+    assert err < 0.05 * relerr_inf(dy.download(), ref)
+
+
+def test_mf_split_f16_needs_factored_kernels(device, monkeypatch):
+    monkeypatch.setenv("NGSAMD_MF_KERNEL", "ref")
+    P = problem(2, 4, 2, seed_kappa=3)
+    with pytest.raises(RuntimeError, match="F16X3"):
+        mf_operator(P, precision=4)
 
 
 @pytest.mark.parametrize("dim,n,p", [(2, 6, 3), (3, 3, 4)])
