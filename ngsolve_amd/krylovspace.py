@@ -169,9 +169,11 @@ class DevCGSolver(LinearSolver):
 class IRCGSolver(LinearSolver):
     """FP64 iterative refinement around a low-precision matrix-free inner CG.
 
-    `mat` is the FP64 operator for the residual, `inner` the FP32/FP16/BF16
-    matrix-free operator; stops when ||f - A u||_free <= tol ||f||_free.
-    inner_tol=None uses the solver's default for the inner precision."""
+    `mat` is the FP64 operator for the residual, `inner` the FP32 / FP16 /
+    BF16 / split-FP16 ("f16x3") matrix-free operator; stops when
+    ||f - A u||_free <= tol ||f||_free.  inner_tol=None uses the solver's
+    default for the inner precision (FP32 and split FP16 3e-4, FP16 1e-3,
+    BF16 3e-2)."""
 
     name = "IRCG"
 
