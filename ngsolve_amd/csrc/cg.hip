@@ -355,15 +355,28 @@ struct CgWork {
 // tolerances and counters from the device scalars, so one instance serves
 // every solve on the same operators and vectors (the refinement's inner
 // solves capture it once per refinement, not once per step).
+// It also keeps the pinned host copies of the device scalars and the two
+// events the host polls, so that the inner solves of a refinement allocate
+// them once.
 struct CgGraph {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   bool failed = false;  // capture not possible: launch the kernels directly
+  CGScalars* pin = nullptr;  // 2 pinned host copies of the device scalars
+  hipEvent_t ev[2] = {nullptr, nullptr};
   CgGraph() = default;
   CgGraph(const CgGraph&) = delete;
   ~CgGraph() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
+    if (pin) (void)hipHostFree(pin);
+    for (hipEvent_t e : ev)
+      if (e) (void)hipEventDestroy(e);
+  }
+  void Prepare() {
+    if (!pin) NGSAMD_HIP(hipHostMalloc((void**)&pin, 2 * sizeof(CGScalars), hipHostMallocDefault));
+    for (hipEvent_t& e : ev)
+      if (!e) NGSAMD_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   }
 };
 
@@ -385,18 +398,14 @@ static CgResult IterateCg(ngsamd_op_t A, const V* inv, ngsamd_op_t pre, V* u, Cg
   CgInitCheckKernel<<<1, kCgBlock, 0, st>>>(LibraryPartials().w, g, W.S.p, W.residuals.p);
   NGSAMD_CHECK_LAUNCH();
 
-  CGScalars* pin = nullptr;
-  NGSAMD_HIP(hipHostMalloc((void**)&pin, 2 * sizeof(CGScalars), hipHostMallocDefault));
-  struct PinGuard {
-    CGScalars* p;
-    ~PinGuard() { (void)hipHostFree(p); }
-  } pin_guard{pin};
-
   // graphs hold an even number of iterations so that every replay starts
   // at parity 0
   if (chunk > 0) chunk += chunk & 1;
   CgGraph local;
   CgGraph& G = cache ? *cache : local;
+  G.Prepare();
+  CGScalars* pin = G.pin;
+  hipEvent_t* ev = G.ev;
   if (chunk > 0 && !G.exec && !G.failed) {
     hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
     if (e == hipSuccess) {
@@ -430,13 +439,6 @@ static CgResult IterateCg(ngsamd_op_t A, const V* inv, ngsamd_op_t pre, V* u, Cg
       EnqueueIteration<V>(A, inv, pre, u, W.d.p, W.w.p, W.s.p, n, W.S.p, W.residuals.p, (int)(enqueued++ & 1), st);
     NGSAMD_HIP(hipMemcpyAsync(&pin[slot], W.S.p, sizeof(CGScalars), hipMemcpyDeviceToHost, st));
   };
-  hipEvent_t ev[2];
-  NGSAMD_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
-  NGSAMD_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
-  struct EvGuard {
-    hipEvent_t* e;
-    ~EvGuard() { (void)hipEventDestroy(e[0]); (void)hipEventDestroy(e[1]); }
-  } ev_guard{ev};
 
   NGSAMD_HIP(hipMemcpyAsync(&pin[0], W.S.p, sizeof(CGScalars), hipMemcpyDeviceToHost, st));
   NGSAMD_HIP(hipStreamSynchronize(st));
