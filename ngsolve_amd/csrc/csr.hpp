@@ -41,8 +41,9 @@ struct CsrOp : ngsamd_op_s {
   CsrOp(int64_t h, int64_t w, const int64_t* rp, const int* c, const double* v);
   void MultAdd(double s, const double* x, double* y, hipStream_t st) override;
   void Mult(const double* x, double* y, hipStream_t st) override;
-  // y = A x plus per-workgroup partial sums of <x, A x> in partials[];
-  // returns the number of partials; skipped entirely when *d_stop != 0.
+  // y = A x plus per-workgroup partial sums of <x, A x> in partials[] (room
+  // for kMaxReduceBlocks); returns the number of partials; skipped entirely
+  // when *d_stop != 0.
   // wide: room for kMaxWideBlocks partials of the wide grid (null: the
   // library's)
   int MultDot(const double* x, double* y, double* partials, const int* d_stop, hipStream_t st,

@@ -748,8 +748,22 @@ void CsrOp::Mult(const double* x, double* y, hipStream_t st) {
   flip ^= 1;
 }
 
+// Workgroups of the DOT variant's launch (one partial each), as LaunchRuns,
+// LaunchStream and LaunchSpmvG choose them.
+static int DotWorkgroups(const CsrOp& A) {
+  if (A.algo == 0) return kMaxReduceBlocks;  // at most this many (LaunchSpmvG)
+  const int cap = std::min(A.max_grid, kMaxWideBlocks);
+  int wg = A.nblocks < cap ? A.nblocks : cap;
+  if (wg > 8) wg &= ~7;
+  return wg < 1 ? 1 : wg;
+}
+
 int CsrOp::MultDot(const double* x, double* y, double* partials, const int* d_stop, hipStream_t st,
                    double* wide) {
+  // a small grid (at most 1,024 workgroups, e.g. 2D 64^2 p=3) writes its
+  // partials straight to CG's, whose update kernel sums them: one launch
+  // less per iteration
+  if (DotWorkgroups(*this) <= 1024) return LaunchSpmv<false, true>(*this, 1.0, x, y, st, partials, d_stop);
   // the operator on its wide grid writes one partial per workgroup; a
   // 64-workgroup kernel folds them so that CG's kernels read 64 values
   if (!wide) wide = State().wide_partials;
