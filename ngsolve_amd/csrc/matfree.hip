@@ -1131,7 +1131,9 @@ __global__ __launch_bounds__(kScalarBlock) void MfScalarKernel(
 
 // y[r] = (ACC ? y[r] : 0) + s * sum of the contributions of dof r, in slot
 // order; with DOT also the per-workgroup partial sums of x[r] * y[r] (the
-// <s, A s> of CG) into partials[].
+// <s, A s> of CG) into partials[].  With fin: the first min(64, grid)
+// workgroups also fold the nfin partials fin[] (the contraction's) into
+// fout[] (consecutive groups, summed in index order by one thread).
 constexpr int kGatherBlock = 256;
 constexpr int kGatherWidth = 4;  // 8 measured slower (FP32 gather 38.5 vs 34.7 us)
 
@@ -1139,8 +1141,19 @@ template <class C, class V, bool DOT>
 __global__ __launch_bounds__(kGatherBlock) void MfGatherKernel(
     const int* __restrict__ rowptr, const int* __restrict__ slots, const int* __restrict__ order,
     const C* __restrict__ contrib, V* __restrict__ y, int64_t n, double s, int accumulate,
-    const int* __restrict__ stop, const V* __restrict__ x, double* __restrict__ partials) {
+    const int* __restrict__ stop, const V* __restrict__ x, double* __restrict__ partials,
+    const double* __restrict__ fin = nullptr, int nfin = 0, double* __restrict__ fout = nullptr) {
   if (stop && *stop) return;
+  if (fin && threadIdx.x == 0) {
+    const int nout = min(kFoldedPartials, (int)gridDim.x);
+    if ((int)blockIdx.x < nout) {
+      const int per = (nfin + nout - 1) / nout;
+      const int b0 = (int)blockIdx.x * per, b1 = min(b0 + per, nfin);
+      double v = 0.0;
+      for (int i = b0; i < b1; i++) v += fin[i];
+      fout[blockIdx.x] = v;
+    }
+  }
   double dotacc = 0.0;
   // contiguous ranges of the gather order per XCD
   const int64_t per = (n + gridDim.x - 1) / gridDim.x;
@@ -2164,13 +2177,14 @@ int MatFreeOp::Apply(double s, const V* x, V* y, bool accumulate, const int* sto
   else contrib = reinterpret_cast<V*>(contrib32.p);
   // with partials (CG): one partial of <x, y> per workgroup.  For y = A x
   // the factored kernels form it themselves as sum_e U_e . V_e =
-  // sum_e x_e^T K_e x_e (no second read of x by dof), straight into
-  // partials[] (at most kMaxReduceBlocks, one per workgroup of the
-  // contraction's resident grid); otherwise the gather forms
-  // sum_r x[r] y[r] on its wide grid, folded to kFoldedPartials.
+  // sum_e x_e^T K_e x_e (no second read of x by dof), one per workgroup of
+  // the contraction's resident grid, which the gather's first workgroups
+  // fold into partials[] (at most kFoldedPartials); otherwise the gather
+  // forms sum_r x[r] y[r] on its wide grid, folded to kFoldedPartials by a
+  // kernel of its own.
   const bool fac_dot = partials && use_mfma && fac_kernel && !accumulate && s == 1.0 &&
-                       MfGrid(*this, 0, sizeof(V) == 8, nrounds) <= kMaxReduceBlocks;
-  double* dotp = partials ? (fac_dot ? partials : wide ? wide : State().wide_partials) : nullptr;
+                       MfGrid(*this, 0, sizeof(V) == 8, nrounds) <= kMaxWideBlocks;
+  double* dotp = partials ? (wide ? wide : State().wide_partials) : nullptr;
   int ndot = 0;
   if (fac_dot) {
     MfSched S = OwnSched(*this);
@@ -2191,9 +2205,9 @@ int MatFreeOp::Apply(double s, const V* x, V* y, bool accumulate, const int* sto
   const int* slots = use_mfma && fac_kernel && half ? g_tslots.p : g_slots.p;
   if (fac_dot) {
     MfGatherKernel<V, V, false><<<np, kGatherBlock, 0, st>>>(g_rowptr.p, slots, g_order.p, contrib, y, height, s,
-                                                             0, stop, nullptr, nullptr);
+                                                             0, stop, nullptr, nullptr, dotp, ndot, partials);
     NGSAMD_CHECK_LAUNCH();
-    return ndot;
+    return std::min(kFoldedPartials, np);
   }
   if (partials)
     MfGatherKernel<V, V, true><<<np, kGatherBlock, 0, st>>>(g_rowptr.p, slots, g_order.p, contrib, y, height, s,
