@@ -109,17 +109,18 @@ constexpr int kMaxLdsClasses = 64;
 // Each thread issues 4 loads before its first LDS write (from clamped
 // indices, so no load sits behind a branch): a plain load-then-store loop
 // waits for every load in turn.
+template <int BLOCK = kMfBlock>
 __device__ __forceinline__ void CopyToLds(void* dst, const void* src, int n) {
   constexpr int U = 4;
   const uint4* s = reinterpret_cast<const uint4*>(src);
   uint4* d = reinterpret_cast<uint4*>(dst);
-  for (int base = 0; base < n; base += kMfBlock * U) {
+  for (int base = 0; base < n; base += BLOCK * U) {
     uint4 v[U];
 #pragma unroll
-    for (int j = 0; j < U; j++) v[j] = s[min(base + j * kMfBlock + (int)threadIdx.x, n - 1)];
+    for (int j = 0; j < U; j++) v[j] = s[min(base + j * BLOCK + (int)threadIdx.x, n - 1)];
 #pragma unroll
     for (int j = 0; j < U; j++) {
-      const int i = base + j * kMfBlock + (int)threadIdx.x;
+      const int i = base + j * BLOCK + (int)threadIdx.x;
       if (i < n) d[i] = v[j];
     }
   }
@@ -891,8 +892,8 @@ __global__ __launch_bounds__(kMfBlock) void MfFacKernel(
 // each product is D_hi X_hi + D_hi X_lo + D_lo X_hi in the FP32
 // accumulators: about 22 significant bits, the FP32 kernel's accuracy, at
 // three times the 16-bit MFMA work (still a few percent of the MFMA pipe).
-template <class H, int DIM, int KS1, int MT, int KG, int ND, int MODE, class V, bool SPLIT>
-__global__ __launch_bounds__(kMfBlock) void MfFac16Kernel(
+template <class H, int DIM, int KS1, int MT, int KG, int ND, int MODE, class V, bool SPLIT, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void MfFac16Kernel(
     const uint16_t* __restrict__ fmat, int64_t fstride, MfRounds R, const int* __restrict__ dofs,
     const double* __restrict__ metric, const double* __restrict__ kappa, const int* __restrict__ elnr, int nd,
     const V* __restrict__ x, V* __restrict__ out, double s, const int* __restrict__ stop,
@@ -913,11 +914,11 @@ __global__ __launch_bounds__(kMfBlock) void MfFac16Kernel(
   __shared__ int s_round0[kMaxLdsClasses + 1];
   __shared__ int s_tbase[kMaxLdsClasses + 1];
   __shared__ int64_t s_range[2 * kMaxLdsClasses];
-  for (int i = threadIdx.x; i <= R.nclass; i += kMfBlock) {
+  for (int i = threadIdx.x; i <= R.nclass; i += (64 * WAVES)) {
     s_round0[i] = R.round0[i];
     if (TILED) s_tbase[i] = R.tbase[i];
   }
-  for (int i = threadIdx.x; i < 2 * R.nclass; i += kMfBlock) s_range[i] = R.range[i];
+  for (int i = threadIdx.x; i < 2 * R.nclass; i += (64 * WAVES)) s_range[i] = R.range[i];
   __syncthreads();
   const h8* a1 = reinterpret_cast<const h8*>(lds_raw) + lane;
   const h8* a2 = a1 + TT * KS1 * 64;
@@ -934,12 +935,12 @@ __global__ __launch_bounds__(kMfBlock) void MfFac16Kernel(
     if (cl != cls) {
       if (cls >= 0) __syncthreads();
       // fstride 16-bit words, a multiple of 8
-      CopyToLds(lds_raw, fmat + (int64_t)cl * fstride, (int)(fstride / 8));
+      CopyToLds<64 * WAVES>(lds_raw, fmat + (int64_t)cl * fstride, (int)(fstride / 8));
       __syncthreads();
       cls = cl;
     }
     const int64_t end = s_range[2 * cl + 1];
-    const int64_t first = s_range[2 * cl] + ((int64_t)(rd - s_round0[cl]) * kMfWaves + wave) * 16;
+    const int64_t first = s_range[2 * cl] + ((int64_t)(rd - s_round0[cl]) * WAVES + wave) * 16;
     if (first >= end) continue;
     const int64_t e = first + col;
     const bool valid = e < end;
@@ -1080,8 +1081,8 @@ __global__ __launch_bounds__(kMfBlock) void MfFac16Kernel(
     }
   }
   if (MODE == 0 && partials) {
-    __shared__ double s_dot[kMfWaves];
-    StorePartial<kMfBlock>(dacc, partials, s_dot);
+    __shared__ double s_dot[WAVES];
+    StorePartial<(64 * WAVES)>(dacc, partials, s_dot);
   }
 }
 
@@ -1720,14 +1721,18 @@ MatFreeOp::MatFreeOp(int dim_, int nd_, int nq_, int64_t ndof, int nclass, const
   // at least one entry (operators without elements)
   contrib64.alloc(std::max<size_t>((size_t)nel * nd, 1));
   contrib32.alloc(std::max<size_t>((size_t)nel * nd, 1));
-  // round schedule of the MFMA kernels: 16-element tiles, kMfWaves tiles per
-  // round, rounds never straddle a class
+  // round schedule of the MFMA kernels: 16-element tiles, mf_waves tiles per
+  // round (one per wavefront of a workgroup), rounds never straddle a class.
+  // Split FP16 runs 4-wave workgroups: at 132 VGPRs an 8-wave one fits once
+  // per CU, 4-wave ones three times (55.5 against 57.8 us per apply at
+  // 32^3 p=4, FP32 vectors); the other kernels measured slower that way.
+  mf_waves = prec == NGSAMD_PREC_F16X3 ? 4 : kMfWaves;
   {
     std::vector<int> r0(nclass + 1, 0);
     std::vector<int64_t> rng(2 * (size_t)nclass);
     for (int c = 0; c < nclass; c++) {
       const int64_t tiles = (off[c + 1] - off[c] + 15) / 16;
-      r0[c + 1] = r0[c] + (int)((tiles + kMfWaves - 1) / kMfWaves);
+      r0[c + 1] = r0[c] + (int)((tiles + mf_waves - 1) / mf_waves);
       rng[2 * c] = off[c];
       rng[2 * c + 1] = off[c + 1];
     }
@@ -1828,10 +1833,11 @@ static MfSched OwnSched(const MatFreeOp& A) {
 // prepare: enable the kernel's dynamic LDS size and record its occupancy
 // (once, outside any stream capture)
 template <class K>
-static void PrepareKernel(const MatFreeOp& A, K kern, size_t lds, int mode, bool f64_vectors) {
+static void PrepareKernel(const MatFreeOp& A, K kern, size_t lds, int mode, bool f64_vectors,
+                          int block = kMfBlock) {
   NGSAMD_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   int nb = 0;
-  NGSAMD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, kMfBlock, lds));
+  NGSAMD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, block, lds));
   A.occ[mode][f64_vectors ? 1 : 0] = nb;
 }
 
@@ -1965,10 +1971,13 @@ template <class H, int DIM, int KS1, int MT, int KG, int ND, int MODE, class V, 
 static void LaunchFac16(const MatFreeOp& A, const V* x, V* out, double s, const int* stop, hipStream_t st,
                         bool prepare, const MfSched& S) {
   const size_t lds = (size_t)A.fstride * 2;
-  auto kern = MfFac16Kernel<H, DIM, KS1, MT, KG, ND, MODE, V, SPLIT>;
-  if (prepare) return PrepareKernel(A, kern, lds, MODE, sizeof(V) == 8);
+  // split FP16: 4-wave workgroups (the operator's rounds hold 4 tiles)
+  constexpr int W = SPLIT ? 4 : kMfWaves;
+  auto kern = MfFac16Kernel<H, DIM, KS1, MT, KG, ND, MODE, V, SPLIT, W>;
+  if (prepare) return PrepareKernel(A, kern, lds, MODE, sizeof(V) == 8, 64 * W);
+  NGSAMD_REQUIRE(A.mf_waves == W, "matrix-free: round schedule does not match the kernel's workgroup");
   const MfRounds R{S.round0, S.range, S.nrounds, (int)A.classes.size(), S.tbase};
-  kern<<<MfGrid(A, MODE, sizeof(V) == 8, S.nrounds), kMfBlock, lds, st>>>(
+  kern<<<MfGrid(A, MODE, sizeof(V) == 8, S.nrounds), 64 * W, lds, st>>>(
       A.fmat16.p, A.fstride, R, S.dofs, S.metric, A.Kappa(), S.elnr, A.nd, x, out, s, stop, S.partials);
   NGSAMD_CHECK_LAUNCH();
 }
@@ -2122,7 +2131,7 @@ void MatFreeOp::SetColoring(const int* colors) {
       rng[(size_t)c * 2 * nclass + 2 * cl] = b;
       rng[(size_t)c * 2 * nclass + 2 * cl + 1] = k;
       const int64_t tiles = (k - b + 15) / 16;
-      rc[cl + 1] = rc[cl] + (int)((tiles + kMfWaves - 1) / kMfWaves);
+      rc[cl + 1] = rc[cl] + (int)((tiles + mf_waves - 1) / mf_waves);
     }
     c_nrounds[c] = rc[nclass];
   }

@@ -25,6 +25,9 @@ struct MatFreeOp : ngsamd_op_s {
   DevBuf<int> round0;         // first round of each class, nclass + 1
   DevBuf<int64_t> cls_range;  // [begin, end) of each class
   int nrounds = 0;
+  // tiles per round = wavefronts per workgroup of the MFMA kernels: 8, or 4
+  // for split FP16 (whose 132 VGPRs allow one 8-wave workgroup per CU)
+  int mf_waves = 8;
   mutable int occ[3][2] = {{1, 1}, {1, 1}, {1, 1}};  // resident workgroups per CU [mode][FP64 vectors]
   // per class reference gradients, MFMA layout [nqp][dim][ndp] (+1 pad per q),
   // in the contraction type (double / float / bf16 bits)
