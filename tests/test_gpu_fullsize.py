@@ -40,7 +40,7 @@ def _release():
 NX, P_ORDER, ORDER = 32, 4, 8
 
 
-def _config(kappa_seed):
+def _cube32(kappa_seed):
     """Product host setup (pattern, values, dof tables) for kappa = 1
     (kappa_seed None) or lognormal kappa; the oracle's mesh / dofs / geometry
     and per-class reference gradients are shared."""
@@ -84,8 +84,8 @@ def _mf(P, precision):
     return P[key]
 
 
-def test_config2_sizes_and_dofs_match_oracle(device):
-    P = _config(None)
+def test_32cube_p4_sizes_and_dofs_match_oracle(device):
+    P = _cube32(None)
     pr = P["oracle_problem"]
     assert P["ndof"] == pr.ndof == 2146689
     assert int(P["rowptr"][-1]) == 159230209
@@ -95,8 +95,8 @@ def test_config2_sizes_and_dofs_match_oracle(device):
     assert int((~pr.free.astype(bool)).sum()) == 98306
 
 
-def test_config2_mf_f64_apply(device):
-    P = _config(None)
+def test_32cube_p4_mf_f64_apply(device):
+    P = _cube32(None)
     x = np.random.default_rng(1).uniform(-1, 1, P["ndof"])
     ref = _oracle_apply(P, x)
     # the host-assembled matrix is the oracle's operator at full size
@@ -113,8 +113,8 @@ def test_config2_mf_f64_apply(device):
 # BF16 (8-bit significand) 3e-2; split FP16 1e-5, relative in the inf-norm,
 # FP32 accumulation
 @pytest.mark.parametrize("precision,tol", [(1, 1e-5), (2, 3e-2), (3, 4e-3), (4, 1e-5)])
-def test_config2_mf_low_precision_apply(device, precision, tol):
-    P = _config(None)
+def test_32cube_p4_mf_low_precision_apply(device, precision, tol):
+    P = _cube32(None)
     x = np.random.default_rng(1).uniform(-1, 1, P["ndof"])
     ref = _oracle_apply(P, x)
     A = _mf(P, precision)
@@ -131,7 +131,7 @@ def test_config2_mf_low_precision_apply(device, precision, tol):
 @pytest.mark.parametrize("op", ["csr", "mf"])
 @pytest.mark.parametrize("kappa_seed", [None, 2])
 def test_fp64_cg_matches_oracle(device, op, kappa_seed):
-    P = _config(kappa_seed)
+    P = _cube32(kappa_seed)
     u_ref, its_ref, res_ref = _oracle_pcg(P, 1e-10)
     A = N.CreateCSR(P["ndof"], P["ndof"], P["rowptr"], P["cols"], P["vals"]) if op == "csr" else _mf(P, 0)
     C = N.CreateJacobi(P["diag"], P["fes"].freedofs)
@@ -146,10 +146,10 @@ def test_fp64_cg_matches_oracle(device, op, kappa_seed):
 
 
 @pytest.mark.parametrize("precision", [1, 2, 3, 4])
-def test_config3_refinement_default_inner_tolerance(device, precision):
+def test_32cube_p4_lognormal_refinement_default_inner_tolerance(device, precision):
     """FP64 refinement around FP32 / BF16 / FP16 / split-FP16 matrix-free inner
     solves, inner tolerance left to its default (inner_tol <= 0)."""
-    P = _config(2)
+    P = _cube32(2)
     A64 = _mf(P, 0)
     A_in = _mf(P, precision)
     C = N.CreateJacobi(P["diag"], P["fes"].freedofs)

@@ -22,7 +22,7 @@ pytestmark = [pytest.mark.gpu, pytest.mark.slow]
 NX, P_ORDER = 128, 3
 
 
-def test_config4_adjoint_gradient_full_size(device):
+def test_128sq_p3_adjoint_gradient_full_size(device):
     import torch
     import ngsolve_amd as ngs
     from ngsolve_amd.autograd import ApplySymmetric, PoissonProblem, SolvePoisson

@@ -48,7 +48,7 @@ def _solved():
     return _state
 
 
-def test_config5_persistent_launch_matches_graph(device):
+def test_512_problems_persistent_launch_matches_graph(device):
     """The persistent form (one cooperative launch) on all 512 problems: the
     graph form's iteration counts, residuals and solutions bit for bit."""
     S = _solved()
@@ -58,7 +58,7 @@ def test_config5_persistent_launch_matches_graph(device):
 
 
 @pytest.mark.parametrize("part", [0, 1, 2, 3])
-def test_config5_every_problem_vs_oracle(device, part):
+def test_512_problems_every_problem_vs_oracle(device, part):
     S = _solved()
     probs, off = S["probs"], S["offsets"]
     worst_its, worst_err, worst_res = 0, 0.0, 0.0
