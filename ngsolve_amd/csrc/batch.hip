@@ -946,10 +946,14 @@ static void LaunchPersistent(const PersistentArgs& P, hipStream_t st) {
   int occ = 0;
   NGSAMD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, kBatchBlock, 0));
   NGSAMD_REQUIRE(occ >= 1, "batch cg: persistent kernel does not fit a compute unit");
-  // NGSAMD_BATCH_PERSISTENT_WG: workgroups per compute unit (at most the
-  // occupancy)
+  // Two workgroups per compute unit by default (NGSAMD_BATCH_PERSISTENT_WG:
+  // another count, at most the occupancy).  A grid at the full occupancy
+  // has once timed out at a barrier (a workgroup not resident while the
+  // others waited); well below it every workgroup finds a slot even when
+  // the dispatcher does not spread them evenly, and 2 to 8 per CU measured
+  // the same time.
   const char* wenv = getenv("NGSAMD_BATCH_PERSISTENT_WG");
-  if (wenv) occ = std::max(1, std::min(occ, atoi(wenv)));
+  occ = std::max(1, std::min(occ, wenv ? atoi(wenv) : 2));
   int64_t g = std::min<int64_t>((int64_t)occ * DeviceCUs(), std::max({P.nblocks, P.nchunks, P.nseg}));
   if (g > 8) g &= ~int64_t(7);
   if (g < 1) g = 1;
