@@ -38,3 +38,13 @@ def test_last_error_without_init():
 def test_python_package_imports_native_modules():
     import ngsolve_amd
     assert ngsolve_amd._N.__file__.endswith(".so")
+
+
+def test_precision_names_match_header():
+    """The Python precision names map to the header's NGSAMD_PREC_* values."""
+    with open(os.path.join(ROOT, "include", "ngsamd.h")) as f:
+        consts = {m.group(1): int(m.group(2)) for m in re.finditer(r"#define NGSAMD_PREC_(\w+) (\d+)", f.read())}
+    from ngsolve_amd._native import PREC
+    assert consts == {"F64": 0, "F32": 1, "BF16": 2, "F16": 3, "F16X3": 4}
+    for name, key in [("f64", "F64"), ("f32", "F32"), ("bf16", "BF16"), ("f16", "F16"), ("f16x3", "F16X3")]:
+        assert PREC[name] == consts[key]
