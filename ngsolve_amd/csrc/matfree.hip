@@ -2613,6 +2613,7 @@ extern "C" int ngsamd_mf_create(ngsamd_op_t* op, int dim, int nd, int nq, int64_
                                 int scatter) {
   return Guard([&] {
     RequireInit();
+    NGSAMD_REQUIRE(nclass >= 1, "ngsamd_mf_create: no classes");
     NGSAMD_REQUIRE(op && off && bmat && w && (off[nclass] == 0 || (dofs && metric)), "ngsamd_mf_create: null argument");
     *op = nullptr;
     *op = new MatFreeOp(dim, nd, nq, ndof, nclass, off, bmat, w, dofs, metric, kappa, elnr, precision, scatter);

@@ -114,6 +114,60 @@ def test_mf_split_f16_needs_factored_kernels(device, monkeypatch):
         mf_operator(P, precision=4)
 
 
+def _mf_inputs(P):
+    """The arrays mf_operator passes to CreateMatrixFree, per class."""
+    fes, ir, dim = P["fes"], P["ir"], P["dim"]
+    classes = N.BuildClasses(fes, ir)
+    detj, ginv = N.ComputeGeometry(P["mesh"])
+    out = []
+    for c in classes:
+        el = np.asarray(c["elements"], dtype=np.int32)
+        G = detj[el, None, None] * ginv[el]
+        if dim == 2:
+            met = np.stack([G[:, 0, 0], G[:, 1, 1], G[:, 0, 1]], axis=1)
+        else:
+            met = np.stack([G[:, 0, 0], G[:, 1, 1], G[:, 2, 2], G[:, 0, 1], G[:, 0, 2], G[:, 1, 2]], axis=1)
+        out.append((el, np.asarray(c["bmat"]).ravel(), met, fes.el_dofs[el]))
+    return out
+
+
+def _create(P, parts, precision):
+    fes, ir, dim = P["fes"], P["ir"], P["dim"]
+    off = np.cumsum([0] + [len(p[0]) for p in parts]).astype(np.int64)
+    el = np.ascontiguousarray(np.concatenate([p[0] for p in parts]).astype(np.int32))
+    return N.CreateMatrixFree(dim, fes.nd_el, len(ir), fes.ndof, off,
+                              np.ascontiguousarray(np.concatenate([p[1] for p in parts])),
+                              np.ascontiguousarray(ir.weights),
+                              np.ascontiguousarray(np.concatenate([p[3] for p in parts]).astype(np.int32)),
+                              np.ascontiguousarray(np.concatenate([p[2] for p in parts])), None, el, precision, 0)
+
+
+# Ragged inputs: a class without elements between two others gives the same
+# bits (rounds and tiles skip it); an operator without elements applies 0.
+@pytest.mark.parametrize("precision", [0, 2])
+def test_mf_empty_class_and_no_elements(device, precision):
+    say(f"[mf empty class] precision={precision}")
+    P = problem(3, 2, 3, seed_kappa=None)
+    parts = _mf_inputs(P)
+    assert len(parts) >= 2
+    A = _create(P, parts, precision)
+    empty = (np.zeros(0, np.int32), parts[0][1], parts[0][2][:0], parts[0][3][:0])
+    B = _create(P, parts[:1] + [empty] + parts[1:], precision)
+    x = np.random.default_rng(2).uniform(-1, 1, P["ndof"])
+    dx, ya, yb = dvec(x), N.DeviceVector(P["ndof"], 8), N.DeviceVector(P["ndof"], 8)
+    A.mult(dx, ya)
+    B.mult(dx, yb)
+    assert np.array_equal(ya.download(), yb.download())
+    Z = _create(P, [empty], precision)
+    y0 = np.random.default_rng(3).uniform(-1, 1, P["ndof"])
+    dz = dvec(y0)
+    Z.mult(dx, dz)
+    assert not np.any(dz.download())
+    dz = dvec(y0)
+    Z.mult_add(2.0, dx, dz)
+    assert np.array_equal(dz.download(), y0)
+
+
 @pytest.mark.parametrize("dim,n,p", [(2, 6, 3), (3, 3, 4)])
 def test_mf_diagonal(device, dim, n, p):
     say(f"[mf diag] dim={dim} p={p}")
