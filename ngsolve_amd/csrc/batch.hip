@@ -1469,6 +1469,7 @@ extern "C" int ngsamd_batch_csr_create(ngsamd_op_t* op, int64_t n, const int64_t
                                        const double* vals, int nseg, const int64_t* seg, int with_f32) {
   return Guard([&] {
     RequireInit();
+    NGSAMD_REQUIRE(n >= 0 && nseg >= 0, "ngsamd_batch_csr_create: negative size");
     NGSAMD_REQUIRE(op && rp && seg && (rp[n] == 0 || (cols && vals)), "ngsamd_batch_csr_create: null argument");
     *op = nullptr;
     *op = new BatchOp(n, rp, cols, vals, nseg, seg, with_f32 != 0);
