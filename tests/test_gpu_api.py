@@ -149,3 +149,28 @@ def test_matrix_free_bilinearform(device):
     y2.data = amf.mat * x
     d = y1.NumPy() - y2.NumPy()
     assert np.abs(d).max() <= 1e-13 * np.abs(y1.NumPy()).max()
+
+
+@pytest.mark.parametrize("precision", ["f32", "f16x3", "bf16"])
+def test_ircgsolver_with_matrix_free_inner(device, precision):
+    """IRCGSolver: FP64 matrix-free residuals around a low-precision
+    matrix-free inner CG (MFOpts(precision=...)) reaches the FP64 CG's
+    solution."""
+    say(f"[api] IRCGSolver inner {precision}")
+    m = ngs.MakeStructured3DMesh(nx=4)
+    fes = ngs.H1(m, order=4, dirichlet=".*")
+    u, v = fes.TnT()
+    kappa = ngs.ElementCoefficient(np.exp(np.random.default_rng(2).standard_normal(m.ne)))
+    form = kappa * ngs.grad(u) * ngs.grad(v) * ngs.dx(bonus_intorder=2)
+    a64 = ngs.BilinearForm(form, mf=ngs.MFOpts()).Assemble()
+    alo = ngs.BilinearForm(form, mf=ngs.MFOpts(precision=precision)).Assemble()
+    f = ngs.LinearForm(1 * v * ngs.dx).Assemble()
+    pre = a64.mat.CreateSmoother(fes.FreeDofs)
+    ir = ngs.IRCGSolver(a64.mat, alo.mat, pre, tol=1e-10)
+    cg = ngs.DevCGSolver(a64.mat, pre, tol=1e-12, maxiter=5000)
+    u1, u2 = f.vec.CreateVector(), f.vec.CreateVector()
+    u1.data = ir * f.vec
+    u2.data = cg * f.vec
+    say(f"[api] outer {ir.outer_iterations} inner {ir.inner_iterations} residual {ir.relative_residual:.2e}")
+    assert ir.relative_residual <= 1e-10
+    assert np.linalg.norm(u1.NumPy() - u2.NumPy()) <= 1e-8 * np.linalg.norm(u2.NumPy())
