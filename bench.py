@@ -404,8 +404,10 @@ def main():
     ap.add_argument("--profile-cg", action="store_true",
                     help="one FP64 CSR CG solve (kappa = 1, tol 1e-10) without graph capture, for a kernel trace")
     ap.add_argument("--profile-ir", action="store_true",
-                    help="one FP32-inner refinement solve (lognormal kappa, tol 1e-10) without graph capture, "
-                         "for a kernel trace")
+                    help="one refinement solve (lognormal kappa, tol 1e-10) without graph capture, for a kernel "
+                         "trace; inner precision from --profile-ir-precision")
+    ap.add_argument("--profile-ir-precision", default="f32", choices=["f32", "bf16", "f16", "f16x3"],
+                    help="inner operator precision of --profile-ir (default f32)")
     ap.add_argument("--profile-mode", action="store_true",
                     help="short run for rocprofv3 counter passes: a few CSR and matrix-free applies, one "
                          "1 GiB device-to-device copy and one 1 GiB streaming read (the calibration kernels of "
@@ -469,7 +471,8 @@ def main():
         f = N.AssembleLinearForm(fes, ir, np.ones((ne, len(ir))))
         df, du = N.DeviceVector(ndof, 8), N.DeviceVector(ndof, 8)
         df.upload(f)
-        M64, M32 = build_mf(N, fes, mesh, ir, kappa, 0), build_mf(N, fes, mesh, ir, kappa, 1)
+        inner_prec = {"f32": 1, "bf16": 2, "f16": 3, "f16x3": 4}[args.profile_ir_precision]
+        M64, M32 = build_mf(N, fes, mesh, ir, kappa, 0), build_mf(N, fes, mesh, ir, kappa, inner_prec)
         outer, inner, rel = N.IRCGSolve(M64, M32, Ck, df, du, 1e-10, -1.0, 60, 200000)
         N.synchronize()
         log(f"[bench] profile IR done: outer {outer} inner {inner} rel {rel:.1e}")
